@@ -1,0 +1,663 @@
+/*
+ * dgx_api.cpp -- host side of the solver C-ABI (include/dgx.h): device-resident mesh and fields,
+ * launch sequencing and the step graph.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "dgx.h"
+#include "dgx_kernels.h"
+
+extern "C" {
+hipError_t dgx_launch_f64(const void *args, int nshots, hipStream_t st, int which);
+hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int which);
+hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st);
+hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st);
+hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
+}
+
+/* ---- error reporting ---- */
+static thread_local char g_err[1024] = "";
+
+extern "C" void dgx_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *dgx_last_error(void) { return g_err; }
+
+#define HIP_TRY(call)                                                                                          \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess) {                                                                                \
+            dgx_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__);          \
+            return DGX_EHIP;                                                                                   \
+        }                                                                                                      \
+    } while (0)
+
+namespace {
+
+enum { K_VEL_FUSED = 0, K_STRESS_FUSED = 1, K_VEL = 2, K_STRESS = 3, K_ABSORB = 4 };
+const int kGraphSteps = 32; /* steps captured per graph replay */
+
+template <typename T>
+hipError_t upload(T **dst, const std::vector<T> &src)
+{
+    *dst = nullptr;
+    if (src.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void **)dst, sizeof(T) * src.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
+}
+
+/* Morton (Z-order) key of a point in [0,1)^2 with 16 bits per axis. */
+uint32_t morton(double u, double v)
+{
+    auto spread = [](uint32_t x) {
+        x &= 0xffff;
+        x = (x | (x << 8)) & 0x00ff00ff;
+        x = (x | (x << 4)) & 0x0f0f0f0f;
+        x = (x | (x << 2)) & 0x33333333;
+        x = (x | (x << 1)) & 0x55555555;
+        return x;
+    };
+    uint32_t a = (uint32_t)std::min(65535.0, std::max(0.0, u * 65536.0));
+    uint32_t b = (uint32_t)std::min(65535.0, std::max(0.0, v * 65536.0));
+    return spread(a) | (spread(b) << 1);
+}
+
+inline long long fidx(long long q, int i) { return ((q >> 6) * 6 + i) * 64 + (q & 63); }
+
+} // namespace
+
+struct dgx_solver {
+    int fp32 = 0;
+    long tri = 0, tri_pad = 0, nshots = 1, nsteps = 0, nrec = 0, nsrc = 0;
+    long step = 0;          /* steps completed */
+    long base_host = 0;     /* mirror of the device step counter */
+    int nper = 0;
+    std::vector<long> perm; /* device element -> caller element (0-based) */
+    std::vector<long> inv;  /* caller element -> device element */
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    /* device memory */
+    void *fields = nullptr; /* 5 * nshots * tri_pad*6 reals */
+    int4 *nbr = nullptr, *vtx = nullptr;
+    double2 *pt = nullptr;
+    void *mat = nullptr;    /* rho_inv | lam | mu, tri_pad reals each */
+    void *absorb = nullptr; /* tri_pad*6 reals */
+    dgx_aux *aux = nullptr;
+    dgx_src_dev *src = nullptr;
+    dgx_rec_dev *rec = nullptr;
+    double *wav = nullptr;
+    double *trace = nullptr;
+    int *step_base = nullptr;
+    /* argument blocks */
+    dgx::StepArgs<double> a64;
+    dgx::StepArgs<float> a32;
+    std::vector<dgx_src_dev> src_host;
+
+    const void *args() const { return fp32 ? (const void *)&a32 : (const void *)&a64; }
+    size_t real_size() const { return fp32 ? 4 : 8; }
+};
+
+static int launch(dgx_solver *s, int which, int step_off)
+{
+    hipError_t e;
+    if (s->fp32) {
+        s->a32.step_off = step_off;
+        e = dgx_launch_f32(&s->a32, (int)s->nshots, s->stream, which);
+    } else {
+        s->a64.step_off = step_off;
+        e = dgx_launch_f64(&s->a64, (int)s->nshots, s->stream, which);
+    }
+    if (e != hipSuccess) {
+        dgx_set_error("kernel launch failed: %s", hipGetErrorString(e));
+        return DGX_EHIP;
+    }
+    return DGX_OK;
+}
+
+template <typename R>
+static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *ops, double dt)
+{
+    /* Dr = Vr*inverV, Ds = Vs*inverV in FP64 from the file values (fun_DG.c:1176-1190 forms the same
+     * products after scaling by the metric terms) */
+    for (int i = 0; i < 6; i++)
+        for (int n = 0; n < 6; n++) {
+            double dr = 0, ds = 0;
+            for (int k = 0; k < 6; k++) {
+                dr += ops->Vr[i * 6 + k] * ops->inverV[k * 6 + n];
+                ds += ops->Vs[i * 6 + k] * ops->inverV[k * 6 + n];
+            }
+            a.op.Dr[i * 6 + n] = (R)dr;
+            a.op.Ds[i * 6 + n] = (R)ds;
+        }
+    const double *S[3] = {ops->S1, ops->S2, ops->S3};
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 18; k++) a.op.L[j * 18 + k] = (R)S[j][k];
+    R *f = (R *)s->fields;
+    size_t per_field = (size_t)s->nshots * s->tri_pad * 6;
+    for (int c = 0; c < 5; c++) a.f[c] = f + c * per_field;
+    a.fstride = s->tri_pad;
+    a.nbr = s->nbr;
+    a.vtx = s->vtx;
+    a.pt = s->pt;
+    R *m = (R *)s->mat;
+    a.rho_inv = m;
+    a.lam = m + s->tri_pad;
+    a.mu = m + 2 * s->tri_pad;
+    a.absorb = (const R *)s->absorb;
+    a.aux = s->aux;
+    a.src = s->src;
+    a.rec = s->rec;
+    a.wav = s->wav;
+    a.trace = s->trace;
+    a.step_base = s->step_base;
+    a.step_off = 0;
+    a.nsteps = (int)s->nsteps;
+    a.nrec = (int)s->nrec;
+    a.tri = (int)s->tri;
+    a.nper = s->nper;
+    a.dt = dt;
+}
+
+static void free_solver(dgx_solver *s)
+{
+    if (!s) return;
+    if (s->graph) hipGraphExecDestroy(s->graph);
+    for (auto &e : s->ev)
+        if (e) hipEventDestroy(e);
+    void *ptrs[] = {s->fields, s->nbr, s->vtx, s->pt, s->mat, s->absorb, s->aux, s->src, s->rec, s->wav, s->trace,
+                    s->step_base};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+template <typename R>
+static std::vector<R> convert(const std::vector<double> &v)
+{
+    return std::vector<R>(v.begin(), v.end());
+}
+
+extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_solver **out)
+{
+    if (!p || !ops || !out) { dgx_set_error("dgx_create: null argument"); return DGX_EARG; }
+    *out = nullptr;
+    const long T = p->tri;
+    if (T <= 0 || p->point_num <= 0 || !p->point || !p->triangle || !p->neighbor || !p->lambda || !p->miu ||
+        !p->den) {
+        dgx_set_error("dgx_create: empty mesh or missing arrays");
+        return DGX_EARG;
+    }
+    if (T > (1L << 28) || p->point_num > (1L << 30)) { dgx_set_error("dgx_create: mesh too large"); return DGX_EARG; }
+    if (p->nshots < 1 || p->nsteps < 0 || p->nrec < 0 || p->nsrc < 0 || (p->nsrc > 0 && !p->src) ||
+        (p->nrec > 0 && !p->rec) || (p->nsrc > 0 && (!p->wavelet || p->nwavelets < 1)) ||
+        (p->precision != DGX_FP64 && p->precision != DGX_FP32) || !(p->dt > 0)) {
+        dgx_set_error("dgx_create: invalid shots/steps/sources/receivers/precision/dt");
+        return DGX_EARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        dgx_set_error("dgx_create: no HIP device available");
+        return DGX_EHIP;
+    }
+
+    dgx_solver *s = new dgx_solver();
+    s->fp32 = (p->precision == DGX_FP32);
+    s->tri = T;
+    s->nshots = p->nshots;
+    s->nsteps = p->nsteps;
+    s->nrec = p->nrec;
+    s->nsrc = p->nsrc;
+    long nblk = (T + DGX_BLOCK - 1) / DGX_BLOCK;
+    s->nper = (int)((nblk + DGX_NXCD - 1) / DGX_NXCD);
+    s->tri_pad = (long)s->nper * DGX_NXCD * DGX_BLOCK;
+
+    /* ---- validate vertices, orientation (fun_DG.c:447 assumes J > 0) ---- */
+    const double *P = p->point;
+    const long *Tv = p->triangle;
+    double xmin = P[0], xmax = P[0], zmin = P[1], zmax = P[1];
+    for (long i = 1; i < p->point_num; i++) {
+        xmin = std::min(xmin, P[2 * i]); xmax = std::max(xmax, P[2 * i]);
+        zmin = std::min(zmin, P[2 * i + 1]); zmax = std::max(zmax, P[2 * i + 1]);
+    }
+    std::vector<uint64_t> key(T);
+    for (long e = 0; e < T; e++) {
+        for (int k = 0; k < 3; k++)
+            if (Tv[3 * e + k] < 1 || Tv[3 * e + k] > p->point_num) {
+                dgx_set_error("element %ld: vertex id %ld out of range", e + 1, Tv[3 * e + k]);
+                free_solver(s);
+                return DGX_EMESH;
+            }
+        const double *a = &P[2 * (Tv[3 * e] - 1)], *b = &P[2 * (Tv[3 * e + 1] - 1)], *c = &P[2 * (Tv[3 * e + 2] - 1)];
+        double D = (b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]);
+        if (!(D > 0)) {
+            dgx_set_error("element %ld is not counter-clockwise (2*area = %g)", e + 1, D);
+            free_solver(s);
+            return DGX_EMESH;
+        }
+        double cx = (a[0] + b[0] + c[0]) / 3, cz = (a[1] + b[1] + c[1]) / 3;
+        double ext = std::max(xmax - xmin, zmax - zmin);
+        if (!(ext > 0)) ext = 1;
+        key[e] = ((uint64_t)morton((cx - xmin) / ext, (cz - zmin) / ext) << 32) | (uint64_t)e;
+    }
+    /* ---- locality order: sort by Morton key (ties by caller index) ---- */
+    std::sort(key.begin(), key.end());
+    s->perm.resize(T);
+    s->inv.resize(T);
+    for (long q = 0; q < T; q++) {
+        s->perm[q] = (long)(key[q] & 0xffffffffu);
+        s->inv[s->perm[q]] = q;
+    }
+
+    /* ---- sponge flags ---- */
+    std::vector<char> sponge(T, 0);
+    if (p->absorb_fun)
+        for (long e = 0; e < T; e++)
+            for (int i = 0; i < 6; i++)
+                if (p->absorb_fun[6 * e + i] != 1.0) sponge[e] = 1;
+
+    /* ---- sources / receivers per element ---- */
+    std::vector<std::vector<int>> src_of(0), rec_of(0);
+    std::vector<char> explosive(T, 0);
+    std::vector<int> elem_aux(T, -1);
+    std::vector<long> aux_elems;
+    auto aux_slot = [&](long e) {
+        if (elem_aux[e] < 0) {
+            elem_aux[e] = (int)aux_elems.size();
+            aux_elems.push_back(e);
+            src_of.emplace_back();
+            rec_of.emplace_back();
+        }
+        return elem_aux[e];
+    };
+    for (long k = 0; k < p->nsrc; k++) {
+        const dgx_source &sr = p->src[k];
+        if (sr.elem < 1 || sr.elem > T || sr.shot < 0 || sr.shot >= p->nshots || sr.wavelet < 0 ||
+            sr.wavelet >= p->nwavelets || (sr.kind != DGX_SRC_FORCE_Z && sr.kind != DGX_SRC_EXPLOSIVE)) {
+            dgx_set_error("source %ld: invalid element/shot/wavelet/kind", k);
+            free_solver(s);
+            return DGX_EARG;
+        }
+        src_of[aux_slot(sr.elem - 1)].push_back((int)k);
+        if (sr.kind == DGX_SRC_EXPLOSIVE) explosive[sr.elem - 1] = 1;
+    }
+    for (long k = 0; k < p->nrec; k++) {
+        const dgx_receiver &rc = p->rec[k];
+        if (rc.elem < 1 || rc.elem > T) {
+            dgx_set_error("receiver %ld: element %ld out of range", k, rc.elem);
+            free_solver(s);
+            return DGX_EARG;
+        }
+        rec_of[aux_slot(rc.elem - 1)].push_back((int)k);
+    }
+    if (aux_elems.size() > DGX_MAX_AUX) { dgx_set_error("too many source/receiver elements"); free_solver(s); return DGX_EARG; }
+    std::vector<dgx_aux> aux_host(aux_elems.size());
+    std::vector<dgx_src_dev> src_host;
+    std::vector<dgx_rec_dev> rec_host;
+    for (size_t x = 0; x < aux_elems.size(); x++) {
+        long q = s->inv[aux_elems[x]];
+        aux_host[x].src_begin = (int)src_host.size();
+        for (int k : src_of[x]) {
+            const dgx_source &sr = p->src[k];
+            dgx_src_dev d;
+            d.elem = (int)q;
+            d.kind = sr.kind;
+            d.shot = sr.shot;
+            d.wavelet = (int)sr.wavelet;
+            for (int i = 0; i < 6; i++) d.fun[i] = sr.fun[i];
+            src_host.push_back(d);
+        }
+        aux_host[x].src_count = (int)src_host.size() - aux_host[x].src_begin;
+        aux_host[x].rec_begin = (int)rec_host.size();
+        for (int k : rec_of[x]) {
+            dgx_rec_dev d;
+            d.elem = (int)q;
+            d.index = k;
+            d.pad0 = d.pad1 = 0;
+            for (int i = 0; i < 6; i++) d.w[i] = p->rec[k].weight[i];
+            rec_host.push_back(d);
+        }
+        aux_host[x].rec_count = (int)rec_host.size() - aux_host[x].rec_begin;
+    }
+    /* split-path source list in the caller's order (k_source applies them in list order) */
+    std::vector<dgx_src_dev> src_list(p->nsrc);
+    for (long k = 0; k < p->nsrc; k++) {
+        const dgx_source &sr = p->src[k];
+        src_list[k].elem = (int)s->inv[sr.elem - 1];
+        src_list[k].kind = sr.kind;
+        src_list[k].shot = sr.shot;
+        src_list[k].wavelet = (int)sr.wavelet;
+        for (int i = 0; i < 6; i++) src_list[k].fun[i] = sr.fun[i];
+    }
+    s->src_host = src_list;
+
+    /* ---- topology: face j of element e runs v_j -> v_{j+1} (j = 0, 1) or v0 -> v2 (j = 2); Triangle's
+     * .neigh column c holds the element opposite vertex c (fun_DG.c:576-578) ---- */
+    static const int col_of_face[3] = {2, 0, 1};
+    std::vector<int4> nbr_host(s->tri_pad), vtx_host(s->tri_pad);
+    for (long q = 0; q < s->tri_pad; q++) {
+        nbr_host[q] = make_int4(-1, -1, -1, 0);
+        vtx_host[q] = make_int4(0, 0, 0, 0);
+    }
+    for (long q = 0; q < T; q++) {
+        long e = s->perm[q];
+        const long *v = &Tv[3 * e];
+        int nbq[3] = {-1, -1, -1};
+        unsigned code = 0;
+        for (int j = 0; j < 3; j++) {
+            long nb = p->neighbor[3 * e + col_of_face[j]];
+            if (nb == -1 || nb == 0) continue;
+            if (nb < 1 || nb > T) {
+                dgx_set_error("element %ld: neighbour %ld out of range", e + 1, nb);
+                free_solver(s);
+                return DGX_EMESH;
+            }
+            const long *w = &Tv[3 * (nb - 1)];
+            long fa = (j == 2) ? v[0] : v[j], fb = (j == 2) ? v[2] : v[j + 1];
+            int m = -1;
+            const long nb_face[3][2] = {{w[0], w[1]}, {w[1], w[2]}, {w[0], w[2]}};
+            for (int t = 0; t < 3; t++)
+                if ((nb_face[t][0] == fa && nb_face[t][1] == fb) || (nb_face[t][0] == fb && nb_face[t][1] == fa)) m = t;
+            if (m < 0) {
+                dgx_set_error("element %ld face %d: neighbour %ld does not share that edge", e + 1, j, nb);
+                free_solver(s);
+                return DGX_EMESH;
+            }
+            unsigned rev = (nb_face[m][0] != fa) ? 1u : 0u; /* start vertices differ (fun_DG.c:1225,1251) */
+            nbq[j] = (int)s->inv[nb - 1];
+            code |= ((unsigned)m | (rev << 2)) << (3 * j);
+            if (sponge[nb - 1] || explosive[nb - 1]) code |= DGX_CODE_NBSPECIAL(j);
+        }
+        if (sponge[e]) code |= DGX_CODE_SPONGE;
+        if (elem_aux[e] >= 0) code |= (unsigned)(elem_aux[e] + 1) << DGX_CODE_AUX_SHIFT;
+        nbr_host[q] = make_int4(nbq[0], nbq[1], nbq[2], (int)code);
+        vtx_host[q] = make_int4((int)(v[0] - 1), (int)(v[1] - 1), (int)(v[2] - 1), 0);
+    }
+    std::vector<double2> pt_host(p->point_num);
+    for (long i = 0; i < p->point_num; i++) pt_host[i] = make_double2(P[2 * i], P[2 * i + 1]);
+
+    /* ---- materials (Elastic_A_B, fun_DG.c:1071-1088): 1/rho, lambda, mu per element ---- */
+    std::vector<double> mat_host(3 * s->tri_pad, 0.0), absorb_host(6 * s->tri_pad, 1.0);
+    for (long q = 0; q < T; q++) {
+        long e = s->perm[q];
+        if (!(p->den[e] > 0)) { dgx_set_error("element %ld: density must be positive", e + 1); free_solver(s); return DGX_EARG; }
+        mat_host[q] = 1 / p->den[e];
+        mat_host[s->tri_pad + q] = p->lambda[e];
+        mat_host[2 * s->tri_pad + q] = p->miu[e];
+        if (p->absorb_fun)
+            for (int i = 0; i < 6; i++) absorb_host[fidx(q, i)] = p->absorb_fun[6 * e + i];
+    }
+
+    /* ---- device allocations ---- */
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    for (auto &ev : s->ev) HIP_TRY(hipEventCreate(&ev));
+    size_t nreal_fields = (size_t)5 * s->nshots * s->tri_pad * 6;
+    HIP_TRY(hipMalloc(&s->fields, nreal_fields * s->real_size()));
+    HIP_TRY(hipMemset(s->fields, 0, nreal_fields * s->real_size()));
+    HIP_TRY(upload(&s->nbr, nbr_host));
+    HIP_TRY(upload(&s->vtx, vtx_host));
+    HIP_TRY(upload(&s->pt, pt_host));
+    if (s->fp32) {
+        float *m, *a;
+        HIP_TRY(upload(&m, convert<float>(mat_host)));
+        HIP_TRY(upload(&a, convert<float>(absorb_host)));
+        s->mat = m;
+        s->absorb = a;
+    } else {
+        double *m, *a;
+        HIP_TRY(upload(&m, mat_host));
+        HIP_TRY(upload(&a, absorb_host));
+        s->mat = m;
+        s->absorb = a;
+    }
+    HIP_TRY(upload(&s->aux, aux_host));
+    {
+        /* fused kernels read sources through the aux ranges; the split source kernel reads the list in
+         * caller order: keep both, aux-ordered first */
+        std::vector<dgx_src_dev> all = src_host;
+        all.insert(all.end(), src_list.begin(), src_list.end());
+        HIP_TRY(upload(&s->src, all));
+    }
+    HIP_TRY(upload(&s->rec, rec_host));
+    if (p->nsrc > 0) {
+        std::vector<double> w(p->wavelet, p->wavelet + (size_t)p->nwavelets * p->nsteps);
+        HIP_TRY(upload(&s->wav, w));
+    }
+    size_t ntrace = (size_t)s->nshots * s->nsteps * 2 * s->nrec;
+    if (ntrace) {
+        HIP_TRY(hipMalloc((void **)&s->trace, ntrace * sizeof(double)));
+        HIP_TRY(hipMemset(s->trace, 0, ntrace * sizeof(double)));
+    }
+    HIP_TRY(hipMalloc((void **)&s->step_base, sizeof(int)));
+    HIP_TRY(hipMemset(s->step_base, 0, sizeof(int)));
+    HIP_TRY(hipDeviceSynchronize());
+
+    if (s->fp32) fill_args(s, s->a32, ops, p->dt);
+    else fill_args(s, s->a64, ops, p->dt);
+    *out = s;
+    return DGX_OK;
+}
+
+extern "C" int dgx_destroy(dgx_solver *s)
+{
+    if (!s) return DGX_EARG;
+    hipStreamSynchronize(s->stream);
+    free_solver(s);
+    return DGX_OK;
+}
+
+static int step_eager(dgx_solver *s)
+{
+    int off = (int)(s->step - s->base_host);
+    int rc = launch(s, K_VEL_FUSED, off);
+    if (rc) return rc;
+    rc = launch(s, K_STRESS_FUSED, off);
+    if (rc) return rc;
+    s->step++;
+    return DGX_OK;
+}
+
+static int build_graph(dgx_solver *s)
+{
+    /* K steps with offsets 0..K-1 from the device counter, then advance the counter by K */
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DGX_OK;
+    for (int k = 0; k < kGraphSteps && rc == DGX_OK; k++) {
+        rc = launch(s, K_VEL_FUSED, k);
+        if (rc == DGX_OK) rc = launch(s, K_STRESS_FUSED, k);
+    }
+    if (rc == DGX_OK && dgx_launch_advance(s->step_base, kGraphSteps, s->stream) != hipSuccess) rc = DGX_EHIP;
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (rc != DGX_OK) return rc;
+    if (e != hipSuccess) { dgx_set_error("graph capture failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
+    e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) { dgx_set_error("graph instantiate failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
+    return DGX_OK;
+}
+
+extern "C" int dgx_step(dgx_solver *s, long n)
+{
+    if (!s || n < 0) return DGX_EARG;
+    if (s->step + n > s->nsteps) {
+        dgx_set_error("dgx_step: %ld + %ld steps exceed the capacity of %ld", s->step, n, s->nsteps);
+        return DGX_ESTATE;
+    }
+    while (n > 0) {
+        if (n >= kGraphSteps) {
+            /* a replay works on steps *step_base + 0..K-1: bring the device counter up to date first */
+            if (s->step != s->base_host) {
+                if (dgx_launch_advance(s->step_base, (int)(s->step - s->base_host), s->stream) != hipSuccess) {
+                    dgx_set_error("step counter update failed");
+                    return DGX_EHIP;
+                }
+                s->base_host = s->step;
+            }
+            if (!s->graph) {
+                int rc = build_graph(s);
+                if (rc) return rc;
+            }
+            HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+            s->step += kGraphSteps;
+            s->base_host += kGraphSteps;
+            n -= kGraphSteps;
+            continue;
+        }
+        int rc = step_eager(s);
+        if (rc) return rc;
+        n--;
+    }
+    return DGX_OK;
+}
+
+extern "C" int dgx_absorb_boundary(dgx_solver *s) { return s ? launch(s, K_ABSORB, 0) : DGX_EARG; }
+extern "C" int dgx_lu_velocity(dgx_solver *s) { return s ? launch(s, K_VEL, 0) : DGX_EARG; }
+extern "C" int dgx_lu_stress_elastic(dgx_solver *s) { return s ? launch(s, K_STRESS, 0) : DGX_EARG; }
+
+extern "C" int dgx_source_kernel(dgx_solver *s, double w)
+{
+    if (!s) return DGX_EARG;
+    if (s->nsrc == 0) return DGX_OK;
+    /* the caller-ordered list follows the aux-ordered copy in the same allocation */
+    dgx::StepArgs<double> a64 = s->a64;
+    dgx::StepArgs<float> a32 = s->a32;
+    a64.src = s->src + s->nsrc;
+    a32.src = s->src + s->nsrc;
+    hipError_t e = dgx_launch_source(s->fp32 ? (const void *)&a32 : (const void *)&a64, s->fp32, (int)s->nsrc, w,
+                                     s->stream);
+    if (e != hipSuccess) { dgx_set_error("source launch failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
+    return DGX_OK;
+}
+
+extern "C" int dgx_record(dgx_solver *s, long t)
+{
+    if (!s || t < 0 || t >= s->nsteps) return DGX_EARG;
+    hipError_t e = dgx_launch_record(s->args(), s->fp32, (int)t, (int)s->nshots, (int)s->nrec, s->stream);
+    if (e != hipSuccess) { dgx_set_error("record launch failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
+    return DGX_OK;
+}
+
+template <typename R>
+static void unpack(const dgx_solver *s, const std::vector<char> &raw, double *out)
+{
+    const R *r = (const R *)raw.data();
+    for (long q = 0; q < s->tri; q++) {
+        long e = s->perm[q];
+        for (int i = 0; i < 6; i++) out[6 * e + i] = (double)r[fidx(q, i)];
+    }
+}
+
+template <typename R>
+static void pack(const dgx_solver *s, const double *in, std::vector<char> &raw)
+{
+    R *r = (R *)raw.data();
+    for (long q = 0; q < s->tri; q++) {
+        long e = s->perm[q];
+        for (int i = 0; i < 6; i++) r[fidx(q, i)] = (R)in[6 * e + i];
+    }
+}
+
+extern "C" int dgx_get_fields(dgx_solver *s, long shot, double *vx, double *vz, double *txx, double *tzz, double *txz)
+{
+    if (!s || shot < 0 || shot >= s->nshots) return DGX_EARG;
+    double *outs[5] = {vx, vz, txx, tzz, txz};
+    size_t rs = s->real_size(), n = (size_t)s->tri_pad * 6;
+    std::vector<char> raw(n * rs);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int c = 0; c < 5; c++) {
+        if (!outs[c]) continue;
+        const char *src = (const char *)s->fields + ((size_t)c * s->nshots + shot) * n * rs;
+        HIP_TRY(hipMemcpy(raw.data(), src, n * rs, hipMemcpyDeviceToHost));
+        if (s->fp32) unpack<float>(s, raw, outs[c]);
+        else unpack<double>(s, raw, outs[c]);
+    }
+    return DGX_OK;
+}
+
+extern "C" int dgx_set_fields(dgx_solver *s, long shot, const double *vx, const double *vz, const double *txx,
+                              const double *tzz, const double *txz)
+{
+    if (!s || shot < 0 || shot >= s->nshots) return DGX_EARG;
+    const double *ins[5] = {vx, vz, txx, tzz, txz};
+    size_t rs = s->real_size(), n = (size_t)s->tri_pad * 6;
+    std::vector<char> raw(n * rs, 0);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int c = 0; c < 5; c++) {
+        if (!ins[c]) continue;
+        std::fill(raw.begin(), raw.end(), 0);
+        if (s->fp32) pack<float>(s, ins[c], raw);
+        else pack<double>(s, ins[c], raw);
+        char *dst = (char *)s->fields + ((size_t)c * s->nshots + shot) * n * rs;
+        HIP_TRY(hipMemcpy(dst, raw.data(), n * rs, hipMemcpyHostToDevice));
+    }
+    return DGX_OK;
+}
+
+extern "C" int dgx_get_traces(dgx_solver *s, long shot, double *rec_x, double *rec_z)
+{
+    if (!s || shot < 0 || shot >= s->nshots) return DGX_EARG;
+    if (s->nrec == 0 || s->nsteps == 0) return DGX_OK;
+    std::vector<double> t((size_t)s->nsteps * 2 * s->nrec);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(t.data(), s->trace + (size_t)shot * s->nsteps * 2 * s->nrec, t.size() * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    for (long st = 0; st < s->nsteps; st++)
+        for (long k = 0; k < s->nrec; k++) {
+            if (rec_x) rec_x[st * s->nrec + k] = t[(st * 2 + 0) * s->nrec + k];
+            if (rec_z) rec_z[st * s->nrec + k] = t[(st * 2 + 1) * s->nrec + k];
+        }
+    return DGX_OK;
+}
+
+extern "C" int dgx_synchronize(dgx_solver *s)
+{
+    if (!s) return DGX_EARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return DGX_OK;
+}
+
+extern "C" long dgx_current_step(const dgx_solver *s) { return s ? s->step : -1; }
+
+extern "C" int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stress)
+{
+    /* n fused steps, each kernel bracketed by events on the solver's stream */
+    if (!s || n <= 0 || s->step + n > s->nsteps) return DGX_EARG;
+    double tv = 0, ts = 0;
+    for (long k = 0; k < n; k++) {
+        int off = (int)(s->step - s->base_host);
+        HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+        int rc = launch(s, K_VEL_FUSED, off);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+        rc = launch(s, K_STRESS_FUSED, off);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(s->ev[2], s->stream));
+        HIP_TRY(hipEventSynchronize(s->ev[2]));
+        float a = 0, b = 0;
+        HIP_TRY(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+        tv += a;
+        ts += b;
+        s->step++;
+    }
+    if (ms_velocity) *ms_velocity = tv / n;
+    if (ms_stress) *ms_stress = ts / n;
+    return DGX_OK;
+}

@@ -1,0 +1,83 @@
+/*
+ * dgx_kernels.h -- device-side data layout shared by the HIP kernels and the host code of libdgx.
+ *
+ * Element order on the device is a locality order (Morton order of element centroids) chosen at
+ * dgx_create; all per-element arrays use that order.  Fields of one shot are stored in 64-element
+ * blocks, node-major inside a block: value (q, i) lives at ((q/64)*6 + i)*64 + q%64, so one
+ * wavefront loading node i of its 64 elements reads 64 consecutive values.
+ */
+#ifndef DGX_KERNELS_H
+#define DGX_KERNELS_H
+
+#include <stdint.h>
+
+#define DGX_BLOCK 256          /* threads (= elements) per workgroup in the plain kernels */
+#define DGX_NXCD 8             /* MI355X accelerator complex dies; workgroups are dealt round-robin */
+
+/* Per-element code word (the .w of the int4 neighbour record):
+ *   bits 3j..3j+2  face j: matched neighbour face (2 bits) | reversed (1 bit)
+ *   bit  9+j       neighbour across face j needs conditioning (sponge or explosive source)
+ *   bit  12        own element lies in the sponge (some factor != 1)
+ *   bits 13..31    aux record index + 1 (0 = no source / receiver on this element) */
+#define DGX_CODE_NBSPECIAL(j) (1u << (9 + (j)))
+#define DGX_CODE_SPONGE (1u << 12)
+#define DGX_CODE_AUX_SHIFT 13
+#define DGX_MAX_AUX ((1u << 19) - 2)
+
+/* Aux record of one element: its sources and receivers as index ranges. */
+typedef struct {
+    int src_begin, src_count, rec_begin, rec_count;
+} dgx_aux;
+
+typedef struct {
+    int elem;    /* device element index */
+    int kind;    /* 0 force (Vz), 1 explosive (Txx, Tzz) */
+    int shot;
+    int wavelet; /* wavelet row */
+    double fun[6];
+} dgx_src_dev;
+
+typedef struct {
+    int elem;  /* device element index */
+    int index; /* receiver index in the caller's list */
+    int pad0, pad1;
+    double w[6];
+} dgx_rec_dev;
+
+
+#ifdef __cplusplus
+#include <hip/hip_runtime.h>
+namespace dgx {
+
+template <typename R>
+struct Ops {
+    R Dr[36], Ds[36];
+    R L[54]; /* L[j*18 + i*3 + k] = S_{j+1}[i][k] */
+};
+
+template <typename R>
+struct StepArgs {
+    Ops<R> op;
+    R *f[5];               /* Vx, Vz, Txx, Tzz, Txz of shot 0 */
+    long long fstride;     /* elements between shots in each field array */
+    const int4 *nbr;       /* {nb0, nb1, nb2, code} */
+    const int4 *vtx;       /* {v0, v1, v2, -} */
+    const double2 *pt;     /* vertex coordinates */
+    const R *rho_inv, *lam, *mu;
+    const R *absorb;       /* blocked like a field, read only where the sponge bit is set */
+    const dgx_aux *aux;
+    const dgx_src_dev *src;
+    const dgx_rec_dev *rec;
+    const double *wav;     /* [nwavelets][nsteps] */
+    double *trace;         /* [shot][step][2][nrec] */
+    const int *step_base;  /* device step counter; this launch works on step *step_base + step_off */
+    int step_off;
+    int nsteps, nrec;
+    int tri, nper;         /* elements; blocks per XCD slice */
+    double dt;
+};
+
+} // namespace dgx
+#endif
+
+#endif

@@ -1,0 +1,451 @@
+/*
+ * dgx_plain.hip -- element-per-thread kernels of the velocity and stress half-steps (gfx950).
+ *
+ * One thread advances one element of one shot.  The reference-element operators Dr = Vr*inverV,
+ * Ds = Vs*inverV and the face lifts S1..S3 arrive as kernel arguments (uniform, so they sit in
+ * scalar registers / the scalar cache).  Geometric factors are recomputed in FP64 from the three
+ * vertex coordinates; with D = 2*area = 4J (J as in fun_DG.c:447):
+ *   rx = 2(z2-z0)/D, rz = -2(x2-x0)/D, sx = -2(z1-z0)/D, sz = 2(x1-x0)/D     (fun_DG.c:1157-1165)
+ *   face j with counter-clockwise edge vector (ex_j, ez_j):
+ *   0.5 * n_j * (L_j/2) / J = (ez_j, -ex_j) / D                              (fun_DG.c:1118-1131)
+ * so the reference's per-element Dx/Dz/Q arrays (fun_DG.c:1091-1192) are never materialised.
+ *
+ * Velocity (OpenMP_Function.c:75-185), with the material factored out and the Dx/Dz products
+ * regrouped as Dr(rx u + rz w) + Ds(sx u + sz w):
+ *   Vx += dt/rho [ Dr(rx Txx + rz Txz) + Ds(sx Txx + sz Txz) + 1/D sum_j S_j (ez_j dTxx_j - ex_j dTxz_j) ]
+ *   Vz += dt/rho [ Dr(rx Txz + rz Tzz) + Ds(sx Txz + sz Tzz) + 1/D sum_j S_j (ez_j dTxz_j - ex_j dTzz_j) ]
+ * Stress (OpenMP_Function.c:188-306) in strain-rate form:
+ *   exx = Dx Vx + 1/D sum_j S_j(ez_j dVx_j),  ezz = Dz Vz + 1/D sum_j S_j(-ex_j dVz_j),
+ *   exz = Dx Vz + Dz Vx + 1/D sum_j S_j(ez_j dVz_j - ex_j dVx_j)
+ *   Txx += dt[(l+2m) exx + l ezz],  Tzz += dt[l exx + (l+2m) ezz],  Txz += dt m exz
+ * The face jump is dU = U_nb - U_own (U_nb = 0 on the outer boundary, fun_DG.c:724-736).
+ *
+ * Fused variants fold the rest of the reference step into these two kernels: the sponge multiply
+ * (OpenMP_Function.c:25-46) is applied to every value as it is loaded, the source increment
+ * (OpenMP_Function.c:49-58) right after it, and the receiver pick (OpenMP_Function.c:61-72) is done by
+ * the stress kernel from the velocities it already holds.  Each field is read once per half-step.
+ */
+#include <hip/hip_runtime.h>
+
+#include "dgx_kernels.h"
+
+namespace dgx {
+
+__device__ __forceinline__ int fidx(int q, int i) { return ((q >> 6) * 6 + i) * 64 + (q & 63); }
+
+/* Workgroups are dealt to the 8 XCDs round-robin; give each XCD one contiguous slice of the
+ * (Morton-ordered) elements so that face neighbours mostly share that XCD's L2. */
+__device__ __forceinline__ int element_of_thread(int nper)
+{
+    const int b = blockIdx.x;
+    return ((b & (DGX_NXCD - 1)) * nper + (b >> 3)) * DGX_BLOCK + threadIdx.x;
+}
+
+/* Local node of face j, position k, in the reference's face order (ibe, fun_DG.c:1048-1056). */
+__device__ __forceinline__ constexpr int face_node(int j, int k)
+{
+    return (j == 0) ? (k == 0 ? 5 : k == 1 ? 3 : 0) : (j == 1) ? k : (k == 0 ? 5 : k == 1 ? 4 : 2);
+}
+
+/* Neighbour's node facing own face position k, given the neighbour's face m and orientation. */
+__device__ __forceinline__ int nb_node(unsigned m, unsigned rev, int k)
+{
+    const unsigned long long tab = 0x245210035ull; /* nibbles: 5,3,0, 0,1,2, 5,4,2 (low first) */
+    const int kk = rev ? 2 - k : k;
+    return (int)((tab >> (4 * (3 * m + kk))) & 15);
+}
+
+struct Geom {
+    double rx, rz, sx, sz, invD;
+    double ex[3], ez[3];
+};
+
+__device__ __forceinline__ Geom geometry(const double2 *pt, int4 v)
+{
+    const double2 p0 = pt[v.x], p1 = pt[v.y], p2 = pt[v.z];
+    Geom g;
+    g.ex[0] = p1.x - p0.x; g.ez[0] = p1.y - p0.y;
+    g.ex[1] = p2.x - p1.x; g.ez[1] = p2.y - p1.y;
+    g.ex[2] = p0.x - p2.x; g.ez[2] = p0.y - p2.y;
+    const double D = g.ex[0] * (p2.y - p0.y) - (p2.x - p0.x) * g.ez[0];
+    g.invD = 1.0 / D;
+    const double t = 2.0 * g.invD;
+    g.rx = (p2.y - p0.y) * t;
+    g.rz = -(p2.x - p0.x) * t;
+    g.sx = -g.ez[0] * t;
+    g.sz = g.ex[0] * t;
+    return g;
+}
+
+/* Sponge and explosive source applied to a stress value of element q at node i (fused step). */
+template <typename R>
+__device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned code, int q, int i, int shot, int step,
+                                                 R &txx, R &tzz, R &txz)
+{
+    if (code & DGX_CODE_SPONGE) {
+        const R f = a.absorb[fidx(q, i)];
+        txx *= f; tzz *= f; txz *= f;
+    }
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (ax) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+            const dgx_src_dev &src = a.src[s];
+            if (src.shot != shot || src.kind != 1) continue;
+            const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
+            const R inc = (R)(src.fun[i] * w);
+            txx += inc;
+            tzz += inc;
+        }
+    }
+}
+
+template <typename R, bool FUSED>
+__global__ void __launch_bounds__(DGX_BLOCK) k_velocity(StepArgs<R> a)
+{
+    const int q = element_of_thread(a.nper);
+    if (q >= a.tri) return;
+    const int shot = blockIdx.y;
+    const long long so = (long long)shot * a.fstride * 6;
+    R *__restrict__ Vx = a.f[0] + so;
+    R *__restrict__ Vz = a.f[1] + so;
+    const R *__restrict__ Txx = a.f[2] + so;
+    const R *__restrict__ Tzz = a.f[3] + so;
+    const R *__restrict__ Txz = a.f[4] + so;
+    const int step = FUSED ? (*a.step_base + a.step_off) : 0;
+
+    const int4 nb = a.nbr[q];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[q]);
+
+    R txx[6], tzz[6], txz[6], vx[6], vz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    if (FUSED) {
+        if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const R f = a.absorb[fidx(q, i)];
+                txx[i] *= f; tzz[i] *= f; txz[i] *= f; vx[i] *= f; vz[i] *= f;
+            }
+        }
+        const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+        if (ax) {
+            const dgx_aux x = a.aux[ax - 1];
+            for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+                const dgx_src_dev &src = a.src[s];
+                if (src.shot != shot) continue;
+                const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    const R inc = (R)(src.fun[i] * w);
+                    if (src.kind == 0) vz[i] += inc;
+                    else { txx[i] += inc; tzz[i] += inc; }
+                }
+            }
+        }
+    }
+
+    const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
+    R ux[6], wx[6], uz[6], wz[6];
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        ux[n] = rx * txx[n] + rz * txz[n];
+        wx[n] = sx * txx[n] + sz * txz[n];
+        uz[n] = rx * txz[n] + rz * tzz[n];
+        wz[n] = sx * txz[n] + sz * tzz[n];
+    }
+    R rhx[6], rhz[6], flx[6], flz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        R sx_ = 0, sz_ = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            sx_ += a.op.Dr[i * 6 + n] * ux[n] + a.op.Ds[i * 6 + n] * wx[n];
+            sz_ += a.op.Dr[i * 6 + n] * uz[n] + a.op.Ds[i * 6 + n] * wz[n];
+        }
+        rhx[i] = sx_;
+        rhz[i] = sz_;
+        flx[i] = 0;
+        flz[i] = 0;
+    }
+
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        R dxx[3], dzz[3], dxz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(j, k);
+            dxx[k] = -txx[on]; dzz[k] = -tzz[on]; dxz[k] = -txz[on];
+        }
+        const int e = nbs[j];
+        if (e >= 0) {
+            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+            const bool special = FUSED && (code & DGX_CODE_NBSPECIAL(j));
+            const unsigned ncode = special ? (unsigned)a.nbr[e].w : 0u;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int nn = nb_node(m, rev, k);
+                const int o = fidx(e, nn);
+                R nxx = Txx[o], nzz = Tzz[o], nxz = Txz[o];
+                if (special) condition_stress(a, ncode, e, nn, shot, step, nxx, nzz, nxz);
+                dxx[k] += nxx; dzz[k] += nzz; dxz[k] += nxz;
+            }
+        }
+        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+        R gx[3], gz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            gx[k] = ez * dxx[k] - ex * dxz[k];
+            gz[k] = ez * dxz[k] - ex * dzz[k];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                flx[i] += a.op.L[j * 18 + i * 3 + k] * gx[k];
+                flz[i] += a.op.L[j * 18 + i * 3 + k] * gz[k];
+            }
+        }
+    }
+
+    const R invD = (R)g.invD;
+    const R c = a.rho_inv[q] * (R)a.dt;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        Vx[o] = (rhx[i] + flx[i] * invD) * c + vx[i];
+        Vz[o] = (rhz[i] + flz[i] * invD) * c + vz[i];
+    }
+}
+
+template <typename R, bool FUSED>
+__global__ void __launch_bounds__(DGX_BLOCK) k_stress(StepArgs<R> a)
+{
+    const int q = element_of_thread(a.nper);
+    if (q >= a.tri) return;
+    const int shot = blockIdx.y;
+    const long long so = (long long)shot * a.fstride * 6;
+    const R *__restrict__ Vx = a.f[0] + so;
+    const R *__restrict__ Vz = a.f[1] + so;
+    R *__restrict__ Txx = a.f[2] + so;
+    R *__restrict__ Tzz = a.f[3] + so;
+    R *__restrict__ Txz = a.f[4] + so;
+    const int step = FUSED ? (*a.step_base + a.step_off) : 0;
+
+    const int4 nb = a.nbr[q];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[q]);
+
+    R vx[6], vz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
+    R exx[6], ezz[6], exz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        R rvx = 0, svx = 0, rvz = 0, svz = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            rvx += a.op.Dr[i * 6 + n] * vx[n];
+            svx += a.op.Ds[i * 6 + n] * vx[n];
+            rvz += a.op.Dr[i * 6 + n] * vz[n];
+            svz += a.op.Ds[i * 6 + n] * vz[n];
+        }
+        exx[i] = rx * rvx + sx * svx;
+        ezz[i] = rz * rvz + sz * svz;
+        exz[i] = rx * rvz + sx * svz + rz * rvx + sz * svx;
+    }
+
+    R fx[6], fz[6], fxz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        R dvx[3], dvz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(j, k);
+            dvx[k] = -vx[on]; dvz[k] = -vz[on];
+        }
+        const int e = nbs[j];
+        if (e >= 0) {
+            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int o = fidx(e, nb_node(m, rev, k));
+                dvx[k] += Vx[o];
+                dvz[k] += Vz[o];
+            }
+        }
+        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+        R px[3], pz[3], pxz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            px[k] = ez * dvx[k];
+            pz[k] = -ex * dvz[k];
+            pxz[k] = ez * dvz[k] - ex * dvx[k];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const R l = a.op.L[j * 18 + i * 3 + k];
+                fx[i] += l * px[k];
+                fz[i] += l * pz[k];
+                fxz[i] += l * pxz[k];
+            }
+        }
+    }
+
+    const R invD = (R)g.invD;
+    const R lam = a.lam[q], mu = a.mu[q], l2m = lam + 2 * mu, dt = (R)a.dt;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        R txx = Txx[o], tzz = Tzz[o], txz = Txz[o];
+        if (FUSED) condition_stress(a, code, q, i, shot, step, txx, tzz, txz);
+        const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
+        Txx[o] = (l2m * axx + lam * azz) * dt + txx;
+        Tzz[o] = (lam * axx + l2m * azz) * dt + tzz;
+        Txz[o] = (mu * axz) * dt + txz;
+    }
+
+    if (FUSED) {
+        const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+        if (ax && step < a.nsteps) {
+            const dgx_aux x = a.aux[ax - 1];
+            for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
+                const dgx_rec_dev &rc = a.rec[r];
+                double sx_ = 0, sz_ = 0;
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    sx_ += rc.w[i] * (double)vx[i];
+                    sz_ += rc.w[i] * (double)vz[i];
+                }
+                double *t = a.trace + ((long long)shot * a.nsteps + step) * 2 * a.nrec;
+                t[rc.index] = sx_;
+                t[a.nrec + rc.index] = sz_;
+            }
+        }
+    }
+}
+
+/* ---- split entry points (unfused reference step functions) ---- */
+
+template <typename R>
+__global__ void __launch_bounds__(DGX_BLOCK) k_absorb(StepArgs<R> a)
+{
+    const int q = element_of_thread(a.nper);
+    if (q >= a.tri) return;
+    if (!((unsigned)a.nbr[q].w & DGX_CODE_SPONGE)) return;
+    const long long so = (long long)blockIdx.y * a.fstride * 6;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        const R f = a.absorb[o];
+#pragma unroll
+        for (int c = 0; c < 5; c++) a.f[c][so + o] *= f;
+    }
+}
+
+/* All sources in one thread, in list order, so that two sources on one element add in a fixed order. */
+template <typename R>
+__global__ void k_source(StepArgs<R> a, int nsrc, double w)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int s = 0; s < nsrc; s++) {
+        const dgx_src_dev &src = a.src[s];
+        const long long so = (long long)src.shot * a.fstride * 6;
+        for (int i = 0; i < 6; i++) {
+            const R inc = (R)(src.fun[i] * w);
+            const int o = fidx(src.elem, i);
+            if (src.kind == 0) a.f[1][so + o] += inc;
+            else { a.f[2][so + o] += inc; a.f[3][so + o] += inc; }
+        }
+    }
+}
+
+template <typename R>
+__global__ void k_record(StepArgs<R> a, int t, int nshots)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nrec * nshots) return;
+    const int shot = k / a.nrec, r = k % a.nrec;
+    const dgx_rec_dev &rc = a.rec[r];
+    const long long so = (long long)shot * a.fstride * 6;
+    double sx_ = 0, sz_ = 0;
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(rc.elem, i);
+        sx_ += rc.w[i] * (double)a.f[0][so + o];
+        sz_ += rc.w[i] * (double)a.f[1][so + o];
+    }
+    double *tr = a.trace + ((long long)shot * a.nsteps + t) * 2 * a.nrec;
+    tr[rc.index] = sx_;
+    tr[a.nrec + rc.index] = sz_;
+}
+
+__global__ void k_advance(int *step_base, int n)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *step_base += n;
+}
+
+} // namespace dgx
+
+/* ---- host-side launchers (C linkage inside the library only) ---- */
+using dgx::StepArgs;
+
+template <typename R>
+static hipError_t launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st, int which)
+{
+    dim3 grid(DGX_NXCD * a.nper, nshots), block(DGX_BLOCK);
+    switch (which) {
+    case 0: dgx::k_velocity<R, true><<<grid, block, 0, st>>>(a); break;
+    case 1: dgx::k_stress<R, true><<<grid, block, 0, st>>>(a); break;
+    case 2: dgx::k_velocity<R, false><<<grid, block, 0, st>>>(a); break;
+    case 3: dgx::k_stress<R, false><<<grid, block, 0, st>>>(a); break;
+    case 4: dgx::k_absorb<R><<<grid, block, 0, st>>>(a); break;
+    }
+    return hipGetLastError();
+}
+
+extern "C" {
+hipError_t dgx_launch_f64(const void *args, int nshots, hipStream_t st, int which)
+{
+    return launch_step_kernels(*(const StepArgs<double> *)args, nshots, st, which);
+}
+hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int which)
+{
+    return launch_step_kernels(*(const StepArgs<float> *)args, nshots, st, which);
+}
+hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st)
+{
+    if (fp32) dgx::k_source<float><<<1, 64, 0, st>>>(*(const StepArgs<float> *)args, nsrc, w);
+    else dgx::k_source<double><<<1, 64, 0, st>>>(*(const StepArgs<double> *)args, nsrc, w);
+    return hipGetLastError();
+}
+hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st)
+{
+    int n = nrec * nshots;
+    if (n <= 0) return hipSuccess;
+    int blocks = (n + 255) / 256;
+    if (fp32) dgx::k_record<float><<<blocks, 256, 0, st>>>(*(const StepArgs<float> *)args, t, nshots);
+    else dgx::k_record<double><<<blocks, 256, 0, st>>>(*(const StepArgs<double> *)args, t, nshots);
+    return hipGetLastError();
+}
+hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st)
+{
+    dgx::k_advance<<<1, 64, 0, st>>>(step_base, n);
+    return hipGetLastError();
+}
+size_t dgx_step_args_size(int fp32) { return fp32 ? sizeof(StepArgs<float>) : sizeof(StepArgs<double>); }
+}

@@ -1,0 +1,581 @@
+/*
+ * dgx_setup.c -- host setup and I/O for libdgx (declared in include/dgx_setup.h).
+ *
+ * Compiled with gcc -O2 and no -march, like the reference's OpenMP build, so the few quantities that
+ * the reference computes on the host with libm (lengths via pow/sqrt, the CFL step, the Ricker
+ * samples, the sponge cosines) come out bit-identical to the reference's when the same steps are
+ * taken.  Those steps are kept where the reference's result depends on them; the data structures
+ * around them (streaming reader, grid locator, vertex/element incidence) are this library's own.
+ */
+#include "dgx_setup.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+void dgx_set_error(const char *fmt, ...);
+
+/* ------------------------------------------------------------------------------------------------
+ * MATLAB "save -ascii" text.  Each value is a 16-byte field "   d.ddddddde+XX" ("  -d..." when
+ * negative).  The value is rebuilt from its digits the way the reference does it (lead digit, then
+ * the seven fraction digits from the last to the first with weights 10^-7..10^-1, then times
+ * 10^exp times the sign, fun_save_to_file.c:83-93), not with strtod, so both readers yield the same
+ * doubles.  Integers weight the digits by the exponent (fun_save_to_file.c:177-185).
+ * ---------------------------------------------------------------------------------------------- */
+static char *slurp(const char *path, long *len)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return NULL;
+    if (fseek(f, 0, SEEK_END) != 0) { fclose(f); return NULL; }
+    long n = ftell(f);
+    if (n < 0) { fclose(f); return NULL; }
+    rewind(f);
+    char *buf = (char *)malloc((size_t)n + 1);
+    if (!buf) { fclose(f); return NULL; }
+    if (n > 0 && fread(buf, 1, (size_t)n, f) != (size_t)n) { free(buf); fclose(f); return NULL; }
+    fclose(f);
+    buf[n] = 0;
+    *len = n;
+    return buf;
+}
+
+static double field_double(const char *s, long i, long *sign)
+{
+    if (s[i - 10] == ' ') *sign = 1;
+    else if (s[i - 10] == '-') *sign = -1;
+    long esign = (s[i + 1] == '-') ? -1 : 1;
+    double v = (double)(s[i - 9] - '0');
+    for (long j = i - 1, k = -7; j >= i - 7; j--, k++) v = v + (s[j] - '0') * pow(10, k);
+    long p = esign * ((s[i + 2] - '0') * 10 + (s[i + 3] - '0'));
+    return v * pow(10, p) * (*sign);
+}
+
+static long field_long(const char *s, long i, long *sign)
+{
+    if (s[i - 10] == ' ') *sign = 1;
+    else if (s[i - 10] == '-') *sign = -1;
+    long esign = (s[i + 1] == '-') ? -1 : 1;
+    long p = esign * ((s[i + 2] - '0') * 10 + (s[i + 3] - '0'));
+    long v = (long)((s[i - 9] - '0') * pow(10, p) * (*sign));
+    for (long j = i - 7, k = p - 1; j <= i - 8 + p; j++, k--) v = (long)(v + (s[j] - '0') * pow(10, k));
+    return v;
+}
+
+/* Scan text, calling back per value with its (row, col).  Returns values seen (capped at want). */
+typedef void (*field_cb)(const char *s, long i, long row, long col, void *ctx);
+static long scan_fields(const char *s, long len, long want, field_cb cb, void *ctx)
+{
+    long got = 0, row = 0, col = 0;
+    for (long i = 0; i < len && got < want; i++) {
+        char c = s[i];
+        if (c == 'e' && i >= 10 && i + 3 < len) {
+            cb(s, i, row, col, ctx);
+            col++;
+            got++;
+        } else if (c == '\n') {
+            row++;
+            col = 0;
+        }
+    }
+    return got;
+}
+
+typedef struct { double *m; long rows, cols; long sign; } dctx;
+typedef struct { long *m; long rows, cols; long sign; } lctx;
+static void put_double(const char *s, long i, long row, long col, void *c)
+{
+    dctx *d = (dctx *)c;
+    double v = field_double(s, i, &d->sign);
+    if (row < d->rows && col < d->cols) d->m[row * d->cols + col] = v;
+}
+static void put_long(const char *s, long i, long row, long col, void *c)
+{
+    lctx *d = (lctx *)c;
+    long v = field_long(s, i, &d->sign);
+    if (row < d->rows && col < d->cols) d->m[row * d->cols + col] = v;
+}
+
+long dgx_read_ascii(const char *path, double *m, long rows, long cols)
+{
+    long len = 0;
+    char *buf = slurp(path, &len);
+    if (!buf) { dgx_set_error("cannot read %s", path); return DGX_EIO; }
+    dctx c = {m, rows, cols, 1};
+    long got = scan_fields(buf, len, rows * cols, put_double, &c);
+    free(buf);
+    if (got != rows * cols) { dgx_set_error("%s: %ld of %ld values", path, got, rows * cols); return DGX_EIO; }
+    return got;
+}
+
+long dgx_read_ascii_long(const char *path, long *m, long rows, long cols)
+{
+    long len = 0;
+    char *buf = slurp(path, &len);
+    if (!buf) { dgx_set_error("cannot read %s", path); return DGX_EIO; }
+    lctx c = {m, rows, cols, 1};
+    long got = scan_fields(buf, len, rows * cols, put_long, &c);
+    free(buf);
+    if (got != rows * cols) { dgx_set_error("%s: %ld of %ld values", path, got, rows * cols); return DGX_EIO; }
+    return got;
+}
+
+int dgx_read_operators(const char *dir, dgx_operators *ops)
+{
+    struct { const char *name; double *dst; long r, c; } tab[] = {
+        {"V.txt", ops->V, 6, 6},         {"inverV.txt", ops->inverV, 6, 6}, {"Vr.txt", ops->Vr, 6, 6},
+        {"Vs.txt", ops->Vs, 6, 6},       {"S1.txt", ops->S1, 6, 3},         {"S2.txt", ops->S2, 6, 3},
+        {"S3.txt", ops->S3, 6, 3},       {"FeketeNew.txt", ops->FeketeNew, 6, 2},
+    };
+    char path[4096];
+    for (size_t k = 0; k < sizeof(tab) / sizeof(tab[0]); k++) {
+        snprintf(path, sizeof(path), "%s/%s", dir, tab[k].name);
+        long rc = dgx_read_ascii(path, tab[k].dst, tab[k].r, tab[k].c);
+        if (rc < 0) return (int)rc;
+    }
+    return DGX_OK;
+}
+
+long dgx_mesh_count(const char *path)
+{
+    long v = 0;
+    long rc = dgx_read_ascii_long(path, &v, 1, 1);
+    return rc < 0 ? rc : v;
+}
+
+int dgx_read_mesh(const char *ele, const char *node, const char *neigh, long tri, long point_num, double *point,
+                  long *triangle, long *neighbor, long *attribute)
+{
+    long *E = (long *)malloc(sizeof(long) * (tri + 1) * 5);
+    long *G = (long *)malloc(sizeof(long) * (tri + 1) * 4);
+    double *P = (double *)malloc(sizeof(double) * (point_num + 1) * 4);
+    int rc = DGX_OK;
+    if (!E || !G || !P) { rc = DGX_ENOMEM; goto done; }
+    if (dgx_read_ascii_long(ele, E, tri + 1, 5) < 0 || dgx_read_ascii_long(neigh, G, tri + 1, 4) < 0 ||
+        dgx_read_ascii(node, P, point_num + 1, 4) < 0) {
+        rc = DGX_EIO;
+        goto done;
+    }
+    /* row 0 is the header; columns: id, v1, v2, v3, attribute / id, n1, n2, n3 / id, x, z, marker */
+    for (long e = 0; e < tri; e++) {
+        for (int j = 0; j < 3; j++) {
+            triangle[e * 3 + j] = E[(e + 1) * 5 + 1 + j];
+            neighbor[e * 3 + j] = G[(e + 1) * 4 + 1 + j];
+        }
+        if (attribute) attribute[e] = E[(e + 1) * 5 + 4];
+    }
+    for (long i = 0; i < point_num; i++) {
+        point[i * 2 + 0] = P[(i + 1) * 4 + 1];
+        point[i * 2 + 1] = P[(i + 1) * 4 + 2];
+    }
+done:
+    free(E); free(G); free(P);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Writers.
+ * ---------------------------------------------------------------------------------------------- */
+int dgx_save_data(const char *path, const double *m, long rows, long cols)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) { dgx_set_error("cannot write %s", path); return DGX_EIO; }
+    for (long i = 0; i < rows; i++) {
+        for (long j = 0; j < cols; j++) fprintf(f, "%0.14f ", m[i * cols + j]);
+        fputc('\n', f);
+    }
+    return fclose(f) == 0 ? DGX_OK : DGX_EIO;
+}
+
+int dgx_save_record(const char *path, const double *m, long rows) { return dgx_save_data(path, m, rows, 1); }
+
+int dgx_save_binary(const char *path, const double *m, long n)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) { dgx_set_error("cannot write %s", path); return DGX_EIO; }
+    size_t w = fwrite(m, sizeof(double), (size_t)n, f);
+    int ok = (fclose(f) == 0) && (w == (size_t)n);
+    return ok ? DGX_OK : DGX_EIO;
+}
+
+/* Node coordinates: x = -(r+s)/2 v0 + (r+1)/2 v1 + (s+1)/2 v2, evaluated in this order so the
+ * coordinates the sponge tests see are the reference's (fun_DG.c:857-858, 873-874). */
+static void node_xy(const double v[3][2], const double *fekete, int i, double *xy)
+{
+    double r = fekete[i * 2 + 0], s = fekete[i * 2 + 1];
+    for (int c = 0; c < 2; c++) xy[c] = -(r + s) / 2 * v[0][c] + (r + 1) / 2 * v[1][c] + (s + 1) / 2 * v[2][c];
+}
+
+static void element_vertices(const double *point, const long *triangle, long e, double v[3][2])
+{
+    for (int k = 0; k < 3; k++) {
+        long p = triangle[e * 3 + k] - 1;
+        v[k][0] = point[p * 2 + 0];
+        v[k][1] = point[p * 2 + 1];
+    }
+}
+
+void dgx_all_node(long tri, const double *point, const long *triangle, const double *fekete, double *all_node)
+{
+    for (long e = 0; e < tri; e++) {
+        double v[3][2];
+        element_vertices(point, triangle, e, v);
+        for (int i = 0; i < 6; i++) node_xy(v, fekete, i, &all_node[e * 12 + i * 2]);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Sponge.  As a function of a node's position (x,z) relative to the interior box, with d the
+ * distance to the box (to its corner point in the corner regions):
+ *   side strips (open intervals along the box side): f = 0.5 + 0.5 cos(beta d pi / L_side)
+ *   corner regions, d < bound_x:                     f = 0.5 + 0.5 cos(beta d pi / bound_x)
+ *   otherwise f = 1, also for corner nodes with d >= bound_x and nodes on a strip's end line.
+ * Only elements whose first vertex lies outside the box are visited (fun_DG.c:266), which equals
+ * "any vertex outside" for the meshes this library accepts (checked below).
+ * ---------------------------------------------------------------------------------------------- */
+static int outside_box(const double *p, const dgx_sponge_params *sp)
+{
+    return p[0] > (sp->bound_x + sp->model_w) || p[0] < sp->bound_x || p[1] > (sp->bound_z + sp->model_h) ||
+           p[1] < sp->bound_z;
+}
+
+static double taper(double d, double L, const dgx_sponge_params *sp)
+{
+    return 0.5 + 0.5 * cos(sp->beta * d * sp->pi / L);
+}
+
+static double sponge_at(double x, double z, const dgx_sponge_params *sp)
+{
+    const double x0 = sp->bound_x, x1 = sp->bound_x + sp->model_w;
+    const double z0 = sp->bound_z, z1 = sp->bound_z + sp->model_h;
+    const int in_x = (x > x0 && x < x1), in_z = (z > z0 && z < z1);
+    if (x < x0 && in_z) return taper(x0 - x, sp->bound_x, sp);
+    if (x > x1 && in_z) return taper(x - x0 - sp->model_w, sp->bound_x, sp);
+    if (z < z0 && in_x) return taper(z0 - z, sp->bound_z, sp);
+    if (z > z1 && in_x) return taper(z - z0 - sp->model_h, sp->bound_z, sp);
+    if ((x < x0 || x > x1) && (z < z0 || z > z1)) {
+        double cx = (x < x0) ? x0 : x1, cz = (z < z0) ? z0 : z1;
+        double dx = fabs(x - cx), dz = fabs(z - cz);
+        double d = sqrt(dx * dx + dz * dz);
+        if (d < sp->bound_x) return taper(d, sp->bound_x, sp);
+    }
+    return 1.0;
+}
+
+long dgx_sponge(long tri, const double *point, const long *triangle, const double *fekete,
+                const dgx_sponge_params *sp, double *absorb_fun)
+{
+    long touched = 0;
+    for (long e = 0; e < tri; e++) {
+        const long *v = &triangle[e * 3];
+        int first = outside_box(&point[(v[0] - 1) * 2], sp);
+        int any = first || outside_box(&point[(v[1] - 1) * 2], sp) || outside_box(&point[(v[2] - 1) * 2], sp);
+        if (any && !first) {
+            dgx_set_error("element %ld has a vertex outside the sponge box but not its first vertex; rotate its "
+                          "vertex list (the reference's absorbing-element list would be short)", e + 1);
+            return DGX_EMESH;
+        }
+        double vx[3][2];
+        element_vertices(point, triangle, e, vx);
+        int any_damped = 0;
+        for (int i = 0; i < 6; i++) {
+            double f = 1.0;
+            if (first) {
+                double xy[2];
+                node_xy(vx, fekete, i, xy);
+                f = sponge_at(xy[0], xy[1], sp);
+            }
+            absorb_fun[e * 6 + i] = f;
+            any_damped |= (f != 1.0);
+        }
+        touched += any_damped;
+    }
+    return touched;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * CFL step and Ricker wavelet.
+ * ---------------------------------------------------------------------------------------------- */
+double dgx_cfl_dt(long tri, const double *point, const long *triangle, const double *vp)
+{
+    double vmax = vp[0], dmin = 0;
+    for (long e = 1; e < tri; e++) vmax = vp[e] > vmax ? vp[e] : vmax;
+    for (long e = 0; e < tri; e++) {
+        double v[3][2];
+        element_vertices(point, triangle, e, v);
+        double a = sqrt(pow(v[0][0] - v[1][0], 2) + pow(v[0][1] - v[1][1], 2));
+        double b = sqrt(pow(v[1][0] - v[2][0], 2) + pow(v[1][1] - v[2][1], 2));
+        double c = sqrt(pow(v[0][0] - v[2][0], 2) + pow(v[0][1] - v[2][1], 2));
+        double s = 0.5 * (a + b + c);
+        double area = sqrt(s * (s - a) * (s - b) * (s - c));
+        double d = 2 * area / s; /* in-circle diameter */
+        if (e == 0 || d < dmin) dmin = d;
+    }
+    return dmin / vmax / (2 * 2 + 1) * 0.99;
+}
+
+double dgx_ricker(long step, double dt, double t0, double f0, double pi)
+{
+    double t = dt * step, u = t - t0;
+    return (1 - 2 * pi * pi * f0 * f0 * u * u) * exp(-pi * pi * f0 * f0 * u * u);
+}
+
+double dgx_ricker_delay(double dt, double f0, double pi)
+{
+    /* first step T+1 at which the undelayed wavelet rises while still below 1e-8 */
+    double prev = dgx_ricker(0, dt, 0.0, f0, pi);
+    for (long T = 0; T < 100000000L; T++) {
+        double t2 = (T + 1) * dt;
+        double cur = dgx_ricker(T + 1, dt, 0.0, f0, pi);
+        if (cur > prev && cur < 1e-8) return t2 * 2.6;
+        prev = cur;
+    }
+    return 0.0;
+}
+
+int dgx_wavelet(long nsteps, double dt, double t0, double f0, double pi, double *w)
+{
+    if (nsteps < 0 || !w) return DGX_EARG;
+    for (long n = 0; n < nsteps; n++) w[n] = dgx_ricker(n, dt, t0, f0, pi);
+    return DGX_OK;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Point location: uniform grid over the mesh bounding box; each cell lists, in increasing element
+ * order, the elements whose bounding box overlaps it, and the vertices inside it.
+ * ---------------------------------------------------------------------------------------------- */
+struct dgx_locator {
+    long tri, point_num;
+    const double *point;
+    const long *triangle;
+    double x0, z0, h;
+    long nx, nz;
+    long *cell_start, *cell_elem;    /* element CSR per cell */
+    long *vcell_start, *vcell_vert;  /* vertex CSR per cell */
+    long *inc_start, *inc_elem;      /* vertex -> elements holding it, increasing */
+};
+
+static long clampl(long v, long lo, long hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+static void cell_of(const dgx_locator *L, double x, double z, long *ix, long *iz)
+{
+    *ix = clampl((long)floor((x - L->x0) / L->h), 0, L->nx - 1);
+    *iz = clampl((long)floor((z - L->z0) / L->h), 0, L->nz - 1);
+}
+
+int dgx_locator_create(long tri, long point_num, const double *point, const long *triangle, dgx_locator **out)
+{
+    if (tri <= 0 || point_num <= 0 || !point || !triangle || !out) return DGX_EARG;
+    dgx_locator *L = (dgx_locator *)calloc(1, sizeof(*L));
+    if (!L) return DGX_ENOMEM;
+    L->tri = tri; L->point_num = point_num; L->point = point; L->triangle = triangle;
+    double xmin = point[0], xmax = point[0], zmin = point[1], zmax = point[1];
+    for (long i = 1; i < point_num; i++) {
+        xmin = fmin(xmin, point[i * 2]); xmax = fmax(xmax, point[i * 2]);
+        zmin = fmin(zmin, point[i * 2 + 1]); zmax = fmax(zmax, point[i * 2 + 1]);
+    }
+    double area = (xmax - xmin) * (zmax - zmin);
+    double h = sqrt((area > 0 ? area : 1.0) / (double)tri) * 2.0;
+    if (!(h > 0)) h = 1.0;
+    L->h = h;
+    L->x0 = xmin - 1e-6 * h;
+    L->z0 = zmin - 1e-6 * h;
+    L->nx = (long)((xmax - L->x0) / h) + 1;
+    L->nz = (long)((zmax - L->z0) / h) + 1;
+    long ncell = L->nx * L->nz;
+    L->cell_start = (long *)calloc(ncell + 1, sizeof(long));
+    L->vcell_start = (long *)calloc(ncell + 1, sizeof(long));
+    L->inc_start = (long *)calloc(point_num + 1, sizeof(long));
+    if (!L->cell_start || !L->vcell_start || !L->inc_start) { dgx_locator_destroy(L); return DGX_ENOMEM; }
+    /* pass 1: counts */
+    for (int pass = 0; pass < 2; pass++) {
+        long *fill = NULL, *vfill = NULL, *ifill = NULL;
+        if (pass == 1) {
+            for (long c = 0; c < ncell; c++) {
+                L->cell_start[c + 1] += L->cell_start[c];
+                L->vcell_start[c + 1] += L->vcell_start[c];
+            }
+            for (long p = 0; p < point_num; p++) L->inc_start[p + 1] += L->inc_start[p];
+            L->cell_elem = (long *)malloc(sizeof(long) * (L->cell_start[ncell] + 1));
+            L->vcell_vert = (long *)malloc(sizeof(long) * (L->vcell_start[ncell] + 1));
+            L->inc_elem = (long *)malloc(sizeof(long) * (L->inc_start[point_num] + 1));
+            fill = (long *)calloc(ncell, sizeof(long));
+            vfill = (long *)calloc(ncell, sizeof(long));
+            ifill = (long *)calloc(point_num, sizeof(long));
+            if (!L->cell_elem || !L->vcell_vert || !L->inc_elem || !fill || !vfill || !ifill) {
+                free(fill); free(vfill); free(ifill);
+                dgx_locator_destroy(L);
+                return DGX_ENOMEM;
+            }
+        }
+        for (long e = 0; e < tri; e++) {
+            double bx0 = 1e300, bx1 = -1e300, bz0 = 1e300, bz1 = -1e300;
+            for (int k = 0; k < 3; k++) {
+                long p = triangle[e * 3 + k] - 1;
+                if (p < 0 || p >= point_num) { dgx_set_error("element %ld: vertex id out of range", e + 1); dgx_locator_destroy(L); return DGX_EMESH; }
+                bx0 = fmin(bx0, point[p * 2]); bx1 = fmax(bx1, point[p * 2]);
+                bz0 = fmin(bz0, point[p * 2 + 1]); bz1 = fmax(bz1, point[p * 2 + 1]);
+                if (pass == 0) L->inc_start[p + 1]++;
+                else L->inc_elem[L->inc_start[p] + ifill[p]++] = e;
+            }
+            long ix0, iz0, ix1, iz1;
+            cell_of(L, bx0 - 1e-9 * h, bz0 - 1e-9 * h, &ix0, &iz0);
+            cell_of(L, bx1 + 1e-9 * h, bz1 + 1e-9 * h, &ix1, &iz1);
+            for (long iz = iz0; iz <= iz1; iz++)
+                for (long ix = ix0; ix <= ix1; ix++) {
+                    long c = iz * L->nx + ix;
+                    if (pass == 0) L->cell_start[c + 1]++;
+                    else L->cell_elem[L->cell_start[c] + fill[c]++] = e;
+                }
+        }
+        for (long p = 0; p < point_num; p++) {
+            long ix, iz;
+            cell_of(L, point[p * 2], point[p * 2 + 1], &ix, &iz);
+            long c = iz * L->nx + ix;
+            if (pass == 0) L->vcell_start[c + 1]++;
+            else L->vcell_vert[L->vcell_start[c] + vfill[c]++] = p;
+        }
+        free(fill); free(vfill); free(ifill);
+    }
+    *out = L;
+    return DGX_OK;
+}
+
+void dgx_locator_destroy(dgx_locator *L)
+{
+    if (!L) return;
+    free(L->cell_start); free(L->cell_elem); free(L->vcell_start); free(L->vcell_vert);
+    free(L->inc_start); free(L->inc_elem);
+    free(L);
+}
+
+/* Vertex within tol of (x,z) in both coordinates; first (lowest id) or last (highest id). */
+static long match_vertex(const dgx_locator *L, double x, double z, double tol, int last)
+{
+    long ix, iz, best = -1;
+    cell_of(L, x, z, &ix, &iz);
+    for (long cz = iz - 1; cz <= iz + 1; cz++)
+        for (long cx = ix - 1; cx <= ix + 1; cx++) {
+            if (cx < 0 || cz < 0 || cx >= L->nx || cz >= L->nz) continue;
+            long c = cz * L->nx + cx;
+            for (long k = L->vcell_start[c]; k < L->vcell_start[c + 1]; k++) {
+                long p = L->vcell_vert[k];
+                if (fabs(L->point[p * 2] - x) < tol && fabs(L->point[p * 2 + 1] - z) < tol)
+                    if (best < 0 || (last ? p > best : p < best)) best = p;
+            }
+        }
+    return best;
+}
+
+/* Barycentric test with the same formulas and tolerance as the oracle's locator. */
+static int inside(const dgx_locator *L, long e, double x, double z, double *r, double *s)
+{
+    double v[3][2];
+    element_vertices(L->point, L->triangle, e, v);
+    double xr = v[1][0] - v[0][0], zr = v[1][1] - v[0][1];
+    double xs = v[2][0] - v[0][0], zs = v[2][1] - v[0][1];
+    double det = xr * zs - xs * zr;
+    double dx = x - v[0][0], dz = z - v[0][1];
+    double a = (dx * zs - xs * dz) / det, b = (xr * dz - dx * zr) / det;
+    const double tol = 1e-10;
+    if (a >= -tol && b >= -tol && a + b <= 1 + tol) {
+        *r = 2 * a - 1;
+        *s = 2 * b - 1;
+        return 1;
+    }
+    return 0;
+}
+
+static long locate_point(const dgx_locator *L, double x, double z, double *r, double *s)
+{
+    long ix, iz;
+    cell_of(L, x, z, &ix, &iz);
+    long c = iz * L->nx + ix;
+    for (long k = L->cell_start[c]; k < L->cell_start[c + 1]; k++)
+        if (inside(L, L->cell_elem[k], x, z, r, s)) return L->cell_elem[k];
+    return -1;
+}
+
+void dgx_p2_basis(double r, double s, double *phi)
+{
+    double l0 = -(r + s) / 2, l1 = (r + 1) / 2, l2 = (s + 1) / 2;
+    phi[0] = l1 * (2 * l1 - 1); /* node 1: v1 */
+    phi[1] = 4 * l1 * l2;       /* node 2: mid v1v2 */
+    phi[2] = l2 * (2 * l2 - 1); /* node 3: v2 */
+    phi[3] = 4 * l0 * l1;       /* node 4: mid v0v1 */
+    phi[4] = 4 * l0 * l2;       /* node 5: mid v0v2 */
+    phi[5] = l0 * (2 * l0 - 1); /* node 6: v0 */
+}
+
+static const int node_of_vertex[3] = {5, 0, 2}; /* v0 -> node 6, v1 -> node 1, v2 -> node 3 */
+
+static int vertex_slot(const long *tv, long p)
+{
+    for (int k = 0; k < 3; k++)
+        if (tv[k] == p + 1) return k;
+    return -1;
+}
+
+static double element_jacobian(const dgx_locator *L, long e)
+{
+    double v[3][2];
+    element_vertices(L->point, L->triangle, e, v);
+    double xr = (v[1][0] - v[0][0]) * 0.5, yr = (v[1][1] - v[0][1]) * 0.5;
+    double xs = (v[2][0] - v[0][0]) * 0.5, ys = (v[2][1] - v[0][1]) * 0.5;
+    return xr * ys - xs * yr; /* fun_DG.c:441-447 */
+}
+
+int dgx_locate_source(const dgx_locator *L, const dgx_operators *ops, double x, double z, int kind, int shot,
+                      long wavelet, dgx_source *out)
+{
+    if (!L || !ops || !out) return DGX_EARG;
+    double phi[6] = {0, 0, 0, 0, 0, 0};
+    long e = -1;
+    long p = match_vertex(L, x, z, 1e-3, 0);
+    if (p >= 0 && L->inc_start[p] < L->inc_start[p + 1]) {
+        e = L->inc_elem[L->inc_start[p]]; /* first element holding the vertex */
+        phi[node_of_vertex[vertex_slot(&L->triangle[e * 3], p)]] = 1.0;
+    } else {
+        double r, s;
+        e = locate_point(L, x, z, &r, &s);
+        if (e < 0) { dgx_set_error("source (%g, %g) lies outside the mesh", x, z); return DGX_EARG; }
+        dgx_p2_basis(r, s, phi);
+    }
+    double J = element_jacobian(L, e);
+    for (int i = 0; i < 6; i++) {
+        double acc = 0;
+        for (int k = 0; k < 6; k++) {
+            double m = 0; /* (V V^T)_{ik}, summed like DG_leap_Frog_OpenMP.c:227-230 */
+            for (int j = 0; j < 6; j++) m = m + ops->V[i * 6 + j] * ops->V[k * 6 + j];
+            acc = acc + m * phi[k];
+        }
+        out->fun[i] = acc / J;
+    }
+    out->elem = e + 1;
+    out->kind = kind;
+    out->shot = shot;
+    out->wavelet = wavelet;
+    return DGX_OK;
+}
+
+int dgx_locate_receivers(const dgx_locator *L, long n, const double *xz, dgx_receiver *out)
+{
+    if (!L || (n > 0 && (!xz || !out))) return DGX_EARG;
+    for (long k = 0; k < n; k++) {
+        double x = xz[k * 2], z = xz[k * 2 + 1];
+        for (int i = 0; i < 6; i++) out[k].weight[i] = 0;
+        long p = match_vertex(L, x, z, 1e-2, 1);
+        if (p >= 0 && L->inc_start[p] < L->inc_start[p + 1]) {
+            long e = L->inc_elem[L->inc_start[p + 1] - 1]; /* last element holding the vertex */
+            out[k].elem = e + 1;
+            out[k].weight[node_of_vertex[vertex_slot(&L->triangle[e * 3], p)]] = 1.0;
+            continue;
+        }
+        double r, s;
+        long e = locate_point(L, x, z, &r, &s);
+        if (e < 0) { dgx_set_error("receiver %ld at (%g, %g) lies outside the mesh", k, x, z); return DGX_EARG; }
+        out[k].elem = e + 1;
+        dgx_p2_basis(r, s, out[k].weight);
+    }
+    return DGX_OK;
+}

@@ -1,0 +1,116 @@
+/*
+ * dgx.h -- C-ABI of the MI355X time-stepping library (libdgx.so) for the 2-D nodal P2
+ * velocity-stress discontinuous Galerkin solver.
+ *
+ * The library replaces the per-step calls of the reference's time loop
+ * (DG_leap_Frog_OpenMP.c:405-425, prototypes OpenMP_Function.c:5-20) with device-resident HIP
+ * kernels.  The caller keeps the reference's setup and file formats; it hands the mesh, materials,
+ * sponge factors, source and receiver descriptors over once (dgx_create), then advances whole steps
+ * (dgx_step) or calls the split entry points that mirror the reference's five step functions one to
+ * one.  Plain C types only; every call returns 0 or a negative DGX_E* code, with a message from
+ * dgx_last_error().  Caller-owned memory is never retained after a call returns; the solver handle
+ * owns all device memory.  One host thread per handle; calls on a handle are stream-ordered and
+ * only dgx_get_* / dgx_synchronize block.
+ */
+#ifndef DGX_H
+#define DGX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGX_OK 0
+#define DGX_EARG (-1)   /* invalid argument */
+#define DGX_EMESH (-2)  /* inconsistent mesh (orientation, neighbours, sizes) */
+#define DGX_EHIP (-3)   /* HIP runtime error or no usable GPU */
+#define DGX_ENOMEM (-4) /* host or device allocation failed */
+#define DGX_EIO (-5)    /* file could not be read or written */
+#define DGX_ESTATE (-6) /* call not valid in the solver's current state */
+
+enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1 };
+enum dgx_source_kind { DGX_SRC_FORCE_Z = 0, DGX_SRC_EXPLOSIVE = 1 };
+
+/* Reference-element operators as read from input_files/precomputed_matrix/ (row-major), the arrays
+ * DG_leap_Frog_OpenMP.c:52-81 reads.  Vb/inverVb are read by the reference but never used. */
+typedef struct {
+    double V[36], inverV[36], Vr[36], Vs[36];
+    double S1[18], S2[18], S3[18]; /* per-face lift, 6x3 each */
+    double FeketeNew[12];          /* node (r,s), 6x2 */
+} dgx_operators;
+
+/* One point source.  fun is the per-node source vector ((M^-1 phi(x_s))_i / J, which is the
+ * reference's fun at DG_leap_Frog_OpenMP.c:220-233 when x_s is a vertex).  Each step adds
+ * fun*w[step] to Vz (force, OpenMP_Function.c:49-58) or to Txx and Tzz (explosive). */
+typedef struct {
+    long elem;    /* 1-based element in the caller's numbering */
+    int kind;     /* enum dgx_source_kind */
+    int shot;     /* 0-based shot this source belongs to */
+    long wavelet; /* row of dgx_problem.wavelet to use */
+    double fun[6];
+} dgx_source;
+
+/* One receiver: trace = sum_i weight[i] * V[elem][i] after each step (OpenMP_Function.c:61-72 is the
+ * unit-weight case). */
+typedef struct {
+    long elem; /* 1-based element */
+    double weight[6];
+} dgx_receiver;
+
+typedef struct {
+    long tri;              /* Tri */
+    long point_num;        /* Point_Num */
+    const double *point;   /* [point_num*2] x,z */
+    const long *triangle;  /* [tri*3] 1-based vertex ids, counter-clockwise */
+    const long *neighbor;  /* [tri*3] Triangle .neigh columns (element opposite v0,v1,v2), -1 = none */
+    const double *lambda;  /* [tri] */
+    const double *miu;     /* [tri] */
+    const double *den;     /* [tri] */
+    const double *absorb_fun; /* [tri*6] sponge factors (Absorbing_Fun, fun_DG.c:906) or NULL */
+    double dt;
+    int precision;         /* enum dgx_precision: arithmetic and storage type of the fields */
+    long nshots;           /* S >= 1 independent wavefields on this mesh */
+    long nsteps;           /* trace capacity in steps (steps beyond it are not recorded) */
+    long nwavelets;
+    const double *wavelet; /* [nwavelets*nsteps] FP64 source samples */
+    long nsrc;
+    const dgx_source *src;
+    long nrec;
+    const dgx_receiver *rec; /* one receiver spread shared by all shots */
+} dgx_problem;
+
+typedef struct dgx_solver dgx_solver;
+
+/* Upload the problem and build the device-side mesh (replaces the arrays the reference builds at
+ * DG_leap_Frog_OpenMP.c:190-367 for the kernels).  Fields start at zero
+ * (DG_leap_Frog_OpenMP.c:399-403). */
+int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_solver **out);
+int dgx_destroy(dgx_solver *s);
+
+/* Advance n whole steps: absorb -> source -> velocity -> stress -> record, one step being the body of
+ * DG_leap_Frog_OpenMP.c:405-425.  Fused into two kernels per step. */
+int dgx_step(dgx_solver *s, long n);
+
+/* Split entry points, one per reference step function (same order, same semantics).  They are the
+ * unfused form of dgx_step, for parity tests and for callers that interleave their own work. */
+int dgx_absorb_boundary(dgx_solver *s);         /* Absorb_Boundary, OpenMP_Function.c:25-46 */
+int dgx_source_kernel(dgx_solver *s, double w); /* source_kernel, OpenMP_Function.c:49-58 (all sources) */
+int dgx_lu_velocity(dgx_solver *s);             /* LU_Velocity, OpenMP_Function.c:75-185 */
+int dgx_lu_stress_elastic(dgx_solver *s);       /* LU_Stress_Elastic, OpenMP_Function.c:188-306 */
+int dgx_record(dgx_solver *s, long t);          /* Record, OpenMP_Function.c:61-72 (trace slot t) */
+
+/* Fields of one shot in the caller's element order, [tri*6] each (any pointer may be NULL). */
+int dgx_get_fields(dgx_solver *s, long shot, double *vx, double *vz, double *txx, double *tzz, double *txz);
+int dgx_set_fields(dgx_solver *s, long shot, const double *vx, const double *vz, const double *txx,
+                   const double *tzz, const double *txz);
+/* Traces of one shot, [nsteps][nrec] like record_x/record_z (DG_leap_Frog_OpenMP.c:166-171). */
+int dgx_get_traces(dgx_solver *s, long shot, double *rec_x, double *rec_z);
+int dgx_synchronize(dgx_solver *s);
+long dgx_current_step(const dgx_solver *s);
+/* Average device time per dgx_step call kernel, measured with HIP events on the solver's stream. */
+int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stress);
+const char *dgx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
