@@ -211,12 +211,6 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         dgx_set_error("dgx_create: invalid shots/steps/sources/receivers/precision/dt");
         return DGX_EARG;
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        dgx_set_error("dgx_create: no HIP device available");
-        return DGX_EHIP;
-    }
-
     dgx_solver *s = new dgx_solver();
     s->fp32 = (p->precision == DGX_FP32);
     s->tri = T;
@@ -404,7 +398,13 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
             for (int i = 0; i < 6; i++) absorb_host[fidx(q, i)] = p->absorb_fun[6 * e + i];
     }
 
-    /* ---- device allocations ---- */
+    /* ---- device allocations (all host-side validation is done above) ---- */
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        dgx_set_error("dgx_create: no HIP device available");
+        free_solver(s);
+        return DGX_EHIP;
+    }
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (auto &ev : s->ev) HIP_TRY(hipEventCreate(&ev));
     size_t nreal_fields = (size_t)5 * s->nshots * s->tri_pad * 6;

@@ -112,6 +112,7 @@ void orc_run(const orc_problem *p, long step0, long nsteps, double *Vx, double *
 void orc_step(const orc_problem *p, long step, double *Vx, double *Vz, double *Txx, double *Tzz, double *Txz);
 
 int orc_num_threads(void);
+void orc_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -78,8 +78,8 @@ def oracle() -> C.CDLL:
         _orc.orc_locate.argtypes = [C.c_long, dptr, lptr, C.c_double, C.c_double, dptr, dptr]
         _orc.orc_get_source.argtypes = [C.c_long, C.c_long, dptr, lptr, C.c_double, C.c_double, lptr, lptr]
         _orc.orc_get_receiver.argtypes = [C.c_long, C.c_long, dptr, lptr, C.c_double, C.c_double, lptr, lptr]
-        _orc.orc_lu_velocity.argtypes = [C.c_long, C.c_double] + [dptr] * 10 + [lptr] * 4
-        _orc.orc_lu_stress.argtypes = [C.c_long, C.c_double] + [dptr] * 10 + [lptr] * 4
+        _orc.orc_lu_velocity.argtypes = [C.c_long, C.c_double] + [dptr] * 11 + [lptr] * 4
+        _orc.orc_lu_stress.argtypes = [C.c_long, C.c_double] + [dptr] * 11 + [lptr] * 4
         _orc.orc_source_kernel.argtypes = [dptr, C.c_long, dptr, C.c_double]
         _orc.orc_num_threads.restype = C.c_int
     return _orc

@@ -14,7 +14,8 @@
  *     is skipped; its result is overwritten at DG_leap_Frog_OpenMP.c:324 anyway;
  *   - Edge is zero-initialised (it is never written on boundary faces, fun_DG.c:589-717), and the
  *     element array carries a 3-entry front pad because Neighbor_Edge reads Triangle[-3..-1] on
- *     boundary faces before testing the neighbour (fun_DG.c:584-587).
+ *     boundary faces before testing the neighbour (fun_DG.c:584-587);
+ *   - the partition-boundary list is sized by Tri (see the comment at its allocation).
  */
 #include "dg.h"
 #include <stdio.h>
@@ -192,7 +193,11 @@ int ref_case(const char *opdir, long Tri, long Point_Num, const double *point_in
     Dx_Dz(Tri, Dx, Dz, Vr, Vs, inverV, Triangle, Point);
     Neighbor_Partition_Mark(Tri, Neighbor, epart, Neighbor_Mark);
     long Boundary_Num = Calculate_Boundary_Num(Tri, Neighbor_Mark);
-    long *Boundary_Triangle = (long *)calloc(Boundary_Num + 1, sizeof(long));
+    /* Calculate_Boundary_Num (fun_DG.c:761-794) does not count elements whose three neighbours all lie
+     * in another partition, but Partition_Boundary_Neighbor (fun_DG.c:815-824) lists them: size the list
+     * by Tri so such partitions do not overrun it. */
+    long *Boundary_Triangle = (long *)calloc(Tri + 1, sizeof(long));
+    (void)Boundary_Num;
     Partition_Boundary_Neighbor(Tri, Neighbor, epart, Neighbor_Mark, Series, Series_order, Bound_Series,
                                 Triangle_part, Boundary_Triangle);
     Central_flux(Tri, Jacobi, Jacobi_Edge, Q, A, B, normal1, normal2, normal3);

@@ -1,0 +1,130 @@
+"""GPU parity: libdgx (HIP, gfx950) against the CPU oracle on the same inputs.
+
+Bars: FP64 plain path relative L2 <= 1e-12 on every trace set and snapshot; FP32 path relative L2 <= 1e-4
+on traces versus the FP64 oracle."""
+import numpy as np
+import pytest
+
+import dgx
+import oracle_py as O
+from dgx import lib as L
+from helpers import OracleCase, product_run, product_setup, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def homog_case():
+    w = dgx.configs.homogeneous_2km(1500)
+    print(f"case homogeneous_2km tri={w.mesh.tri} steps={w.nsteps}", flush=True)
+    oc = OracleCase(w)
+    ox, oz, osnap = oc.run(snap_every=250)
+    return w, oc, ox, oz, osnap
+
+
+def test_homogeneous_fp64_traces_and_snapshots(homog_case):
+    w, oc, ox, oz, osnap = homog_case
+    st = product_setup(w)
+    assert st.dt == oc.dt
+    gx, gz, gsnap = product_run(st, L.DGX_FP64, snap_every=250)
+    ex, ez = rel_l2(gx, ox), rel_l2(gz, oz)
+    print(f"fp64 traces rel L2: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
+    assert ex <= FP64_TOL and ez <= FP64_TOL
+    for k, f in osnap.items():
+        for c in range(5):
+            e = rel_l2(gsnap[k][c], f[c])
+            assert e <= FP64_TOL, (k, c, e)
+
+
+def test_homogeneous_fp32_traces(homog_case):
+    w, oc, ox, oz, osnap = homog_case
+    st = product_setup(w)
+    gx, gz, gsnap = product_run(st, L.DGX_FP32, snap_every=250)
+    ex, ez = rel_l2(gx, ox), rel_l2(gz, oz)
+    print(f"fp32 traces rel L2: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
+    assert ex <= FP32_TOL and ez <= FP32_TOL
+    for k, f in osnap.items():
+        print(f"fp32 snapshot {k}: Vx {rel_l2(gsnap[k][0], f[0]):.3e} Vz {rel_l2(gsnap[k][1], f[1]):.3e}")
+
+
+def two_media_workload(nsteps=400):
+    """The reference's own configuration: water (attr 1) over rock (attr 3), vertical force at
+    (1200, 1200), one vertex receiver at (1600, 1600), dg.h sponge on a 2 km square."""
+    sp = dgx.Sponge.for_domain(2000.0, 2000.0, 100.0)
+    mesh = dgx.structured(40, 40, 50.0, diagonal="random", seed=3, sponge_box=sp.box(),
+                          attribute=lambda c: np.where(c[:, 1] > 1500, 1, 3))
+    model = dgx.Model.by_attribute(mesh.attribute)
+    src = [dgx.SourceSpec(1200.0, 1200.0, L.DGX_SRC_FORCE_Z, 0, 15.0)]
+    return dgx.configs.Workload("two_media", mesh, model, sp, src, np.array([[1600.0, 1600.0]]), nsteps, 100)
+
+
+def test_reference_configuration_fp64():
+    w = two_media_workload(400)
+    oc = OracleCase(w)
+    ox, oz, osnap = oc.run(snap_every=100)
+    gx, gz, gsnap = product_run(product_setup(w), L.DGX_FP64, snap_every=100)
+    assert rel_l2(gx, ox) <= FP64_TOL and rel_l2(gz, oz) <= FP64_TOL
+    for k, f in osnap.items():
+        for c in range(5):
+            assert rel_l2(gsnap[k][c], f[c]) <= FP64_TOL, (k, c)
+
+
+def test_split_entry_points_match_oracle_functions():
+    """dgx_absorb_boundary / dgx_source_kernel / dgx_lu_velocity / dgx_lu_stress_elastic on a random
+    state against the oracle's restatement of each reference function."""
+    w = two_media_workload(10)
+    oc = OracleCase(w)
+    st = product_setup(w)
+    rng = np.random.default_rng(7)
+    T = w.mesh.tri
+    f0 = rng.standard_normal((5, T * 6)) * np.array([1, 1, 1e6, 1e6, 1e6])[:, None]
+    s = dgx.Solver(st, L.DGX_FP64)
+    s.set_fields(f0)
+    ref = f0.copy()
+    s.absorb_boundary()
+    O.absorb(oc.lay, ref)
+    assert np.array_equal(s.fields(), ref)
+    e, kind, fun, _, _ = oc.src[0]
+    s.source_kernel(0.37)
+    ref[1, (e - 1) * 6:(e - 1) * 6 + 6] += fun * 0.37
+    assert rel_l2(s.fields()[1], ref[1]) <= 1e-15
+    s.lu_velocity()
+    O.lu_velocity(oc.lay, oc.dt, ref)
+    g = s.fields()
+    for c in (0, 1):
+        assert rel_l2(g[c], ref[c]) <= 1e-13, c
+    s.set_fields(ref)
+    s.lu_stress_elastic()
+    O.lu_stress(oc.lay, oc.dt, ref)
+    g = s.fields()
+    for c in (2, 3, 4):
+        assert rel_l2(g[c], ref[c]) <= 1e-13, c
+    s.close()
+
+
+def test_graph_replay_equals_eager():
+    w = two_media_workload(96)
+    st = product_setup(w)
+    a = product_run(st, L.DGX_FP64, chunk=96)       # 3 graph replays
+    b = product_run(st, L.DGX_FP64, chunk=7)        # eager launches only (chunks < 32)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_shots_are_independent():
+    w = two_media_workload(200)
+    w.sources = [dgx.SourceSpec(1200.0, 1200.0, L.DGX_SRC_FORCE_Z, 0, 15.0),
+                 dgx.SourceSpec(800.0, 1000.0, L.DGX_SRC_EXPLOSIVE, 1, 15.0)]
+    st = product_setup(w)
+    both = dgx.Solver(st, L.DGX_FP64)
+    both.step(200)
+    for shot in (0, 1):
+        w1 = two_media_workload(200)
+        w1.sources = [w.sources[shot]]
+        w1.sources[0].shot = 0
+        alone = product_run(product_setup(w1), L.DGX_FP64)
+        x, z = both.traces(shot)
+        assert np.array_equal(x, alone[0]) and np.array_equal(z, alone[1]), shot
+    both.close()

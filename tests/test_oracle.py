@@ -1,0 +1,347 @@
+"""Pins of the CPU oracle (oracle/dg_oracle.c).
+
+How each oracle function is pinned:
+  reader (orc_parse_*)           reference ReadDouble/ReadInt outputs (live build) bit for bit; numpy
+                                 text parse to 1 ulp; integer round trip
+  geometry (jacobian, normals)   reference fixtures (SHA-256) + closed forms (area/2, rotated edge / L)
+  topology (neighbor_edge,       reference fixtures + brute force: matched face nodes coincide in space
+   neighbor_nodes, ibe)
+  sponge (absorbing_fun)         reference fixtures + hand-evaluated taper values + the end-line quirk
+  dt (time_step)                 reference fixtures + closed form h(2-sqrt2)*0.99/(5 Vp) on a right mesh
+  Ricker (ricker, ricker_delay)  reference fixtures + analytic trough time sqrt(3/2)/(pi f0)
+  source/receiver (get_*,        reference fixtures + M^-1 = V V^T against an exact P2 mass matrix
+   source_vector)
+  operators Dx/Dz, Q, A/B        reference fixtures (SHA-256)
+  step kernels, composition      reference fixtures (all five fields, bit for bit) + invariance to the
+                                 thread count and to element relabelling (bit for bit)
+  [EXT] p2 basis, locate,        Kronecker property, partition of unity, exact reproduction of
+   source_vector_at              quadratics; reduction to the reference case at vertices
+"""
+import ctypes
+import hashlib
+import math
+import pathlib
+
+import numpy as np
+import pytest
+
+import dgx
+import oracle_py as O
+from helpers import OPDIR
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+FIXTURES = sorted(GOLDEN.glob("ref_*.npz"))
+REF_SPONGE = O.SpongeParams(100.0, 100.0, 1800.0, 1800.0, 0.1, 3.1415)  # dg.h:6-12, beta at main:287
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def load_fixture(path):
+    d = np.load(path, allow_pickle=False)
+    mesh = dgx.Mesh(d["point"], d["triangle"], d["neighbor"], d["attribute"])
+    return d, mesh
+
+
+# ---------------------------------------------------------------------------------------------- reader
+def test_operator_files_parse_like_numpy():
+    ops = O.read_operators(OPDIR)
+    for name, shape in O.OP_SHAPES.items():
+        ref = np.loadtxt(OPDIR / f"{name}.txt").reshape(shape)
+        assert np.allclose(ops[name], ref, rtol=4.5e-16, atol=0), name
+        assert np.array_equal(np.signbit(ops[name]), np.signbit(ref)), name  # -0.0 kept
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="reference build (oracle/_ref) not present")
+def test_operator_files_parse_like_reference_reader():
+    import ctypes as C
+    r = O.reference()
+    for name, (rows, cols) in O.OP_SHAPES.items():
+        out = np.zeros(rows * cols)
+        r.ReadDouble(str(OPDIR / f"{name}.txt").encode(), O.dp(out), C.c_long(rows), C.c_long(cols))
+        assert np.array_equal(out.reshape(rows, cols), O.read_operators(OPDIR)[name]), name
+
+
+def test_integer_parse_roundtrip(tmp_path):
+    vals = np.array([[20000, 3, 1, 0, 0], [1, 19802, 19803, 19903, 4], [2, -1, 7, 99999999, 3]], dtype=np.int64)
+    p = tmp_path / "x.ele"
+    dgx.formats.write_ascii(p, vals)
+    got = O.parse_long_text(p.read_bytes(), 3, 5)
+    assert np.array_equal(got, vals)
+
+
+# ------------------------------------------------------------------------------- operator invariants
+def test_operator_file_invariants():
+    ops = O.read_operators(OPDIR)
+    V, iV, Vr, Vs = ops["V"], ops["inverV"], ops["Vr"], ops["Vs"]
+    assert np.abs(V @ iV - np.eye(6)).max() < 2e-8
+    Dr, Ds = Vr @ iV, Vs @ iV
+    r, s = ops["FeketeNew"][:, 0], ops["FeketeNew"][:, 1]
+    assert np.abs(Dr @ np.ones(6)).max() < 1e-7 and np.abs(Ds @ np.ones(6)).max() < 1e-7
+    # exact for linear data up to the 8-digit rounding of the files
+    assert np.abs(Dr @ r - 1).max() < 5e-8 and np.abs(Ds @ s - 1).max() < 5e-8
+    assert np.abs(Dr @ s).max() < 1e-7 and np.abs(Ds @ r).max() < 1e-7
+    assert np.allclose(np.diag(V @ V.T), [18, 4.875, 18, 4.875, 4.875, 18], rtol=1e-7)
+    assert np.allclose(ops["S1"].sum(axis=0), [5.5, 10, 5.5])
+
+
+def exact_p2_mass_reference():
+    """Mass matrix of the P2 Lagrange basis on the reference triangle (-1,-1),(1,-1),(-1,1) (area 2),
+    by a degree-4-exact quadrature (Strang-Fix 6 points), independent of the operator files."""
+    a, b = 0.816847572980459, 0.091576213509771
+    c, d = 0.108103018168070, 0.445948490915965
+    w1, w2 = 0.109951743655322, 0.223381589678011
+    bary = [(a, b, b), (b, a, b), (b, b, a), (c, d, d), (d, c, d), (d, d, c)]
+    wts = [w1] * 3 + [w2] * 3
+    M = np.zeros((6, 6))
+    for (l0, l1, l2), wq in zip(bary, wts):
+        r, s = 2 * l1 - 1, 2 * l2 - 1
+        phi = np.zeros(6)
+        O.oracle().orc_p2_basis(r, s, O.dp(phi))
+        M += wq * 2.0 * np.outer(phi, phi)  # weights sum to 1 -> scale by the area
+    return M
+
+
+def test_inverse_mass_is_VVt():
+    ops = O.read_operators(OPDIR)
+    M = exact_p2_mass_reference()
+    assert np.abs(ops["V"] @ ops["V"].T @ M - np.eye(6)).max() < 1e-6
+
+
+def test_face_lift_is_inverse_mass_times_face_mass():
+    """S_j = M^-1 E_j with E_j the 1-D P2 mass matrix of face j (length 2 on the reference element,
+    except the hypotenuse face 1, length 2*sqrt2, whose scaling the reference folds into Q via L/2)."""
+    ops = O.read_operators(OPDIR)
+    Minv = ops["V"] @ ops["V"].T
+    m1d = np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) / 15.0  # P2 on [-1, 1]
+    faces = [[5, 3, 0], [0, 1, 2], [5, 4, 2]]
+    for j, fn in enumerate(faces):
+        S = ops[f"S{j + 1}"]
+        assert np.abs(Minv[:, fn] @ m1d - S).max() < 1e-5, j
+
+
+# -------------------------------------------------------------------------- fixtures of the reference
+@pytest.mark.parametrize("path", FIXTURES, ids=[p.stem for p in FIXTURES])
+def test_oracle_matches_reference_fixture(path):
+    d, mesh = load_fixture(path)
+    ops = O.read_operators(OPDIR)
+    model = dgx.Model.by_attribute(mesh.attribute)
+    lay = O.setup_reference_layout(mesh.point, mesh.triangle, mesh.neighbor, model.lam, model.miu, model.den, ops,
+                                   REF_SPONGE)
+    mine = dict(Dx=lay.Dx, Dz=lay.Dz, Q=lay.Q, A=lay.A, B=lay.B, absorb_fun=lay.absorb_fun, jacobi=lay.jacobi,
+                jacobi_edge=lay.jacobi_edge, n1=lay.normals[0].ravel(), n2=lay.normals[1].ravel(),
+                n3=lay.normals[2].ravel(), neighbor=lay.neighbor, edge_flag=lay.edge_flag,
+                node_neighbor=lay.node_neighbor)
+    for name, dg in zip(d["hashed"], d["digests"]):
+        assert digest(mine[str(name)]) == str(dg), name
+    dt = O.time_step(mesh.point, mesh.triangle, model.vp)
+    t0 = O.ricker_delay(dt, 15.0)
+    assert dt == float(d["dt"]) and t0 == float(d["t0"])
+    e, fun = O.locate_source(mesh.point, mesh.triangle, ops["V"], *d["src"])
+    assert e == int(d["source"][0]) and np.array_equal(fun, d["fun"])
+    er, w = O.locate_receiver(mesh.point, mesh.triangle, *d["rec"])
+    assert er == int(d["receiver"][0]) and w[int(d["receiver"][1]) - 1] == 1.0
+    n, every = int(d["nsteps"]), int(d["snap_every"])
+    run = O.Run(lay, dt, [(e, 0, fun)], O.wavelet(n, dt, t0, 15.0), [(er, w)])
+    xs, zs = [], []
+    for k in range(n // every):
+        x, z = run.run(every)
+        xs.append(x[:, 0])
+        zs.append(z[:, 0])
+        assert np.array_equal(run.f, d["snaps"][k]), k
+    assert np.array_equal(np.concatenate(xs), d["rec_x"]) and np.array_equal(np.concatenate(zs), d["rec_z"])
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="reference build (oracle/_ref) not present")
+def test_oracle_matches_live_reference_explicit_materials():
+    """A case the fixtures do not cover: per-element materials (elastic gradient) on a random mesh."""
+    sp = dgx.Sponge.for_domain(2000.0, 2000.0, 100.0)
+    mesh = dgx.structured(30, 30, 2000 / 30, diagonal="random", seed=11, sponge_box=sp.box(), attribute=3)
+    depth = 2000 - mesh.centroids()[:, 1]
+    vp = 2500 + depth
+    model = dgx.Model.from_velocities(vp, vp / 1.8, 310 * vp ** 0.25)
+    p = mesh.point
+    src = p[np.argmin(np.hypot(p[:, 0] - 900, p[:, 1] - 1100))]
+    rec = p[np.argmin(np.hypot(p[:, 0] - 1300, p[:, 1] - 1800))]
+    epart = (np.arange(mesh.tri) % 2).astype(np.int64)
+    res, rx, rz, snaps = O.ref_case(OPDIR, mesh, epart, tuple(src), tuple(rec), 120, 60, model=model)
+    ops = O.read_operators(OPDIR)
+    lay = O.setup_reference_layout(mesh.point, mesh.triangle, mesh.neighbor, model.lam, model.miu, model.den, ops,
+                                   REF_SPONGE)
+    dt = O.time_step(mesh.point, mesh.triangle, model.vp)
+    assert dt == res["dt"] and np.array_equal(lay.Q, res["Q"]) and np.array_equal(lay.Dx, res["Dx"])
+    e, fun = O.locate_source(mesh.point, mesh.triangle, ops["V"], *src)
+    er, w = O.locate_receiver(mesh.point, mesh.triangle, *rec)
+    run = O.Run(lay, dt, [(e, 0, fun)], O.wavelet(120, dt, O.ricker_delay(dt, 15.0), 15.0), [(er, w)])
+    x1, z1 = run.run(60)
+    assert np.array_equal(run.f, snaps[0])
+    x2, z2 = run.run(60)
+    assert np.array_equal(run.f, snaps[1])
+    assert np.array_equal(np.concatenate([x1, x2])[:, 0], rx)
+
+
+# ------------------------------------------------------------------------------------- closed forms
+def test_geometry_closed_forms():
+    mesh = dgx.structured(12, 9, 37.0, diagonal="random", jitter=8.0, seed=4)
+    lay = O.setup_reference_layout(mesh.point, mesh.triangle, mesh.neighbor, *([np.ones(mesh.tri)] * 3),
+                                   O.read_operators(OPDIR), None)
+    v = mesh.vertices()
+    area = 0.5 * mesh.area2()
+    assert np.allclose(lay.jacobi, area / 2, rtol=1e-14, atol=0)
+    edges = [(0, 1), (1, 2), (2, 0)]
+    for j, (a, b) in enumerate(edges):
+        d = v[:, b] - v[:, a]
+        L = np.hypot(d[:, 0], d[:, 1])
+        n = np.stack([d[:, 1], -d[:, 0]], 1) / L[:, None]
+        assert np.abs(lay.normals[j] - n).max() < 1e-13, j
+        assert np.allclose(lay.jacobi_edge.reshape(-1, 3)[:, j], L / 2, rtol=1e-15)
+
+
+def test_topology_matched_face_nodes_coincide():
+    mesh = dgx.structured(9, 7, 50.0, diagonal="random", jitter=10.0, seed=2)
+    lay = O.setup_reference_layout(mesh.point, mesh.triangle, mesh.neighbor, *([np.ones(mesh.tri)] * 3),
+                                   O.read_operators(OPDIR), None)
+    xy = np.zeros(mesh.tri * 12)
+    O.oracle().orc_all_node(mesh.tri, O.dp(np.ascontiguousarray(mesh.point)), O.lp(np.ascontiguousarray(mesh.triangle)),
+                            O.dp(np.ascontiguousarray(O.read_operators(OPDIR)["FeketeNew"])), O.dp(xy))
+    xy = xy.reshape(mesh.tri, 6, 2)
+    nn = lay.node_neighbor.reshape(-1, 3, 3)
+    ibe = lay.ibe.reshape(3, 3)
+    interior = 0
+    for e in range(mesh.tri):
+        for j in range(3):
+            if lay.edge_flag[3 * e + j] == 0:
+                assert mesh.neighbor[e, [2, 0, 1][j]] == -1
+                continue
+            interior += 1
+            nb = lay.neighbor[3 * e + j] - 1
+            for i in range(3):
+                own = xy[e, ibe[j, i] - 1]
+                other = xy[nb, nn[e, j, i] - 1]
+                assert np.abs(own - other).max() < 1e-9
+    assert interior > 0
+
+
+def test_sponge_hand_values_and_quirks():
+    """Strip node at distance d gets 0.5+0.5cos(0.1*d*3.1415/100); the box interior and the strip end
+    lines (z == bound_z with x < bound_x) keep 1; corners beyond radius 100 keep 1."""
+    mesh = dgx.structured(40, 40, 50.0, diagonal="same", sponge_box=(100, 1900, 100, 1900), attribute=3)
+    ab = np.zeros(mesh.tri * 6)
+    ops = O.read_operators(OPDIR)
+    O.oracle().orc_absorbing_fun(mesh.tri, O.dp(np.ascontiguousarray(mesh.point)),
+                                 O.lp(np.ascontiguousarray(mesh.triangle)), O.dp(np.ascontiguousarray(ops["FeketeNew"])),
+                                 ctypes.byref(REF_SPONGE), O.dp(ab))
+    xy = np.zeros(mesh.tri * 12)
+    O.oracle().orc_all_node(mesh.tri, O.dp(np.ascontiguousarray(mesh.point)), O.lp(np.ascontiguousarray(mesh.triangle)),
+                            O.dp(np.ascontiguousarray(ops["FeketeNew"])), O.dp(xy))
+    xy = xy.reshape(-1, 2)
+
+    def taper(d):
+        return 0.5 + 0.5 * math.cos(0.1 * d * 3.1415 / 100.0)
+    checked = 0
+    for (x, z), f in zip(xy, ab):
+        if 100 < x < 1900 and 100 < z < 1900:
+            assert f == 1.0
+        elif x < 100 and 100 < z < 1900:
+            assert abs(f - taper(100 - x)) < 1e-15
+            checked += 1
+        elif z == 100.0 and x < 100:
+            assert f == 1.0
+        elif x < 100 and z < 100:
+            d = math.hypot(100 - x, 100 - z)
+            assert (f == 1.0) if d >= 100 else abs(f - taper(d)) < 1e-15
+    assert checked > 50
+    # with the reference's Pi = 3.1415 the factor at d = 100 is cos(0.31415): 1.4e-6 above the exact-pi value
+    assert abs(taper(100.0) - 0.9755296897038572) < 1e-15
+
+
+def test_time_step_closed_form():
+    mesh = dgx.structured(10, 10, 20.0, diagonal="same")
+    dt = O.time_step(mesh.point, mesh.triangle, np.full(mesh.tri, 3000.0))
+    assert abs(dt - 20.0 * (2 - math.sqrt(2)) / 3000.0 / 5 * 0.99) < 1e-18
+    assert abs(dt - 7.73238e-4) < 1e-9
+
+
+@pytest.mark.parametrize("f0", [10.0, 15.0, 25.0])
+def test_ricker_delay_is_first_rising_sample_after_trough(f0):
+    pi = 3.1415
+    dt = 7.73238e-4
+    t0 = O.ricker_delay(dt, f0, pi)
+    t_trough = math.sqrt(1.5) / (pi * f0)
+    k = round(t0 / 2.6 / dt)
+    assert abs(t0 - 2.6 * k * dt) < 1e-15
+    assert t_trough - dt <= k * dt <= t_trough + 2 * dt
+
+
+# ------------------------------------------------------------------------------- [EXT] interpolation
+def test_p2_basis_kronecker_unity_and_quadratics():
+    nodes = O.read_operators(OPDIR)["FeketeNew"]
+    for k, (r, s) in enumerate(nodes):
+        phi = np.zeros(6)
+        O.oracle().orc_p2_basis(r, s, O.dp(phi))
+        e = np.zeros(6)
+        e[k] = 1
+        assert np.array_equal(phi, e)
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        a, b = rng.random(2) * 0.5
+        r, s = 2 * a - 1, 2 * b - 1
+        phi = np.zeros(6)
+        O.oracle().orc_p2_basis(r, s, O.dp(phi))
+        assert abs(phi.sum() - 1) < 1e-14
+        for f in (lambda x, y: x * x, lambda x, y: x * y, lambda x, y: 3 * y * y - x + 2):
+            assert abs(phi @ f(nodes[:, 0], nodes[:, 1]) - f(r, s)) < 1e-13
+
+
+def test_source_vector_at_vertex_reduces_to_reference():
+    ops = O.read_operators(OPDIR)
+    for node in (1, 3, 6):
+        a, b = np.zeros(6), np.zeros(6)
+        O.oracle().orc_source_vector(O.dp(np.ascontiguousarray(ops["V"])), node, 123.5, O.dp(a))
+        phi = np.zeros(6)
+        phi[node - 1] = 1
+        O.oracle().orc_source_vector_at(O.dp(np.ascontiguousarray(ops["V"])), O.dp(phi), 123.5, O.dp(b))
+        assert np.array_equal(a, b)
+
+
+# -------------------------------------------------------------------------------------- invariances
+def _small_run(mesh, threads=None, nsteps=60, src=None):
+    if threads:
+        O.oracle().orc_set_threads(threads)
+    ops = O.read_operators(OPDIR)
+    model = dgx.Model.by_attribute(mesh.attribute)
+    lay = O.setup_reference_layout(mesh.point, mesh.triangle, mesh.neighbor, model.lam, model.miu, model.den, ops,
+                                   REF_SPONGE)
+    dt = O.time_step(mesh.point, mesh.triangle, model.vp)
+    e, fun = src or O.locate_source(mesh.point, mesh.triangle, ops["V"], 1200.0, 1200.0)
+    run = O.Run(lay, dt, [(e, 0, fun)], O.wavelet(nsteps, dt, O.ricker_delay(dt, 15.0), 15.0), [])
+    run.run(nsteps)
+    return run.f, (e, fun)
+
+
+def test_thread_count_invariance():
+    sp = dgx.Sponge.for_domain(2000.0, 2000.0, 100.0)
+    mesh = dgx.structured(20, 20, 100.0, diagonal="random", seed=5, sponge_box=sp.box(),
+                          attribute=lambda c: np.where(c[:, 1] > 1500, 1, 3))
+    n0 = O.oracle().orc_num_threads()
+    a, _ = _small_run(mesh, 1)
+    b, _ = _small_run(mesh, max(2, n0))
+    O.oracle().orc_set_threads(n0)
+    assert np.array_equal(a, b)
+
+
+def test_element_relabelling_invariance():
+    sp = dgx.Sponge.for_domain(2000.0, 2000.0, 100.0)
+    mesh = dgx.structured(20, 20, 100.0, diagonal="random", seed=5, sponge_box=sp.box(),
+                          attribute=lambda c: np.where(c[:, 1] > 1500, 1, 3))
+    a, (e, fun) = _small_run(mesh)
+    perm = np.random.default_rng(1).permutation(mesh.tri)  # new element k = old perm[k]
+    inv = np.argsort(perm)
+    tri2 = mesh.triangle[perm]
+    nb2 = mesh.neighbor[perm]
+    nb2 = np.where(nb2 > 0, inv[np.maximum(nb2, 1) - 1] + 1, -1)
+    m2 = dgx.Mesh(mesh.point, tri2, nb2, mesh.attribute[perm])
+    b, _ = _small_run(m2, src=(int(inv[e - 1]) + 1, fun))
+    b = b.reshape(5, mesh.tri, 6)[:, inv].reshape(5, -1)
+    assert np.array_equal(a, b)
