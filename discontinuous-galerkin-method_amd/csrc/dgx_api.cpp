@@ -22,6 +22,8 @@ hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int whic
 hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st);
 hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
+hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
+hipError_t dgx_launch_batched_f32(const void *args, hipStream_t st, int which);
 }
 
 /* ---- error reporting ---- */
@@ -83,6 +85,11 @@ inline long long fidx(long long q, int i) { return ((q >> 6) * 6 + i) * 64 + (q 
 
 struct dgx_solver {
     int fp32 = 0;
+    int layout = DGX_LAYOUT_ELEMENT;
+    int shot_tile = 0;               /* shot-innermost: shots per lane group */
+    long spad = 1;                   /* shot-innermost: padded shot count */
+    std::vector<int> tune_tiles;     /* autotuner candidates and their ms per step */
+    std::vector<double> tune_ms;
     long tri = 0, tri_pad = 0, nshots = 1, nsteps = 0, nrec = 0, nsrc = 0;
     long step = 0;          /* steps completed */
     long base_host = 0;     /* mirror of the device step counter */
@@ -116,6 +123,20 @@ struct dgx_solver {
 static int launch(dgx_solver *s, int which, int step_off)
 {
     hipError_t e;
+    if (s->layout == DGX_LAYOUT_SHOT_INNER) {
+        if (which != K_VEL_FUSED && which != K_STRESS_FUSED) {
+            dgx_set_error("the split entry points need the element layout (DGX_LAYOUT_ELEMENT)");
+            return DGX_ESTATE;
+        }
+        s->a32.step_off = s->a64.step_off = step_off;
+        e = s->fp32 ? dgx_launch_batched_f32(&s->a32, s->stream, which == K_VEL_FUSED ? 0 : 1)
+                    : dgx_launch_batched_f64(&s->a64, s->stream, which == K_VEL_FUSED ? 0 : 1);
+        if (e != hipSuccess) {
+            dgx_set_error("kernel launch failed: %s", hipGetErrorString(e));
+            return DGX_EHIP;
+        }
+        return DGX_OK;
+    }
     if (s->fp32) {
         s->a32.step_off = step_off;
         e = dgx_launch_f32(&s->a32, (int)s->nshots, s->stream, which);
@@ -149,7 +170,8 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     for (int j = 0; j < 3; j++)
         for (int k = 0; k < 18; k++) a.op.L[j * 18 + k] = (R)S[j][k];
     R *f = (R *)s->fields;
-    size_t per_field = (size_t)s->nshots * s->tri_pad * 6;
+    size_t per_field = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)s->tri * 6 * s->spad
+                                                             : (size_t)s->nshots * s->tri_pad * 6;
     for (int c = 0; c < 5; c++) a.f[c] = f + c * per_field;
     a.fstride = s->tri_pad;
     a.nbr = s->nbr;
@@ -172,6 +194,18 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     a.tri = (int)s->tri;
     a.nper = s->nper;
     a.dt = dt;
+    a.spad = (int)s->spad;
+    a.shot_tile = s->shot_tile;
+    a.shot_start = nullptr;
+    a.shot_len = nullptr;
+}
+
+/* Blocks per XCD slice for the current layout and shot tile. */
+static int blocks_per_xcd(const dgx_solver *s)
+{
+    long per_block = (s->layout == DGX_LAYOUT_SHOT_INNER) ? 4L * (64 / s->shot_tile) : DGX_BLOCK;
+    long nblk = (s->tri + per_block - 1) / per_block;
+    return (int)((nblk + DGX_NXCD - 1) / DGX_NXCD);
 }
 
 static void free_solver(dgx_solver *s)
@@ -192,6 +226,54 @@ template <typename R>
 static std::vector<R> convert(const std::vector<double> &v)
 {
     return std::vector<R>(v.begin(), v.end());
+}
+
+static void set_tile(dgx_solver *s, int tile)
+{
+    s->shot_tile = tile;
+    s->nper = blocks_per_xcd(s);
+    s->a64.shot_tile = s->a32.shot_tile = tile;
+    s->a64.nper = s->a32.nper = s->nper;
+}
+
+/* Shot-innermost autotuner: time a few real steps with every shot tile that divides the padded shot
+ * count (the tile fixes elements per workgroup, 4*64/tile, and shots per lane group), keep the
+ * fastest, then restore the initial state (zero fields and traces, step 0). */
+static int autotune(dgx_solver *s)
+{
+    const int cands[] = {4, 8, 16, 32, 64};
+    const int nt = (int)std::min<long>(3, s->nsteps);
+    int best = 0;
+    double best_ms = 1e300;
+    for (int tile : cands) {
+        if (s->spad % tile) continue;
+        set_tile(s, tile);
+        for (int rep = 0; rep < 2; rep++) { /* first pass warms up */
+            HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+            for (int k = 0; k < nt; k++) {
+                int rc = launch(s, K_VEL_FUSED, k);
+                if (rc == DGX_OK) rc = launch(s, K_STRESS_FUSED, k);
+                if (rc) return rc;
+            }
+            HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+            HIP_TRY(hipEventSynchronize(s->ev[1]));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+            if (rep == 1) {
+                s->tune_tiles.push_back(tile);
+                s->tune_ms.push_back(ms / nt);
+                if (ms < best_ms) { best_ms = ms; best = tile; }
+            }
+        }
+    }
+    set_tile(s, best);
+    size_t rs = s->real_size();
+    HIP_TRY(hipMemsetAsync(s->fields, 0, (size_t)5 * s->tri * 6 * s->spad * rs, s->stream));
+    if (s->trace)
+        HIP_TRY(hipMemsetAsync(s->trace, 0, (size_t)s->nshots * s->nsteps * 2 * s->nrec * sizeof(double), s->stream));
+    HIP_TRY(hipMemsetAsync(s->step_base, 0, sizeof(int), s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return DGX_OK;
 }
 
 extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_solver **out)
@@ -218,9 +300,30 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     s->nsteps = p->nsteps;
     s->nrec = p->nrec;
     s->nsrc = p->nsrc;
-    long nblk = (T + DGX_BLOCK - 1) / DGX_BLOCK;
-    s->nper = (int)((nblk + DGX_NXCD - 1) / DGX_NXCD);
-    s->tri_pad = (long)s->nper * DGX_NXCD * DGX_BLOCK;
+    if (p->layout != DGX_LAYOUT_AUTO && p->layout != DGX_LAYOUT_ELEMENT && p->layout != DGX_LAYOUT_SHOT_INNER) {
+        dgx_set_error("dgx_create: unknown layout %d", p->layout);
+        free_solver(s);
+        return DGX_EARG;
+    }
+    s->layout = (p->layout == DGX_LAYOUT_AUTO) ? (p->nshots >= 8 ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT) : p->layout;
+    if (s->layout == DGX_LAYOUT_SHOT_INNER) {
+        const int t = p->shot_tile;
+        if (t != 0 && t != 4 && t != 8 && t != 16 && t != 32 && t != 64) {
+            dgx_set_error("dgx_create: shot_tile must be 0 (autotune), 4, 8, 16, 32 or 64");
+            free_solver(s);
+            return DGX_EARG;
+        }
+        const long unit = t ? t : 8;
+        s->spad = (p->nshots + unit - 1) / unit * unit;
+        s->shot_tile = t ? t : 8;
+        s->tri_pad = T; /* shot-innermost fields are element-major: no block padding */
+    } else if (p->shot_tile) {
+        dgx_set_error("dgx_create: shot_tile applies to the shot-innermost layout only");
+        free_solver(s);
+        return DGX_EARG;
+    }
+    s->nper = blocks_per_xcd(s);
+    if (s->layout == DGX_LAYOUT_ELEMENT) s->tri_pad = (long)s->nper * DGX_NXCD * DGX_BLOCK;
 
     /* ---- validate vertices, orientation (fun_DG.c:447 assumes J > 0) ---- */
     const double *P = p->point;
@@ -395,7 +498,10 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         mat_host[s->tri_pad + q] = p->lambda[e];
         mat_host[2 * s->tri_pad + q] = p->miu[e];
         if (p->absorb_fun)
-            for (int i = 0; i < 6; i++) absorb_host[fidx(q, i)] = p->absorb_fun[6 * e + i];
+            for (int i = 0; i < 6; i++) {
+                long long o = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (long long)q * 6 + i : fidx(q, i);
+                absorb_host[o] = p->absorb_fun[6 * e + i];
+            }
     }
 
     /* ---- device allocations (all host-side validation is done above) ---- */
@@ -407,7 +513,8 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (auto &ev : s->ev) HIP_TRY(hipEventCreate(&ev));
-    size_t nreal_fields = (size_t)5 * s->nshots * s->tri_pad * 6;
+    size_t nreal_fields = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)5 * s->tri * 6 * s->spad
+                                                                 : (size_t)5 * s->nshots * s->tri_pad * 6;
     HIP_TRY(hipMalloc(&s->fields, nreal_fields * s->real_size()));
     HIP_TRY(hipMemset(s->fields, 0, nreal_fields * s->real_size()));
     HIP_TRY(upload(&s->nbr, nbr_host));
@@ -450,6 +557,10 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
 
     if (s->fp32) fill_args(s, s->a32, ops, p->dt);
     else fill_args(s, s->a64, ops, p->dt);
+    if (s->layout == DGX_LAYOUT_SHOT_INNER && p->shot_tile == 0 && s->nsteps > 0) {
+        int rc = autotune(s);
+        if (rc) { free_solver(s); return rc; }
+    }
     *out = s;
     return DGX_OK;
 }
@@ -554,23 +665,44 @@ extern "C" int dgx_record(dgx_solver *s, long t)
     return DGX_OK;
 }
 
+/* Index of (q, i) of one shot inside the raw copy of a field: for the element layout the raw copy is
+ * that shot's array; for the shot-innermost layout it is the whole field and the shot is added. */
+static inline long long raw_index(const dgx_solver *s, long q, int i, long shot)
+{
+    if (s->layout == DGX_LAYOUT_SHOT_INNER) return ((long long)q * 6 + i) * s->spad + shot;
+    return fidx(q, i);
+}
+
 template <typename R>
-static void unpack(const dgx_solver *s, const std::vector<char> &raw, double *out)
+static void unpack(const dgx_solver *s, const std::vector<char> &raw, long shot, double *out)
 {
     const R *r = (const R *)raw.data();
     for (long q = 0; q < s->tri; q++) {
         long e = s->perm[q];
-        for (int i = 0; i < 6; i++) out[6 * e + i] = (double)r[fidx(q, i)];
+        for (int i = 0; i < 6; i++) out[6 * e + i] = (double)r[raw_index(s, q, i, shot)];
     }
 }
 
 template <typename R>
-static void pack(const dgx_solver *s, const double *in, std::vector<char> &raw)
+static void pack(const dgx_solver *s, const double *in, long shot, std::vector<char> &raw)
 {
     R *r = (R *)raw.data();
     for (long q = 0; q < s->tri; q++) {
         long e = s->perm[q];
-        for (int i = 0; i < 6; i++) r[fidx(q, i)] = (R)in[6 * e + i];
+        for (int i = 0; i < 6; i++) r[raw_index(s, q, i, shot)] = (R)in[6 * e + i];
+    }
+}
+
+/* Bytes and byte offset of the raw copy of field c used for `shot`. */
+static void raw_span(const dgx_solver *s, int c, long shot, size_t *off, size_t *len)
+{
+    size_t rs = s->real_size();
+    if (s->layout == DGX_LAYOUT_SHOT_INNER) {
+        *len = (size_t)s->tri * 6 * s->spad * rs;
+        *off = (size_t)c * *len;
+    } else {
+        *len = (size_t)s->tri_pad * 6 * rs;
+        *off = ((size_t)c * s->nshots + shot) * *len;
     }
 }
 
@@ -578,15 +710,16 @@ extern "C" int dgx_get_fields(dgx_solver *s, long shot, double *vx, double *vz, 
 {
     if (!s || shot < 0 || shot >= s->nshots) return DGX_EARG;
     double *outs[5] = {vx, vz, txx, tzz, txz};
-    size_t rs = s->real_size(), n = (size_t)s->tri_pad * 6;
-    std::vector<char> raw(n * rs);
     HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<char> raw;
     for (int c = 0; c < 5; c++) {
         if (!outs[c]) continue;
-        const char *src = (const char *)s->fields + ((size_t)c * s->nshots + shot) * n * rs;
-        HIP_TRY(hipMemcpy(raw.data(), src, n * rs, hipMemcpyDeviceToHost));
-        if (s->fp32) unpack<float>(s, raw, outs[c]);
-        else unpack<double>(s, raw, outs[c]);
+        size_t off, len;
+        raw_span(s, c, shot, &off, &len);
+        raw.resize(len);
+        HIP_TRY(hipMemcpy(raw.data(), (const char *)s->fields + off, len, hipMemcpyDeviceToHost));
+        if (s->fp32) unpack<float>(s, raw, shot, outs[c]);
+        else unpack<double>(s, raw, shot, outs[c]);
     }
     return DGX_OK;
 }
@@ -596,16 +729,18 @@ extern "C" int dgx_set_fields(dgx_solver *s, long shot, const double *vx, const 
 {
     if (!s || shot < 0 || shot >= s->nshots) return DGX_EARG;
     const double *ins[5] = {vx, vz, txx, tzz, txz};
-    size_t rs = s->real_size(), n = (size_t)s->tri_pad * 6;
-    std::vector<char> raw(n * rs, 0);
     HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<char> raw;
     for (int c = 0; c < 5; c++) {
         if (!ins[c]) continue;
-        std::fill(raw.begin(), raw.end(), 0);
-        if (s->fp32) pack<float>(s, ins[c], raw);
-        else pack<double>(s, ins[c], raw);
-        char *dst = (char *)s->fields + ((size_t)c * s->nshots + shot) * n * rs;
-        HIP_TRY(hipMemcpy(dst, raw.data(), n * rs, hipMemcpyHostToDevice));
+        size_t off, len;
+        raw_span(s, c, shot, &off, &len);
+        raw.assign(len, 0);
+        /* read-modify-write so the other shots sharing the array keep their values */
+        HIP_TRY(hipMemcpy(raw.data(), (const char *)s->fields + off, len, hipMemcpyDeviceToHost));
+        if (s->fp32) pack<float>(s, ins[c], shot, raw);
+        else pack<double>(s, ins[c], shot, raw);
+        HIP_TRY(hipMemcpy((char *)s->fields + off, raw.data(), len, hipMemcpyHostToDevice));
     }
     return DGX_OK;
 }
@@ -634,6 +769,20 @@ extern "C" int dgx_synchronize(dgx_solver *s)
 }
 
 extern "C" long dgx_current_step(const dgx_solver *s) { return s ? s->step : -1; }
+
+extern "C" int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile, int *tiles, double *ms_per_step,
+                               int cap)
+{
+    if (!s) return DGX_EARG;
+    if (layout) *layout = s->layout;
+    if (shot_tile) *shot_tile = s->shot_tile;
+    int n = (int)s->tune_tiles.size();
+    for (int k = 0; k < n && k < cap; k++) {
+        if (tiles) tiles[k] = s->tune_tiles[k];
+        if (ms_per_step) ms_per_step[k] = s->tune_ms[k];
+    }
+    return n;
+}
 
 extern "C" int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stress)
 {

@@ -29,6 +29,10 @@ extern "C" {
 
 enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1 };
 enum dgx_source_kind { DGX_SRC_FORCE_Z = 0, DGX_SRC_EXPLOSIVE = 1 };
+/* Field layout on the device.  ELEMENT: 64-element blocks, one thread per element and shot (the
+ * single-shot path).  SHOT_INNER: shots innermost, lanes over shots, for many shots on one mesh.
+ * AUTO picks SHOT_INNER from 8 shots on. */
+enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_INNER = 2 };
 
 /* Reference-element operators as read from input_files/precomputed_matrix/ (row-major), the arrays
  * DG_leap_Frog_OpenMP.c:52-81 reads.  Vb/inverVb are read by the reference but never used. */
@@ -76,6 +80,8 @@ typedef struct {
     const dgx_source *src;
     long nrec;
     const dgx_receiver *rec; /* one receiver spread shared by all shots */
+    int layout;            /* enum dgx_layout */
+    int shot_tile;         /* shots per lane group in the shot-innermost layout; 0 = autotune at create */
 } dgx_problem;
 
 typedef struct dgx_solver dgx_solver;
@@ -106,8 +112,12 @@ int dgx_set_fields(dgx_solver *s, long shot, const double *vx, const double *vz,
 int dgx_get_traces(dgx_solver *s, long shot, double *rec_x, double *rec_z);
 int dgx_synchronize(dgx_solver *s);
 long dgx_current_step(const dgx_solver *s);
-/* Average device time per dgx_step call kernel, measured with HIP events on the solver's stream. */
+/* Average device time of the two fused kernels over n steps, measured with HIP events on the solver's
+ * stream (the steps are real steps: the state advances). */
 int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stress);
+/* Layout and shot tile in use (after autotuning), and the autotuner's timings: for each candidate
+ * tile k, tiles[k] and ms_per_step[k]; returns the number of candidates (0 if none were timed). */
+int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile, int *tiles, double *ms_per_step, int cap);
 const char *dgx_last_error(void);
 
 #ifdef __cplusplus
