@@ -38,22 +38,30 @@ def parse():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--shots", type=int, default=64, help="shots of the batched FP32 leg (0 = skip)")
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--copy-probe", type=int, default=1,
+                    help="also time a plain device copy (achievable bandwidth, counter calibration)")
     return ap.parse_args()
 
 
-def run_path(setup, precision, warmup, steps):
-    s = dgx.Solver(setup, precision)
+def run_path(setup, precision, warmup, steps, layout=L.DGX_LAYOUT_AUTO, bench_launches=100):
+    """Warm up, time `steps` whole steps (hipGraph replays), then time each fused kernel alone over
+    back-to-back launches between two HIP events on the solver's stream."""
+    s = dgx.Solver(setup, precision, layout)
+    info = s.layout_info()
     s.step(warmup)
     s.synchronize()
     t = time.perf_counter()
     s.step(steps)
     s.synchronize()
     sec = time.perf_counter() - t
-    kv, ks = s.kernel_times(min(50, setup.nsteps - warmup - steps))
     x, z = s.traces(0)
+    kv = s.kernel_bench(0, bench_launches)
+    ks = s.kernel_bench(1, bench_launches)
     s.close()
     n = warmup + steps
-    return sec, kv, ks, x[:n], z[:n]
+    return sec, kv, ks, x[:n], z[:n], info
 
 
 def cpu_baseline(w, setup, budget_s):
@@ -102,13 +110,13 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design note "
                                    "(shots and requests shard with no exchange), not a measured path"}))
         sys.exit(2)
-    w = dgx.configs.gradient_8x4km(a.warmup + a.steps + 60)
+    w = dgx.configs.gradient_8x4km(a.warmup + a.steps + 5)
     setup = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
     T = w.mesh.tri
     dof = T * 30
 
-    sec64, kv64, ks64, x64, z64 = run_path(setup, L.DGX_FP64, a.warmup, a.steps)
-    sec32, kv32, ks32, x32, z32 = run_path(setup, L.DGX_FP32, a.warmup, a.steps)
+    sec64, kv64, ks64, x64, z64, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps)
+    sec32, kv32, ks32, x32, z32, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps)
     rate64 = dof * a.steps / sec64
     rate32 = dof * a.steps / sec32
 
@@ -148,6 +156,7 @@ def main():
         "data": "synthetic (seeded jittered mesh and gradient model generated in-process)",
         "config": {"workload": w.name, "elements": T, "dof": dof, "shots": 1, "receivers": int(len(w.receivers)),
                    "path": "plain FP64, fused velocity+stress kernels, hipGraph step replay"},
+        "kernel_timing": "each fused kernel timed alone over back-to-back launches between HIP events",
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "achieved_GBps": achieved / 1e9, "kernel_ms": kms,
@@ -155,6 +164,35 @@ def main():
         "fp32": {"value": rate32, "ms_per_step": sec32 * 1e3 / a.steps, "trace_rel_l2_vs_fp64": err32,
                  "velocity_ms": kv32, "stress_ms": ks32, "speedup_vs_fp64": rate32 / rate64},
     }
+    if a.shots > 0:
+        # batched FP32: S shots on the same mesh, sources spread along x at the same depth, shot 0
+        # identical to the single-shot source, shot-innermost layout with the autotuned shot tile
+        S = a.shots
+        nb = a.batched_steps
+        xs = np.linspace(500.0, 7500.0, S)
+        xs[0] = w.sources[0].x
+        H = float(w.mesh.point[:, 1].max())
+        srcs = [dgx.SourceSpec(float(x), w.sources[0].z, L.DGX_SRC_EXPLOSIVE, k, 20.0) for k, x in enumerate(xs)]
+        bset = dgx.build_setup(w.mesh, w.model, srcs, w.receivers, 20 + nb + 5, sponge=w.sponge, dt=setup.dt)
+        secb, kvb, ksb, xb, zb, info = run_path(bset, L.DGX_FP32, 20, nb, L.DGX_LAYOUT_SHOT_INNER, 20)
+        rateb = dof * S * nb / secb
+        nn = min(len(xb), len(x64))
+        errb = max(rel(xb[:nn], x64[:nn]), rel(zb[:nn], z64[:nn]))
+        bytes_b = T * 6 * 4 * S
+        out["batched_fp32"] = {
+            "value": rateb, "unit": "DOF-updates/s", "shots": S, "steps": nb, "ms_per_step": secb * 1e3 / nb,
+            "x_vs_plain_fp64": rateb / rate64, "shot0_trace_rel_l2_vs_fp64": errb,
+            "layout": "shot-innermost", "shot_tile": info[1],
+            "autotune_ms_per_step": {str(k): v for k, v in info[2].items()},
+            "velocity_ms": kvb, "stress_ms": ksb,
+            "velocity_GBps": bytes_b * 7 / (kvb * 1e-3) / 1e9, "stress_GBps": bytes_b * 8 / (ksb * 1e-3) / 1e9}
+    if a.copy_probe:
+        import ctypes as C
+        nbytes, reps = 1 << 30, 20
+        ms = C.c_double()
+        L.check(L.load().dgx_copy_probe(nbytes, reps, C.byref(ms)), "dgx_copy_probe")
+        out["copy_probe"] = {"bytes_copied": nbytes, "ms": ms.value,
+                             "GBps_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}
     out["roofline"]["achieved"] = achieved / 1e9
     out["roofline"]["peak"] = HBM_PEAK / 1e9
     if not a.no_cpu:

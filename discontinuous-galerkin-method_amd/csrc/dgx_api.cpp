@@ -22,6 +22,7 @@ hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int whic
 hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st);
 hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
+hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int blocks, hipStream_t st);
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
 hipError_t dgx_launch_batched_f32(const void *args, hipStream_t st, int which);
 }
@@ -769,6 +770,56 @@ extern "C" int dgx_synchronize(dgx_solver *s)
 }
 
 extern "C" long dgx_current_step(const dgx_solver *s) { return s ? s->step : -1; }
+
+extern "C" int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms)
+{
+    /* n back-to-back launches of one fused kernel between two events: the kernel's own duration
+     * without per-launch event overhead.  The launches repeat the current step, so the fields are
+     * no longer a valid state afterwards. */
+    if (!s || n <= 0 || (which != 0 && which != 1) || s->step >= s->nsteps) return DGX_EARG;
+    int off = (int)(s->step - s->base_host);
+    int k = which == 0 ? K_VEL_FUSED : K_STRESS_FUSED;
+    int rc = launch(s, k, off); /* warm */
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+    for (long i = 0; i < n; i++) {
+        rc = launch(s, k, off);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+    HIP_TRY(hipEventSynchronize(s->ev[1]));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, s->ev[0], s->ev[1]));
+    if (ms) *ms = t / n;
+    return DGX_OK;
+}
+
+extern "C" int dgx_copy_probe(long bytes, long reps, double *ms)
+{
+    if (bytes < 1024 || reps < 1) return DGX_EARG;
+    long n32 = bytes / 32;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipMalloc(&a, n32 * 32));
+    HIP_TRY(hipMalloc(&b, n32 * 32));
+    HIP_TRY(hipMemset(a, 0, n32 * 32));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int blocks = 256 * 8;
+    HIP_TRY(dgx_launch_copy(a, b, n32, blocks, 0));
+    HIP_TRY(hipEventRecord(e0, 0));
+    for (long r = 0; r < reps; r++) HIP_TRY(dgx_launch_copy(a, b, n32, blocks, 0));
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return DGX_OK;
+}
 
 extern "C" int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile, int *tiles, double *ms_per_step,
                                int cap)

@@ -1,0 +1,365 @@
+/*
+ * dgx_batched.hip -- shot-innermost kernels: many shots (independent wavefields) on one mesh.
+ *
+ * Layout: value (element q, node i, shot s) of a field lives at (q*6 + i)*S + s, S the padded shot
+ * count.  A wavefront covers EW = 64/ST elements x ST consecutive shots (lane = e_sub*ST + shot), so
+ * every load of one node of one field for one element is ST consecutive values, and so is every face
+ * gather from a neighbour: the irregular part of the access pattern is amortised over ST shots.  The
+ * per-element data (neighbours, code word, vertices, material) is identical across a lane group and is
+ * read once per ST shots; with ST = 64 it is wave-uniform.
+ *
+ * The arithmetic is the same as the element-per-thread kernels (dgx_plain.hip), term for term, so
+ * the two layouts agree to rounding; the tests compare them.
+ */
+#include <hip/hip_runtime.h>
+
+#include "dgx_kernels.h"
+
+namespace dgxb {
+
+using dgx::StepArgs;
+
+__device__ __forceinline__ constexpr int face_node(int j, int k)
+{
+    return (j == 0) ? (k == 0 ? 5 : k == 1 ? 3 : 0) : (j == 1) ? k : (k == 0 ? 5 : k == 1 ? 4 : 2);
+}
+
+__device__ __forceinline__ int nb_node(unsigned m, unsigned rev, int k)
+{
+    const unsigned long long tab = 0x245210035ull;
+    const int kk = rev ? 2 - k : k;
+    return (int)((tab >> (4 * (3 * m + kk))) & 15);
+}
+
+struct Geom {
+    double rx, rz, sx, sz, invD;
+    double ex[3], ez[3];
+};
+
+__device__ __forceinline__ Geom geometry(const double2 *pt, int4 v)
+{
+    const double2 p0 = pt[v.x], p1 = pt[v.y], p2 = pt[v.z];
+    Geom g;
+    g.ex[0] = p1.x - p0.x; g.ez[0] = p1.y - p0.y;
+    g.ex[1] = p2.x - p1.x; g.ez[1] = p2.y - p1.y;
+    g.ex[2] = p0.x - p2.x; g.ez[2] = p0.y - p2.y;
+    const double D = g.ex[0] * (p2.y - p0.y) - (p2.x - p0.x) * g.ez[0];
+    g.invD = 1.0 / D;
+    const double t = 2.0 * g.invD;
+    g.rx = (p2.y - p0.y) * t;
+    g.rz = -(p2.x - p0.x) * t;
+    g.sx = -g.ez[0] * t;
+    g.sz = g.ex[0] * t;
+    return g;
+}
+
+/* Element and shot of this lane; false when outside the mesh. */
+template <int ST, typename R>
+__device__ __forceinline__ bool my_work(const StepArgs<R> &a, int &q, int &s)
+{
+    constexpr int EW = 64 / ST;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x;
+    const int tile = (b & (DGX_NXCD - 1)) * a.nper + (b >> 3); /* XCD-contiguous element tiles */
+    q = (tile * 4 + wave) * EW + lane / ST;
+    s = blockIdx.y * ST + (lane % ST);
+    return q < a.tri;
+}
+
+template <typename R>
+__device__ __forceinline__ long long bidx(const StepArgs<R> &a, int q, int i, int s)
+{
+    return ((long long)q * 6 + i) * a.spad + s;
+}
+
+/* local step of shot s at the launch's global step; -1 when the shot is not running */
+template <typename R>
+__device__ __forceinline__ int shot_step(const StepArgs<R> &a, int s, int gstep)
+{
+    if (!a.shot_start) return gstep;
+    const int t = gstep - a.shot_start[s];
+    return (t >= 0 && t < a.shot_len[s]) ? t : -1;
+}
+
+template <typename R>
+__device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned code, int q, int i, int s, int lstep,
+                                                 R &txx, R &tzz, R &txz)
+{
+    if (code & DGX_CODE_SPONGE) {
+        const R f = a.absorb[q * 6 + i];
+        txx *= f; tzz *= f; txz *= f;
+    }
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (ax && lstep >= 0) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
+            const dgx_src_dev &src = a.src[k];
+            if (src.shot != s || src.kind != 1) continue;
+            const R inc = (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + lstep]);
+            txx += inc;
+            tzz += inc;
+        }
+    }
+}
+
+template <typename R, int ST>
+__global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
+{
+    int q, s;
+    if (!my_work<ST>(a, q, s)) return;
+    R *__restrict__ Vx = a.f[0];
+    R *__restrict__ Vz = a.f[1];
+    const R *__restrict__ Txx = a.f[2];
+    const R *__restrict__ Tzz = a.f[3];
+    const R *__restrict__ Txz = a.f[4];
+    const int lstep = shot_step(a, s, *a.step_base + a.step_off);
+
+    const int4 nb = a.nbr[q];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[q]);
+
+    R txx[6], tzz[6], txz[6], vx[6], vz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = bidx(a, q, i, s);
+        txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const R f = a.absorb[q * 6 + i];
+            txx[i] *= f; tzz[i] *= f; txz[i] *= f; vx[i] *= f; vz[i] *= f;
+        }
+    }
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (ax && lstep >= 0) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
+            const dgx_src_dev &src = a.src[k];
+            if (src.shot != s) continue;
+            const double w = a.wav[(long long)src.wavelet * a.nsteps + lstep];
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const R inc = (R)(src.fun[i] * w);
+                if (src.kind == 0) vz[i] += inc;
+                else { txx[i] += inc; tzz[i] += inc; }
+            }
+        }
+    }
+
+    const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
+    R ux[6], wx[6], uz[6], wz[6];
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        ux[n] = rx * txx[n] + rz * txz[n];
+        wx[n] = sx * txx[n] + sz * txz[n];
+        uz[n] = rx * txz[n] + rz * tzz[n];
+        wz[n] = sx * txz[n] + sz * tzz[n];
+    }
+    R rhx[6], rhz[6], flx[6], flz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        R sx_ = 0, sz_ = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            sx_ += a.op.Dr[i * 6 + n] * ux[n] + a.op.Ds[i * 6 + n] * wx[n];
+            sz_ += a.op.Dr[i * 6 + n] * uz[n] + a.op.Ds[i * 6 + n] * wz[n];
+        }
+        rhx[i] = sx_;
+        rhz[i] = sz_;
+        flx[i] = 0;
+        flz[i] = 0;
+    }
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        R dxx[3], dzz[3], dxz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(j, k);
+            dxx[k] = -txx[on]; dzz[k] = -tzz[on]; dxz[k] = -txz[on];
+        }
+        const int e = nbs[j];
+        if (e >= 0) {
+            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+            const bool special = (code & DGX_CODE_NBSPECIAL(j)) != 0;
+            const unsigned ncode = special ? (unsigned)a.nbr[e].w : 0u;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int nn = nb_node(m, rev, k);
+                const long long o = bidx(a, e, nn, s);
+                R nxx = Txx[o], nzz = Tzz[o], nxz = Txz[o];
+                if (special) condition_stress(a, ncode, e, nn, s, lstep, nxx, nzz, nxz);
+                dxx[k] += nxx; dzz[k] += nzz; dxz[k] += nxz;
+            }
+        }
+        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+        R gx[3], gz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            gx[k] = ez * dxx[k] - ex * dxz[k];
+            gz[k] = ez * dxz[k] - ex * dzz[k];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                flx[i] += a.op.L[j * 18 + i * 3 + k] * gx[k];
+                flz[i] += a.op.L[j * 18 + i * 3 + k] * gz[k];
+            }
+        }
+    }
+    const R invD = (R)g.invD;
+    const R c = a.rho_inv[q] * (R)a.dt;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = bidx(a, q, i, s);
+        Vx[o] = (rhx[i] + flx[i] * invD) * c + vx[i];
+        Vz[o] = (rhz[i] + flz[i] * invD) * c + vz[i];
+    }
+}
+
+template <typename R, int ST>
+__global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
+{
+    int q, s;
+    if (!my_work<ST>(a, q, s)) return;
+    const R *__restrict__ Vx = a.f[0];
+    const R *__restrict__ Vz = a.f[1];
+    R *__restrict__ Txx = a.f[2];
+    R *__restrict__ Tzz = a.f[3];
+    R *__restrict__ Txz = a.f[4];
+    const int lstep = shot_step(a, s, *a.step_base + a.step_off);
+
+    const int4 nb = a.nbr[q];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[q]);
+
+    R vx[6], vz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = bidx(a, q, i, s);
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
+    R exx[6], ezz[6], exz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        R rvx = 0, svx = 0, rvz = 0, svz = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            rvx += a.op.Dr[i * 6 + n] * vx[n];
+            svx += a.op.Ds[i * 6 + n] * vx[n];
+            rvz += a.op.Dr[i * 6 + n] * vz[n];
+            svz += a.op.Ds[i * 6 + n] * vz[n];
+        }
+        exx[i] = rx * rvx + sx * svx;
+        ezz[i] = rz * rvz + sz * svz;
+        exz[i] = rx * rvz + sx * svz + rz * rvx + sz * svx;
+    }
+    R fx[6], fz[6], fxz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        R dvx[3], dvz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(j, k);
+            dvx[k] = -vx[on]; dvz[k] = -vz[on];
+        }
+        const int e = nbs[j];
+        if (e >= 0) {
+            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const long long o = bidx(a, e, nb_node(m, rev, k), s);
+                dvx[k] += Vx[o];
+                dvz[k] += Vz[o];
+            }
+        }
+        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+        R px[3], pz[3], pxz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            px[k] = ez * dvx[k];
+            pz[k] = -ex * dvz[k];
+            pxz[k] = ez * dvz[k] - ex * dvx[k];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const R l = a.op.L[j * 18 + i * 3 + k];
+                fx[i] += l * px[k];
+                fz[i] += l * pz[k];
+                fxz[i] += l * pxz[k];
+            }
+        }
+    }
+    const R invD = (R)g.invD;
+    const R lam = a.lam[q], mu = a.mu[q], l2m = lam + 2 * mu, dt = (R)a.dt;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = bidx(a, q, i, s);
+        R txx = Txx[o], tzz = Tzz[o], txz = Txz[o];
+        condition_stress(a, code, q, i, s, lstep, txx, tzz, txz);
+        const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
+        Txx[o] = (l2m * axx + lam * azz) * dt + txx;
+        Tzz[o] = (lam * axx + l2m * azz) * dt + tzz;
+        Txz[o] = (mu * axz) * dt + txz;
+    }
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (ax && lstep >= 0 && lstep < a.nsteps) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
+            const dgx_rec_dev &rc = a.rec[r];
+            double sx_ = 0, sz_ = 0;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                sx_ += rc.w[i] * (double)vx[i];
+                sz_ += rc.w[i] * (double)vz[i];
+            }
+            double *t = a.trace + ((long long)s * a.nsteps + lstep) * 2 * a.nrec;
+            t[rc.index] = sx_;
+            t[a.nrec + rc.index] = sz_;
+        }
+    }
+}
+
+} // namespace dgxb
+
+template <typename R, int ST>
+static hipError_t launch_b(const dgx::StepArgs<R> &a, hipStream_t st, int which)
+{
+    dim3 grid(DGX_NXCD * a.nper, a.spad / ST), block(DGX_BLOCK);
+    if (which == 0) dgxb::kb_velocity<R, ST><<<grid, block, 0, st>>>(a);
+    else dgxb::kb_stress<R, ST><<<grid, block, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t dispatch_b(const dgx::StepArgs<R> &a, hipStream_t st, int which)
+{
+    switch (a.shot_tile) {
+    case 4: return launch_b<R, 4>(a, st, which);
+    case 8: return launch_b<R, 8>(a, st, which);
+    case 16: return launch_b<R, 16>(a, st, which);
+    case 32: return launch_b<R, 32>(a, st, which);
+    case 64: return launch_b<R, 64>(a, st, which);
+    }
+    return hipErrorInvalidValue;
+}
+
+extern "C" {
+hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which)
+{
+    return dispatch_b(*(const dgx::StepArgs<double> *)args, st, which);
+}
+hipError_t dgx_launch_batched_f32(const void *args, hipStream_t st, int which)
+{
+    return dispatch_b(*(const dgx::StepArgs<float> *)args, st, which);
+}
+}

@@ -118,62 +118,39 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_velocity(StepArgs<R> a)
     const unsigned code = (unsigned)nb.w;
     const Geom g = geometry(a.pt, a.vtx[q]);
 
-    R txx[6], tzz[6], txz[6], vx[6], vz[6];
+    /* stresses, conditioned (sponge, explosive source) as the reference leaves them before
+     * LU_Velocity reads them */
+    R txx[6], tzz[6], txz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const int o = fidx(q, i);
         txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
-        vx[i] = Vx[o]; vz[i] = Vz[o];
     }
     if (FUSED) {
-        if (code & DGX_CODE_SPONGE) {
 #pragma unroll
-            for (int i = 0; i < 6; i++) {
-                const R f = a.absorb[fidx(q, i)];
-                txx[i] *= f; tzz[i] *= f; txz[i] *= f; vx[i] *= f; vz[i] *= f;
-            }
-        }
-        const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
-        if (ax) {
-            const dgx_aux x = a.aux[ax - 1];
-            for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
-                const dgx_src_dev &src = a.src[s];
-                if (src.shot != shot) continue;
-                const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
-#pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    const R inc = (R)(src.fun[i] * w);
-                    if (src.kind == 0) vz[i] += inc;
-                    else { txx[i] += inc; tzz[i] += inc; }
-                }
-            }
-        }
+        for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, shot, step, txx[i], tzz[i], txz[i]);
     }
 
+    /* volume term: Dr(rx u + rz w) + Ds(sx u + sz w) */
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
-    R ux[6], wx[6], uz[6], wz[6];
+    R rhx[6], rhz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { rhx[i] = 0; rhz[i] = 0; }
 #pragma unroll
     for (int n = 0; n < 6; n++) {
-        ux[n] = rx * txx[n] + rz * txz[n];
-        wx[n] = sx * txx[n] + sz * txz[n];
-        uz[n] = rx * txz[n] + rz * tzz[n];
-        wz[n] = sx * txz[n] + sz * tzz[n];
-    }
-    R rhx[6], rhz[6], flx[6], flz[6];
+        const R ux = rx * txx[n] + rz * txz[n], wx = sx * txx[n] + sz * txz[n];
+        const R uz = rx * txz[n] + rz * tzz[n], wz = sx * txz[n] + sz * tzz[n];
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
-        R sx_ = 0, sz_ = 0;
-#pragma unroll
-        for (int n = 0; n < 6; n++) {
-            sx_ += a.op.Dr[i * 6 + n] * ux[n] + a.op.Ds[i * 6 + n] * wx[n];
-            sz_ += a.op.Dr[i * 6 + n] * uz[n] + a.op.Ds[i * 6 + n] * wz[n];
+        for (int i = 0; i < 6; i++) {
+            rhx[i] += a.op.Dr[i * 6 + n] * ux + a.op.Ds[i * 6 + n] * wx;
+            rhz[i] += a.op.Dr[i * 6 + n] * uz + a.op.Ds[i * 6 + n] * wz;
         }
-        rhx[i] = sx_;
-        rhz[i] = sz_;
-        flx[i] = 0;
-        flz[i] = 0;
     }
 
+    /* surface term, scaled by 1/D at the end */
+    R flx[6], flz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
     const int nbs[3] = {nb.x, nb.y, nb.z};
 #pragma unroll
     for (int j = 0; j < 3; j++) {
@@ -198,29 +175,43 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_velocity(StepArgs<R> a)
             }
         }
         const R ex = (R)g.ex[j], ez = (R)g.ez[j];
-        R gx[3], gz[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            gx[k] = ez * dxx[k] - ex * dxz[k];
-            gz[k] = ez * dxz[k] - ex * dzz[k];
-        }
+            const R gx = ez * dxx[k] - ex * dxz[k];
+            const R gz = ez * dxz[k] - ex * dzz[k];
 #pragma unroll
-        for (int i = 0; i < 6; i++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                flx[i] += a.op.L[j * 18 + i * 3 + k] * gx[k];
-                flz[i] += a.op.L[j * 18 + i * 3 + k] * gz[k];
+            for (int i = 0; i < 6; i++) {
+                flx[i] += a.op.L[j * 18 + i * 3 + k] * gx;
+                flz[i] += a.op.L[j * 18 + i * 3 + k] * gz;
             }
         }
     }
 
+    /* V = (sponge(V) + source) + dt/rho * rhs */
     const R invD = (R)g.invD;
     const R c = a.rho_inv[q] * (R)a.dt;
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const int o = fidx(q, i);
-        Vx[o] = (rhx[i] + flx[i] * invD) * c + vx[i];
-        Vz[o] = (rhz[i] + flz[i] * invD) * c + vz[i];
+        R vx = Vx[o], vz = Vz[o];
+        if (FUSED) {
+            if (code & DGX_CODE_SPONGE) {
+                const R f = a.absorb[o];
+                vx *= f;
+                vz *= f;
+            }
+            if (ax) {
+                const dgx_aux x = a.aux[ax - 1];
+                for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+                    const dgx_src_dev &src = a.src[s];
+                    if (src.shot == shot && src.kind == 0)
+                        vz += (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + step]);
+                }
+            }
+        }
+        Vx[o] = (rhx[i] + flx[i] * invD) * c + vx;
+        Vz[o] = (rhz[i] + flz[i] * invD) * c + vz;
     }
 }
 
@@ -394,6 +385,13 @@ __global__ void k_record(StepArgs<R> a, int t, int nshots)
     tr[a.nrec + rc.index] = sz_;
 }
 
+/* Plain streaming copy, the reference point for HBM counters and achievable bandwidth. */
+__global__ void __launch_bounds__(256) k_copy(const double4 *__restrict__ src, double4 *__restrict__ dst, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_advance(int *step_base, int n)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *step_base += n;
@@ -440,6 +438,11 @@ hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int 
     int blocks = (n + 255) / 256;
     if (fp32) dgx::k_record<float><<<blocks, 256, 0, st>>>(*(const StepArgs<float> *)args, t, nshots);
     else dgx::k_record<double><<<blocks, 256, 0, st>>>(*(const StepArgs<double> *)args, t, nshots);
+    return hipGetLastError();
+}
+hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int blocks, hipStream_t st)
+{
+    dgx::k_copy<<<blocks, 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
     return hipGetLastError();
 }
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st)

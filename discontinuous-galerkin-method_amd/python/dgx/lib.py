@@ -17,6 +17,7 @@ OPERATOR_DIR = PKG_DIR / "input_files" / "precomputed_matrix"
 
 DGX_FP64, DGX_FP32 = 0, 1
 DGX_SRC_FORCE_Z, DGX_SRC_EXPLOSIVE = 0, 1
+DGX_LAYOUT_AUTO, DGX_LAYOUT_ELEMENT, DGX_LAYOUT_SHOT_INNER = 0, 1, 2
 
 dptr = C.POINTER(C.c_double)
 lptr = C.POINTER(C.c_long)
@@ -47,7 +48,7 @@ class ProblemC(C.Structure):
                 ("neighbor", lptr), ("lam", dptr), ("miu", dptr), ("den", dptr), ("absorb_fun", dptr),
                 ("dt", C.c_double), ("precision", C.c_int), ("nshots", C.c_long), ("nsteps", C.c_long),
                 ("nwavelets", C.c_long), ("wavelet", dptr), ("nsrc", C.c_long), ("src", C.POINTER(Source)),
-                ("nrec", C.c_long), ("rec", C.POINTER(Receiver))]
+                ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("layout", C.c_int), ("shot_tile", C.c_int)]
 
 
 class SpongeParams(C.Structure):
@@ -86,6 +87,9 @@ def load() -> C.CDLL:
         "dgx_current_step": (C.c_long, [vp]),
         "dgx_kernel_times": (C.c_int, [vp, C.c_long, dptr, dptr]),
         "dgx_last_error": (C.c_char_p, []),
+        "dgx_kernel_bench": (C.c_int, [vp, C.c_int, C.c_long, dptr]),
+        "dgx_copy_probe": (C.c_int, [C.c_long, C.c_long, dptr]),
+        "dgx_layout_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), dptr, C.c_int]),
         "dgx_read_ascii": (C.c_long, [C.c_char_p, dptr, C.c_long, C.c_long]),
         "dgx_read_ascii_long": (C.c_long, [C.c_char_p, lptr, C.c_long, C.c_long]),
         "dgx_read_operators": (C.c_int, [C.c_char_p, C.POINTER(Operators)]),

@@ -148,7 +148,8 @@ def build_setup(mesh: Mesh, model: Model, sources: Sequence[SourceSpec], receive
 class Solver:
     """Owns one dgx_solver handle (device-resident fields for ``nshots`` shots on one mesh)."""
 
-    def __init__(self, setup: Setup, precision: int = L.DGX_FP64):
+    def __init__(self, setup: Setup, precision: int = L.DGX_FP64, layout: int = L.DGX_LAYOUT_AUTO,
+                 shot_tile: int = 0):
         self.lib = L.load()
         self.setup = setup
         m = setup.mesh
@@ -167,7 +168,8 @@ class Solver:
         self._keep += [srcs, recs]
         p = L.ProblemC(m.tri, m.point_num, L.dp(pt), L.lp(tr), L.lp(nb), L.dp(la), L.dp(mi), L.dp(de),
                        L.dp(ab) if ab is not None else L.dptr(), setup.dt, precision, setup.nshots, setup.nsteps,
-                       wv.shape[0], L.dp(wv), len(setup.sources), srcs, len(setup.receivers), recs)
+                       wv.shape[0], L.dp(wv), len(setup.sources), srcs, len(setup.receivers), recs, layout,
+                       shot_tile)
         self.handle = C.c_void_p()
         L.check(self.lib.dgx_create(C.byref(p), C.byref(setup.ops), C.byref(self.handle)), "dgx_create")
         self.precision = precision
@@ -226,6 +228,21 @@ class Solver:
         z = np.zeros((n, max(r, 1)), np.float64)
         L.check(self.lib.dgx_get_traces(self.handle, shot, L.dp(x), L.dp(z)), "dgx_get_traces")
         return x[:, :r], z[:, :r]
+
+    def kernel_bench(self, which: int, n: int) -> float:
+        """Average ms of one fused kernel (0 velocity, 1 stress) over n back-to-back launches.
+        Leaves the fields invalid."""
+        ms = C.c_double()
+        L.check(self.lib.dgx_kernel_bench(self.handle, which, n, C.byref(ms)), "dgx_kernel_bench")
+        return ms.value
+
+    def layout_info(self):
+        """(layout, shot_tile, {tile: ms per step} from the autotuner)."""
+        lay, tile = C.c_int(), C.c_int()
+        tiles = (C.c_int * 8)()
+        ms = (C.c_double * 8)()
+        n = L.check(self.lib.dgx_layout_info(self.handle, C.byref(lay), C.byref(tile), tiles, ms, 8), "layout_info")
+        return lay.value, tile.value, {tiles[k]: ms[k] for k in range(n)}
 
     def kernel_times(self, n: int) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()

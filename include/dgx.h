@@ -115,6 +115,12 @@ long dgx_current_step(const dgx_solver *s);
 /* Average device time of the two fused kernels over n steps, measured with HIP events on the solver's
  * stream (the steps are real steps: the state advances). */
 int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stress);
+/* Average duration of one fused kernel (which: 0 velocity, 1 stress) over n back-to-back launches
+ * between two HIP events on the solver's stream.  Invalidates the fields (the launches repeat a step). */
+int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms);
+/* Streaming copy of `bytes` bytes (read + write) repeated `reps` times on the default stream: the
+ * achievable-bandwidth reference and the calibration point of the HBM byte counters. */
+int dgx_copy_probe(long bytes, long reps, double *ms);
 /* Layout and shot tile in use (after autotuning), and the autotuner's timings: for each candidate
  * tile k, tiles[k] and ms_per_step[k]; returns the number of candidates (0 if none were timed). */
 int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile, int *tiles, double *ms_per_step, int cap);
