@@ -196,6 +196,7 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     a.nper = s->nper;
     a.dt = dt;
     a.spad = (int)s->spad;
+    a.nshots = (int)s->nshots;
     a.shot_tile = s->shot_tile;
     a.shot_start = nullptr;
     a.shot_len = nullptr;

@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
         Txz[o] = (mu * axz) * dt + txz;
     }
     const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
-    if (ax && lstep >= 0 && lstep < a.nsteps) {
+    if (ax && lstep >= 0 && lstep < a.nsteps && s < a.nshots) {
         const dgx_aux x = a.aux[ax - 1];
         for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
             const dgx_rec_dev &rc = a.rec[r];

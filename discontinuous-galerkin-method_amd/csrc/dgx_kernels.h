@@ -77,6 +77,7 @@ struct StepArgs {
     double dt;
     /* shot-innermost layout only */
     int spad;              /* padded shot count: stride between nodes */
+    int nshots;            /* real shots; padded lanes s >= nshots never record */
     int shot_tile;         /* shots per lane group (ST); a wavefront covers 64/ST elements */
     const int *shot_start; /* optional per-shot first global step (NULL: all shots start at 0) */
     const int *shot_len;   /* optional per-shot step count */
