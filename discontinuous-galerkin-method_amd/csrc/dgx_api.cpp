@@ -644,9 +644,17 @@ extern "C" int dgx_absorb_boundary(dgx_solver *s) { return s ? launch(s, K_ABSOR
 extern "C" int dgx_lu_velocity(dgx_solver *s) { return s ? launch(s, K_VEL, 0) : DGX_EARG; }
 extern "C" int dgx_lu_stress_elastic(dgx_solver *s) { return s ? launch(s, K_STRESS, 0) : DGX_EARG; }
 
+static int needs_element_layout(const dgx_solver *s)
+{
+    if (s->layout == DGX_LAYOUT_ELEMENT) return DGX_OK;
+    dgx_set_error("the split entry points need the element layout (DGX_LAYOUT_ELEMENT)");
+    return DGX_ESTATE;
+}
+
 extern "C" int dgx_source_kernel(dgx_solver *s, double w)
 {
     if (!s) return DGX_EARG;
+    if (int rc = needs_element_layout(s)) return rc;
     if (s->nsrc == 0) return DGX_OK;
     /* the caller-ordered list follows the aux-ordered copy in the same allocation */
     dgx::StepArgs<double> a64 = s->a64;
@@ -662,6 +670,7 @@ extern "C" int dgx_source_kernel(dgx_solver *s, double w)
 extern "C" int dgx_record(dgx_solver *s, long t)
 {
     if (!s || t < 0 || t >= s->nsteps) return DGX_EARG;
+    if (int rc = needs_element_layout(s)) return rc;
     hipError_t e = dgx_launch_record(s->args(), s->fp32, (int)t, (int)s->nshots, (int)s->nrec, s->stream);
     if (e != hipSuccess) { dgx_set_error("record launch failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
     return DGX_OK;
