@@ -22,7 +22,7 @@ hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int whic
 hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st);
 hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
-hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int blocks, hipStream_t st);
+hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int variant, hipStream_t st);
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
 hipError_t dgx_launch_batched_f32(const void *args, hipStream_t st, int which);
 }
@@ -488,8 +488,21 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         nbr_host[q] = make_int4(nbq[0], nbq[1], nbq[2], (int)code);
         vtx_host[q] = make_int4((int)(v[0] - 1), (int)(v[1] - 1), (int)(v[2] - 1), 0);
     }
-    std::vector<double2> pt_host(p->point_num);
-    for (long i = 0; i < p->point_num; i++) pt_host[i] = make_double2(P[2 * i], P[2 * i + 1]);
+    /* vertices renumbered in order of first use along the element order, so the three vertex loads
+     * of neighbouring elements hit the same cache lines */
+    std::vector<int> vnew(p->point_num, -1);
+    std::vector<double2> pt_host;
+    pt_host.reserve(p->point_num);
+    for (long q = 0; q < T; q++)
+        for (int k = 0; k < 3; k++) {
+            int &slot = k == 0 ? vtx_host[q].x : k == 1 ? vtx_host[q].y : vtx_host[q].z;
+            int v = slot;
+            if (vnew[v] < 0) {
+                vnew[v] = (int)pt_host.size();
+                pt_host.push_back(make_double2(P[2 * v], P[2 * v + 1]));
+            }
+            slot = vnew[v];
+        }
 
     /* ---- materials (Elastic_A_B, fun_DG.c:1071-1088): 1/rho, lambda, mu per element ---- */
     std::vector<double> mat_host(3 * s->tri_pad, 0.0), absorb_host(6 * s->tri_pad, 1.0);
@@ -804,9 +817,9 @@ extern "C" int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms)
     return DGX_OK;
 }
 
-extern "C" int dgx_copy_probe(long bytes, long reps, double *ms)
+extern "C" int dgx_copy_probe(long bytes, long reps, int variant, double *ms)
 {
-    if (bytes < 1024 || reps < 1) return DGX_EARG;
+    if (bytes < 1024 || reps < 1 || variant < 0 || variant > 1) return DGX_EARG;
     long n32 = bytes / 32;
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0, e1;
@@ -815,10 +828,9 @@ extern "C" int dgx_copy_probe(long bytes, long reps, double *ms)
     HIP_TRY(hipMemset(a, 0, n32 * 32));
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    int blocks = 256 * 8;
-    HIP_TRY(dgx_launch_copy(a, b, n32, blocks, 0));
+    HIP_TRY(dgx_launch_copy(a, b, n32, variant, 0));
     HIP_TRY(hipEventRecord(e0, 0));
-    for (long r = 0; r < reps; r++) HIP_TRY(dgx_launch_copy(a, b, n32, blocks, 0));
+    for (long r = 0; r < reps; r++) HIP_TRY(dgx_launch_copy(a, b, n32, variant, 0));
     HIP_TRY(hipEventRecord(e1, 0));
     HIP_TRY(hipEventSynchronize(e1));
     float t = 0;

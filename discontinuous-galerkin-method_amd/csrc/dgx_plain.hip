@@ -120,11 +120,12 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_velocity(StepArgs<R> a)
 
     /* stresses, conditioned (sponge, explosive source) as the reference leaves them before
      * LU_Velocity reads them */
-    R txx[6], tzz[6], txz[6];
+    R txx[6], tzz[6], txz[6], vxe[6], vze[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const int o = fidx(q, i);
         txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
+        if (sizeof(R) == 4) { vxe[i] = Vx[o]; vze[i] = Vz[o]; } /* FP32: issue every load up front */
     }
     if (FUSED) {
 #pragma unroll
@@ -194,7 +195,9 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_velocity(StepArgs<R> a)
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const int o = fidx(q, i);
-        R vx = Vx[o], vz = Vz[o];
+        R vx, vz;
+        if (sizeof(R) == 4) { vx = vxe[i]; vz = vze[i]; }
+        else { vx = Vx[o]; vz = Vz[o]; } /* FP64: load late, keeping registers for occupancy */
         if (FUSED) {
             if (code & DGX_CODE_SPONGE) {
                 const R f = a.absorb[o];
@@ -385,11 +388,33 @@ __global__ void k_record(StepArgs<R> a, int t, int nshots)
     tr[a.nrec + rc.index] = sz_;
 }
 
-/* Plain streaming copy, the reference point for HBM counters and achievable bandwidth. */
+/* Plain streaming copies, the reference point for HBM counters and achievable bandwidth.
+ * variant 0: grid-stride loop, one 32-byte vector per iteration;
+ * variant 1: one workgroup per 32 KiB, four 32-byte vectors per thread, non-temporal. */
 __global__ void __launch_bounds__(256) k_copy(const double4 *__restrict__ src, double4 *__restrict__ dst, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_copy4(const double4 *__restrict__ src, double4 *__restrict__ dst, long long n)
+{
+    const v4d *s = (const v4d *)src;
+    v4d *d = (v4d *)dst;
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    v4d v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const long long i = base + k * 256;
+        if (i < n) v[k] = __builtin_nontemporal_load(&s[i]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const long long i = base + k * 256;
+        if (i < n) __builtin_nontemporal_store(v[k], &d[i]);
+    }
 }
 
 __global__ void k_advance(int *step_base, int n)
@@ -440,9 +465,10 @@ hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int 
     else dgx::k_record<double><<<blocks, 256, 0, st>>>(*(const StepArgs<double> *)args, t, nshots);
     return hipGetLastError();
 }
-hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int blocks, hipStream_t st)
+hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int variant, hipStream_t st)
 {
-    dgx::k_copy<<<blocks, 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
+    if (variant == 0) dgx::k_copy<<<256 * 8, 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
+    else dgx::k_copy4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
     return hipGetLastError();
 }
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st)

@@ -119,8 +119,9 @@ int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stre
  * between two HIP events on the solver's stream.  Invalidates the fields (the launches repeat a step). */
 int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms);
 /* Streaming copy of `bytes` bytes (read + write) repeated `reps` times on the default stream: the
- * achievable-bandwidth reference and the calibration point of the HBM byte counters. */
-int dgx_copy_probe(long bytes, long reps, double *ms);
+ * achievable-bandwidth reference and the calibration point of the HBM byte counters.  variant 0:
+ * grid-stride loop; variant 1: 32 KiB per workgroup, non-temporal loads and stores. */
+int dgx_copy_probe(long bytes, long reps, int variant, double *ms);
 /* Layout and shot tile in use (after autotuning), and the autotuner's timings: for each candidate
  * tile k, tiles[k] and ms_per_step[k]; returns the number of candidates (0 if none were timed). */
 int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile, int *tiles, double *ms_per_step, int cap);
