@@ -84,6 +84,8 @@ inline long long fidx(long long q, int i) { return ((q >> 6) * 6 + i) * 64 + (q 
 
 } // namespace
 
+#include "dgx_internal.h"
+
 struct dgx_solver {
     int fp32 = 0;
     int layout = DGX_LAYOUT_ELEMENT;
@@ -278,6 +280,127 @@ static int autotune(dgx_solver *s)
     return DGX_OK;
 }
 
+/*
+ * Host-side mesh build shared by the solver and the server: validation (vertex ids, counter-clockwise
+ * orientation, neighbour consistency, density), locality order (Morton key of element centroids),
+ * face codes from the .neigh columns, vertices renumbered along that order, per-element material and
+ * sponge factors.  `shot_inner` selects the element-major sponge layout of the shot-innermost kernels
+ * (otherwise the 64-element blocked layout).  `explosive` (caller numbering, may be NULL) marks
+ * elements whose neighbours must condition their face values.
+ */
+int dgx_build_mesh_host(const dgx_problem *p, int shot_inner, long tri_pad, const std::vector<char> *explosive,
+                        dgx_mesh_host *out)
+{
+    const long T = p->tri;
+    const double *P = p->point;
+    const long *Tv = p->triangle;
+    out->tri = T;
+    out->tri_pad = tri_pad;
+    double xmin = P[0], xmax = P[0], zmin = P[1], zmax = P[1];
+    for (long i = 1; i < p->point_num; i++) {
+        xmin = std::min(xmin, P[2 * i]); xmax = std::max(xmax, P[2 * i]);
+        zmin = std::min(zmin, P[2 * i + 1]); zmax = std::max(zmax, P[2 * i + 1]);
+    }
+    std::vector<uint64_t> key(T);
+    for (long e = 0; e < T; e++) {
+        for (int k = 0; k < 3; k++)
+            if (Tv[3 * e + k] < 1 || Tv[3 * e + k] > p->point_num) {
+                dgx_set_error("element %ld: vertex id %ld out of range", e + 1, Tv[3 * e + k]);
+                return DGX_EMESH;
+            }
+        const double *a = &P[2 * (Tv[3 * e] - 1)], *b = &P[2 * (Tv[3 * e + 1] - 1)], *c = &P[2 * (Tv[3 * e + 2] - 1)];
+        double D = (b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]);
+        if (!(D > 0)) {
+            dgx_set_error("element %ld is not counter-clockwise (2*area = %g)", e + 1, D);
+            return DGX_EMESH;
+        }
+        double cx = (a[0] + b[0] + c[0]) / 3, cz = (a[1] + b[1] + c[1]) / 3;
+        double ext = std::max(xmax - xmin, zmax - zmin);
+        if (!(ext > 0)) ext = 1;
+        key[e] = ((uint64_t)morton((cx - xmin) / ext, (cz - zmin) / ext) << 32) | (uint64_t)e;
+    }
+    std::sort(key.begin(), key.end());
+    out->perm.resize(T);
+    out->inv.resize(T);
+    for (long q = 0; q < T; q++) {
+        out->perm[q] = (long)(key[q] & 0xffffffffu);
+        out->inv[out->perm[q]] = q;
+    }
+    std::vector<char> sponge(T, 0);
+    if (p->absorb_fun)
+        for (long e = 0; e < T; e++)
+            for (int i = 0; i < 6; i++)
+                if (p->absorb_fun[6 * e + i] != 1.0) sponge[e] = 1;
+
+    /* face j of element e runs v_j -> v_{j+1} (j = 0, 1) or v0 -> v2 (j = 2); Triangle's .neigh column c
+     * holds the element opposite vertex c (fun_DG.c:576-578) */
+    static const int col_of_face[3] = {2, 0, 1};
+    out->nbr.assign(tri_pad, make_int4(-1, -1, -1, 0));
+    out->vtx.assign(tri_pad, make_int4(0, 0, 0, 0));
+    for (long q = 0; q < T; q++) {
+        long e = out->perm[q];
+        const long *v = &Tv[3 * e];
+        int nbq[3] = {-1, -1, -1};
+        unsigned code = 0;
+        for (int j = 0; j < 3; j++) {
+            long nb = p->neighbor[3 * e + col_of_face[j]];
+            if (nb == -1 || nb == 0) continue;
+            if (nb < 1 || nb > T) {
+                dgx_set_error("element %ld: neighbour %ld out of range", e + 1, nb);
+                return DGX_EMESH;
+            }
+            const long *w = &Tv[3 * (nb - 1)];
+            long fa = (j == 2) ? v[0] : v[j], fb = (j == 2) ? v[2] : v[j + 1];
+            int m = -1;
+            const long nb_face[3][2] = {{w[0], w[1]}, {w[1], w[2]}, {w[0], w[2]}};
+            for (int t = 0; t < 3; t++)
+                if ((nb_face[t][0] == fa && nb_face[t][1] == fb) || (nb_face[t][0] == fb && nb_face[t][1] == fa)) m = t;
+            if (m < 0) {
+                dgx_set_error("element %ld face %d: neighbour %ld does not share that edge", e + 1, j, nb);
+                return DGX_EMESH;
+            }
+            unsigned rev = (nb_face[m][0] != fa) ? 1u : 0u; /* start vertices differ (fun_DG.c:1225,1251) */
+            nbq[j] = (int)out->inv[nb - 1];
+            code |= ((unsigned)m | (rev << 2)) << (3 * j);
+            if (sponge[nb - 1] || (explosive && (*explosive)[nb - 1])) code |= DGX_CODE_NBSPECIAL(j);
+        }
+        if (sponge[e]) code |= DGX_CODE_SPONGE;
+        out->nbr[q] = make_int4(nbq[0], nbq[1], nbq[2], (int)code);
+        out->vtx[q] = make_int4((int)(v[0] - 1), (int)(v[1] - 1), (int)(v[2] - 1), 0);
+    }
+    /* vertices renumbered in order of first use along the element order, so the three vertex loads
+     * of neighbouring elements hit the same cache lines */
+    std::vector<int> vnew(p->point_num, -1);
+    out->pt.clear();
+    out->pt.reserve(p->point_num);
+    for (long q = 0; q < T; q++)
+        for (int k = 0; k < 3; k++) {
+            int &slot = k == 0 ? out->vtx[q].x : k == 1 ? out->vtx[q].y : out->vtx[q].z;
+            int v = slot;
+            if (vnew[v] < 0) {
+                vnew[v] = (int)out->pt.size();
+                out->pt.push_back(make_double2(P[2 * v], P[2 * v + 1]));
+            }
+            slot = vnew[v];
+        }
+    /* materials (Elastic_A_B, fun_DG.c:1071-1088): 1/rho, lambda, mu per element */
+    out->mat.assign(3 * tri_pad, 0.0);
+    out->absorb.assign(6 * tri_pad, 1.0);
+    for (long q = 0; q < T; q++) {
+        long e = out->perm[q];
+        if (!(p->den[e] > 0)) { dgx_set_error("element %ld: density must be positive", e + 1); return DGX_EARG; }
+        out->mat[q] = 1 / p->den[e];
+        out->mat[tri_pad + q] = p->lambda[e];
+        out->mat[2 * tri_pad + q] = p->miu[e];
+        if (p->absorb_fun)
+            for (int i = 0; i < 6; i++) {
+                long long o = shot_inner ? (long long)q * 6 + i : fidx(q, i);
+                out->absorb[o] = p->absorb_fun[6 * e + i];
+            }
+    }
+    return DGX_OK;
+}
+
 extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_solver **out)
 {
     if (!p || !ops || !out) { dgx_set_error("dgx_create: null argument"); return DGX_EARG; }
@@ -327,51 +450,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     s->nper = blocks_per_xcd(s);
     if (s->layout == DGX_LAYOUT_ELEMENT) s->tri_pad = (long)s->nper * DGX_NXCD * DGX_BLOCK;
 
-    /* ---- validate vertices, orientation (fun_DG.c:447 assumes J > 0) ---- */
-    const double *P = p->point;
-    const long *Tv = p->triangle;
-    double xmin = P[0], xmax = P[0], zmin = P[1], zmax = P[1];
-    for (long i = 1; i < p->point_num; i++) {
-        xmin = std::min(xmin, P[2 * i]); xmax = std::max(xmax, P[2 * i]);
-        zmin = std::min(zmin, P[2 * i + 1]); zmax = std::max(zmax, P[2 * i + 1]);
-    }
-    std::vector<uint64_t> key(T);
-    for (long e = 0; e < T; e++) {
-        for (int k = 0; k < 3; k++)
-            if (Tv[3 * e + k] < 1 || Tv[3 * e + k] > p->point_num) {
-                dgx_set_error("element %ld: vertex id %ld out of range", e + 1, Tv[3 * e + k]);
-                free_solver(s);
-                return DGX_EMESH;
-            }
-        const double *a = &P[2 * (Tv[3 * e] - 1)], *b = &P[2 * (Tv[3 * e + 1] - 1)], *c = &P[2 * (Tv[3 * e + 2] - 1)];
-        double D = (b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]);
-        if (!(D > 0)) {
-            dgx_set_error("element %ld is not counter-clockwise (2*area = %g)", e + 1, D);
-            free_solver(s);
-            return DGX_EMESH;
-        }
-        double cx = (a[0] + b[0] + c[0]) / 3, cz = (a[1] + b[1] + c[1]) / 3;
-        double ext = std::max(xmax - xmin, zmax - zmin);
-        if (!(ext > 0)) ext = 1;
-        key[e] = ((uint64_t)morton((cx - xmin) / ext, (cz - zmin) / ext) << 32) | (uint64_t)e;
-    }
-    /* ---- locality order: sort by Morton key (ties by caller index) ---- */
-    std::sort(key.begin(), key.end());
-    s->perm.resize(T);
-    s->inv.resize(T);
-    for (long q = 0; q < T; q++) {
-        s->perm[q] = (long)(key[q] & 0xffffffffu);
-        s->inv[s->perm[q]] = q;
-    }
-
-    /* ---- sponge flags ---- */
-    std::vector<char> sponge(T, 0);
-    if (p->absorb_fun)
-        for (long e = 0; e < T; e++)
-            for (int i = 0; i < 6; i++)
-                if (p->absorb_fun[6 * e + i] != 1.0) sponge[e] = 1;
-
-    /* ---- sources / receivers per element ---- */
+    /* ---- sources / receivers per element (caller numbering) ---- */
     std::vector<std::vector<int>> src_of(0), rec_of(0);
     std::vector<char> explosive(T, 0);
     std::vector<int> elem_aux(T, -1);
@@ -406,11 +485,26 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         rec_of[aux_slot(rc.elem - 1)].push_back((int)k);
     }
     if (aux_elems.size() > DGX_MAX_AUX) { dgx_set_error("too many source/receiver elements"); free_solver(s); return DGX_EARG; }
+
+    /* ---- mesh: order, topology, vertices, materials, sponge ---- */
+    dgx_mesh_host mh;
+    {
+        int rc = dgx_build_mesh_host(p, s->layout == DGX_LAYOUT_SHOT_INNER, s->tri_pad, &explosive, &mh);
+        if (rc) { free_solver(s); return rc; }
+    }
+    s->perm = mh.perm;
+    s->inv = mh.inv;
+    std::vector<int4> &nbr_host = mh.nbr;
+    std::vector<int4> &vtx_host = mh.vtx;
+    std::vector<double2> &pt_host = mh.pt;
+    std::vector<double> &mat_host = mh.mat, &absorb_host = mh.absorb;
+
     std::vector<dgx_aux> aux_host(aux_elems.size());
     std::vector<dgx_src_dev> src_host;
     std::vector<dgx_rec_dev> rec_host;
     for (size_t x = 0; x < aux_elems.size(); x++) {
         long q = s->inv[aux_elems[x]];
+        nbr_host[q].w = (int)((unsigned)nbr_host[q].w | ((unsigned)(x + 1) << DGX_CODE_AUX_SHIFT));
         aux_host[x].src_begin = (int)src_host.size();
         for (int k : src_of[x]) {
             const dgx_source &sr = p->src[k];
@@ -445,79 +539,6 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         for (int i = 0; i < 6; i++) src_list[k].fun[i] = sr.fun[i];
     }
     s->src_host = src_list;
-
-    /* ---- topology: face j of element e runs v_j -> v_{j+1} (j = 0, 1) or v0 -> v2 (j = 2); Triangle's
-     * .neigh column c holds the element opposite vertex c (fun_DG.c:576-578) ---- */
-    static const int col_of_face[3] = {2, 0, 1};
-    std::vector<int4> nbr_host(s->tri_pad), vtx_host(s->tri_pad);
-    for (long q = 0; q < s->tri_pad; q++) {
-        nbr_host[q] = make_int4(-1, -1, -1, 0);
-        vtx_host[q] = make_int4(0, 0, 0, 0);
-    }
-    for (long q = 0; q < T; q++) {
-        long e = s->perm[q];
-        const long *v = &Tv[3 * e];
-        int nbq[3] = {-1, -1, -1};
-        unsigned code = 0;
-        for (int j = 0; j < 3; j++) {
-            long nb = p->neighbor[3 * e + col_of_face[j]];
-            if (nb == -1 || nb == 0) continue;
-            if (nb < 1 || nb > T) {
-                dgx_set_error("element %ld: neighbour %ld out of range", e + 1, nb);
-                free_solver(s);
-                return DGX_EMESH;
-            }
-            const long *w = &Tv[3 * (nb - 1)];
-            long fa = (j == 2) ? v[0] : v[j], fb = (j == 2) ? v[2] : v[j + 1];
-            int m = -1;
-            const long nb_face[3][2] = {{w[0], w[1]}, {w[1], w[2]}, {w[0], w[2]}};
-            for (int t = 0; t < 3; t++)
-                if ((nb_face[t][0] == fa && nb_face[t][1] == fb) || (nb_face[t][0] == fb && nb_face[t][1] == fa)) m = t;
-            if (m < 0) {
-                dgx_set_error("element %ld face %d: neighbour %ld does not share that edge", e + 1, j, nb);
-                free_solver(s);
-                return DGX_EMESH;
-            }
-            unsigned rev = (nb_face[m][0] != fa) ? 1u : 0u; /* start vertices differ (fun_DG.c:1225,1251) */
-            nbq[j] = (int)s->inv[nb - 1];
-            code |= ((unsigned)m | (rev << 2)) << (3 * j);
-            if (sponge[nb - 1] || explosive[nb - 1]) code |= DGX_CODE_NBSPECIAL(j);
-        }
-        if (sponge[e]) code |= DGX_CODE_SPONGE;
-        if (elem_aux[e] >= 0) code |= (unsigned)(elem_aux[e] + 1) << DGX_CODE_AUX_SHIFT;
-        nbr_host[q] = make_int4(nbq[0], nbq[1], nbq[2], (int)code);
-        vtx_host[q] = make_int4((int)(v[0] - 1), (int)(v[1] - 1), (int)(v[2] - 1), 0);
-    }
-    /* vertices renumbered in order of first use along the element order, so the three vertex loads
-     * of neighbouring elements hit the same cache lines */
-    std::vector<int> vnew(p->point_num, -1);
-    std::vector<double2> pt_host;
-    pt_host.reserve(p->point_num);
-    for (long q = 0; q < T; q++)
-        for (int k = 0; k < 3; k++) {
-            int &slot = k == 0 ? vtx_host[q].x : k == 1 ? vtx_host[q].y : vtx_host[q].z;
-            int v = slot;
-            if (vnew[v] < 0) {
-                vnew[v] = (int)pt_host.size();
-                pt_host.push_back(make_double2(P[2 * v], P[2 * v + 1]));
-            }
-            slot = vnew[v];
-        }
-
-    /* ---- materials (Elastic_A_B, fun_DG.c:1071-1088): 1/rho, lambda, mu per element ---- */
-    std::vector<double> mat_host(3 * s->tri_pad, 0.0), absorb_host(6 * s->tri_pad, 1.0);
-    for (long q = 0; q < T; q++) {
-        long e = s->perm[q];
-        if (!(p->den[e] > 0)) { dgx_set_error("element %ld: density must be positive", e + 1); free_solver(s); return DGX_EARG; }
-        mat_host[q] = 1 / p->den[e];
-        mat_host[s->tri_pad + q] = p->lambda[e];
-        mat_host[2 * s->tri_pad + q] = p->miu[e];
-        if (p->absorb_fun)
-            for (int i = 0; i < 6; i++) {
-                long long o = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (long long)q * 6 + i : fidx(q, i);
-                absorb_host[o] = p->absorb_fun[6 * e + i];
-            }
-    }
 
     /* ---- device allocations (all host-side validation is done above) ---- */
     int ndev = 0;

@@ -81,6 +81,19 @@ __device__ __forceinline__ int shot_step(const StepArgs<R> &a, int s, int gstep)
     return (t >= 0 && t < a.shot_len[s]) ? t : -1;
 }
 
+/* aux record index + 1 of element q (0: none) */
+template <typename R>
+__device__ __forceinline__ unsigned aux_of(const StepArgs<R> &a, int q, unsigned code)
+{
+    return a.aux_of_elem ? (unsigned)(a.aux_of_elem[q] + 1) : (code >> DGX_CODE_AUX_SHIFT);
+}
+
+template <typename R>
+__device__ __forceinline__ double wavelet_at(const StepArgs<R> &a, const dgx_src_dev &src, int s, int lstep)
+{
+    return a.slot_wav ? a.slot_wav[s][lstep] : a.wav[(long long)src.wavelet * a.nsteps + lstep];
+}
+
 template <typename R>
 __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned code, int q, int i, int s, int lstep,
                                                  R &txx, R &tzz, R &txz)
@@ -89,30 +102,28 @@ __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned 
         const R f = a.absorb[q * 6 + i];
         txx *= f; tzz *= f; txz *= f;
     }
-    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    const unsigned ax = aux_of(a, q, code);
     if (ax && lstep >= 0) {
         const dgx_aux x = a.aux[ax - 1];
         for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
             const dgx_src_dev &src = a.src[k];
             if (src.shot != s || src.kind != 1) continue;
-            const R inc = (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + lstep]);
+            const R inc = (R)(src.fun[i] * wavelet_at(a, src, s, lstep));
             txx += inc;
             tzz += inc;
         }
     }
 }
 
-template <typename R, int ST>
-__global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
+template <typename R>
+__device__ __forceinline__ void velocity_body(const StepArgs<R> &a, int q, int s, int gstep)
 {
-    int q, s;
-    if (!my_work<ST>(a, q, s)) return;
     R *__restrict__ Vx = a.f[0];
     R *__restrict__ Vz = a.f[1];
     const R *__restrict__ Txx = a.f[2];
     const R *__restrict__ Tzz = a.f[3];
     const R *__restrict__ Txz = a.f[4];
-    const int lstep = shot_step(a, s, *a.step_base + a.step_off);
+    const int lstep = shot_step(a, s, gstep);
 
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
@@ -132,13 +143,13 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
             txx[i] *= f; tzz[i] *= f; txz[i] *= f; vx[i] *= f; vz[i] *= f;
         }
     }
-    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    const unsigned ax = aux_of(a, q, code);
     if (ax && lstep >= 0) {
         const dgx_aux x = a.aux[ax - 1];
         for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
             const dgx_src_dev &src = a.src[k];
             if (src.shot != s) continue;
-            const double w = a.wav[(long long)src.wavelet * a.nsteps + lstep];
+            const double w = wavelet_at(a, src, s, lstep);
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 const R inc = (R)(src.fun[i] * w);
@@ -183,7 +194,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
         const int e = nbs[j];
         if (e >= 0) {
             const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-            const bool special = (code & DGX_CODE_NBSPECIAL(j)) != 0;
+            const bool special = (code & DGX_CODE_NBSPECIAL(j)) != 0 || (a.aux_of_elem && a.aux_of_elem[e] >= 0);
             const unsigned ncode = special ? (unsigned)a.nbr[e].w : 0u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -220,17 +231,15 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
     }
 }
 
-template <typename R, int ST>
-__global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
+template <typename R>
+__device__ __forceinline__ void stress_body(const StepArgs<R> &a, int q, int s, int gstep)
 {
-    int q, s;
-    if (!my_work<ST>(a, q, s)) return;
     const R *__restrict__ Vx = a.f[0];
     const R *__restrict__ Vz = a.f[1];
     R *__restrict__ Txx = a.f[2];
     R *__restrict__ Tzz = a.f[3];
     R *__restrict__ Txz = a.f[4];
-    const int lstep = shot_step(a, s, *a.step_base + a.step_off);
+    const int lstep = shot_step(a, s, gstep);
 
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
@@ -311,21 +320,86 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
         Tzz[o] = (lam * axx + l2m * azz) * dt + tzz;
         Txz[o] = (mu * axz) * dt + txz;
     }
-    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
-    if (ax && lstep >= 0 && lstep < a.nsteps && s < a.nshots) {
+    const unsigned ax = aux_of(a, q, code);
+    if (ax && lstep >= 0 && s < a.nshots && (a.slot_trace || lstep < a.nsteps)) {
         const dgx_aux x = a.aux[ax - 1];
         for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
             const dgx_rec_dev &rc = a.rec[r];
+            if (a.slot_trace && rc.pad0 != s) continue;
             double sx_ = 0, sz_ = 0;
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 sx_ += rc.w[i] * (double)vx[i];
                 sz_ += rc.w[i] * (double)vz[i];
             }
-            double *t = a.trace + ((long long)s * a.nsteps + lstep) * 2 * a.nrec;
+            const int nrec = a.slot_trace ? a.slot_nrec[s] : a.nrec;
+            double *t = a.slot_trace ? a.slot_trace[s] + (long long)lstep * 2 * nrec
+                                     : a.trace + ((long long)s * a.nsteps + lstep) * 2 * nrec;
             t[rc.index] = sx_;
-            t[a.nrec + rc.index] = sz_;
+            t[nrec + rc.index] = sz_;
         }
+    }
+}
+
+template <typename R, int ST>
+__global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
+{
+    int q, s;
+    if (!my_work<ST>(a, q, s)) return;
+    velocity_body(a, q, s, *a.step_base + a.step_off);
+}
+
+template <typename R, int ST>
+__global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
+{
+    int q, s;
+    if (!my_work<ST>(a, q, s)) return;
+    stress_body(a, q, s, *a.step_base + a.step_off);
+}
+
+/*
+ * Serving: one launch per half-step over every active (mesh, slot group) batch.  sched = {nbatch,
+ * total tiles, tile offset of batch 0, ..., tile offset of batch nbatch}; a tile is 16 elements x 16
+ * shots (4 elements per wavefront).  The grid is fixed (a persistent grid-stride loop), so a captured
+ * step graph stays valid while batches are added and retired: only the table and sched change.  The
+ * tiles are dealt in 8 contiguous ranges, one per XCD.
+ */
+template <typename R, bool VEL>
+__global__ void __launch_bounds__(DGX_BLOCK) ks_step(const StepArgs<R> *table, const int *sched,
+                                                        const int *step_base, int step_off)
+{
+    const int nb = sched[0], total = sched[1];
+    const int gstep = *step_base + step_off;
+    const int xcd = blockIdx.x & (DGX_NXCD - 1), per_xcd = gridDim.x / DGX_NXCD;
+    const int chunk = (total + DGX_NXCD - 1) / DGX_NXCD;
+    const int t_end = min(total, (xcd + 1) * chunk);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int b = 0;
+    for (int tile = xcd * chunk + (int)(blockIdx.x >> 3); tile < t_end; tile += per_xcd) {
+        while (b + 1 < nb && sched[2 + b + 1] <= tile) b++;
+        const StepArgs<R> &a = table[b];
+        const int q = (tile - sched[2 + b]) * 16 + wave * 4 + (lane >> 4), s = lane & 15;
+        if (q >= a.tri) continue;
+        if (VEL) velocity_body(a, q, s, gstep);
+        else stress_body(a, q, s, gstep);
+    }
+}
+
+/* dst[pairs[k].x] = pairs[k].y */
+__global__ void k_scatter_int(int *dst, const int2 *pairs, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[pairs[k].x] = pairs[k].y;
+}
+
+/* zero the field columns of the slots in `mask` (a 16-slot group) before they are reused */
+template <typename R>
+__global__ void k_clear_slots(R *f0, R *f1, R *f2, R *f3, R *f4, long long n_nodes, unsigned mask)
+{
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n_nodes * 16;
+         k += (long long)gridDim.x * blockDim.x) {
+        if (!((mask >> (k & 15)) & 1u)) continue;
+        f0[k] = 0; f1[k] = 0; f2[k] = 0; f3[k] = 0; f4[k] = 0;
     }
 }
 
@@ -354,6 +428,30 @@ static hipError_t dispatch_b(const dgx::StepArgs<R> &a, hipStream_t st, int whic
 }
 
 extern "C" {
+hipError_t dgx_launch_serve(int fp32, const void *table, const int *sched, const int *step_base, int step_off,
+                            int vel, int grid, hipStream_t st)
+{
+    if (fp32) {
+        if (vel) dgxb::ks_step<float, true><<<grid, DGX_BLOCK, 0, st>>>((const dgx::StepArgs<float> *)table, sched, step_base, step_off);
+        else dgxb::ks_step<float, false><<<grid, DGX_BLOCK, 0, st>>>((const dgx::StepArgs<float> *)table, sched, step_base, step_off);
+    } else {
+        if (vel) dgxb::ks_step<double, true><<<grid, DGX_BLOCK, 0, st>>>((const dgx::StepArgs<double> *)table, sched, step_base, step_off);
+        else dgxb::ks_step<double, false><<<grid, DGX_BLOCK, 0, st>>>((const dgx::StepArgs<double> *)table, sched, step_base, step_off);
+    }
+    return hipGetLastError();
+}
+hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    dgxb::k_scatter_int<<<(n + 255) / 256, 256, 0, st>>>(dst, pairs, n);
+    return hipGetLastError();
+}
+hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, unsigned mask, hipStream_t st)
+{
+    if (fp32) dgxb::k_clear_slots<float><<<1024, 256, 0, st>>>((float *)f[0], (float *)f[1], (float *)f[2], (float *)f[3], (float *)f[4], n_nodes, mask);
+    else dgxb::k_clear_slots<double><<<1024, 256, 0, st>>>((double *)f[0], (double *)f[1], (double *)f[2], (double *)f[3], (double *)f[4], n_nodes, mask);
+    return hipGetLastError();
+}
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which)
 {
     return dispatch_b(*(const dgx::StepArgs<double> *)args, st, which);

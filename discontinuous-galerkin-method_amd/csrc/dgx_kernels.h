@@ -81,6 +81,13 @@ struct StepArgs {
     int shot_tile;         /* shots per lane group (ST); a wavefront covers 64/ST elements */
     const int *shot_start; /* optional per-shot first global step (NULL: all shots start at 0) */
     const int *shot_len;   /* optional per-shot step count */
+    /* serving only (NULL otherwise): sources/receivers change as requests come and go, so they are
+     * indexed per element by aux_of_elem (-1: none) instead of the code word, every receiver record
+     * names its slot (dgx_rec_dev.pad0), and each slot has its own wavelet and trace buffers */
+    const int *aux_of_elem;
+    const double *const *slot_wav;
+    double *const *slot_trace;
+    const int *slot_nrec;
 };
 
 } // namespace dgx

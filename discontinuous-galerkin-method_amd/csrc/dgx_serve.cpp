@@ -1,0 +1,701 @@
+/*
+ * dgx_serve.cpp -- request loop (include/dgx_serve.h): resident meshes, slot groups of 16 shots in the
+ * shot-innermost layout, memory-bounded admission, one persistent kernel per half-step over all active
+ * batches, hipGraph-captured blocks of steps, retirement with trace read-back.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "dgx.h"
+#include "dgx_internal.h"
+#include "dgx_kernels.h"
+#include "dgx_serve.h"
+
+extern "C" {
+hipError_t dgx_launch_serve(int fp32, const void *table, const int *sched, const int *step_base, int step_off,
+                            int vel, int grid, hipStream_t st);
+hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, unsigned mask, hipStream_t st);
+hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
+hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st);
+}
+
+#define SV_TRY(call)                                                                                           \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess) {                                                                                \
+            dgx_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__);          \
+            return DGX_EHIP;                                                                                   \
+        }                                                                                                      \
+    } while (0)
+
+namespace {
+
+const int GROUP = 16;     /* shots per batch (= shot tile of the serving kernel) */
+const int GRID = 2048;    /* persistent grid: 8 XCDs x 256 blocks */
+const int MAX_BATCHES = 4096;
+
+struct MeshDev {
+    long tri = 0;
+    double dt = 0;
+    std::vector<long> inv; /* caller element -> device element */
+    int4 *nbr = nullptr, *vtx = nullptr;
+    double2 *pt = nullptr;
+    void *mat = nullptr, *absorb = nullptr;
+    std::vector<int> batches;
+};
+
+struct Request {
+    long id = 0;
+    int mesh = 0, nshots = 0;
+    long nsteps = 0, nrec = 0;
+    std::vector<dgx_source> src;
+    std::vector<double> wav;
+    std::vector<dgx_receiver> rec;
+    int state = DGX_REQ_QUEUED;
+    double arrival = 0, t_admit = 0, t_done = 0;
+    long start = 0;
+    std::vector<std::pair<int, int>> slots; /* (batch, slot) per shot */
+    std::vector<double *> d_wav, d_trace;
+    std::vector<double> res_x, res_z;
+    long long bytes = 0;
+};
+
+struct Batch {
+    int mesh = 0;
+    void *fields = nullptr;
+    long long field_bytes = 0;
+    long req[GROUP];
+    int shot[GROUP];
+    int h_start[GROUP], h_len[GROUP], h_nrec[GROUP];
+    double *h_wav[GROUP], *h_trace[GROUP];
+    int *d_slot_int = nullptr;      /* start[16] | len[16] | nrec[16] */
+    double **d_slot_ptr = nullptr;  /* wav[16] | trace[16] */
+    int *d_aux_of_elem = nullptr;
+    std::vector<int> aux_elems;     /* device elements that currently have an aux record */
+    dgx_aux *d_aux = nullptr;
+    dgx_src_dev *d_src = nullptr;
+    dgx_rec_dev *d_rec = nullptr;
+    int2 *d_pairs = nullptr;
+    long aux_cap = 0, src_cap = 0, rec_cap = 0, pair_cap = 0;
+    bool dirty = false;
+    int used() const
+    {
+        int n = 0;
+        for (int k = 0; k < GROUP; k++) n += req[k] >= 0;
+        return n;
+    }
+};
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+hipError_t grow(T **p, long *cap, long need)
+{
+    if (need <= *cap) return hipSuccess;
+    long c = std::max<long>(need, *cap * 2 + 16);
+    if (*p) {
+        hipError_t e = hipFree(*p);
+        if (e != hipSuccess) return e;
+    }
+    *p = nullptr;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * c);
+    if (e != hipSuccess) return e;
+    *cap = c;
+    return hipSuccess;
+}
+
+} // namespace
+
+struct dgx_server {
+    int fp32 = 1;
+    long long budget = 0, in_use = 0;
+    int K = 100;
+    dgx_operators ops;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int *d_step_base = nullptr;
+    long global_step = 0;
+    void *d_table = nullptr;
+    int *d_sched = nullptr;
+    std::vector<MeshDev> meshes;
+    std::vector<Batch> batches;
+    std::map<long, Request> reqs;
+    std::deque<long> queue;
+    long next_id = 1;
+    long long dof_updates = 0;
+    double busy = 0;
+    long blocks = 0;
+    bool table_dirty = true;
+    dgx::Ops<double> op64;
+    dgx::Ops<float> op32;
+    /* memory pool of per-request device buffers, by size */
+    std::multimap<size_t, double *> pool;
+
+    size_t rsize() const { return fp32 ? 4 : 8; }
+};
+
+static void make_ops(dgx_server *sv)
+{
+    for (int i = 0; i < 6; i++)
+        for (int n = 0; n < 6; n++) {
+            double dr = 0, ds = 0;
+            for (int k = 0; k < 6; k++) {
+                dr += sv->ops.Vr[i * 6 + k] * sv->ops.inverV[k * 6 + n];
+                ds += sv->ops.Vs[i * 6 + k] * sv->ops.inverV[k * 6 + n];
+            }
+            sv->op64.Dr[i * 6 + n] = dr;
+            sv->op64.Ds[i * 6 + n] = ds;
+            sv->op32.Dr[i * 6 + n] = (float)dr;
+            sv->op32.Ds[i * 6 + n] = (float)ds;
+        }
+    const double *S[3] = {sv->ops.S1, sv->ops.S2, sv->ops.S3};
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 18; k++) {
+            sv->op64.L[j * 18 + k] = S[j][k];
+            sv->op32.L[j * 18 + k] = (float)S[j][k];
+        }
+}
+
+static double *pool_get(dgx_server *sv, size_t n)
+{
+    auto it = sv->pool.lower_bound(n);
+    if (it != sv->pool.end() && it->first <= 2 * n) {
+        double *p = it->second;
+        sv->pool.erase(it);
+        return p;
+    }
+    double *p = nullptr;
+    if (hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return nullptr;
+    return p;
+}
+
+static void pool_put(dgx_server *sv, double *p, size_t n) { sv->pool.emplace(n, p); }
+
+extern "C" double dgx_server_now(void) { return now_s(); }
+
+extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operators *ops, dgx_server **out)
+{
+    if (!ops || !out) return DGX_EARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        dgx_set_error("dgx_server_create: no HIP device available");
+        return DGX_EHIP;
+    }
+    dgx_server *sv = new dgx_server();
+    sv->fp32 = cfg ? (cfg->precision == DGX_FP64 ? 0 : 1) : 1;
+    sv->K = (cfg && cfg->steps_per_block > 0) ? cfg->steps_per_block : 100;
+    sv->ops = *ops;
+    make_ops(sv);
+    size_t fr = 0, tot = 0;
+    SV_TRY(hipMemGetInfo(&fr, &tot));
+    sv->budget = (cfg && cfg->hbm_budget > 0) ? cfg->hbm_budget : (long long)(0.8 * fr);
+    SV_TRY(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+    SV_TRY(hipMalloc((void **)&sv->d_step_base, sizeof(int)));
+    SV_TRY(hipMemset(sv->d_step_base, 0, sizeof(int)));
+    size_t args = sv->fp32 ? sizeof(dgx::StepArgs<float>) : sizeof(dgx::StepArgs<double>);
+    SV_TRY(hipMalloc(&sv->d_table, args * MAX_BATCHES));
+    SV_TRY(hipMalloc((void **)&sv->d_sched, sizeof(int) * (MAX_BATCHES + 3)));
+    int zero[3] = {0, 0, 0};
+    SV_TRY(hipMemcpy(sv->d_sched, zero, sizeof(zero), hipMemcpyHostToDevice));
+    *out = sv;
+    return DGX_OK;
+}
+
+extern "C" int dgx_server_destroy(dgx_server *sv)
+{
+    if (!sv) return DGX_EARG;
+    (void)hipStreamSynchronize(sv->stream);
+    if (sv->graph) (void)hipGraphExecDestroy(sv->graph);
+    for (auto &m : sv->meshes) {
+        void *p[] = {m.nbr, m.vtx, m.pt, m.mat, m.absorb};
+        for (void *x : p)
+            if (x) (void)hipFree(x);
+    }
+    for (auto &b : sv->batches) {
+        void *p[] = {b.fields, b.d_slot_int, b.d_slot_ptr, b.d_aux_of_elem, b.d_aux, b.d_src, b.d_rec, b.d_pairs};
+        for (void *x : p)
+            if (x) (void)hipFree(x);
+    }
+    for (auto &r : sv->reqs) {
+        for (double *p : r.second.d_wav) (void)hipFree(p);
+        for (double *p : r.second.d_trace) (void)hipFree(p);
+    }
+    for (auto &kv : sv->pool) (void)hipFree(kv.second);
+    void *p[] = {sv->d_step_base, sv->d_table, sv->d_sched};
+    for (void *x : p)
+        if (x) (void)hipFree(x);
+    (void)hipStreamDestroy(sv->stream);
+    delete sv;
+    return DGX_OK;
+}
+
+template <typename T>
+static hipError_t upload_vec(T **dst, const std::vector<T> &v)
+{
+    hipError_t e = hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+}
+
+extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *mesh_id)
+{
+    if (!sv || !p || !mesh_id) return DGX_EARG;
+    if (p->tri <= 0 || !p->point || !p->triangle || !p->neighbor || !p->lambda || !p->miu || !p->den ||
+        !(p->dt > 0)) {
+        dgx_set_error("dgx_server_add_mesh: incomplete mesh");
+        return DGX_EARG;
+    }
+    dgx_mesh_host mh;
+    int rc = dgx_build_mesh_host(p, 1, p->tri, nullptr, &mh);
+    if (rc) return rc;
+    MeshDev m;
+    m.tri = p->tri;
+    m.dt = p->dt;
+    m.inv = mh.inv;
+    SV_TRY(upload_vec(&m.nbr, mh.nbr));
+    SV_TRY(upload_vec(&m.vtx, mh.vtx));
+    SV_TRY(upload_vec(&m.pt, mh.pt));
+    if (sv->fp32) {
+        std::vector<float> mat(mh.mat.begin(), mh.mat.end()), ab(mh.absorb.begin(), mh.absorb.end());
+        float *a, *b;
+        SV_TRY(upload_vec(&a, mat));
+        SV_TRY(upload_vec(&b, ab));
+        m.mat = a;
+        m.absorb = b;
+    } else {
+        double *a, *b;
+        SV_TRY(upload_vec(&a, mh.mat));
+        SV_TRY(upload_vec(&b, mh.absorb));
+        m.mat = a;
+        m.absorb = b;
+    }
+    sv->meshes.push_back(std::move(m));
+    *mesh_id = (int)sv->meshes.size() - 1;
+    return DGX_OK;
+}
+
+extern "C" int dgx_server_submit(dgx_server *sv, const dgx_request *rq, long *id)
+{
+    if (!sv || !rq || !id) return DGX_EARG;
+    if (rq->mesh < 0 || rq->mesh >= (int)sv->meshes.size() || rq->nshots < 1 || rq->nshots > GROUP ||
+        rq->nsteps < 1 || rq->nrec < 0 || !rq->src || !rq->wavelet || (rq->nrec > 0 && !rq->rec)) {
+        dgx_set_error("dgx_server_submit: invalid request");
+        return DGX_EARG;
+    }
+    const long T = sv->meshes[rq->mesh].tri;
+    for (int k = 0; k < rq->nshots; k++)
+        if (rq->src[k].elem < 1 || rq->src[k].elem > T ||
+            (rq->src[k].kind != DGX_SRC_FORCE_Z && rq->src[k].kind != DGX_SRC_EXPLOSIVE)) {
+            dgx_set_error("dgx_server_submit: source %d invalid", k);
+            return DGX_EARG;
+        }
+    for (long k = 0; k < rq->nrec; k++)
+        if (rq->rec[k].elem < 1 || rq->rec[k].elem > T) {
+            dgx_set_error("dgx_server_submit: receiver %ld invalid", k);
+            return DGX_EARG;
+        }
+    Request r;
+    r.id = sv->next_id++;
+    r.mesh = rq->mesh;
+    r.nshots = rq->nshots;
+    r.nsteps = rq->nsteps;
+    r.nrec = rq->nrec;
+    r.src.assign(rq->src, rq->src + rq->nshots);
+    r.wav.assign(rq->wavelet, rq->wavelet + (size_t)rq->nshots * rq->nsteps);
+    if (rq->nrec) r.rec.assign(rq->rec, rq->rec + rq->nrec);
+    r.arrival = rq->arrival;
+    r.bytes = (long long)rq->nshots * rq->nsteps * (1 + 2 * rq->nrec) * 8;
+    *id = r.id;
+    sv->queue.push_back(r.id);
+    sv->reqs.emplace(r.id, std::move(r));
+    return DGX_OK;
+}
+
+static int new_batch(dgx_server *sv, int mesh, int *bi)
+{
+    const MeshDev &m = sv->meshes[mesh];
+    if ((long)sv->batches.size() >= MAX_BATCHES) return DGX_ENOMEM;
+    Batch b;
+    b.mesh = mesh;
+    b.field_bytes = (long long)5 * m.tri * 6 * GROUP * sv->rsize();
+    for (int k = 0; k < GROUP; k++) {
+        b.req[k] = -1;
+        b.shot[k] = 0;
+        b.h_start[k] = 0;
+        b.h_len[k] = 0;
+        b.h_nrec[k] = 0;
+        b.h_wav[k] = nullptr;
+        b.h_trace[k] = nullptr;
+    }
+    SV_TRY(hipMalloc(&b.fields, b.field_bytes));
+    SV_TRY(hipMemsetAsync(b.fields, 0, b.field_bytes, sv->stream));
+    SV_TRY(hipMalloc((void **)&b.d_slot_int, sizeof(int) * 3 * GROUP));
+    SV_TRY(hipMalloc((void **)&b.d_slot_ptr, sizeof(double *) * 2 * GROUP));
+    SV_TRY(hipMalloc((void **)&b.d_aux_of_elem, sizeof(int) * m.tri));
+    SV_TRY(hipMemsetAsync(b.d_aux_of_elem, 0xff, sizeof(int) * m.tri, sv->stream)); /* all -1 */
+    sv->in_use += b.field_bytes + 4 * m.tri;
+    sv->batches.push_back(std::move(b));
+    *bi = (int)sv->batches.size() - 1;
+    sv->meshes[mesh].batches.push_back(*bi);
+    sv->table_dirty = true;
+    return DGX_OK;
+}
+
+/* Rebuild a batch's source/receiver tables and per-slot parameters after admissions/retirements. */
+static int refresh_batch(dgx_server *sv, Batch &b)
+{
+    const MeshDev &m = sv->meshes[b.mesh];
+    std::map<int, std::pair<std::vector<dgx_src_dev>, std::vector<dgx_rec_dev>>> by_elem;
+    for (int k = 0; k < GROUP; k++) {
+        if (b.req[k] < 0) continue;
+        const Request &r = sv->reqs.at(b.req[k]);
+        const dgx_source &sr = r.src[b.shot[k]];
+        dgx_src_dev d;
+        d.elem = (int)m.inv[sr.elem - 1];
+        d.kind = sr.kind;
+        d.shot = k;
+        d.wavelet = k;
+        for (int i = 0; i < 6; i++) d.fun[i] = sr.fun[i];
+        by_elem[d.elem].first.push_back(d);
+        for (long j = 0; j < r.nrec; j++) {
+            dgx_rec_dev rr;
+            rr.elem = (int)m.inv[r.rec[j].elem - 1];
+            rr.index = (int)j;
+            rr.pad0 = k;
+            rr.pad1 = 0;
+            for (int i = 0; i < 6; i++) rr.w[i] = r.rec[j].weight[i];
+            by_elem[rr.elem].second.push_back(rr);
+        }
+    }
+    std::vector<dgx_aux> aux;
+    std::vector<dgx_src_dev> src;
+    std::vector<dgx_rec_dev> rec;
+    std::vector<int> elems;
+    for (auto &kv : by_elem) {
+        dgx_aux x;
+        x.src_begin = (int)src.size();
+        x.src_count = (int)kv.second.first.size();
+        x.rec_begin = (int)rec.size();
+        x.rec_count = (int)kv.second.second.size();
+        src.insert(src.end(), kv.second.first.begin(), kv.second.first.end());
+        rec.insert(rec.end(), kv.second.second.begin(), kv.second.second.end());
+        aux.push_back(x);
+        elems.push_back(kv.first);
+    }
+    std::vector<int2> pairs;
+    for (int q : b.aux_elems) pairs.push_back(make_int2(q, -1));
+    for (size_t x = 0; x < elems.size(); x++) pairs.push_back(make_int2(elems[x], (int)x));
+    long old_a = b.aux_cap, old_s = b.src_cap, old_r = b.rec_cap;
+    SV_TRY(grow(&b.d_aux, &b.aux_cap, (long)aux.size()));
+    SV_TRY(grow(&b.d_src, &b.src_cap, (long)src.size()));
+    SV_TRY(grow(&b.d_rec, &b.rec_cap, (long)rec.size()));
+    SV_TRY(grow(&b.d_pairs, &b.pair_cap, (long)pairs.size()));
+    if (old_a != b.aux_cap || old_s != b.src_cap || old_r != b.rec_cap) sv->table_dirty = true;
+    if (!aux.empty()) SV_TRY(hipMemcpyAsync(b.d_aux, aux.data(), sizeof(dgx_aux) * aux.size(), hipMemcpyHostToDevice, sv->stream));
+    if (!src.empty()) SV_TRY(hipMemcpyAsync(b.d_src, src.data(), sizeof(dgx_src_dev) * src.size(), hipMemcpyHostToDevice, sv->stream));
+    if (!rec.empty()) SV_TRY(hipMemcpyAsync(b.d_rec, rec.data(), sizeof(dgx_rec_dev) * rec.size(), hipMemcpyHostToDevice, sv->stream));
+    if (!pairs.empty()) {
+        SV_TRY(hipMemcpyAsync(b.d_pairs, pairs.data(), sizeof(int2) * pairs.size(), hipMemcpyHostToDevice, sv->stream));
+        SV_TRY(dgx_launch_scatter(b.d_aux_of_elem, b.d_pairs, (int)pairs.size(), sv->stream));
+    }
+    b.aux_elems = elems;
+    int si[3 * GROUP];
+    double *sp[2 * GROUP];
+    for (int k = 0; k < GROUP; k++) {
+        si[k] = b.h_start[k];
+        si[GROUP + k] = b.h_len[k];
+        si[2 * GROUP + k] = b.h_nrec[k];
+        sp[k] = b.h_wav[k];
+        sp[GROUP + k] = b.h_trace[k];
+    }
+    SV_TRY(hipMemcpyAsync(b.d_slot_int, si, sizeof(si), hipMemcpyHostToDevice, sv->stream));
+    SV_TRY(hipMemcpyAsync(b.d_slot_ptr, sp, sizeof(sp), hipMemcpyHostToDevice, sv->stream));
+    /* the staging copies above read host memory asynchronously: wait before it goes out of scope */
+    SV_TRY(hipStreamSynchronize(sv->stream));
+    b.dirty = false;
+    return DGX_OK;
+}
+
+template <typename R>
+static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops<R> &op, dgx::StepArgs<R> &a)
+{
+    const MeshDev &m = sv->meshes[b.mesh];
+    std::memset(&a, 0, sizeof(a));
+    a.op = op;
+    R *f = (R *)b.fields;
+    size_t per = (size_t)m.tri * 6 * GROUP;
+    for (int c = 0; c < 5; c++) a.f[c] = f + c * per;
+    a.fstride = m.tri;
+    a.nbr = m.nbr;
+    a.vtx = m.vtx;
+    a.pt = m.pt;
+    const R *mat = (const R *)m.mat;
+    a.rho_inv = mat;
+    a.lam = mat + m.tri;
+    a.mu = mat + 2 * m.tri;
+    a.absorb = (const R *)m.absorb;
+    a.aux = b.d_aux;
+    a.src = b.d_src;
+    a.rec = b.d_rec;
+    a.nsteps = 0x7fffffff;
+    a.tri = (int)m.tri;
+    a.dt = m.dt;
+    a.spad = GROUP;
+    a.shot_tile = GROUP;
+    a.nshots = GROUP;
+    a.shot_start = b.d_slot_int;
+    a.shot_len = b.d_slot_int + GROUP;
+    a.slot_nrec = b.d_slot_int + 2 * GROUP;
+    a.slot_wav = (const double *const *)b.d_slot_ptr;
+    a.slot_trace = b.d_slot_ptr + GROUP;
+    a.aux_of_elem = b.d_aux_of_elem;
+}
+
+static int upload_table(dgx_server *sv)
+{
+    std::vector<int> sched(2);
+    std::vector<char> table;
+    int nb = 0, tiles = 0;
+    size_t asz = sv->fp32 ? sizeof(dgx::StepArgs<float>) : sizeof(dgx::StepArgs<double>);
+    for (auto &b : sv->batches) {
+        if (b.used() == 0) continue;
+        table.resize((nb + 1) * asz);
+        if (sv->fp32) fill_batch_args(sv, b, sv->op32, *(dgx::StepArgs<float> *)(table.data() + nb * asz));
+        else fill_batch_args(sv, b, sv->op64, *(dgx::StepArgs<double> *)(table.data() + nb * asz));
+        sched.push_back(tiles);
+        tiles += (int)((sv->meshes[b.mesh].tri + 15) / 16);
+        nb++;
+    }
+    sched.push_back(tiles);
+    sched[0] = nb;
+    sched[1] = tiles;
+    if (nb) SV_TRY(hipMemcpyAsync(sv->d_table, table.data(), table.size(), hipMemcpyHostToDevice, sv->stream));
+    SV_TRY(hipMemcpyAsync(sv->d_sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice, sv->stream));
+    SV_TRY(hipStreamSynchronize(sv->stream));
+    sv->table_dirty = false;
+    return DGX_OK;
+}
+
+static int admit(dgx_server *sv)
+{
+    std::deque<long> still;
+    while (!sv->queue.empty()) {
+        long id = sv->queue.front();
+        sv->queue.pop_front();
+        Request &r = sv->reqs.at(id);
+        const MeshDev &m = sv->meshes[r.mesh];
+        /* best fit among this mesh's batches */
+        int best = -1, best_free = GROUP + 1;
+        for (int bi : m.batches) {
+            int fr = GROUP - sv->batches[bi].used();
+            if (fr >= r.nshots && fr < best_free) { best = bi; best_free = fr; }
+        }
+        long long need = r.bytes;
+        if (best < 0) need += (long long)5 * m.tri * 6 * GROUP * sv->rsize() + 4 * m.tri;
+        if (sv->in_use + need > sv->budget) { still.push_back(id); continue; }
+        if (best < 0) {
+            int rc = new_batch(sv, r.mesh, &best);
+            if (rc) return rc;
+        }
+        Batch &b = sv->batches[best];
+        unsigned mask = 0;
+        int k = 0;
+        for (int slot = 0; slot < GROUP && k < r.nshots; slot++) {
+            if (b.req[slot] >= 0) continue;
+            double *w = pool_get(sv, r.nsteps);
+            double *t = pool_get(sv, (size_t)r.nsteps * 2 * std::max<long>(r.nrec, 1));
+            if (!w || !t) { dgx_set_error("out of device memory for request buffers"); return DGX_ENOMEM; }
+            SV_TRY(hipMemcpyAsync(w, &r.wav[(size_t)k * r.nsteps], sizeof(double) * r.nsteps, hipMemcpyHostToDevice,
+                                  sv->stream));
+            b.req[slot] = id;
+            b.shot[slot] = k;
+            b.h_start[slot] = (int)sv->global_step;
+            b.h_len[slot] = (int)r.nsteps;
+            b.h_nrec[slot] = (int)r.nrec;
+            b.h_wav[slot] = w;
+            b.h_trace[slot] = t;
+            r.slots.push_back({best, slot});
+            r.d_wav.push_back(w);
+            r.d_trace.push_back(t);
+            mask |= 1u << slot;
+            k++;
+        }
+        void *f[5];
+        size_t per = (size_t)m.tri * 6 * GROUP * sv->rsize();
+        for (int c = 0; c < 5; c++) f[c] = (char *)b.fields + c * per;
+        SV_TRY(dgx_launch_clear_slots(sv->fp32, f, (long long)m.tri * 6, mask, sv->stream));
+        b.dirty = true;
+        sv->table_dirty = true;
+        r.state = DGX_REQ_RUNNING;
+        r.start = sv->global_step;
+        r.t_admit = now_s();
+        sv->in_use += r.bytes;
+    }
+    sv->queue = still;
+    return DGX_OK;
+}
+
+static int retire(dgx_server *sv, int *completed)
+{
+    for (auto &kv : sv->reqs) {
+        Request &r = kv.second;
+        if (r.state != DGX_REQ_RUNNING || sv->global_step < r.start + r.nsteps) continue;
+        const long n = r.nsteps, nr = r.nrec;
+        r.res_x.assign((size_t)r.nshots * n * nr, 0.0);
+        r.res_z.assign((size_t)r.nshots * n * nr, 0.0);
+        std::vector<double> buf((size_t)n * 2 * std::max<long>(nr, 1));
+        for (int k = 0; k < r.nshots; k++) {
+            if (nr > 0) {
+                SV_TRY(hipMemcpy(buf.data(), r.d_trace[k], sizeof(double) * n * 2 * nr, hipMemcpyDeviceToHost));
+                for (long t = 0; t < n; t++)
+                    for (long j = 0; j < nr; j++) {
+                        r.res_x[((size_t)k * n + t) * nr + j] = buf[(t * 2 + 0) * nr + j];
+                        r.res_z[((size_t)k * n + t) * nr + j] = buf[(t * 2 + 1) * nr + j];
+                    }
+            }
+            Batch &b = sv->batches[r.slots[k].first];
+            int slot = r.slots[k].second;
+            b.req[slot] = -1;
+            b.h_len[slot] = 0;
+            b.h_wav[slot] = nullptr;
+            b.h_trace[slot] = nullptr;
+            b.dirty = true;
+            pool_put(sv, r.d_wav[k], n);
+            pool_put(sv, r.d_trace[k], (size_t)n * 2 * std::max<long>(nr, 1));
+        }
+        r.d_wav.clear();
+        r.d_trace.clear();
+        r.state = DGX_REQ_DONE;
+        r.t_done = now_s();
+        sv->in_use -= r.bytes;
+        sv->table_dirty = true;
+        (*completed)++;
+    }
+    return DGX_OK;
+}
+
+static int build_graph(dgx_server *sv)
+{
+    hipGraph_t g;
+    SV_TRY(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < sv->K && e == hipSuccess; k++) {
+        e = dgx_launch_serve(sv->fp32, sv->d_table, sv->d_sched, sv->d_step_base, k, 1, GRID, sv->stream);
+        if (e == hipSuccess) e = dgx_launch_serve(sv->fp32, sv->d_table, sv->d_sched, sv->d_step_base, k, 0, GRID, sv->stream);
+    }
+    if (e == hipSuccess) e = dgx_launch_advance(sv->d_step_base, sv->K, sv->stream);
+    hipError_t e2 = hipStreamEndCapture(sv->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        dgx_set_error("serving graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        return DGX_EHIP;
+    }
+    e = hipGraphInstantiate(&sv->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { dgx_set_error("graph instantiate failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
+    return DGX_OK;
+}
+
+extern "C" int dgx_server_advance(dgx_server *sv, int nblocks)
+{
+    if (!sv || nblocks < 0) return DGX_EARG;
+    double t0 = now_s();
+    int completed = 0;
+    for (int it = 0; it < nblocks; it++) {
+        int rc = admit(sv);
+        if (rc) return rc;
+        for (auto &b : sv->batches)
+            if (b.dirty) {
+                rc = refresh_batch(sv, b);
+                if (rc) return rc;
+            }
+        if (sv->table_dirty) {
+            rc = upload_table(sv);
+            if (rc) return rc;
+        }
+        long running = 0;
+        long long dof = 0;
+        for (auto &kv : sv->reqs)
+            if (kv.second.state == DGX_REQ_RUNNING) {
+                const Request &r = kv.second;
+                running++;
+                long left = r.start + r.nsteps - sv->global_step;
+                dof += (long long)r.nshots * sv->meshes[r.mesh].tri * 30 * std::min<long>(left, sv->K);
+            }
+        if (!running) break;
+        if (!sv->graph) {
+            rc = build_graph(sv);
+            if (rc) return rc;
+        }
+        SV_TRY(hipGraphLaunch(sv->graph, sv->stream));
+        SV_TRY(hipStreamSynchronize(sv->stream));
+        sv->global_step += sv->K;
+        sv->dof_updates += dof;
+        sv->blocks++;
+        rc = retire(sv, &completed);
+        if (rc) return rc;
+    }
+    sv->busy += now_s() - t0;
+    return completed;
+}
+
+extern "C" int dgx_server_status(const dgx_server *sv, long id, int *state, double *t_admit, double *t_done)
+{
+    if (!sv) return DGX_EARG;
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return DGX_EARG;
+    if (state) *state = it->second.state;
+    if (t_admit) *t_admit = it->second.t_admit;
+    if (t_done) *t_done = it->second.t_done;
+    return DGX_OK;
+}
+
+extern "C" int dgx_server_result(dgx_server *sv, long id, double *rec_x, double *rec_z)
+{
+    if (!sv) return DGX_EARG;
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end() || it->second.state != DGX_REQ_DONE) {
+        dgx_set_error("request %ld is not finished", id);
+        return DGX_ESTATE;
+    }
+    const Request &r = it->second;
+    if (rec_x && !r.res_x.empty()) std::memcpy(rec_x, r.res_x.data(), sizeof(double) * r.res_x.size());
+    if (rec_z && !r.res_z.empty()) std::memcpy(rec_z, r.res_z.data(), sizeof(double) * r.res_z.size());
+    return DGX_OK;
+}
+
+extern "C" int dgx_server_release(dgx_server *sv, long id)
+{
+    if (!sv) return DGX_EARG;
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end() || it->second.state != DGX_REQ_DONE) return DGX_ESTATE;
+    sv->reqs.erase(it);
+    return DGX_OK;
+}
+
+extern "C" int dgx_server_get_stats(const dgx_server *sv, dgx_server_stats *st)
+{
+    if (!sv || !st) return DGX_EARG;
+    std::memset(st, 0, sizeof(*st));
+    for (auto &kv : sv->reqs) {
+        if (kv.second.state == DGX_REQ_QUEUED) st->queued++;
+        else if (kv.second.state == DGX_REQ_RUNNING) st->running++;
+        else st->done++;
+    }
+    st->batches = (long)sv->batches.size();
+    st->bytes_in_use = sv->in_use;
+    st->dof_updates = sv->dof_updates;
+    st->busy_seconds = sv->busy;
+    st->blocks = sv->blocks;
+    st->global_step = sv->global_step;
+    return DGX_OK;
+}

@@ -51,6 +51,21 @@ class ProblemC(C.Structure):
                 ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("layout", C.c_int), ("shot_tile", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("precision", C.c_int), ("hbm_budget", C.c_longlong), ("steps_per_block", C.c_int)]
+
+
+class RequestC(C.Structure):
+    _fields_ = [("mesh", C.c_int), ("nshots", C.c_int), ("nsteps", C.c_long), ("src", C.POINTER(Source)),
+                ("wavelet", dptr), ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("arrival", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_long), ("running", C.c_long), ("done", C.c_long), ("batches", C.c_long),
+                ("bytes_in_use", C.c_longlong), ("dof_updates", C.c_longlong), ("busy_seconds", C.c_double),
+                ("blocks", C.c_long), ("global_step", C.c_long)]
+
+
 class SpongeParams(C.Structure):
     _fields_ = [("bound_x", C.c_double), ("bound_z", C.c_double), ("model_w", C.c_double),
                 ("model_h", C.c_double), ("beta", C.c_double), ("pi", C.c_double)]
@@ -110,6 +125,16 @@ def load() -> C.CDLL:
                                         C.c_long, C.POINTER(Source)]),
         "dgx_locate_receivers": (C.c_int, [vp, C.c_long, dptr, C.POINTER(Receiver)]),
         "dgx_p2_basis": (None, [C.c_double, C.c_double, dptr]),
+        "dgx_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(Operators), C.POINTER(vp)]),
+        "dgx_server_destroy": (C.c_int, [vp]),
+        "dgx_server_add_mesh": (C.c_int, [vp, C.POINTER(ProblemC), C.POINTER(C.c_int)]),
+        "dgx_server_submit": (C.c_int, [vp, C.POINTER(RequestC), C.POINTER(C.c_long)]),
+        "dgx_server_advance": (C.c_int, [vp, C.c_int]),
+        "dgx_server_status": (C.c_int, [vp, C.c_long, C.POINTER(C.c_int), dptr, dptr]),
+        "dgx_server_result": (C.c_int, [vp, C.c_long, dptr, dptr]),
+        "dgx_server_release": (C.c_int, [vp, C.c_long]),
+        "dgx_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "dgx_server_now": (C.c_double, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

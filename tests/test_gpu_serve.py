@@ -1,0 +1,114 @@
+"""Request loop (include/dgx_serve.h): continuous batching must not change any result.
+
+* every request's traces are bit-identical alone and in a staggered mix with other requests on the same
+  and on other meshes;
+* they equal the standalone shot-innermost FP32 solver's traces for the same shots;
+* they are within 1e-4 relative L2 of the FP64 oracle;
+* a tight memory budget queues requests instead of failing, and all of them complete.
+"""
+import numpy as np
+import pytest
+
+import dgx
+import dgx.serve as S
+from dgx import lib as L
+from helpers import OracleCase, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def meshes():
+    sp = dgx.Sponge.for_domain(2000.0, 2000.0, 100.0)
+    m0 = dgx.structured(40, 40, 50.0, diagonal="random", seed=3, sponge_box=sp.box(),
+                        attribute=lambda c: np.where(c[:, 1] > 1500, 1, 3))
+    m1 = dgx.structured(25, 25, 80.0, diagonal="same", jitter=10.0, seed=9, sponge_box=sp.box(), attribute=3)
+    return sp, [(m0, dgx.Model.by_attribute(m0.attribute)), (m1, dgx.Model.by_attribute(m1.attribute))]
+
+
+REQS = [  # (mesh, shots, steps, f0)
+    (0, 3, 230, 15.0), (1, 5, 170, 20.0), (0, 16, 120, 12.0), (0, 2, 260, 25.0), (1, 1, 90, 30.0),
+]
+
+
+def build(server, rms, spec, k):
+    mi, ns, nsteps, f0 = spec
+    rng = np.random.default_rng(100 + k)
+    xz = np.stack([rng.uniform(300, 1700, ns), rng.uniform(600, 1400, ns)], 1)
+    rec = np.stack([np.linspace(200, 1800, 17), np.full(17, 1450.0 + 10 * k)], 1)
+    return server.make_request(rms[mi], xz, [f0] * ns, nsteps, rec), xz, rec
+
+
+def new_server(precision=L.DGX_FP32, budget=0, K=50):
+    sp, ms = meshes()
+    sv = S.Server(precision, budget, K)
+    rms = [sv.add_mesh(m, md, sp) for m, md in ms]
+    return sv, rms, sp, ms
+
+
+def run_alone(k):
+    sv, rms, _, _ = new_server()
+    req, xz, rec = build(sv, rms, REQS[k], k)
+    rid = sv.submit(req)
+    sv.drain()
+    out = sv.result(rid, req)
+    sv.close()
+    return out, xz, rec
+
+
+def test_results_identical_alone_and_in_the_mix():
+    sv, rms, _, _ = new_server()
+    built = [build(sv, rms, spec, k) for k, spec in enumerate(REQS)]
+    ids = [sv.submit(built[0][0]), sv.submit(built[1][0])]
+    sv.advance(1)
+    ids.append(sv.submit(built[2][0]))
+    sv.advance(2)
+    ids += [sv.submit(built[3][0]), sv.submit(built[4][0])]
+    sv.drain()
+    st = sv.stats()
+    assert st["done"] == len(REQS) and st["running"] == 0
+    mixed = [sv.result(rid, built[k][0]) for k, rid in enumerate(ids)]
+    sv.close()
+    for k in range(len(REQS)):
+        (ax, az), _, _ = run_alone(k)
+        assert np.array_equal(mixed[k][0], ax) and np.array_equal(mixed[k][1], az), k
+
+
+def test_request_equals_standalone_solver_and_oracle():
+    k = 0
+    (x, z), xz, rec = run_alone(k)
+    sp, ms = meshes()
+    mesh, model = ms[REQS[k][0]]
+    nsteps, f0 = REQS[k][2], REQS[k][3]
+    srcs = [dgx.SourceSpec(float(a), float(b), L.DGX_SRC_EXPLOSIVE, s, f0) for s, (a, b) in enumerate(xz)]
+    st = dgx.build_setup(mesh, model, srcs, rec, nsteps, sponge=sp)
+    solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, 16)
+    solver.step(nsteps)
+    for s in range(len(xz)):
+        gx, gz = solver.traces(s)
+        assert np.array_equal(gx, x[s]) and np.array_equal(gz, z[s]), s
+    solver.close()
+    w = dgx.configs.Workload("serve", mesh, model, sp, srcs, rec, nsteps, 0)
+    oc = OracleCase(w)
+    worst = 0.0
+    for s in range(len(xz)):
+        ox, oz, _ = oc.run(shot=s)
+        worst = max(worst, rel_l2(x[s], ox), rel_l2(z[s], oz))
+    print(f"served fp32 vs fp64 oracle: worst trace rel L2 {worst:.3e}")
+    assert worst <= 1e-4
+
+
+def test_tight_budget_queues_and_completes():
+    sp, ms = meshes()
+    T0 = ms[0][0].tri
+    budget = 5 * T0 * 6 * 16 * 4 + 4 * T0 + 2_000_000  # one slot group of mesh 0 plus small buffers
+    sv = S.Server(L.DGX_FP32, budget, 50)
+    rm0 = sv.add_mesh(ms[0][0], ms[0][1], sp)
+    reqs = [sv.make_request(rm0, [(500.0 + 100 * k, 900.0)], [15.0], 120, [[1000.0, 1500.0]]) for k in range(6)]
+    reqs.append(sv.make_request(rm0, [(700.0, 800.0)] * 12, [15.0] * 12, 100, [[1000.0, 1500.0]]))
+    ids = [sv.submit(r) for r in reqs]
+    sv.advance(1)
+    st = sv.stats()
+    assert st["batches"] == 1 and st["queued"] >= 1  # the 12-shot request must wait for slots
+    sv.drain()
+    assert all(sv.status(i)[0] == 2 for i in ids)
+    sv.close()
