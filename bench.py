@@ -7,7 +7,9 @@ half-step, receiver pick) over the whole mesh; DOF-updates/s = Tri * 6 nodes * 5
 steps / seconds, with all inputs resident in HBM when the timed region starts.
 
 Prints one JSON line.  `value` is the plain FP64 single-shot path; the FP32 path and its trace error
-against FP64 are reported beside it.  `roofline` is for the dominant kernel (algorithmic field bytes
+against FP64 are reported beside it, then the batched FP32 path (64 shots on the ~1.2 M-element slope
+model, shot-innermost layout) and a sample of the open-loop serving stream on the eight resident meshes
+(p50/p99 latency and aggregate DOF-updates/s), each with its ratio to the plain FP64 rate.  `roofline` is for the dominant kernel (algorithmic field bytes
 per launch over its HIP-event-measured duration); `cpu_baseline` is the reference's own step loop
 (oracle/_ref, compiled from the reference sources) or, without it, the oracle, timed on the host.
 """
@@ -39,7 +41,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--shots", type=int, default=64, help="shots of the batched FP32 leg (0 = skip)")
-    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--batched-steps", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the seeded serving stream to run (0 = skip; 2000 = the whole stream)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=1,
                     help="also time a plain device copy (achievable bandwidth, counter calibration)")
     return ap.parse_args()
@@ -62,6 +67,70 @@ def run_path(setup, precision, warmup, steps, layout=L.DGX_LAYOUT_AUTO, bench_la
     s.close()
     n = warmup + steps
     return sec, kv, ks, x[:n], z[:n], info
+
+
+def rel(u, v):
+    nv = np.linalg.norm(v)
+    return float(np.linalg.norm(u - v) / nv) if nv > 0 else 0.0
+
+
+def batched_leg(a, rate64):
+    """Batched FP32 on the slope model (water over three dipping layers, ~1.2 M elements): S shots in
+    the shot-innermost layout with the autotuned shot tile; FP32 traces compared with an FP64 run of the
+    same shots."""
+    w = dgx.configs.slope_6x3km(a.batched_steps + 25, nshots=a.shots)
+    S, nb = a.shots, a.batched_steps
+    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
+    T = w.mesh.tri
+    secb, kvb, ksb, xb, zb, info = run_path(st, L.DGX_FP32, 20, nb, L.DGX_LAYOUT_SHOT_INNER, 20)
+    rateb = T * 30 * S * nb / secb
+    # FP64 reference on 4 of the shots (same mesh, sources, steps), element layout
+    picks = [0, S // 3, (2 * S) // 3, S - 1]
+    worst = 0.0
+    s32 = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, info[1])
+    s32.step(20 + nb)
+    for k in picks:
+        one = dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge, [dgx.SourceSpec(
+            w.sources[k].x, w.sources[k].z, w.sources[k].kind, 0, w.sources[k].f0)], w.receivers, w.nsteps, 0)
+        s1 = dgx.build_setup(one.mesh, one.model, one.sources, one.receivers, one.nsteps, sponge=one.sponge,
+                             dt=st.dt)
+        r64 = dgx.Solver(s1, L.DGX_FP64, L.DGX_LAYOUT_ELEMENT)
+        r64.step(20 + nb)
+        x64, z64 = r64.traces(0)
+        r64.close()
+        x32, z32 = s32.traces(k)
+        n = 20 + nb
+        worst = max(worst, rel(x32[:n], x64[:n]), rel(z32[:n], z64[:n]))
+    s32.close()
+    bytes_b = T * 6 * 4 * S
+    return {"workload": w.name, "elements": T, "value": rateb, "unit": "DOF-updates/s", "dtype": "f32",
+            "shots": S, "steps": nb, "ms_per_step": secb * 1e3 / nb, "x_vs_plain_fp64": rateb / rate64,
+            "trace_rel_l2_vs_fp64_worst_of_4_shots": worst, "layout": "shot-innermost", "shot_tile": info[1],
+            "autotune_ms_per_step": {str(k): v for k, v in info[2].items()},
+            "velocity_ms": kvb, "stress_ms": ksb,
+            "velocity_GBps": bytes_b * 7 / (kvb * 1e-3) / 1e9, "stress_GBps": bytes_b * 8 / (ksb * 1e-3) / 1e9}
+
+
+def serving_leg(a, rate64):
+    """Open-loop serving on the eight resident meshes (20 k .. 2 M elements, random layered models):
+    the first `serve_requests` requests of the seeded Poisson stream (mean rate per second), FP32."""
+    import dgx.serve as SV
+    ms = dgx.configs.serving_meshes()
+    arrivals, specs = dgx.configs.serving_requests(ms, n=max(a.serve_requests, 1), rate=a.serve_rate)
+    arrivals, specs = arrivals[:a.serve_requests], specs[:a.serve_requests]
+    sv = SV.Server(L.DGX_FP32, 0, 100)
+    rms = [sv.add_mesh(m, md, sp) for m, md, sp in ms]
+    reqs = [sv.make_request(rms[r["mesh"]], r["xz"], [r["f0"]] * len(r["xz"]), r["nsteps"], r["rec"])
+            for r in specs]
+    lat, wall, st, _ = SV.run_open_loop(sv, reqs, arrivals)
+    sv.close()
+    dof = st["dof_updates"]
+    return {"requests": len(reqs), "arrival_rate_per_s": a.serve_rate, "meshes": [m.tri for m, _, _ in ms],
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()), "wall_s": wall, "dof_updates": dof,
+            "value": dof / wall, "unit": "DOF-updates/s", "dtype": "f32",
+            "x_vs_plain_fp64": dof / wall / rate64, "busy_s": st["busy_seconds"],
+            "sample": f"first {len(reqs)} requests of the seeded stream (2000 in the full stream)"}
 
 
 def cpu_baseline(w, setup, budget_s):
@@ -120,9 +189,6 @@ def main():
     rate64 = dof * a.steps / sec64
     rate32 = dof * a.steps / sec32
 
-    def rel(u, v):
-        nv = np.linalg.norm(v)
-        return float(np.linalg.norm(u - v) / nv) if nv > 0 else 0.0
     err32 = max(rel(x32, x64), rel(z32, z64))
 
     # dominant kernel of the FP64 step: algorithmic bytes = fields read + written per launch
@@ -165,27 +231,9 @@ def main():
                  "velocity_ms": kv32, "stress_ms": ks32, "speedup_vs_fp64": rate32 / rate64},
     }
     if a.shots > 0:
-        # batched FP32: S shots on the same mesh, sources spread along x at the same depth, shot 0
-        # identical to the single-shot source, shot-innermost layout with the autotuned shot tile
-        S = a.shots
-        nb = a.batched_steps
-        xs = np.linspace(500.0, 7500.0, S)
-        xs[0] = w.sources[0].x
-        H = float(w.mesh.point[:, 1].max())
-        srcs = [dgx.SourceSpec(float(x), w.sources[0].z, L.DGX_SRC_EXPLOSIVE, k, 20.0) for k, x in enumerate(xs)]
-        bset = dgx.build_setup(w.mesh, w.model, srcs, w.receivers, 20 + nb + 5, sponge=w.sponge, dt=setup.dt)
-        secb, kvb, ksb, xb, zb, info = run_path(bset, L.DGX_FP32, 20, nb, L.DGX_LAYOUT_SHOT_INNER, 20)
-        rateb = dof * S * nb / secb
-        nn = min(len(xb), len(x64))
-        errb = max(rel(xb[:nn], x64[:nn]), rel(zb[:nn], z64[:nn]))
-        bytes_b = T * 6 * 4 * S
-        out["batched_fp32"] = {
-            "value": rateb, "unit": "DOF-updates/s", "shots": S, "steps": nb, "ms_per_step": secb * 1e3 / nb,
-            "x_vs_plain_fp64": rateb / rate64, "shot0_trace_rel_l2_vs_fp64": errb,
-            "layout": "shot-innermost", "shot_tile": info[1],
-            "autotune_ms_per_step": {str(k): v for k, v in info[2].items()},
-            "velocity_ms": kvb, "stress_ms": ksb,
-            "velocity_GBps": bytes_b * 7 / (kvb * 1e-3) / 1e9, "stress_GBps": bytes_b * 8 / (ksb * 1e-3) / 1e9}
+        out["batched_fp32"] = batched_leg(a, rate64)
+    if a.serve_requests > 0:
+        out["serving"] = serving_leg(a, rate64)
     if a.copy_probe:
         import ctypes as C
         nbytes, reps = 1 << 30, 20

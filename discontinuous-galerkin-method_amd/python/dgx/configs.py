@@ -222,6 +222,17 @@ def carbonate_4x3km(nsteps: int = 8000, *, nvugs: int = 300, nfrac: int = 50, h_
         bnd += [line + nrm[None], line - nrm[None]]
     from scipy.spatial import cKDTree
     Bd = np.concatenate(bnd)
+    # fracture walls inside a vug are not interfaces; where features touch or cross, keep one of any
+    # two boundary points closer than 0.6 h_min
+    tree_v = cKDTree(vugs[:, :2])
+    dv, iv = tree_v.query(Bd, k=1)
+    on_circle = np.abs(dv - vugs[iv, 2]) < 1e-6
+    Bd = Bd[on_circle | (dv > vugs[iv, 2] + 0.5 * h_min)]
+    drop = np.zeros(len(Bd), bool)
+    for i, j in cKDTree(Bd).query_pairs(0.6 * h_min):
+        if not drop[i]:
+            drop[j] = True
+    Bd = Bd[~drop]
     # where two levels meet, drop coarse points closer than 0.75 of the finer spacing to finer points
     kept = [pts[0]]
     s = h_min
