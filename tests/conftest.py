@@ -11,3 +11,4 @@ for p in (ROOT / "discontinuous-galerkin-method_amd" / "python", ROOT / "oracle"
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libdgx.so on the device)")
+    config.addinivalue_line("markers", "slow: CPU test that takes tens of seconds")
