@@ -238,12 +238,12 @@ def main():
         import ctypes as C
         nbytes, reps = 1 << 30, 20
         probes = {}
-        for variant, name in ((0, "grid_stride"), (1, "nontemporal_32KiB_per_wg")):
+        for variant, name in ((0, "copy_grid_stride"), (1, "copy_nontemporal_32KiB_per_wg"), (2, "read_only")):
             ms = C.c_double()
             L.check(L.load().dgx_copy_probe(nbytes, reps, variant, C.byref(ms)), "dgx_copy_probe")
-            probes[name] = 2 * nbytes / (ms.value * 1e-3) / 1e9
-        out["copy_probe"] = {"bytes_copied": nbytes, "GBps_read_plus_write": probes,
-                             "best_GBps": max(probes.values())}
+            probes[name] = (1 if variant == 2 else 2) * nbytes / (ms.value * 1e-3) / 1e9
+        out["copy_probe"] = {"bytes_per_launch": nbytes, "GBps": probes,
+                             "best_copy_GBps": max(probes["copy_grid_stride"], probes["copy_nontemporal_32KiB_per_wg"])}
     out["roofline"]["achieved"] = achieved / 1e9
     out["roofline"]["peak"] = HBM_PEAK / 1e9
     if not a.no_cpu:

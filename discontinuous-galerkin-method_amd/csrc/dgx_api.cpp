@@ -64,20 +64,35 @@ hipError_t upload(T **dst, const std::vector<T> &src)
     return hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
 }
 
-/* Morton (Z-order) key of a point in [0,1)^2 with 16 bits per axis. */
-uint32_t morton(double u, double v)
+/* Hilbert-curve index of a point in [0,1)^2 on a 2^16 x 2^16 grid.  Consecutive indices are always
+ * adjacent cells (unlike the Z-order curve, which jumps at every power-of-two boundary), so elements
+ * that are close in this order are close in space. */
+uint32_t hilbert(double u, double v)
 {
-    auto spread = [](uint32_t x) {
-        x &= 0xffff;
-        x = (x | (x << 8)) & 0x00ff00ff;
-        x = (x | (x << 4)) & 0x0f0f0f0f;
-        x = (x | (x << 2)) & 0x33333333;
-        x = (x | (x << 1)) & 0x55555555;
-        return x;
-    };
-    uint32_t a = (uint32_t)std::min(65535.0, std::max(0.0, u * 65536.0));
-    uint32_t b = (uint32_t)std::min(65535.0, std::max(0.0, v * 65536.0));
-    return spread(a) | (spread(b) << 1);
+    const uint32_t n = 1u << 16;
+    uint32_t x = (uint32_t)std::min(65535.0, std::max(0.0, u * 65536.0));
+    uint32_t y = (uint32_t)std::min(65535.0, std::max(0.0, v * 65536.0));
+    uint32_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) { /* rotate the quadrant */
+            if (rx == 1) {
+                x = s - 1 - (x & (s - 1)) + (x & ~(s - 1));
+                y = s - 1 - (y & (s - 1)) + (y & ~(s - 1));
+                x &= (s - 1);
+                y &= (s - 1);
+            } else {
+                x &= (s - 1);
+                y &= (s - 1);
+            }
+            std::swap(x, y);
+        } else {
+            x &= (s - 1);
+            y &= (s - 1);
+        }
+    }
+    return d;
 }
 
 inline long long fidx(long long q, int i) { return ((q >> 6) * 6 + i) * 64 + (q & 63); }
@@ -282,7 +297,7 @@ static int autotune(dgx_solver *s)
 
 /*
  * Host-side mesh build shared by the solver and the server: validation (vertex ids, counter-clockwise
- * orientation, neighbour consistency, density), locality order (Morton key of element centroids),
+ * orientation, neighbour consistency, density), locality order (Hilbert index of element centroids),
  * face codes from the .neigh columns, vertices renumbered along that order, per-element material and
  * sponge factors.  `shot_inner` selects the element-major sponge layout of the shot-innermost kernels
  * (otherwise the 64-element blocked layout).  `explosive` (caller numbering, may be NULL) marks
@@ -317,7 +332,7 @@ int dgx_build_mesh_host(const dgx_problem *p, int shot_inner, long tri_pad, cons
         double cx = (a[0] + b[0] + c[0]) / 3, cz = (a[1] + b[1] + c[1]) / 3;
         double ext = std::max(xmax - xmin, zmax - zmin);
         if (!(ext > 0)) ext = 1;
-        key[e] = ((uint64_t)morton((cx - xmin) / ext, (cz - zmin) / ext) << 32) | (uint64_t)e;
+        key[e] = ((uint64_t)hilbert((cx - xmin) / ext, (cz - zmin) / ext) << 32) | (uint64_t)e;
     }
     std::sort(key.begin(), key.end());
     out->perm.resize(T);
@@ -840,7 +855,7 @@ extern "C" int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms)
 
 extern "C" int dgx_copy_probe(long bytes, long reps, int variant, double *ms)
 {
-    if (bytes < 1024 || reps < 1 || variant < 0 || variant > 1) return DGX_EARG;
+    if (bytes < 1024 || reps < 1 || variant < 0 || variant > 2) return DGX_EARG;
     long n32 = bytes / 32;
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0, e1;

@@ -417,6 +417,25 @@ __global__ void __launch_bounds__(256) k_copy4(const double4 *__restrict__ src, 
     }
 }
 
+/* variant 2: read-only stream (sum of 32 KiB per workgroup, one value written per workgroup) */
+__global__ void __launch_bounds__(256) k_read4(const double4 *__restrict__ src, double *__restrict__ out, long long n)
+{
+    const v4d *s = (const v4d *)src;
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    v4d acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const long long i = base + k * 256;
+        if (i < n) acc += __builtin_nontemporal_load(&s[i]);
+    }
+    double t = acc.x + acc.y + acc.z + acc.w;
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
 __global__ void k_advance(int *step_base, int n)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *step_base += n;
@@ -468,7 +487,8 @@ hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int 
 hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int variant, hipStream_t st)
 {
     if (variant == 0) dgx::k_copy<<<256 * 8, 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
-    else dgx::k_copy4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
+    else if (variant == 1) dgx::k_copy4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
+    else dgx::k_read4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double *)dst, n32);
     return hipGetLastError();
 }
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st)

@@ -120,7 +120,8 @@ int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stre
 int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms);
 /* Streaming copy of `bytes` bytes (read + write) repeated `reps` times on the default stream: the
  * achievable-bandwidth reference and the calibration point of the HBM byte counters.  variant 0:
- * grid-stride loop; variant 1: 32 KiB per workgroup, non-temporal loads and stores. */
+ * grid-stride loop; variant 1: 32 KiB per workgroup, non-temporal loads and stores; variant 2: read-only
+ * stream of `bytes` (32 KiB summed per workgroup). */
 int dgx_copy_probe(long bytes, long reps, int variant, double *ms);
 /* Layout and shot tile in use (after autotuning), and the autotuner's timings: for each candidate
  * tile k, tiles[k] and ms_per_step[k]; returns the number of candidates (0 if none were timed). */
