@@ -380,6 +380,9 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const StepArgs<R> *table, c
         const StepArgs<R> &a = table[b];
         const int q = (tile - sched[2 + b]) * 16 + wave * 4 + (lane >> 4), s = lane & 15;
         if (q >= a.tri) continue;
+        /* a slot that is not running (free, or its request has not started / has finished) is skipped:
+         * its fields are never read again before the slot is cleared for a new request */
+        if (shot_step(a, s, gstep) < 0) continue;
         if (VEL) velocity_body(a, q, s, gstep);
         else stress_body(a, q, s, gstep);
     }

@@ -133,12 +133,15 @@ class Server:
             self.advance(1)
 
 
-def run_open_loop(server: Server, requests: list[dict], arrivals: np.ndarray, keep_results: bool = False):
+def run_open_loop(server: Server, requests: list[dict], arrivals: np.ndarray, keep_results: bool = False,
+                  progress_every: float = 10.0):
     """Submit request i when the server's clock passes t_start + arrivals[i]; advance whenever work is
     running, otherwise wait for the next arrival.  Returns per-request latencies (completion minus
     arrival, seconds), the wall time and the server statistics."""
+    import sys
     order = np.argsort(arrivals)
     t_start = server.now()
+    t_log = t_start
     ids = {}
     nxt = 0
     done = set()
@@ -155,6 +158,11 @@ def run_open_loop(server: Server, requests: list[dict], arrivals: np.ndarray, ke
                 time.sleep(max(0.0, min(0.05, t_start + arrivals[order[nxt]] - server.now())))
             continue
         server.advance(1)
+        if progress_every and server.now() - t_log > progress_every:
+            t_log = server.now()
+            s = server.stats()
+            print(f"  t={t_log - t_start:.1f}s submitted={nxt} done={len(done)} running={s['running']} "
+                  f"queued={s['queued']} step={s['global_step']}", file=sys.stderr, flush=True)
         for i, rid in ids.items():
             if i in done:
                 continue
