@@ -17,6 +17,7 @@
 
 namespace dgxb {
 
+using dgx::Ops;
 using dgx::StepArgs;
 
 __device__ __forceinline__ constexpr int face_node(int j, int k)
@@ -116,7 +117,7 @@ __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned 
 }
 
 template <typename R>
-__device__ __forceinline__ void velocity_body(const StepArgs<R> &a, int q, int s, int gstep)
+__device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int gstep)
 {
     R *__restrict__ Vx = a.f[0];
     R *__restrict__ Vz = a.f[1];
@@ -129,35 +130,18 @@ __device__ __forceinline__ void velocity_body(const StepArgs<R> &a, int q, int s
     const unsigned code = (unsigned)nb.w;
     const Geom g = geometry(a.pt, a.vtx[q]);
 
-    R txx[6], tzz[6], txz[6], vx[6], vz[6];
+    /* stresses conditioned (sponge, explosive source) before use; velocities are conditioned at the
+     * update.  FP32 issues the velocity loads up front; FP64 loads them at the update, which keeps
+     * the register count (and so the number of resident wavefronts) where it is without them. */
+    R txx[6], tzz[6], txz[6], vxe[6], vze[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const long long o = bidx(a, q, i, s);
         txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
-        vx[i] = Vx[o]; vz[i] = Vz[o];
+        if (sizeof(R) == 4) { vxe[i] = Vx[o]; vze[i] = Vz[o]; }
     }
-    if (code & DGX_CODE_SPONGE) {
 #pragma unroll
-        for (int i = 0; i < 6; i++) {
-            const R f = a.absorb[q * 6 + i];
-            txx[i] *= f; tzz[i] *= f; txz[i] *= f; vx[i] *= f; vz[i] *= f;
-        }
-    }
-    const unsigned ax = aux_of(a, q, code);
-    if (ax && lstep >= 0) {
-        const dgx_aux x = a.aux[ax - 1];
-        for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
-            const dgx_src_dev &src = a.src[k];
-            if (src.shot != s) continue;
-            const double w = wavelet_at(a, src, s, lstep);
-#pragma unroll
-            for (int i = 0; i < 6; i++) {
-                const R inc = (R)(src.fun[i] * w);
-                if (src.kind == 0) vz[i] += inc;
-                else { txx[i] += inc; tzz[i] += inc; }
-            }
-        }
-    }
+    for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, s, lstep, txx[i], tzz[i], txz[i]);
 
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
     R ux[6], wx[6], uz[6], wz[6];
@@ -174,8 +158,8 @@ __device__ __forceinline__ void velocity_body(const StepArgs<R> &a, int q, int s
         R sx_ = 0, sz_ = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            sx_ += a.op.Dr[i * 6 + n] * ux[n] + a.op.Ds[i * 6 + n] * wx[n];
-            sz_ += a.op.Dr[i * 6 + n] * uz[n] + a.op.Ds[i * 6 + n] * wz[n];
+            sx_ += op.Dr[i * 6 + n] * ux[n] + op.Ds[i * 6 + n] * wx[n];
+            sz_ += op.Dr[i * 6 + n] * uz[n] + op.Ds[i * 6 + n] * wz[n];
         }
         rhx[i] = sx_;
         rhz[i] = sz_;
@@ -216,23 +200,39 @@ __device__ __forceinline__ void velocity_body(const StepArgs<R> &a, int q, int s
         for (int i = 0; i < 6; i++) {
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                flx[i] += a.op.L[j * 18 + i * 3 + k] * gx[k];
-                flz[i] += a.op.L[j * 18 + i * 3 + k] * gz[k];
+                flx[i] += op.L[j * 18 + i * 3 + k] * gx[k];
+                flz[i] += op.L[j * 18 + i * 3 + k] * gz[k];
             }
         }
     }
     const R invD = (R)g.invD;
     const R c = a.rho_inv[q] * (R)a.dt;
+    const unsigned ax = aux_of(a, q, code);
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const long long o = bidx(a, q, i, s);
-        Vx[o] = (rhx[i] + flx[i] * invD) * c + vx[i];
-        Vz[o] = (rhz[i] + flz[i] * invD) * c + vz[i];
+        R vx, vz;
+        if (sizeof(R) == 4) { vx = vxe[i]; vz = vze[i]; }
+        else { vx = Vx[o]; vz = Vz[o]; }
+        if (code & DGX_CODE_SPONGE) {
+            const R f = a.absorb[q * 6 + i];
+            vx *= f;
+            vz *= f;
+        }
+        if (ax && lstep >= 0) {
+            const dgx_aux x = a.aux[ax - 1];
+            for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
+                const dgx_src_dev &src = a.src[k];
+                if (src.shot == s && src.kind == 0) vz += (R)(src.fun[i] * wavelet_at(a, src, s, lstep));
+            }
+        }
+        Vx[o] = (rhx[i] + flx[i] * invD) * c + vx;
+        Vz[o] = (rhz[i] + flz[i] * invD) * c + vz;
     }
 }
 
 template <typename R>
-__device__ __forceinline__ void stress_body(const StepArgs<R> &a, int q, int s, int gstep)
+__device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int gstep)
 {
     const R *__restrict__ Vx = a.f[0];
     const R *__restrict__ Vz = a.f[1];
@@ -258,10 +258,10 @@ __device__ __forceinline__ void stress_body(const StepArgs<R> &a, int q, int s, 
         R rvx = 0, svx = 0, rvz = 0, svz = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            rvx += a.op.Dr[i * 6 + n] * vx[n];
-            svx += a.op.Ds[i * 6 + n] * vx[n];
-            rvz += a.op.Dr[i * 6 + n] * vz[n];
-            svz += a.op.Ds[i * 6 + n] * vz[n];
+            rvx += op.Dr[i * 6 + n] * vx[n];
+            svx += op.Ds[i * 6 + n] * vx[n];
+            rvz += op.Dr[i * 6 + n] * vz[n];
+            svz += op.Ds[i * 6 + n] * vz[n];
         }
         exx[i] = rx * rvx + sx * svx;
         ezz[i] = rz * rvz + sz * svz;
@@ -301,7 +301,7 @@ __device__ __forceinline__ void stress_body(const StepArgs<R> &a, int q, int s, 
         for (int i = 0; i < 6; i++) {
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const R l = a.op.L[j * 18 + i * 3 + k];
+                const R l = op.L[j * 18 + i * 3 + k];
                 fx[i] += l * px[k];
                 fz[i] += l * pz[k];
                 fxz[i] += l * pxz[k];
@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
 {
     int q, s;
     if (!my_work<ST>(a, q, s)) return;
-    velocity_body(a, q, s, *a.step_base + a.step_off);
+    velocity_body(a.op, a, q, s, *a.step_base + a.step_off);
 }
 
 template <typename R, int ST>
@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
 {
     int q, s;
     if (!my_work<ST>(a, q, s)) return;
-    stress_body(a, q, s, *a.step_base + a.step_off);
+    stress_body(a.op, a, q, s, *a.step_base + a.step_off);
 }
 
 /*
@@ -365,8 +365,9 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
  * tiles are dealt in 8 contiguous ranges, one per XCD.
  */
 template <typename R, bool VEL>
-__global__ void __launch_bounds__(DGX_BLOCK) ks_step(const StepArgs<R> *table, const int *sched,
-                                                        const int *step_base, int step_off)
+__global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const StepArgs<R> *__restrict__ table,
+                                                        const int *__restrict__ sched, const int *step_base,
+                                                        int step_off)
 {
     const int nb = sched[0], total = sched[1];
     const int gstep = *step_base + step_off;
@@ -383,8 +384,8 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const StepArgs<R> *table, c
         /* a slot that is not running (free, or its request has not started / has finished) is skipped:
          * its fields are never read again before the slot is cleared for a new request */
         if (shot_step(a, s, gstep) < 0) continue;
-        if (VEL) velocity_body(a, q, s, gstep);
-        else stress_body(a, q, s, gstep);
+        if (VEL) velocity_body(op, a, q, s, gstep);
+        else stress_body(op, a, q, s, gstep);
     }
 }
 
@@ -431,15 +432,21 @@ static hipError_t dispatch_b(const dgx::StepArgs<R> &a, hipStream_t st, int whic
 }
 
 extern "C" {
-hipError_t dgx_launch_serve(int fp32, const void *table, const int *sched, const int *step_base, int step_off,
-                            int vel, int grid, hipStream_t st)
+hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched, const int *step_base,
+                            int step_off, int vel, int grid, hipStream_t st)
 {
+    /* the operators are the same for every mesh: they travel as a kernel argument (scalar registers),
+     * not through the batch table */
     if (fp32) {
-        if (vel) dgxb::ks_step<float, true><<<grid, DGX_BLOCK, 0, st>>>((const dgx::StepArgs<float> *)table, sched, step_base, step_off);
-        else dgxb::ks_step<float, false><<<grid, DGX_BLOCK, 0, st>>>((const dgx::StepArgs<float> *)table, sched, step_base, step_off);
+        const dgx::Ops<float> &op = *(const dgx::Ops<float> *)ops;
+        const dgx::StepArgs<float> *t = (const dgx::StepArgs<float> *)table;
+        if (vel) dgxb::ks_step<float, true><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
+        else dgxb::ks_step<float, false><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
     } else {
-        if (vel) dgxb::ks_step<double, true><<<grid, DGX_BLOCK, 0, st>>>((const dgx::StepArgs<double> *)table, sched, step_base, step_off);
-        else dgxb::ks_step<double, false><<<grid, DGX_BLOCK, 0, st>>>((const dgx::StepArgs<double> *)table, sched, step_base, step_off);
+        const dgx::Ops<double> &op = *(const dgx::Ops<double> *)ops;
+        const dgx::StepArgs<double> *t = (const dgx::StepArgs<double> *)table;
+        if (vel) dgxb::ks_step<double, true><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
+        else dgxb::ks_step<double, false><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
     }
     return hipGetLastError();
 }

@@ -1,7 +1,7 @@
 /*
  * dgx_kernels.h -- device-side data layout shared by the HIP kernels and the host code of libdgx.
  *
- * Element order on the device is a locality order (Morton order of element centroids) chosen at
+ * Element order on the device is a locality order (Hilbert order of element centroids) chosen at
  * dgx_create; all per-element arrays use that order.  Fields of one shot are stored in 64-element
  * blocks, node-major inside a block: value (q, i) lives at ((q/64)*6 + i)*64 + q%64, so one
  * wavefront loading node i of its 64 elements reads 64 consecutive values.

@@ -34,7 +34,7 @@ namespace dgx {
 __device__ __forceinline__ int fidx(int q, int i) { return ((q >> 6) * 6 + i) * 64 + (q & 63); }
 
 /* Workgroups are dealt to the 8 XCDs round-robin; give each XCD one contiguous slice of the
- * (Morton-ordered) elements so that face neighbours mostly share that XCD's L2. */
+ * (Hilbert-ordered) elements so that face neighbours mostly share that XCD's L2. */
 __device__ __forceinline__ int element_of_thread(int nper)
 {
     const int b = blockIdx.x;

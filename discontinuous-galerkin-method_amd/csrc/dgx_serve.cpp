@@ -18,8 +18,8 @@
 #include "dgx_serve.h"
 
 extern "C" {
-hipError_t dgx_launch_serve(int fp32, const void *table, const int *sched, const int *step_base, int step_off,
-                            int vel, int grid, hipStream_t st);
+hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched, const int *step_base,
+                            int step_off, int vel, int grid, hipStream_t st);
 hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, unsigned mask, hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
 hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st);
@@ -589,9 +589,10 @@ static int build_graph(dgx_server *sv)
     hipGraph_t g;
     SV_TRY(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
+    const void *ops = sv->fp32 ? (const void *)&sv->op32 : (const void *)&sv->op64;
     for (int k = 0; k < sv->K && e == hipSuccess; k++) {
-        e = dgx_launch_serve(sv->fp32, sv->d_table, sv->d_sched, sv->d_step_base, k, 1, GRID, sv->stream);
-        if (e == hipSuccess) e = dgx_launch_serve(sv->fp32, sv->d_table, sv->d_sched, sv->d_step_base, k, 0, GRID, sv->stream);
+        e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 1, GRID, sv->stream);
+        if (e == hipSuccess) e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 0, GRID, sv->stream);
     }
     if (e == hipSuccess) e = dgx_launch_advance(sv->d_step_base, sv->K, sv->stream);
     hipError_t e2 = hipStreamEndCapture(sv->stream, &g);
