@@ -230,14 +230,15 @@ static int blocks_per_xcd(const dgx_solver *s)
 static void free_solver(dgx_solver *s)
 {
     if (!s) return;
-    if (s->graph) hipGraphExecDestroy(s->graph);
+    /* teardown: errors here cannot be acted on */
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
     for (auto &e : s->ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {s->fields, s->nbr, s->vtx, s->pt, s->mat, s->absorb, s->aux, s->src, s->rec, s->wav, s->trace,
                     s->step_base};
     for (void *p : ptrs)
-        if (p) hipFree(p);
-    if (s->stream) hipStreamDestroy(s->stream);
+        if (p) (void)hipFree(p);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
@@ -619,7 +620,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
 extern "C" int dgx_destroy(dgx_solver *s)
 {
     if (!s) return DGX_EARG;
-    hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->stream);
     free_solver(s);
     return DGX_OK;
 }
@@ -650,7 +651,7 @@ static int build_graph(dgx_solver *s)
     if (rc != DGX_OK) return rc;
     if (e != hipSuccess) { dgx_set_error("graph capture failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
     e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) { dgx_set_error("graph instantiate failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
     return DGX_OK;
 }

@@ -166,42 +166,46 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
         flx[i] = 0;
         flz[i] = 0;
     }
-    const int nbs[3] = {nb.x, nb.y, nb.z};
+    {
+        const int nbs[3] = {nb.x, nb.y, nb.z};
+        R gxx[3][3], gzz[3][3], gxz[3][3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        R dxx[3], dzz[3], dxz[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int on = face_node(j, k);
-            dxx[k] = -txx[on]; dzz[k] = -tzz[on]; dxz[k] = -txz[on];
-        }
-        const int e = nbs[j];
-        if (e >= 0) {
+        for (int j = 0; j < 3; j++) {
+            const int ee = nbs[j] >= 0 ? nbs[j] : q;
             const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-            const bool special = (code & DGX_CODE_NBSPECIAL(j)) != 0 || (a.aux_of_elem && a.aux_of_elem[e] >= 0);
-            const unsigned ncode = special ? (unsigned)a.nbr[e].w : 0u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const int nn = nb_node(m, rev, k);
-                const long long o = bidx(a, e, nn, s);
-                R nxx = Txx[o], nzz = Tzz[o], nxz = Txz[o];
-                if (special) condition_stress(a, ncode, e, nn, s, lstep, nxx, nzz, nxz);
-                dxx[k] += nxx; dzz[k] += nzz; dxz[k] += nxz;
+                const long long o = bidx(a, ee, nb_node(m, rev, k), s);
+                gxx[j][k] = Txx[o]; gzz[j][k] = Tzz[o]; gxz[j][k] = Txz[o];
             }
         }
-        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
-        R gx[3], gz[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            gx[k] = ez * dxx[k] - ex * dxz[k];
-            gz[k] = ez * dxz[k] - ex * dzz[k];
+        for (int j = 0; j < 3; j++) {
+            const int e = nbs[j];
+            if (e < 0) continue;
+            const bool special = (code & DGX_CODE_NBSPECIAL(j)) != 0 || (a.aux_of_elem && a.aux_of_elem[e] >= 0);
+            if (!special) continue;
+            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+            const unsigned ncode = (unsigned)a.nbr[e].w;
+            for (int k = 0; k < 3; k++)
+                condition_stress(a, ncode, e, nb_node(m, rev, k), s, lstep, gxx[j][k], gzz[j][k], gxz[j][k]);
         }
 #pragma unroll
-        for (int i = 0; i < 6; i++) {
+        for (int j = 0; j < 3; j++) {
+            const bool has = nbs[j] >= 0;
+            const R ex = (R)g.ex[j], ez = (R)g.ez[j];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                flx[i] += op.L[j * 18 + i * 3 + k] * gx[k];
-                flz[i] += op.L[j * 18 + i * 3 + k] * gz[k];
+                const int on = face_node(j, k);
+                R dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
+                if (has) { dxx += gxx[j][k]; dzz += gzz[j][k]; dxz += gxz[j][k]; }
+                const R gx = ez * dxx - ex * dxz;
+                const R gz = ez * dxz - ex * dzz;
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    flx[i] += op.L[j * 18 + i * 3 + k] * gx;
+                    flz[i] += op.L[j * 18 + i * 3 + k] * gz;
+                }
             }
         }
     }
@@ -270,41 +274,42 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
     R fx[6], fz[6], fxz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
-    const int nbs[3] = {nb.x, nb.y, nb.z};
+    {
+        const int nbs[3] = {nb.x, nb.y, nb.z};
+        R gvx[3][3], gvz[3][3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        R dvx[3], dvz[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int on = face_node(j, k);
-            dvx[k] = -vx[on]; dvz[k] = -vz[on];
-        }
-        const int e = nbs[j];
-        if (e >= 0) {
+        for (int j = 0; j < 3; j++) {
+            const int ee = nbs[j] >= 0 ? nbs[j] : q;
             const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const long long o = bidx(a, e, nb_node(m, rev, k), s);
-                dvx[k] += Vx[o];
-                dvz[k] += Vz[o];
+                const long long o = bidx(a, ee, nb_node(m, rev, k), s);
+                gvx[j][k] = Vx[o]; gvz[j][k] = Vz[o];
             }
         }
-        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
-        R px[3], pz[3], pxz[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            px[k] = ez * dvx[k];
-            pz[k] = -ex * dvz[k];
-            pxz[k] = ez * dvz[k] - ex * dvx[k];
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
+        for (int j = 0; j < 3; j++) {
+            const bool has = nbs[j] >= 0;
+            const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+            R px[3], pz[3], pxz[3];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const R l = op.L[j * 18 + i * 3 + k];
-                fx[i] += l * px[k];
-                fz[i] += l * pz[k];
-                fxz[i] += l * pxz[k];
+                const int on = face_node(j, k);
+                R dvx = -vx[on], dvz = -vz[on];
+                if (has) { dvx += gvx[j][k]; dvz += gvz[j][k]; }
+                px[k] = ez * dvx;
+                pz[k] = -ex * dvz;
+                pxz[k] = ez * dvz - ex * dvx;
+            }
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const R l = op.L[j * 18 + i * 3 + k];
+                    fx[i] += l * px[k];
+                    fz[i] += l * pz[k];
+                    fxz[i] += l * pxz[k];
+                }
             }
         }
     }

@@ -100,8 +100,10 @@ __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned 
     }
 }
 
+/* At least 3 wavefronts per SIMD: caps the FP64 kernel at 168 VGPRs (it needs 170 unbounded, which
+ * leaves 2), without spilling. */
 template <typename R, bool FUSED>
-__global__ void __launch_bounds__(DGX_BLOCK) k_velocity(StepArgs<R> a)
+__global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
 {
     const int q = element_of_thread(a.nper);
     if (q >= a.tri) return;
@@ -148,46 +150,53 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_velocity(StepArgs<R> a)
         }
     }
 
-    /* surface term, scaled by 1/D at the end */
     R flx[6], flz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
-    const int nbs[3] = {nb.x, nb.y, nb.z};
+    {
+        const int nbs[3] = {nb.x, nb.y, nb.z};
+        /* all face gathers first (boundary faces read the own element and drop it), so that the 27
+         * loads are in flight together */
+        R gxx[3][3], gzz[3][3], gxz[3][3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        R dxx[3], dzz[3], dxz[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int on = face_node(j, k);
-            dxx[k] = -txx[on]; dzz[k] = -tzz[on]; dxz[k] = -txz[on];
-        }
-        const int e = nbs[j];
-        if (e >= 0) {
+        for (int j = 0; j < 3; j++) {
+            const int ee = nbs[j] >= 0 ? nbs[j] : q;
             const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-            const bool special = FUSED && (code & DGX_CODE_NBSPECIAL(j));
-            const unsigned ncode = special ? (unsigned)a.nbr[e].w : 0u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const int nn = nb_node(m, rev, k);
-                const int o = fidx(e, nn);
-                R nxx = Txx[o], nzz = Tzz[o], nxz = Txz[o];
-                if (special) condition_stress(a, ncode, e, nn, shot, step, nxx, nzz, nxz);
-                dxx[k] += nxx; dzz[k] += nzz; dxz[k] += nxz;
+                const int o = fidx(ee, nb_node(m, rev, k));
+                gxx[j][k] = Txx[o]; gzz[j][k] = Tzz[o]; gxz[j][k] = Txz[o];
             }
         }
-        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+        if (FUSED && (code & (DGX_CODE_NBSPECIAL(0) | DGX_CODE_NBSPECIAL(1) | DGX_CODE_NBSPECIAL(2)))) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const R gx = ez * dxx[k] - ex * dxz[k];
-            const R gz = ez * dxz[k] - ex * dzz[k];
+            for (int j = 0; j < 3; j++) {
+                if (!(code & DGX_CODE_NBSPECIAL(j)) || nbs[j] < 0) continue;
+                const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+                const unsigned ncode = (unsigned)a.nbr[nbs[j]].w;
+                for (int k = 0; k < 3; k++)
+                    condition_stress(a, ncode, nbs[j], nb_node(m, rev, k), shot, step, gxx[j][k], gzz[j][k], gxz[j][k]);
+            }
+        }
 #pragma unroll
-            for (int i = 0; i < 6; i++) {
-                flx[i] += a.op.L[j * 18 + i * 3 + k] * gx;
-                flz[i] += a.op.L[j * 18 + i * 3 + k] * gz;
+        for (int j = 0; j < 3; j++) {
+            const bool has = nbs[j] >= 0;
+            const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int on = face_node(j, k);
+                R dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
+                if (has) { dxx += gxx[j][k]; dzz += gzz[j][k]; dxz += gxz[j][k]; }
+                const R gx = ez * dxx - ex * dxz;
+                const R gz = ez * dxz - ex * dzz;
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    flx[i] += a.op.L[j * 18 + i * 3 + k] * gx;
+                    flz[i] += a.op.L[j * 18 + i * 3 + k] * gz;
+                }
             }
         }
     }
-
     /* V = (sponge(V) + source) + dt/rho * rhs */
     const R invD = (R)g.invD;
     const R c = a.rho_inv[q] * (R)a.dt;
@@ -262,45 +271,45 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_stress(StepArgs<R> a)
     R fx[6], fz[6], fxz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
-    const int nbs[3] = {nb.x, nb.y, nb.z};
+    {
+        const int nbs[3] = {nb.x, nb.y, nb.z};
+        R gvx[3][3], gvz[3][3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        R dvx[3], dvz[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int on = face_node(j, k);
-            dvx[k] = -vx[on]; dvz[k] = -vz[on];
-        }
-        const int e = nbs[j];
-        if (e >= 0) {
+        for (int j = 0; j < 3; j++) {
+            const int ee = nbs[j] >= 0 ? nbs[j] : q;
             const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const int o = fidx(e, nb_node(m, rev, k));
-                dvx[k] += Vx[o];
-                dvz[k] += Vz[o];
+                const int o = fidx(ee, nb_node(m, rev, k));
+                gvx[j][k] = Vx[o]; gvz[j][k] = Vz[o];
             }
         }
-        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
-        R px[3], pz[3], pxz[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            px[k] = ez * dvx[k];
-            pz[k] = -ex * dvz[k];
-            pxz[k] = ez * dvz[k] - ex * dvx[k];
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
+        for (int j = 0; j < 3; j++) {
+            const bool has = nbs[j] >= 0;
+            const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+            R px[3], pz[3], pxz[3];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const R l = a.op.L[j * 18 + i * 3 + k];
-                fx[i] += l * px[k];
-                fz[i] += l * pz[k];
-                fxz[i] += l * pxz[k];
+                const int on = face_node(j, k);
+                R dvx = -vx[on], dvz = -vz[on];
+                if (has) { dvx += gvx[j][k]; dvz += gvz[j][k]; }
+                px[k] = ez * dvx;
+                pz[k] = -ex * dvz;
+                pxz[k] = ez * dvz - ex * dvx;
+            }
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const R l = a.op.L[j * 18 + i * 3 + k];
+                    fx[i] += l * px[k];
+                    fz[i] += l * pz[k];
+                    fxz[i] += l * pxz[k];
+                }
             }
         }
     }
-
     const R invD = (R)g.invD;
     const R lam = a.lam[q], mu = a.mu[q], l2m = lam + 2 * mu, dt = (R)a.dt;
 #pragma unroll
