@@ -50,10 +50,12 @@ def parse():
     return ap.parse_args()
 
 
-def run_path(setup, precision, warmup, steps, layout=L.DGX_LAYOUT_AUTO, bench_launches=100):
-    """Warm up, time `steps` whole steps (hipGraph replays), then time each fused kernel alone over
-    back-to-back launches between two HIP events on the solver's stream."""
-    s = dgx.Solver(setup, precision, layout)
+def run_path(setup, precision, warmup, steps, layout=L.DGX_LAYOUT_AUTO, bench_launches=100,
+             step_kernel=L.DGX_STEP_AUTO):
+    """Warm up, time `steps` whole steps (hipGraph replays), then time each kernel of the step alone over
+    back-to-back launches between two HIP events on the solver's stream: {"full": ms} for the
+    full-step kernel, {"velocity": ms, "stress": ms} for the half-step pair."""
+    s = dgx.Solver(setup, precision, layout, step_kernel=step_kernel)
     info = s.layout_info()
     s.step(warmup)
     s.synchronize()
@@ -62,11 +64,13 @@ def run_path(setup, precision, warmup, steps, layout=L.DGX_LAYOUT_AUTO, bench_la
     s.synchronize()
     sec = time.perf_counter() - t
     x, z = s.traces(0)
-    kv = s.kernel_bench(0, bench_launches)
-    ks = s.kernel_bench(1, bench_launches)
+    if step_kernel == L.DGX_STEP_FULL:
+        kt = {"full": s.kernel_bench(2, bench_launches)}
+    else:
+        kt = {"velocity": s.kernel_bench(0, bench_launches), "stress": s.kernel_bench(1, bench_launches)}
     s.close()
     n = warmup + steps
-    return sec, kv, ks, x[:n], z[:n], info
+    return sec, kt, x[:n], z[:n], info
 
 
 def rel(u, v):
@@ -82,7 +86,8 @@ def batched_leg(a, rate64):
     S, nb = a.shots, a.batched_steps
     st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
     T = w.mesh.tri
-    secb, kvb, ksb, xb, zb, info = run_path(st, L.DGX_FP32, 20, nb, L.DGX_LAYOUT_SHOT_INNER, 20)
+    secb, ktb, xb, zb, info = run_path(st, L.DGX_FP32, 20, nb, L.DGX_LAYOUT_SHOT_INNER, 20)
+    kvb, ksb = ktb["velocity"], ktb["stress"]
     rateb = T * 30 * S * nb / secb
     # FP64 reference on 4 of the shots (same mesh, sources, steps), element layout
     picks = [0, S // 3, (2 * S) // 3, S - 1]
@@ -184,16 +189,20 @@ def main():
     T = w.mesh.tri
     dof = T * 30
 
-    sec64, kv64, ks64, x64, z64, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps)
-    sec32, kv32, ks32, x32, z32, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps)
+    sec64, kt64, x64, z64, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps)
+    sec32, kt32, x32, z32, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps)
+    secf, ktf, xf, zf, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps, step_kernel=L.DGX_STEP_FULL)
     rate64 = dof * a.steps / sec64
     rate32 = dof * a.steps / sec32
+    rate64f = dof * a.steps / secf
 
     err32 = max(rel(x32, x64), rel(z32, z64))
+    err_full = max(rel(xf, x64), rel(zf, z64))
 
     # dominant kernel of the FP64 step: algorithmic bytes = fields read + written per launch
     bytes_vel = T * 6 * 8 * 7   # read Txx,Tzz,Txz,Vx,Vz; write Vx,Vz
     bytes_str = T * 6 * 8 * 8   # read Vx,Vz,Txx,Tzz,Txz; write Txx,Tzz,Txz
+    kv64, ks64 = kt64["velocity"], kt64["stress"]
     if ks64 >= kv64:
         kname, kbytes, kms = "k_stress<double,fused>", bytes_str, ks64
     else:
@@ -222,13 +231,17 @@ def main():
         "data": "synthetic (seeded jittered mesh and gradient model generated in-process)",
         "config": {"workload": w.name, "elements": T, "dof": dof, "shots": 1, "receivers": int(len(w.receivers)),
                    "path": "plain FP64, fused velocity+stress kernels, hipGraph step replay"},
-        "kernel_timing": "each fused kernel timed alone over back-to-back launches between HIP events",
+        "kernel_timing": "each kernel timed alone over back-to-back launches between HIP events",
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "achieved_GBps": achieved / 1e9, "kernel_ms": kms,
-                     "velocity_ms": kv64, "stress_ms": ks64},
+                     "achieved_GBps": achieved / 1e9, "kernel_ms": kms, "algorithmic_bytes_per_launch": kbytes,
+                     "velocity_ms": kv64, "stress_ms": ks64,
+                     "step_frac": T * 30 * 24 / (sec64 / a.steps) / HBM_PEAK},
         "fp32": {"value": rate32, "ms_per_step": sec32 * 1e3 / a.steps, "trace_rel_l2_vs_fp64": err32,
-                 "velocity_ms": kv32, "stress_ms": ks32, "speedup_vs_fp64": rate32 / rate64},
+                 "velocity_ms": kt32["velocity"], "stress_ms": kt32["stress"], "speedup_vs_fp64": rate32 / rate64},
+        "full_step_fp64": {"value": rate64f, "ms_per_step": secf * 1e3 / a.steps, "kernel_ms": ktf["full"],
+                           "trace_rel_l2_vs_half_step": err_full,
+                           "note": "one kernel per step over tiles + recomputed ring (opt-in DGX_STEP_FULL)"},
     }
     if a.shots > 0:
         out["batched_fp32"] = batched_leg(a, rate64)

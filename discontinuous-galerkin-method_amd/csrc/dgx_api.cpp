@@ -51,7 +51,7 @@ extern "C" const char *dgx_last_error(void) { return g_err; }
 
 namespace {
 
-enum { K_VEL_FUSED = 0, K_STRESS_FUSED = 1, K_VEL = 2, K_STRESS = 3, K_ABSORB = 4 };
+enum { K_VEL_FUSED = 0, K_STRESS_FUSED = 1, K_VEL = 2, K_STRESS = 3, K_ABSORB = 4, K_FULL = 5 };
 const int kGraphSteps = 32; /* steps captured per graph replay */
 
 template <typename T>
@@ -115,10 +115,17 @@ struct dgx_solver {
     std::vector<long> perm; /* device element -> caller element (0-based) */
     std::vector<long> inv;  /* caller element -> device element */
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr}; /* step graphs, by which field copy holds the state */
+    int full = 0;           /* one full-step kernel per step (element layout) */
+    int cur = 0;            /* full step: copy of the fields holding the current state */
+    long vrows = 0;         /* element layout: rows per field and shot (elements, then ring rows) */
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     /* device memory */
-    void *fields = nullptr; /* 5 * nshots * tri_pad*6 reals */
+    void *fields = nullptr;  /* 5 * nshots * vrows*6 reals (element layout) */
+    void *fields2 = nullptr; /* full step: the second copy */
+    int4 *nb2 = nullptr, *tiles = nullptr;
+    int *ring = nullptr;
+    long ntile = 0;
     int4 *nbr = nullptr, *vtx = nullptr;
     double2 *pt = nullptr;
     void *mat = nullptr;    /* rho_inv | lam | mu, tri_pad reals each */
@@ -136,14 +143,85 @@ struct dgx_solver {
 
     const void *args() const { return fp32 ? (const void *)&a32 : (const void *)&a64; }
     size_t real_size() const { return fp32 ? 4 : 8; }
+    /* the copy holding the current state */
+    void *state() const { return cur ? fields2 : fields; }
 };
+
+/* Full step: the copy just written becomes the current state. */
+static void swap_copies(dgx_solver *s)
+{
+    s->cur ^= 1;
+    for (int c = 0; c < 5; c++) {
+        std::swap(s->a64.f[c], s->a64.g[c]);
+        std::swap(s->a32.f[c], s->a32.g[c]);
+    }
+}
+
+/*
+ * Tiles of the full-step kernel: runs of consecutive elements (locality order) grown while the run plus
+ * its ring (elements outside the run that share a face with it) fit one element per thread of a
+ * workgroup.  tiles[t] = {first, own, first ring entry, ring entries}; ring = element ids, tile after
+ * tile; nb2[q] = {row of face j's neighbour velocities in the output copy, code}: the neighbour itself
+ * inside the tile, row vrow0 + its ring entry outside it, -1 on the boundary.
+ */
+static void build_tiles(long tri, const std::vector<int4> &nbr, long vrow0, std::vector<int4> &tiles,
+                        std::vector<int> &ring, std::vector<int4> &nb2)
+{
+    std::vector<int> stamp(tri, -1), pos(tri, -1);
+    nb2.assign(tri, int4{-1, -1, -1, 0});
+    long q0 = 0;
+    int t = 0;
+    while (q0 < tri) {
+        long q = q0;
+        long nring = 0;
+        std::vector<int> cand;
+        while (q < tri && q - q0 < DGX_BLOCK) {
+            const int nq[3] = {nbr[q].x, nbr[q].y, nbr[q].z};
+            long add = 0;
+            for (int j = 0; j < 3; j++) {
+                const int e = nq[j];
+                if (e >= 0 && !(e >= q0 && e <= q) && stamp[e] != t) add++;
+            }
+            const long in_ring = stamp[q] == t ? 1 : 0;
+            if ((q - q0 + 1) + nring + add - in_ring > DGX_BLOCK) break;
+            if (in_ring) { stamp[q] = -1; nring--; }
+            for (int j = 0; j < 3; j++) {
+                const int e = nq[j];
+                if (e >= 0 && !(e >= q0 && e <= q) && stamp[e] != t) {
+                    stamp[e] = t;
+                    cand.push_back(e);
+                    nring++;
+                }
+            }
+            q++;
+        }
+        const int rbegin = (int)ring.size();
+        for (int e : cand)
+            if (stamp[e] == t) {
+                pos[e] = (int)ring.size();
+                ring.push_back(e);
+            }
+        tiles.push_back(int4{(int)q0, (int)(q - q0), rbegin, (int)ring.size() - rbegin});
+        for (long x = q0; x < q; x++) {
+            const int nx[3] = {nbr[x].x, nbr[x].y, nbr[x].z};
+            int r[3];
+            for (int j = 0; j < 3; j++) {
+                const int e = nx[j];
+                r[j] = e < 0 ? -1 : (e >= q0 && e < q) ? e : (int)(vrow0 + pos[e]);
+            }
+            nb2[x] = int4{r[0], r[1], r[2], nbr[x].w};
+        }
+        q0 = q;
+        t++;
+    }
+}
 
 static int launch(dgx_solver *s, int which, int step_off)
 {
     hipError_t e;
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         if (which != K_VEL_FUSED && which != K_STRESS_FUSED) {
-            dgx_set_error("the split entry points need the element layout (DGX_LAYOUT_ELEMENT)");
+            dgx_set_error("the split entry points and the full-step kernel need the element layout (DGX_LAYOUT_ELEMENT)");
             return DGX_ESTATE;
         }
         s->a32.step_off = s->a64.step_off = step_off;
@@ -154,6 +232,10 @@ static int launch(dgx_solver *s, int which, int step_off)
             return DGX_EHIP;
         }
         return DGX_OK;
+    }
+    if (which == K_FULL && !s->full) {
+        dgx_set_error("the full-step kernel is not enabled for this solver (dgx_problem.step_kernel)");
+        return DGX_ESTATE;
     }
     if (s->fp32) {
         s->a32.step_off = step_off;
@@ -187,11 +269,20 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     const double *S[3] = {ops->S1, ops->S2, ops->S3};
     for (int j = 0; j < 3; j++)
         for (int k = 0; k < 18; k++) a.op.L[j * 18 + k] = (R)S[j][k];
-    R *f = (R *)s->fields;
+    R *f = (R *)s->fields, *f2 = (R *)s->fields2;
     size_t per_field = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)s->tri * 6 * s->spad
-                                                             : (size_t)s->nshots * s->tri_pad * 6;
-    for (int c = 0; c < 5; c++) a.f[c] = f + c * per_field;
-    a.fstride = s->tri_pad;
+                                                             : (size_t)s->nshots * s->vrows * 6;
+    for (int c = 0; c < 5; c++) {
+        a.f[c] = f + c * per_field;
+        a.g[c] = f2 ? f2 + c * per_field : nullptr;
+    }
+    a.fstride = s->vrows;
+    a.nb2 = s->nb2;
+    a.tiles = s->tiles;
+    a.ring = s->ring;
+    a.ntile = (int)s->ntile;
+    a.tper = (int)((s->ntile + DGX_NXCD - 1) / DGX_NXCD);
+    a.vrow0 = (int)s->tri_pad;
     a.nbr = s->nbr;
     a.vtx = s->vtx;
     a.pt = s->pt;
@@ -231,11 +322,12 @@ static void free_solver(dgx_solver *s)
 {
     if (!s) return;
     /* teardown: errors here cannot be acted on */
-    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    for (auto &g : s->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
-    void *ptrs[] = {s->fields, s->nbr, s->vtx, s->pt, s->mat, s->absorb, s->aux, s->src, s->rec, s->wav, s->trace,
-                    s->step_base};
+    void *ptrs[] = {s->fields, s->fields2, s->nb2, s->tiles, s->ring, s->nbr, s->vtx, s->pt, s->mat, s->absorb,
+                    s->aux, s->src, s->rec, s->wav, s->trace, s->step_base};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -447,6 +539,17 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         return DGX_EARG;
     }
     s->layout = (p->layout == DGX_LAYOUT_AUTO) ? (p->nshots >= 8 ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT) : p->layout;
+    if (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF && p->step_kernel != DGX_STEP_FULL) {
+        dgx_set_error("dgx_create: unknown step_kernel %d", p->step_kernel);
+        free_solver(s);
+        return DGX_EARG;
+    }
+    if (p->step_kernel == DGX_STEP_FULL && s->layout != DGX_LAYOUT_ELEMENT) {
+        dgx_set_error("dgx_create: the full-step kernel needs the element layout");
+        free_solver(s);
+        return DGX_EARG;
+    }
+    s->full = p->step_kernel == DGX_STEP_FULL;
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         const int t = p->shot_tile;
         if (t != 0 && t != 4 && t != 8 && t != 16 && t != 32 && t != 64) {
@@ -556,6 +659,17 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     s->src_host = src_list;
 
+    /* ---- full-step tiles: rows of the element range, then the tiles' ring rows ---- */
+    s->vrows = s->tri_pad;
+    std::vector<int4> tiles_host, nb2_host;
+    std::vector<int> ring_host;
+    if (s->full) {
+        build_tiles(T, nbr_host, s->tri_pad, tiles_host, ring_host, nb2_host);
+        s->ntile = (long)tiles_host.size();
+        s->vrows = s->tri_pad + ((long)ring_host.size() + 63) / 64 * 64;
+        if (s->vrows * 6 >= (1L << 31)) { dgx_set_error("mesh too large for 32-bit row indices"); free_solver(s); return DGX_EARG; }
+    }
+
     /* ---- device allocations (all host-side validation is done above) ---- */
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -566,9 +680,16 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (auto &ev : s->ev) HIP_TRY(hipEventCreate(&ev));
     size_t nreal_fields = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)5 * s->tri * 6 * s->spad
-                                                                 : (size_t)5 * s->nshots * s->tri_pad * 6;
+                                                                 : (size_t)5 * s->nshots * s->vrows * 6;
     HIP_TRY(hipMalloc(&s->fields, nreal_fields * s->real_size()));
     HIP_TRY(hipMemset(s->fields, 0, nreal_fields * s->real_size()));
+    if (s->full) {
+        HIP_TRY(hipMalloc(&s->fields2, nreal_fields * s->real_size()));
+        HIP_TRY(hipMemset(s->fields2, 0, nreal_fields * s->real_size()));
+        HIP_TRY(upload(&s->tiles, tiles_host));
+        HIP_TRY(upload(&s->ring, ring_host));
+        HIP_TRY(upload(&s->nb2, nb2_host));
+    }
     HIP_TRY(upload(&s->nbr, nbr_host));
     HIP_TRY(upload(&s->vtx, vtx_host));
     HIP_TRY(upload(&s->pt, pt_host));
@@ -617,6 +738,28 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     return DGX_OK;
 }
 
+extern "C" int dgx_full_step_tiling(const dgx_problem *p, long *ntile, long *nring, long *vrow0, long *vrows,
+                                    int4 *tiles, int *ring, int4 *nb2, int4 *nbr)
+{
+    if (!p || p->tri <= 0) return DGX_EARG;
+    const long nblk = (p->tri + DGX_BLOCK - 1) / DGX_BLOCK;
+    const long tri_pad = (nblk + DGX_NXCD - 1) / DGX_NXCD * DGX_NXCD * DGX_BLOCK;
+    dgx_mesh_host mh;
+    if (int rc = dgx_build_mesh_host(p, 0, tri_pad, nullptr, &mh)) return rc;
+    std::vector<int4> t, n2;
+    std::vector<int> r;
+    build_tiles(p->tri, mh.nbr, tri_pad, t, r, n2);
+    if (ntile) *ntile = (long)t.size();
+    if (nring) *nring = (long)r.size();
+    if (vrow0) *vrow0 = tri_pad;
+    if (vrows) *vrows = tri_pad + ((long)r.size() + 63) / 64 * 64;
+    if (tiles) std::copy(t.begin(), t.end(), tiles);
+    if (ring) std::copy(r.begin(), r.end(), ring);
+    if (nb2) std::copy(n2.begin(), n2.end(), nb2);
+    if (nbr) std::copy(mh.nbr.begin(), mh.nbr.begin() + p->tri, nbr); /* mh.nbr is padded */
+    return DGX_OK;
+}
+
 extern "C" int dgx_destroy(dgx_solver *s)
 {
     if (!s) return DGX_EARG;
@@ -628,29 +771,45 @@ extern "C" int dgx_destroy(dgx_solver *s)
 static int step_eager(dgx_solver *s)
 {
     int off = (int)(s->step - s->base_host);
-    int rc = launch(s, K_VEL_FUSED, off);
-    if (rc) return rc;
-    rc = launch(s, K_STRESS_FUSED, off);
-    if (rc) return rc;
+    int rc;
+    if (s->full) {
+        rc = launch(s, K_FULL, off);
+        if (rc) return rc;
+        swap_copies(s);
+    } else {
+        rc = launch(s, K_VEL_FUSED, off);
+        if (rc) return rc;
+        rc = launch(s, K_STRESS_FUSED, off);
+        if (rc) return rc;
+    }
     s->step++;
     return DGX_OK;
 }
 
 static int build_graph(dgx_solver *s)
 {
-    /* K steps with offsets 0..K-1 from the device counter, then advance the counter by K */
+    /* K steps with offsets 0..K-1 from the device counter, then advance the counter by K.  A full-step
+     * graph alternates the two copies; K is even, so it ends on the copy it starts from. */
+    static_assert(kGraphSteps % 2 == 0, "a full-step graph must end on the copy it starts from");
     hipGraph_t g;
+    const int c0 = s->cur;
     HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = DGX_OK;
     for (int k = 0; k < kGraphSteps && rc == DGX_OK; k++) {
-        rc = launch(s, K_VEL_FUSED, k);
-        if (rc == DGX_OK) rc = launch(s, K_STRESS_FUSED, k);
+        if (s->full) {
+            rc = launch(s, K_FULL, k);
+            swap_copies(s);
+        } else {
+            rc = launch(s, K_VEL_FUSED, k);
+            if (rc == DGX_OK) rc = launch(s, K_STRESS_FUSED, k);
+        }
     }
+    while (s->cur != c0) swap_copies(s); /* only after a failed launch */
     if (rc == DGX_OK && dgx_launch_advance(s->step_base, kGraphSteps, s->stream) != hipSuccess) rc = DGX_EHIP;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (rc != DGX_OK) return rc;
     if (e != hipSuccess) { dgx_set_error("graph capture failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
-    e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&s->graph[c0], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) { dgx_set_error("graph instantiate failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
     return DGX_OK;
@@ -673,11 +832,11 @@ extern "C" int dgx_step(dgx_solver *s, long n)
                 }
                 s->base_host = s->step;
             }
-            if (!s->graph) {
+            if (!s->graph[s->cur]) {
                 int rc = build_graph(s);
                 if (rc) return rc;
             }
-            HIP_TRY(hipGraphLaunch(s->graph, s->stream));
+            HIP_TRY(hipGraphLaunch(s->graph[s->cur], s->stream));
             s->step += kGraphSteps;
             s->base_host += kGraphSteps;
             n -= kGraphSteps;
@@ -763,7 +922,7 @@ static void raw_span(const dgx_solver *s, int c, long shot, size_t *off, size_t 
         *off = (size_t)c * *len;
     } else {
         *len = (size_t)s->tri_pad * 6 * rs;
-        *off = ((size_t)c * s->nshots + shot) * *len;
+        *off = ((size_t)c * s->nshots + shot) * (size_t)s->vrows * 6 * rs;
     }
 }
 
@@ -778,7 +937,7 @@ extern "C" int dgx_get_fields(dgx_solver *s, long shot, double *vx, double *vz, 
         size_t off, len;
         raw_span(s, c, shot, &off, &len);
         raw.resize(len);
-        HIP_TRY(hipMemcpy(raw.data(), (const char *)s->fields + off, len, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(raw.data(), (const char *)s->state() + off, len, hipMemcpyDeviceToHost));
         if (s->fp32) unpack<float>(s, raw, shot, outs[c]);
         else unpack<double>(s, raw, shot, outs[c]);
     }
@@ -798,10 +957,10 @@ extern "C" int dgx_set_fields(dgx_solver *s, long shot, const double *vx, const 
         raw_span(s, c, shot, &off, &len);
         raw.assign(len, 0);
         /* read-modify-write so the other shots sharing the array keep their values */
-        HIP_TRY(hipMemcpy(raw.data(), (const char *)s->fields + off, len, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(raw.data(), (const char *)s->state() + off, len, hipMemcpyDeviceToHost));
         if (s->fp32) pack<float>(s, ins[c], shot, raw);
         else pack<double>(s, ins[c], shot, raw);
-        HIP_TRY(hipMemcpy((char *)s->fields + off, raw.data(), len, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy((char *)s->state() + off, raw.data(), len, hipMemcpyHostToDevice));
     }
     return DGX_OK;
 }
@@ -836,9 +995,9 @@ extern "C" int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms)
     /* n back-to-back launches of one fused kernel between two events: the kernel's own duration
      * without per-launch event overhead.  The launches repeat the current step, so the fields are
      * no longer a valid state afterwards. */
-    if (!s || n <= 0 || (which != 0 && which != 1) || s->step >= s->nsteps) return DGX_EARG;
+    if (!s || n <= 0 || which < 0 || which > 2 || s->step >= s->nsteps) return DGX_EARG;
     int off = (int)(s->step - s->base_host);
-    int k = which == 0 ? K_VEL_FUSED : K_STRESS_FUSED;
+    int k = which == 0 ? K_VEL_FUSED : which == 1 ? K_STRESS_FUSED : K_FULL;
     int rc = launch(s, k, off); /* warm */
     if (rc) return rc;
     HIP_TRY(hipEventRecord(s->ev[0], s->stream));

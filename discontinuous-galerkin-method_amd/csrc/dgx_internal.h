@@ -28,4 +28,10 @@ int dgx_build_mesh_host(const dgx_problem *p, int shot_inner, long tri_pad, cons
 
 extern "C" void dgx_set_error(const char *fmt, ...);
 
+/* Host-only view of the full-step tiling of a mesh in the element layout (no device needed): counts
+ * when the array pointers are NULL, else also tiles [ntile], ring [nring], nb2 [tri] and the device-order
+ * neighbour records nbr [tri].  For tests of the tiling's invariants; not part of the C-ABI. */
+extern "C" int dgx_full_step_tiling(const dgx_problem *p, long *ntile, long *nring, long *vrow0, long *vrows,
+                                    int4 *tiles, int *ring, int4 *nb2, int4 *nbr);
+
 #endif

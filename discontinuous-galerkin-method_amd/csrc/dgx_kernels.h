@@ -88,6 +88,13 @@ struct StepArgs {
     const double *const *slot_wav;
     double *const *slot_trace;
     const int *slot_nrec;
+    /* full-step kernel only (element layout): the step reads f and writes the other copy g */
+    R *g[5];
+    const int4 *nb2;       /* per element {row in g of face j's neighbour velocities (j = 0..2), code} */
+    const int4 *tiles;     /* per tile {first element, own elements, first ring entry, ring elements} */
+    const int *ring;       /* ring entries: element ids, tile after tile */
+    int ntile, tper;       /* tiles; tiles per XCD slice */
+    int vrow0;             /* velocity row of ring entry 0 (rows below it are elements) */
 };
 
 } // namespace dgx

@@ -100,22 +100,22 @@ __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned 
     }
 }
 
-/* At least 3 wavefronts per SIMD: caps the FP64 kernel at 168 VGPRs (it needs 170 unbounded, which
- * leaves 2), without spilling. */
-template <typename R, bool FUSED>
-__global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
+/*
+ * Velocity update of element q (shot `shot`) from stresses T and velocities Vi, written to row `orow`
+ * of Vo.  In place (Vi == Vo, orow == q) for the half-step kernel; out of place for the full-step
+ * kernel, which also keeps the conditioned stresses of its own elements in LDS (when `save`:
+ * tsave[(c*6 + i)*DGX_BLOCK + threadIdx.x], c = xx, zz, xz).
+ */
+template <typename R, bool FUSED, bool INPLACE>
+__device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int shot, int step, int orow,
+                                                const R *__restrict__ Txx, const R *__restrict__ Tzz,
+                                                const R *__restrict__ Txz, const R *__restrict__ Vxi,
+                                                const R *__restrict__ Vzi, R *__restrict__ Vxo,
+                                                R *__restrict__ Vzo, R *tsave, bool save)
 {
-    const int q = element_of_thread(a.nper);
-    if (q >= a.tri) return;
-    const int shot = blockIdx.y;
-    const long long so = (long long)shot * a.fstride * 6;
-    R *__restrict__ Vx = a.f[0] + so;
-    R *__restrict__ Vz = a.f[1] + so;
-    const R *__restrict__ Txx = a.f[2] + so;
-    const R *__restrict__ Tzz = a.f[3] + so;
-    const R *__restrict__ Txz = a.f[4] + so;
-    const int step = FUSED ? (*a.step_base + a.step_off) : 0;
-
+    /* in place, the velocities are read through the output pointers (Vxi, Vzi unused) */
+    const R *Vxl = INPLACE ? Vxo : Vxi;
+    const R *Vzl = INPLACE ? Vzo : Vzi;
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
     const Geom g = geometry(a.pt, a.vtx[q]);
@@ -127,11 +127,19 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
     for (int i = 0; i < 6; i++) {
         const int o = fidx(q, i);
         txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
-        if (sizeof(R) == 4) { vxe[i] = Vx[o]; vze[i] = Vz[o]; } /* FP32: issue every load up front */
+        if (sizeof(R) == 4) { vxe[i] = Vxl[o]; vze[i] = Vzl[o]; } /* FP32: issue every load up front */
     }
     if (FUSED) {
 #pragma unroll
         for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, shot, step, txx[i], tzz[i], txz[i]);
+    }
+    if (save) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            tsave[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = txx[i];
+            tsave[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = tzz[i];
+            tsave[(2 * 6 + i) * DGX_BLOCK + threadIdx.x] = txz[i];
+        }
     }
 
     /* volume term: Dr(rx u + rz w) + Ds(sx u + sz w) */
@@ -150,6 +158,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
         }
     }
 
+    /* surface term, scaled by 1/D at the end */
     R flx[6], flz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
@@ -197,6 +206,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
             }
         }
     }
+
     /* V = (sponge(V) + source) + dt/rho * rhs */
     const R invD = (R)g.invD;
     const R c = a.rho_inv[q] * (R)a.dt;
@@ -206,7 +216,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
         const int o = fidx(q, i);
         R vx, vz;
         if (sizeof(R) == 4) { vx = vxe[i]; vz = vze[i]; }
-        else { vx = Vx[o]; vz = Vz[o]; } /* FP64: load late, keeping registers for occupancy */
+        else { vx = Vxl[o]; vz = Vzl[o]; } /* FP64: load late, keeping registers for occupancy */
         if (FUSED) {
             if (code & DGX_CODE_SPONGE) {
                 const R f = a.absorb[o];
@@ -222,27 +232,24 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
                 }
             }
         }
-        Vx[o] = (rhx[i] + flx[i] * invD) * c + vx;
-        Vz[o] = (rhz[i] + flz[i] * invD) * c + vz;
+        const int oo = INPLACE ? o : fidx(orow, i);
+        Vxo[oo] = (rhx[i] + flx[i] * invD) * c + vx;
+        Vzo[oo] = (rhz[i] + flz[i] * invD) * c + vz;
     }
 }
 
+/*
+ * Stress update of element q from the new velocities V (own values at row q, face neighbour j's at row
+ * nrow.j; nrow.w is the element's code word).  In place, own stresses come from Txx, Tzz, Txz and are
+ * conditioned here; in the full-step kernel (tload != nullptr) they come conditioned from LDS.  Records the
+ * receivers of q from the velocities it holds.
+ */
 template <typename R, bool FUSED>
-__global__ void __launch_bounds__(DGX_BLOCK) k_stress(StepArgs<R> a)
+__device__ __forceinline__ void stress_update(const StepArgs<R> &a, int q, int shot, int step, int4 nrow,
+                                              const R *__restrict__ Vx, const R *__restrict__ Vz, const R *tload,
+                                              R *__restrict__ Txx, R *__restrict__ Tzz, R *__restrict__ Txz)
 {
-    const int q = element_of_thread(a.nper);
-    if (q >= a.tri) return;
-    const int shot = blockIdx.y;
-    const long long so = (long long)shot * a.fstride * 6;
-    const R *__restrict__ Vx = a.f[0] + so;
-    const R *__restrict__ Vz = a.f[1] + so;
-    R *__restrict__ Txx = a.f[2] + so;
-    R *__restrict__ Tzz = a.f[3] + so;
-    R *__restrict__ Txz = a.f[4] + so;
-    const int step = FUSED ? (*a.step_base + a.step_off) : 0;
-
-    const int4 nb = a.nbr[q];
-    const unsigned code = (unsigned)nb.w;
+    const unsigned code = (unsigned)nrow.w;
     const Geom g = geometry(a.pt, a.vtx[q]);
 
     R vx[6], vz[6];
@@ -272,7 +279,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_stress(StepArgs<R> a)
 #pragma unroll
     for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
     {
-        const int nbs[3] = {nb.x, nb.y, nb.z};
+        const int nbs[3] = {nrow.x, nrow.y, nrow.z};
         R gvx[3][3], gvz[3][3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
@@ -310,13 +317,21 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_stress(StepArgs<R> a)
             }
         }
     }
+
     const R invD = (R)g.invD;
     const R lam = a.lam[q], mu = a.mu[q], l2m = lam + 2 * mu, dt = (R)a.dt;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const int o = fidx(q, i);
-        R txx = Txx[o], tzz = Tzz[o], txz = Txz[o];
-        if (FUSED) condition_stress(a, code, q, i, shot, step, txx, tzz, txz);
+        R txx, tzz, txz;
+        if (tload) {
+            txx = tload[(0 * 6 + i) * DGX_BLOCK + threadIdx.x];
+            tzz = tload[(1 * 6 + i) * DGX_BLOCK + threadIdx.x];
+            txz = tload[(2 * 6 + i) * DGX_BLOCK + threadIdx.x];
+        } else {
+            txx = Txx[o]; tzz = Tzz[o]; txz = Txz[o];
+            if (FUSED) condition_stress(a, code, q, i, shot, step, txx, tzz, txz);
+        }
         const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
         Txx[o] = (l2m * axx + lam * azz) * dt + txx;
         Tzz[o] = (lam * axx + l2m * azz) * dt + tzz;
@@ -341,6 +356,81 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_stress(StepArgs<R> a)
             }
         }
     }
+}
+
+/* The element kernels ask for at least 3 wavefronts per SIMD: this caps the FP64 kernels at 168 VGPRs
+ * (unbounded they take 170-171, which leaves room for 2), without spilling. */
+template <typename R, bool FUSED>
+__global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
+{
+    const int q = element_of_thread(a.nper);
+    if (q >= a.tri) return;
+    const int shot = blockIdx.y;
+    const long long so = (long long)shot * a.fstride * 6;
+    const int step = FUSED ? (*a.step_base + a.step_off) : 0;
+    R *V0 = a.f[0] + so, *V1 = a.f[1] + so;
+    velocity_update<R, FUSED, true>(a, q, shot, step, q, a.f[2] + so, a.f[3] + so, a.f[4] + so, nullptr, nullptr,
+                                    V0, V1, nullptr, false);
+}
+
+template <typename R, bool FUSED>
+__global__ void __launch_bounds__(DGX_BLOCK, 3) k_stress(StepArgs<R> a)
+{
+    const int q = element_of_thread(a.nper);
+    if (q >= a.tri) return;
+    const int shot = blockIdx.y;
+    const long long so = (long long)shot * a.fstride * 6;
+    const int step = FUSED ? (*a.step_base + a.step_off) : 0;
+    stress_update<R, FUSED>(a, q, shot, step, a.nbr[q], a.f[0] + so, a.f[1] + so, nullptr, a.f[2] + so,
+                            a.f[3] + so, a.f[4] + so);
+}
+
+/*
+ * One whole leap-frog step per launch (element layout), reading the fields in f and writing the step's
+ * result to the other copy g.  Workgroup b owns tile t: `own` consecutive elements (in the locality
+ * order) plus the `nring` elements outside the tile that share a face with them, together at most one
+ * element per thread.
+ *   phase 1: every thread computes the new velocity of its element (own or ring) from f, exactly as the
+ *            velocity half-step does, and writes it to g: own elements at their row, ring elements at
+ *            the tile's private rows vrow0 + (ring entry); own threads keep their conditioned stresses
+ *            in LDS;
+ *   barrier;
+ *   phase 2: own threads update their stresses from the new velocities in g (in-tile neighbours at
+ *            their rows, outside neighbours at the tile's ring rows, nb2) and write them to g.
+ * Ring velocities are recomputed by every tile that borders them, from the same inputs with the same
+ * code, so they equal the owner's values.  Since f is only read and every row of g is written by one
+ * workgroup, tiles never wait on each other.  Fields move once per step: 5 reads + 5 writes per
+ * element-node, against 7 + 8 for the two half-step kernels.
+ */
+template <typename R>
+__global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
+{
+    __shared__ R tc[18 * DGX_BLOCK];
+    const int b = blockIdx.x;
+    const int t = (b & (DGX_NXCD - 1)) * a.tper + (b >> 3);
+    if (t >= a.ntile) return; /* uniform over the workgroup */
+    const int4 td = a.tiles[t]; /* {first, own, ring begin, nring} */
+    const int lane = threadIdx.x;
+    const int shot = blockIdx.y;
+    const long long so = (long long)shot * a.fstride * 6;
+    const int step = *a.step_base + a.step_off;
+
+    int q = -1, orow = 0;
+    if (lane < td.y) {
+        q = td.x + lane;
+        orow = q;
+    } else if (lane < td.y + td.w) {
+        const int k = td.z + lane - td.y;
+        q = a.ring[k];
+        orow = a.vrow0 + k;
+    }
+    if (q >= 0)
+        velocity_update<R, true, false>(a, q, shot, step, orow, a.f[2] + so, a.f[3] + so, a.f[4] + so, a.f[0] + so,
+                                 a.f[1] + so, a.g[0] + so, a.g[1] + so, tc, lane < td.y);
+    __syncthreads();
+    if (lane < td.y)
+        stress_update<R, true>(a, q, shot, step, a.nb2[q], a.g[0] + so, a.g[1] + so, tc, a.g[2] + so,
+                               a.g[3] + so, a.g[4] + so);
 }
 
 /* ---- split entry points (unfused reference step functions) ---- */
@@ -465,6 +555,7 @@ static hipError_t launch_step_kernels(const StepArgs<R> &a, int nshots, hipStrea
     case 2: dgx::k_velocity<R, false><<<grid, block, 0, st>>>(a); break;
     case 3: dgx::k_stress<R, false><<<grid, block, 0, st>>>(a); break;
     case 4: dgx::k_absorb<R><<<grid, block, 0, st>>>(a); break;
+    case 5: dgx::k_fullstep<R><<<dim3(DGX_NXCD * a.tper, nshots), block, 0, st>>>(a); break;
     }
     return hipGetLastError();
 }

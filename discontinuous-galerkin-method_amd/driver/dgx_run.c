@@ -24,6 +24,7 @@
  *   receiver <x> <z>         one receiver (repeatable)                   [1600 1600]
  *   receiver_line <x0> <z0> <x1> <z1> <n>   n receivers on a segment
  *   precision <fp64|fp32>                                                [fp64]
+ *   step_kernel <auto|half|full>   one full-step kernel or two half-step kernels per step   [auto]
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -50,6 +51,7 @@ typedef struct {
     long nrec, rec_cap;
     double *rec_xz;
     int precision;
+    int step_kernel;
 } config;
 
 static double now(void)
@@ -123,6 +125,8 @@ static int read_config(const char *path, config *c)
             }
         } else if (!strcmp(key, "precision") && sscanf(line, "%*s %31s", kind) == 1)
             c->precision = !strcmp(kind, "fp32") ? DGX_FP32 : DGX_FP64;
+        else if (!strcmp(key, "step_kernel") && sscanf(line, "%*s %31s", kind) == 1)
+            c->step_kernel = !strcmp(kind, "half") ? DGX_STEP_HALF : !strcmp(kind, "full") ? DGX_STEP_FULL : DGX_STEP_AUTO;
         else {
             fprintf(stderr, "config: cannot parse line: %s", line);
             fclose(f);
@@ -228,7 +232,7 @@ int main(int argc, char **argv)
     if (save_pair(c.out, "All_node.txt", all_node, tri * 6, 2)) { fprintf(stderr, "cannot write to %s\n", c.out); return 1; }
 
     dgx_problem p = {tri, np, point, triangle, neighbor, lam, miu, den, absorb, dt, c.precision, 1, c.steps, 1, wav,
-                     c.nsrc, src, c.nrec, rec};
+                     c.nsrc, src, c.nrec, rec, DGX_LAYOUT_ELEMENT, 0, c.step_kernel};
     dgx_solver *s;
     CHECK(dgx_create(&p, &ops, &s));
 

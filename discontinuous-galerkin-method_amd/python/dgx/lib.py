@@ -18,6 +18,7 @@ OPERATOR_DIR = PKG_DIR / "input_files" / "precomputed_matrix"
 DGX_FP64, DGX_FP32 = 0, 1
 DGX_SRC_FORCE_Z, DGX_SRC_EXPLOSIVE = 0, 1
 DGX_LAYOUT_AUTO, DGX_LAYOUT_ELEMENT, DGX_LAYOUT_SHOT_INNER = 0, 1, 2
+DGX_STEP_AUTO, DGX_STEP_HALF, DGX_STEP_FULL = 0, 1, 2
 
 dptr = C.POINTER(C.c_double)
 lptr = C.POINTER(C.c_long)
@@ -48,7 +49,8 @@ class ProblemC(C.Structure):
                 ("neighbor", lptr), ("lam", dptr), ("miu", dptr), ("den", dptr), ("absorb_fun", dptr),
                 ("dt", C.c_double), ("precision", C.c_int), ("nshots", C.c_long), ("nsteps", C.c_long),
                 ("nwavelets", C.c_long), ("wavelet", dptr), ("nsrc", C.c_long), ("src", C.POINTER(Source)),
-                ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("layout", C.c_int), ("shot_tile", C.c_int)]
+                ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("layout", C.c_int), ("shot_tile", C.c_int),
+                ("step_kernel", C.c_int)]
 
 
 class ServerConfig(C.Structure):

@@ -149,7 +149,7 @@ class Solver:
     """Owns one dgx_solver handle (device-resident fields for ``nshots`` shots on one mesh)."""
 
     def __init__(self, setup: Setup, precision: int = L.DGX_FP64, layout: int = L.DGX_LAYOUT_AUTO,
-                 shot_tile: int = 0):
+                 shot_tile: int = 0, step_kernel: int = L.DGX_STEP_AUTO):
         self.lib = L.load()
         self.setup = setup
         m = setup.mesh
@@ -169,7 +169,7 @@ class Solver:
         p = L.ProblemC(m.tri, m.point_num, L.dp(pt), L.lp(tr), L.lp(nb), L.dp(la), L.dp(mi), L.dp(de),
                        L.dp(ab) if ab is not None else L.dptr(), setup.dt, precision, setup.nshots, setup.nsteps,
                        wv.shape[0], L.dp(wv), len(setup.sources), srcs, len(setup.receivers), recs, layout,
-                       shot_tile)
+                       shot_tile, step_kernel)
         self.handle = C.c_void_p()
         L.check(self.lib.dgx_create(C.byref(p), C.byref(setup.ops), C.byref(self.handle)), "dgx_create")
         self.precision = precision
@@ -230,8 +230,8 @@ class Solver:
         return x[:, :r], z[:, :r]
 
     def kernel_bench(self, which: int, n: int) -> float:
-        """Average ms of one fused kernel (0 velocity, 1 stress) over n back-to-back launches.
-        Leaves the fields invalid."""
+        """Average ms of one fused kernel (0 velocity, 1 stress, 2 full step) over n back-to-back
+        launches.  Leaves the fields invalid."""
         ms = C.c_double()
         L.check(self.lib.dgx_kernel_bench(self.handle, which, n, C.byref(ms)), "dgx_kernel_bench")
         return ms.value

@@ -33,6 +33,13 @@ enum dgx_source_kind { DGX_SRC_FORCE_Z = 0, DGX_SRC_EXPLOSIVE = 1 };
  * single-shot path).  SHOT_INNER: shots innermost, lanes over shots, for many shots on one mesh.
  * AUTO picks SHOT_INNER from 8 shots on. */
 enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_INNER = 2 };
+/* How dgx_step runs a step.  HALF: two kernels per step (velocity, then stress), each updating the
+ * fields in place.  FULL (element layout only): one kernel per step that reads the fields and writes the
+ * next state to a second copy; workgroups recompute the velocities of the elements bordering their tile
+ * instead of waiting for them, so each field is read and written once per step.  It gives the same
+ * results; on MI355X it is slower than HALF (the kernels wait on memory latency, not bandwidth, and the
+ * tile's two phases lengthen that wait).  AUTO: HALF. */
+enum dgx_step_kernel { DGX_STEP_AUTO = 0, DGX_STEP_HALF = 1, DGX_STEP_FULL = 2 };
 
 /* Reference-element operators as read from input_files/precomputed_matrix/ (row-major), the arrays
  * DG_leap_Frog_OpenMP.c:52-81 reads.  Vb/inverVb are read by the reference but never used. */
@@ -82,6 +89,7 @@ typedef struct {
     const dgx_receiver *rec; /* one receiver spread shared by all shots */
     int layout;            /* enum dgx_layout */
     int shot_tile;         /* shots per lane group in the shot-innermost layout; 0 = autotune at create */
+    int step_kernel;       /* enum dgx_step_kernel */
 } dgx_problem;
 
 typedef struct dgx_solver dgx_solver;
@@ -93,7 +101,7 @@ int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_solver **out)
 int dgx_destroy(dgx_solver *s);
 
 /* Advance n whole steps: absorb -> source -> velocity -> stress -> record, one step being the body of
- * DG_leap_Frog_OpenMP.c:405-425.  Fused into two kernels per step. */
+ * DG_leap_Frog_OpenMP.c:405-425.  One or two kernels per step (dgx_problem.step_kernel). */
 int dgx_step(dgx_solver *s, long n);
 
 /* Split entry points, one per reference step function (same order, same semantics).  They are the
@@ -115,8 +123,9 @@ long dgx_current_step(const dgx_solver *s);
 /* Average device time of the two fused kernels over n steps, measured with HIP events on the solver's
  * stream (the steps are real steps: the state advances). */
 int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stress);
-/* Average duration of one fused kernel (which: 0 velocity, 1 stress) over n back-to-back launches
- * between two HIP events on the solver's stream.  Invalidates the fields (the launches repeat a step). */
+/* Average duration of one fused kernel (which: 0 velocity, 1 stress, 2 full step) over n back-to-back
+ * launches between two HIP events on the solver's stream.  Invalidates the fields (the launches repeat
+ * a step).  which = 2 needs the full-step kernel (element layout). */
 int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms);
 /* Streaming copy of `bytes` bytes (read + write) repeated `reps` times on the default stream: the
  * achievable-bandwidth reference and the calibration point of the HBM byte counters.  variant 0:

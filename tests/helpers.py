@@ -66,8 +66,9 @@ def product_setup(w: dgx.configs.Workload, nshots=None) -> dgx.Setup:
     return dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge, nshots=nshots)
 
 
-def product_run(setup: dgx.Setup, precision=L.DGX_FP64, nsteps=None, snap_every=0, shot=0, chunk=None):
-    s = dgx.Solver(setup, precision)
+def product_run(setup: dgx.Setup, precision=L.DGX_FP64, nsteps=None, snap_every=0, shot=0, chunk=None,
+                step_kernel=L.DGX_STEP_AUTO):
+    s = dgx.Solver(setup, precision, step_kernel=step_kernel)
     n = nsteps or setup.nsteps
     snaps = {}
     step = snap_every or chunk or n

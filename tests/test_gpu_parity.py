@@ -113,6 +113,25 @@ def test_graph_replay_equals_eager():
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("precision,tol", [(L.DGX_FP64, 1e-13), (L.DGX_FP32, 1e-5)])
+def test_full_step_kernel_equals_half_step_kernels(precision, tol):
+    """One full-step kernel per step (two field copies, ring velocities recomputed per tile) against the
+    two in-place half-step kernels: traces and every snapshot, at step counts that leave the state in
+    either copy and mix eager steps with graph replays."""
+    w = two_media_workload(141)
+    st = product_setup(w)
+    print(f"case two_media 40x40 full vs half, precision {precision}", flush=True)
+    full = product_run(st, precision, snap_every=47, step_kernel=L.DGX_STEP_FULL)
+    half = product_run(st, precision, snap_every=47, step_kernel=L.DGX_STEP_HALF)
+    for a, b in ((full[0], half[0]), (full[1], half[1])):
+        assert rel_l2(a, b) <= tol
+    assert sorted(full[2]) == [47, 94, 141]
+    for k in full[2]:
+        for c in range(5):
+            e = rel_l2(full[2][k][c], half[2][k][c])
+            assert e <= tol, (k, c, e)
+
+
 def test_shots_are_independent():
     w = two_media_workload(200)
     w.sources = [dgx.SourceSpec(1200.0, 1200.0, L.DGX_SRC_FORCE_Z, 0, 15.0),

@@ -1,0 +1,99 @@
+"""Full-step kernel tiling (CPU): every index the kernel reads or writes stays in its arrays, and the tile
+plus ring of every workgroup is exactly what its stress phase needs.
+
+The full-step kernel gives each workgroup a run of consecutive elements (own) and the elements outside the
+run that share a face with it (ring), at most one per thread; the new velocities of ring elements go to
+the tile's private rows vrow0 + ring entry, and the stress phase finds every face neighbour's new
+velocity through nb2.  These checks run on the host-side tiling the solver uploads."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import dgx
+from dgx import lib as L
+from dgx import unstructured as U
+
+BLOCK = 256
+
+
+def tiling(mesh):
+    lib = L.load()
+    f = lib.dgx_full_step_tiling
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(L.ProblemC)] + [C.POINTER(C.c_long)] * 4 + [C.c_void_p] * 4
+    pt = np.ascontiguousarray(mesh.point, np.float64)
+    tr = np.ascontiguousarray(mesh.triangle, np.int64)
+    nb = np.ascontiguousarray(mesh.neighbor, np.int64)
+    T = mesh.tri
+    ones = np.ones(T)
+    p = L.ProblemC(T, mesh.point_num, L.dp(pt), L.lp(tr), L.lp(nb), L.dp(ones), L.dp(ones), L.dp(ones),
+                   L.dptr(), 1.0, L.DGX_FP64, 1, 1, 0, L.dptr(), 0, None, 0, None, L.DGX_LAYOUT_ELEMENT, 0,
+                   L.DGX_STEP_FULL)
+    nt, nr, v0, vr = C.c_long(), C.c_long(), C.c_long(), C.c_long()
+    assert f(C.byref(p), C.byref(nt), C.byref(nr), C.byref(v0), C.byref(vr), None, None, None, None) == 0
+    tiles = np.zeros((nt.value, 4), np.int32)
+    ring = np.zeros(max(1, nr.value), np.int32)
+    nb2 = np.zeros((T, 4), np.int32)
+    nbr = np.zeros((T, 4), np.int32)
+    assert f(C.byref(p), C.byref(nt), C.byref(nr), C.byref(v0), C.byref(vr), tiles.ctypes.data,
+             ring.ctypes.data, nb2.ctypes.data, nbr.ctypes.data) == 0
+    return tiles, ring[:nr.value], nb2, nbr, v0.value, vr.value
+
+
+def check_tiling(mesh):
+    tiles, ring, nb2, nbr, vrow0, vrows = tiling(mesh)
+    T = mesh.tri
+    # tiles cover 0..T-1 in order, each at most one element per thread
+    assert tiles[0, 0] == 0 and np.all(tiles[1:, 0] == tiles[:-1, 0] + tiles[:-1, 1])
+    assert tiles[-1, 0] + tiles[-1, 1] == T
+    assert np.all(tiles[:, 1] >= 1) and np.all(tiles[:, 1] + tiles[:, 3] <= BLOCK)
+    # ring entries: consecutive ranges, in bounds, rows inside the velocity arrays
+    assert np.all(tiles[1:, 2] == tiles[:-1, 2] + tiles[:-1, 3]) and tiles[-1, 2] + tiles[-1, 3] == len(ring)
+    assert np.all((ring >= 0) & (ring < T))
+    assert vrow0 >= T and vrow0 + len(ring) <= vrows and vrow0 % 64 == 0 and vrows % 64 == 0
+    for t in range(len(tiles)):
+        first, own, rb, nr = tiles[t]
+        members = set(range(first, first + own))
+        rng = ring[rb:rb + nr]
+        assert len(set(rng.tolist())) == nr and not (set(rng.tolist()) & members)
+        outside = set()
+        for q in range(first, first + own):
+            for j in range(3):
+                e = nbr[q, j]
+                if e < 0:
+                    assert nb2[q, j] == -1
+                elif e in members:
+                    assert nb2[q, j] == e
+                else:
+                    outside.add(e)
+                    k = nb2[q, j] - vrow0
+                    assert rb <= k < rb + nr and ring[k] == e
+            assert nb2[q, 3] == nbr[q, 3]
+        assert outside == set(rng.tolist())  # the ring is exactly the outside face neighbours
+    return len(ring) / T
+
+
+def test_tiling_structured_reference_mesh():
+    w = dgx.configs.homogeneous_2km(10)
+    overhead = check_tiling(w.mesh)
+    assert overhead < 0.35
+
+
+def test_tiling_jittered_random_diagonals():
+    w = dgx.configs.gradient_8x4km(10, nx=120, nz=60)
+    overhead = check_tiling(w.mesh)
+    assert overhead < 0.35
+
+
+def test_tiling_unstructured_delaunay():
+    pts, _ = U.layered_point_set(1200.0, 600.0, 12.0, [lambda x: 300.0 + 0.2 * x], seed=4)
+    mesh = U.triangulate(pts)
+    check_tiling(mesh)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_tiling_tiny_meshes(n):
+    mesh = dgx.mesh.structured(n, 1, 10.0)
+    tiles, ring, nb2, nbr, vrow0, vrows = tiling(mesh)
+    assert len(tiles) == 1 and tiles[0, 1] == mesh.tri and len(ring) == 0
