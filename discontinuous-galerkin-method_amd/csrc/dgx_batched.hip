@@ -86,7 +86,7 @@ __device__ __forceinline__ int shot_step(const StepArgs<R> &a, int s, int gstep)
 template <typename R>
 __device__ __forceinline__ unsigned aux_of(const StepArgs<R> &a, int q, unsigned code)
 {
-    return a.aux_of_elem ? (unsigned)(a.aux_of_elem[q] + 1) : (code >> DGX_CODE_AUX_SHIFT);
+    return a.aux_of_elem ? ((unsigned)a.aux_of_elem[q] & DGX_DYN_AUX_MASK) : (code >> DGX_CODE_AUX_SHIFT);
 }
 
 template <typename R>
@@ -179,11 +179,14 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
                 gxx[j][k] = Txx[o]; gzz[j][k] = Tzz[o]; gxz[j][k] = Txz[o];
             }
         }
+        /* faces whose neighbour needs its stresses conditioned: sponge (static, in the code word) or,
+         * when serving, an explosive source of this batch (the element's own word) */
+        const unsigned dyn = a.aux_of_elem ? ((unsigned)a.aux_of_elem[q] >> DGX_DYN_NB_SHIFT) : 0u;
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const int e = nbs[j];
             if (e < 0) continue;
-            const bool special = (code & DGX_CODE_NBSPECIAL(j)) != 0 || (a.aux_of_elem && a.aux_of_elem[e] >= 0);
+            const bool special = (code & DGX_CODE_NBSPECIAL(j)) != 0 || ((dyn >> j) & 1u);
             if (!special) continue;
             const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
             const unsigned ncode = (unsigned)a.nbr[e].w;

@@ -23,6 +23,9 @@
 #define DGX_CODE_SPONGE (1u << 12)
 #define DGX_CODE_AUX_SHIFT 13
 #define DGX_MAX_AUX ((1u << 19) - 2)
+/* serving: per-element word of a batch (StepArgs::aux_of_elem) */
+#define DGX_DYN_AUX_MASK ((1u << 29) - 1)
+#define DGX_DYN_NB_SHIFT 29
 
 /* Aux record of one element: its sources and receivers as index ranges. */
 typedef struct {
@@ -82,7 +85,9 @@ struct StepArgs {
     const int *shot_start; /* optional per-shot first global step (NULL: all shots start at 0) */
     const int *shot_len;   /* optional per-shot step count */
     /* serving only (NULL otherwise): sources/receivers change as requests come and go, so they are
-     * indexed per element by aux_of_elem (-1: none) instead of the code word, every receiver record
+     * indexed per element by aux_of_elem instead of the code word (bits 0..28: aux record index + 1,
+     * 0 = none; bit 29+j: the neighbour across face j has an explosive source of this batch, so its
+     * stresses need conditioning when gathered), every receiver record
      * names its slot (dgx_rec_dev.pad0), and each slot has its own wavelet and trace buffers */
     const int *aux_of_elem;
     const double *const *slot_wav;

@@ -44,6 +44,7 @@ struct MeshDev {
     long tri = 0;
     double dt = 0;
     std::vector<long> inv; /* caller element -> device element */
+    std::vector<int4> nbr_host; /* device-order neighbours (finding the faces next to a source) */
     int4 *nbr = nullptr, *vtx = nullptr;
     double2 *pt = nullptr;
     void *mat = nullptr, *absorb = nullptr;
@@ -77,7 +78,7 @@ struct Batch {
     int *d_slot_int = nullptr;      /* start[16] | len[16] | nrec[16] */
     double **d_slot_ptr = nullptr;  /* wav[16] | trace[16] */
     int *d_aux_of_elem = nullptr;
-    std::vector<int> aux_elems;     /* device elements that currently have an aux record */
+    std::vector<int> aux_elems;     /* device elements whose per-element word is currently nonzero */
     dgx_aux *d_aux = nullptr;
     dgx_src_dev *d_src = nullptr;
     dgx_rec_dev *d_rec = nullptr;
@@ -262,6 +263,7 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     m.tri = p->tri;
     m.dt = p->dt;
     m.inv = mh.inv;
+    m.nbr_host.assign(mh.nbr.begin(), mh.nbr.begin() + mh.tri);
     SV_TRY(upload_vec(&m.nbr, mh.nbr));
     SV_TRY(upload_vec(&m.vtx, mh.vtx));
     SV_TRY(upload_vec(&m.pt, mh.pt));
@@ -342,7 +344,7 @@ static int new_batch(dgx_server *sv, int mesh, int *bi)
     SV_TRY(hipMalloc((void **)&b.d_slot_int, sizeof(int) * 3 * GROUP));
     SV_TRY(hipMalloc((void **)&b.d_slot_ptr, sizeof(double *) * 2 * GROUP));
     SV_TRY(hipMalloc((void **)&b.d_aux_of_elem, sizeof(int) * m.tri));
-    SV_TRY(hipMemsetAsync(b.d_aux_of_elem, 0xff, sizeof(int) * m.tri, sv->stream)); /* all -1 */
+    SV_TRY(hipMemsetAsync(b.d_aux_of_elem, 0, sizeof(int) * m.tri, sv->stream)); /* no aux, no source */
     sv->in_use += b.field_bytes + 4 * m.tri;
     sv->batches.push_back(std::move(b));
     *bi = (int)sv->batches.size() - 1;
@@ -392,9 +394,26 @@ static int refresh_batch(dgx_server *sv, Batch &b)
         aux.push_back(x);
         elems.push_back(kv.first);
     }
+    /* per-element words (aux record + 1, neighbour-source bits); every element whose word changes gets
+     * exactly one pair, so the scatter has no two writes to one address */
+    std::map<int, unsigned> word;
+    for (size_t x = 0; x < elems.size(); x++) word[elems[x]] |= (unsigned)(x + 1);
+    for (const dgx_src_dev &d : src) {
+        if (d.kind != DGX_SRC_EXPLOSIVE) continue;
+        const int4 ne = m.nbr_host[d.elem];
+        for (int n : {ne.x, ne.y, ne.z}) {
+            if (n < 0) continue;
+            const int4 nn = m.nbr_host[n];
+            const int back[3] = {nn.x, nn.y, nn.z};
+            for (int j = 0; j < 3; j++)
+                if (back[j] == d.elem) word[n] |= 1u << (DGX_DYN_NB_SHIFT + j);
+        }
+    }
+    std::map<int, int> fin;
+    for (int q : b.aux_elems) fin[q] = 0;
+    for (auto &kv : word) fin[kv.first] = (int)kv.second;
     std::vector<int2> pairs;
-    for (int q : b.aux_elems) pairs.push_back(make_int2(q, -1));
-    for (size_t x = 0; x < elems.size(); x++) pairs.push_back(make_int2(elems[x], (int)x));
+    for (auto &kv : fin) pairs.push_back(make_int2(kv.first, kv.second));
     long old_a = b.aux_cap, old_s = b.src_cap, old_r = b.rec_cap;
     SV_TRY(grow(&b.d_aux, &b.aux_cap, (long)aux.size()));
     SV_TRY(grow(&b.d_src, &b.src_cap, (long)src.size()));
@@ -408,7 +427,8 @@ static int refresh_batch(dgx_server *sv, Batch &b)
         SV_TRY(hipMemcpyAsync(b.d_pairs, pairs.data(), sizeof(int2) * pairs.size(), hipMemcpyHostToDevice, sv->stream));
         SV_TRY(dgx_launch_scatter(b.d_aux_of_elem, b.d_pairs, (int)pairs.size(), sv->stream));
     }
-    b.aux_elems = elems;
+    b.aux_elems.clear();
+    for (auto &kv : word) b.aux_elems.push_back(kv.first);
     int si[3 * GROUP];
     double *sp[2 * GROUP];
     for (int k = 0; k < GROUP; k++) {
