@@ -12,6 +12,7 @@
  * the two layouts agree to rounding; the tests compare them.
  */
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 #include "dgx_kernels.h"
 
@@ -19,6 +20,17 @@ namespace dgxb {
 
 using dgx::Ops;
 using dgx::StepArgs;
+
+/* Pointers read from a device-side table (the serving batch table, the per-slot wavelet and trace
+ * tables) are generic, so every access through them would be a flat memory instruction.  They all point
+ * to device memory (hipMalloc), where a global pointer has the same value as the generic one: rebuilding
+ * each from its address as a global pointer lets the compiler use global instructions for every access
+ * derived from it. */
+template <typename T>
+__device__ __forceinline__ T *as_global(T *p)
+{
+    return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
 
 __device__ __forceinline__ constexpr int face_node(int j, int k)
 {
@@ -92,7 +104,7 @@ __device__ __forceinline__ unsigned aux_of(const StepArgs<R> &a, int q, unsigned
 template <typename R>
 __device__ __forceinline__ double wavelet_at(const StepArgs<R> &a, const dgx_src_dev &src, int s, int lstep)
 {
-    return a.slot_wav ? a.slot_wav[s][lstep] : a.wav[(long long)src.wavelet * a.nsteps + lstep];
+    return a.slot_wav ? as_global(a.slot_wav[s])[lstep] : a.wav[(long long)src.wavelet * a.nsteps + lstep];
 }
 
 template <typename R>
@@ -341,7 +353,7 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
                 sz_ += rc.w[i] * (double)vz[i];
             }
             const int nrec = a.slot_trace ? a.slot_nrec[s] : a.nrec;
-            double *t = a.slot_trace ? a.slot_trace[s] + (long long)lstep * 2 * nrec
+            double *t = a.slot_trace ? as_global(a.slot_trace[s]) + (long long)lstep * 2 * nrec
                                      : a.trace + ((long long)s * a.nsteps + lstep) * 2 * nrec;
             t[rc.index] = sx_;
             t[nrec + rc.index] = sz_;
@@ -365,6 +377,30 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
     stress_body(a.op, a, q, s, *a.step_base + a.step_off);
 }
 
+template <typename R>
+__device__ __forceinline__ StepArgs<R> global_args(const StepArgs<R> &t)
+{
+    StepArgs<R> a = t;
+    for (int c = 0; c < 5; c++) a.f[c] = as_global(a.f[c]);
+    a.nbr = as_global(a.nbr);
+    a.vtx = as_global(a.vtx);
+    a.pt = as_global(a.pt);
+    a.rho_inv = as_global(a.rho_inv);
+    a.lam = as_global(a.lam);
+    a.mu = as_global(a.mu);
+    a.absorb = as_global(a.absorb);
+    a.aux = as_global(a.aux);
+    a.src = as_global(a.src);
+    a.rec = as_global(a.rec);
+    a.shot_start = as_global(a.shot_start);
+    a.shot_len = as_global(a.shot_len);
+    a.aux_of_elem = as_global(a.aux_of_elem);
+    a.slot_wav = as_global(a.slot_wav);
+    a.slot_trace = as_global(a.slot_trace);
+    a.slot_nrec = as_global(a.slot_nrec);
+    return a;
+}
+
 /*
  * Serving: one launch per half-step over every active (mesh, slot group) batch.  sched = {nbatch,
  * total tiles, tile offset of batch 0, ..., tile offset of batch nbatch}; a tile is 16 elements x 16
@@ -386,7 +422,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const Step
     int b = 0;
     for (int tile = xcd * chunk + (int)(blockIdx.x >> 3); tile < t_end; tile += per_xcd) {
         while (b + 1 < nb && sched[2 + b + 1] <= tile) b++;
-        const StepArgs<R> &a = table[b];
+        const StepArgs<R> a = global_args(table[b]);
         const int q = (tile - sched[2 + b]) * 16 + wave * 4 + (lane >> 4), s = lane & 15;
         if (q >= a.tri) continue;
         /* a slot that is not running (free, or its request has not started / has finished) is skipped:

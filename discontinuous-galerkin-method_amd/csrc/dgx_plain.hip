@@ -100,6 +100,35 @@ __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned 
     }
 }
 
+/* condition_stress for all 6 nodes of element q: one sponge branch and one source branch per element */
+template <typename R>
+__device__ __forceinline__ void condition_stress6(const StepArgs<R> &a, unsigned code, int q, int shot, int step,
+                                                  R *txx, R *tzz, R *txz)
+{
+    if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const R f = a.absorb[fidx(q, i)];
+            txx[i] *= f; tzz[i] *= f; txz[i] *= f;
+        }
+    }
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (ax) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+            const dgx_src_dev &src = a.src[s];
+            if (src.shot != shot || src.kind != 1) continue;
+            const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const R inc = (R)(src.fun[i] * w);
+                txx[i] += inc;
+                tzz[i] += inc;
+            }
+        }
+    }
+}
+
 /*
  * Velocity update of element q (shot `shot`) from stresses T and velocities Vi, written to row `orow`
  * of Vo.  In place (Vi == Vo, orow == q) for the half-step kernel; out of place for the full-step
@@ -130,8 +159,14 @@ __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int
         if (sizeof(R) == 4) { vxe[i] = Vxl[o]; vze[i] = Vzl[o]; } /* FP32: issue every load up front */
     }
     if (FUSED) {
+        /* per element (one branch each for sponge and sources) in FP32; per node in FP64, where the
+         * per-element form needs more registers than the 168 that keep 3 wavefronts per SIMD */
+        if constexpr (sizeof(R) == 4) {
+            condition_stress6(a, code, q, shot, step, txx, tzz, txz);
+        } else {
 #pragma unroll
-        for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, shot, step, txx[i], tzz[i], txz[i]);
+            for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, shot, step, txx[i], tzz[i], txz[i]);
+        }
     }
     if (save) {
 #pragma unroll
@@ -211,30 +246,59 @@ __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int
     const R invD = (R)g.invD;
     const R c = a.rho_inv[q] * (R)a.dt;
     const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int o = fidx(q, i);
-        R vx, vz;
-        if (sizeof(R) == 4) { vx = vxe[i]; vz = vze[i]; }
-        else { vx = Vxl[o]; vz = Vzl[o]; } /* FP64: load late, keeping registers for occupancy */
+    if constexpr (sizeof(R) == 4) {
+        /* FP32: velocities loaded up front, conditioned per element */
         if (FUSED) {
             if (code & DGX_CODE_SPONGE) {
-                const R f = a.absorb[o];
-                vx *= f;
-                vz *= f;
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    const R f = a.absorb[fidx(q, i)];
+                    vxe[i] *= f;
+                    vze[i] *= f;
+                }
             }
             if (ax) {
                 const dgx_aux x = a.aux[ax - 1];
                 for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
                     const dgx_src_dev &src = a.src[s];
-                    if (src.shot == shot && src.kind == 0)
-                        vz += (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + step]);
+                    if (src.shot != shot || src.kind != 0) continue;
+                    const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
+#pragma unroll
+                    for (int i = 0; i < 6; i++) vze[i] += (R)(src.fun[i] * w);
                 }
             }
         }
-        const int oo = INPLACE ? o : fidx(orow, i);
-        Vxo[oo] = (rhx[i] + flx[i] * invD) * c + vx;
-        Vzo[oo] = (rhz[i] + flz[i] * invD) * c + vz;
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const int oo = INPLACE ? fidx(q, i) : fidx(orow, i);
+            Vxo[oo] = (rhx[i] + flx[i] * invD) * c + vxe[i];
+            Vzo[oo] = (rhz[i] + flz[i] * invD) * c + vze[i];
+        }
+    } else {
+        /* FP64: velocities loaded late (registers for occupancy), conditioned per node */
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const int o = fidx(q, i);
+            R vx = Vxl[o], vz = Vzl[o];
+            if (FUSED) {
+                if (code & DGX_CODE_SPONGE) {
+                    const R f = a.absorb[o];
+                    vx *= f;
+                    vz *= f;
+                }
+                if (ax) {
+                    const dgx_aux x = a.aux[ax - 1];
+                    for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+                        const dgx_src_dev &src = a.src[s];
+                        if (src.shot == shot && src.kind == 0)
+                            vz += (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + step]);
+                    }
+                }
+            }
+            const int oo = INPLACE ? o : fidx(orow, i);
+            Vxo[oo] = (rhx[i] + flx[i] * invD) * c + vx;
+            Vzo[oo] = (rhz[i] + flz[i] * invD) * c + vz;
+        }
     }
 }
 
