@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the seeded serving stream to run (0 = skip; 2000 = the whole stream)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-group", type=int, default=64, help="shots per slot group of the server (16 or 64)")
     ap.add_argument("--copy-probe", type=int, default=1,
                     help="also time a plain device copy (achievable bandwidth, counter calibration)")
     return ap.parse_args()
@@ -123,7 +124,7 @@ def serving_leg(a, rate64):
     ms = dgx.configs.serving_meshes()
     arrivals, specs = dgx.configs.serving_requests(ms, n=max(a.serve_requests, 1), rate=a.serve_rate)
     arrivals, specs = arrivals[:a.serve_requests], specs[:a.serve_requests]
-    sv = SV.Server(L.DGX_FP32, 0, 100)
+    sv = SV.Server(L.DGX_FP32, 0, 100, group_shots=a.serve_group)
     rms = [sv.add_mesh(m, md, sp) for m, md, sp in ms]
     reqs = [sv.make_request(rms[r["mesh"]], r["xz"], [r["f0"]] * len(r["xz"]), r["nsteps"], r["rec"])
             for r in specs]
@@ -131,6 +132,7 @@ def serving_leg(a, rate64):
     sv.close()
     dof = st["dof_updates"]
     return {"requests": len(reqs), "arrival_rate_per_s": a.serve_rate, "meshes": [m.tri for m, _, _ in ms],
+            "group_shots": a.serve_group,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "wall_s": wall, "dof_updates": dof,
             "value": dof / wall, "unit": "DOF-updates/s", "dtype": "f32",
@@ -173,7 +175,16 @@ def cpu_baseline(w, setup, budget_s):
     n = max(3, min(2000, int(budget_s / max(t1 / 2, 1e-6))))
     sec = timed(n)
     rate = lay.tri * 30 * n / sec
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": rate, "unit": "DOF-updates/s", "cores": threads, "kind": kind,
+            "host": f"{model}, {os.cpu_count()} logical CPUs visible",
             "sample": f"{n} full steps of the same 640k-element mesh (reference step loop, -O2 -fopenmp, "
                       f"{threads} threads), {sec:.1f} s"}
 

@@ -128,6 +128,35 @@ __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned 
     }
 }
 
+/* condition_stress for all 6 nodes: one sponge branch and one source branch per element */
+template <typename R>
+__device__ __forceinline__ void condition_stress6(const StepArgs<R> &a, unsigned code, int q, int s, int lstep,
+                                                  R *txx, R *tzz, R *txz)
+{
+    if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const R f = a.absorb[q * 6 + i];
+            txx[i] *= f; tzz[i] *= f; txz[i] *= f;
+        }
+    }
+    const unsigned ax = aux_of(a, q, code);
+    if (ax && lstep >= 0) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
+            const dgx_src_dev &src = a.src[k];
+            if (src.shot != s || src.kind != 1) continue;
+            const double w = wavelet_at(a, src, s, lstep);
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const R inc = (R)(src.fun[i] * w);
+                txx[i] += inc;
+                tzz[i] += inc;
+            }
+        }
+    }
+}
+
 template <typename R>
 __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int gstep)
 {
@@ -152,8 +181,12 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
         txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
         if (sizeof(R) == 4) { vxe[i] = Vx[o]; vze[i] = Vz[o]; }
     }
+    if constexpr (sizeof(R) == 4) {
+        condition_stress6(a, code, q, s, lstep, txx, tzz, txz); /* FP32: per element */
+    } else {
 #pragma unroll
-    for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, s, lstep, txx[i], tzz[i], txz[i]);
+        for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, s, lstep, txx[i], tzz[i], txz[i]);
+    }
 
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
     R ux[6], wx[6], uz[6], wz[6];
@@ -227,6 +260,36 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
     const R invD = (R)g.invD;
     const R c = a.rho_inv[q] * (R)a.dt;
     const unsigned ax = aux_of(a, q, code);
+    if constexpr (sizeof(R) == 4) {
+        /* FP32: velocities already loaded; sponge and sources applied per element (one branch each),
+         * which measures faster than per node; FP64 keeps the per-node form below, which needs fewer
+         * registers */
+        if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const R f = a.absorb[q * 6 + i];
+                vxe[i] *= f;
+                vze[i] *= f;
+            }
+        }
+        if (ax && lstep >= 0) {
+            const dgx_aux x = a.aux[ax - 1];
+            for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
+                const dgx_src_dev &src = a.src[k];
+                if (src.shot != s || src.kind != 0) continue;
+                const double w = wavelet_at(a, src, s, lstep);
+#pragma unroll
+                for (int i = 0; i < 6; i++) vze[i] += (R)(src.fun[i] * w);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const long long o = bidx(a, q, i, s);
+            Vx[o] = (rhx[i] + flx[i] * invD) * c + vxe[i];
+            Vz[o] = (rhz[i] + flz[i] * invD) * c + vze[i];
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const long long o = bidx(a, q, i, s);
@@ -330,6 +393,24 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
     }
     const R invD = (R)g.invD;
     const R lam = a.lam[q], mu = a.mu[q], l2m = lam + 2 * mu, dt = (R)a.dt;
+    if constexpr (sizeof(R) == 4) {
+        /* FP32: own stresses loaded together and conditioned per element */
+        R txx[6], tzz[6], txz[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const long long o = bidx(a, q, i, s);
+            txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
+        }
+        condition_stress6(a, code, q, s, lstep, txx, tzz, txz);
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const long long o = bidx(a, q, i, s);
+            const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
+            Txx[o] = (l2m * axx + lam * azz) * dt + txx[i];
+            Tzz[o] = (lam * axx + l2m * azz) * dt + tzz[i];
+            Txz[o] = (mu * axz) * dt + txz[i];
+        }
+    } else
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const long long o = bidx(a, q, i, s);
@@ -403,16 +484,17 @@ __device__ __forceinline__ StepArgs<R> global_args(const StepArgs<R> &t)
 
 /*
  * Serving: one launch per half-step over every active (mesh, slot group) batch.  sched = {nbatch,
- * total tiles, tile offset of batch 0, ..., tile offset of batch nbatch}; a tile is 16 elements x 16
- * shots (4 elements per wavefront).  The grid is fixed (a persistent grid-stride loop), so a captured
- * step graph stays valid while batches are added and retired: only the table and sched change.  The
- * tiles are dealt in 8 contiguous ranges, one per XCD.
+ * total tiles, tile offset of batch 0, ..., tile offset of batch nbatch}; a tile is 256/G elements x G
+ * shots (G = slots per group, 16 or 64: 64/G elements per wavefront).  The grid is fixed (a persistent
+ * grid-stride loop), so a captured step graph stays valid while batches are added and retired: only the
+ * table and sched change.  The tiles are dealt in 8 contiguous ranges, one per XCD.
  */
-template <typename R, bool VEL>
+template <typename R, bool VEL, int G>
 __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const StepArgs<R> *__restrict__ table,
                                                         const int *__restrict__ sched, const int *step_base,
                                                         int step_off)
 {
+    constexpr int EW = 64 / G; /* elements per wavefront */
     const int nb = sched[0], total = sched[1];
     const int gstep = *step_base + step_off;
     const int xcd = blockIdx.x & (DGX_NXCD - 1), per_xcd = gridDim.x / DGX_NXCD;
@@ -423,7 +505,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const Step
     for (int tile = xcd * chunk + (int)(blockIdx.x >> 3); tile < t_end; tile += per_xcd) {
         while (b + 1 < nb && sched[2 + b + 1] <= tile) b++;
         const StepArgs<R> a = global_args(table[b]);
-        const int q = (tile - sched[2 + b]) * 16 + wave * 4 + (lane >> 4), s = lane & 15;
+        const int q = (tile - sched[2 + b]) * (4 * EW) + wave * EW + lane / G, s = lane % G;
         if (q >= a.tri) continue;
         /* a slot that is not running (free, or its request has not started / has finished) is skipped:
          * its fields are never read again before the slot is cleared for a new request */
@@ -440,13 +522,13 @@ __global__ void k_scatter_int(int *dst, const int2 *pairs, int n)
     if (k < n) dst[pairs[k].x] = pairs[k].y;
 }
 
-/* zero the field columns of the slots in `mask` (a 16-slot group) before they are reused */
+/* zero the field columns of the slots in `mask` (of a G-slot group) before they are reused */
 template <typename R>
-__global__ void k_clear_slots(R *f0, R *f1, R *f2, R *f3, R *f4, long long n_nodes, unsigned mask)
+__global__ void k_clear_slots(R *f0, R *f1, R *f2, R *f3, R *f4, long long n_nodes, int G, unsigned long long mask)
 {
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n_nodes * 16;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n_nodes * G;
          k += (long long)gridDim.x * blockDim.x) {
-        if (!((mask >> (k & 15)) & 1u)) continue;
+        if (!((mask >> (k % G)) & 1ull)) continue;
         f0[k] = 0; f1[k] = 0; f2[k] = 0; f3[k] = 0; f4[k] = 0;
     }
 }
@@ -475,24 +557,32 @@ static hipError_t dispatch_b(const dgx::StepArgs<R> &a, hipStream_t st, int whic
     return hipErrorInvalidValue;
 }
 
+template <typename R>
+static hipError_t launch_serve(const void *ops, const void *table, const int *sched, const int *step_base,
+                               int step_off, int vel, int group, int grid, hipStream_t st)
+{
+    const dgx::Ops<R> &op = *(const dgx::Ops<R> *)ops;
+    const dgx::StepArgs<R> *t = (const dgx::StepArgs<R> *)table;
+    if (group == 16) {
+        if (vel) dgxb::ks_step<R, true, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
+        else dgxb::ks_step<R, false, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
+    } else if (group == 64) {
+        if (vel) dgxb::ks_step<R, true, 64><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
+        else dgxb::ks_step<R, false, 64><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 extern "C" {
 hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched, const int *step_base,
-                            int step_off, int vel, int grid, hipStream_t st)
+                            int step_off, int vel, int group, int grid, hipStream_t st)
 {
     /* the operators are the same for every mesh: they travel as a kernel argument (scalar registers),
      * not through the batch table */
-    if (fp32) {
-        const dgx::Ops<float> &op = *(const dgx::Ops<float> *)ops;
-        const dgx::StepArgs<float> *t = (const dgx::StepArgs<float> *)table;
-        if (vel) dgxb::ks_step<float, true><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
-        else dgxb::ks_step<float, false><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
-    } else {
-        const dgx::Ops<double> &op = *(const dgx::Ops<double> *)ops;
-        const dgx::StepArgs<double> *t = (const dgx::StepArgs<double> *)table;
-        if (vel) dgxb::ks_step<double, true><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
-        else dgxb::ks_step<double, false><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
-    }
-    return hipGetLastError();
+    if (fp32) return launch_serve<float>(ops, table, sched, step_base, step_off, vel, group, grid, st);
+    return launch_serve<double>(ops, table, sched, step_base, step_off, vel, group, grid, st);
 }
 hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st)
 {
@@ -500,10 +590,11 @@ hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st
     dgxb::k_scatter_int<<<(n + 255) / 256, 256, 0, st>>>(dst, pairs, n);
     return hipGetLastError();
 }
-hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, unsigned mask, hipStream_t st)
+hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, int group, unsigned long long mask,
+                                  hipStream_t st)
 {
-    if (fp32) dgxb::k_clear_slots<float><<<1024, 256, 0, st>>>((float *)f[0], (float *)f[1], (float *)f[2], (float *)f[3], (float *)f[4], n_nodes, mask);
-    else dgxb::k_clear_slots<double><<<1024, 256, 0, st>>>((double *)f[0], (double *)f[1], (double *)f[2], (double *)f[3], (double *)f[4], n_nodes, mask);
+    if (fp32) dgxb::k_clear_slots<float><<<1024, 256, 0, st>>>((float *)f[0], (float *)f[1], (float *)f[2], (float *)f[3], (float *)f[4], n_nodes, group, mask);
+    else dgxb::k_clear_slots<double><<<1024, 256, 0, st>>>((double *)f[0], (double *)f[1], (double *)f[2], (double *)f[3], (double *)f[4], n_nodes, group, mask);
     return hipGetLastError();
 }
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which)

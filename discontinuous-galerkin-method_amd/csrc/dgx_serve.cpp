@@ -1,5 +1,5 @@
 /*
- * dgx_serve.cpp -- request loop (include/dgx_serve.h): resident meshes, slot groups of 16 shots in the
+ * dgx_serve.cpp -- request loop (include/dgx_serve.h): resident meshes, slot groups of 16 or 64 shots in the
  * shot-innermost layout, memory-bounded admission, one persistent kernel per half-step over all active
  * batches, hipGraph-captured blocks of steps, retirement with trace read-back.
  */
@@ -19,8 +19,9 @@
 
 extern "C" {
 hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched, const int *step_base,
-                            int step_off, int vel, int grid, hipStream_t st);
-hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, unsigned mask, hipStream_t st);
+                            int step_off, int vel, int group, int grid, hipStream_t st);
+hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, int group, unsigned long long mask,
+                                  hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
 hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st);
 }
@@ -36,7 +37,7 @@ hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st
 
 namespace {
 
-const int GROUP = 16;     /* shots per batch (= shot tile of the serving kernel) */
+const int MAXG = 64;      /* largest slot group (shots per batch = shot tile of the serving kernel) */
 const int GRID = 2048;    /* persistent grid: 8 XCDs x 256 blocks */
 const int MAX_BATCHES = 4096;
 
@@ -71,12 +72,13 @@ struct Batch {
     int mesh = 0;
     void *fields = nullptr;
     long long field_bytes = 0;
-    long req[GROUP];
-    int shot[GROUP];
-    int h_start[GROUP], h_len[GROUP], h_nrec[GROUP];
-    double *h_wav[GROUP], *h_trace[GROUP];
-    int *d_slot_int = nullptr;      /* start[16] | len[16] | nrec[16] */
-    double **d_slot_ptr = nullptr;  /* wav[16] | trace[16] */
+    int G = 16;                     /* slots */
+    long req[MAXG];
+    int shot[MAXG];
+    int h_start[MAXG], h_len[MAXG], h_nrec[MAXG];
+    double *h_wav[MAXG], *h_trace[MAXG];
+    int *d_slot_int = nullptr;      /* start[G] | len[G] | nrec[G] */
+    double **d_slot_ptr = nullptr;  /* wav[G] | trace[G] */
     int *d_aux_of_elem = nullptr;
     std::vector<int> aux_elems;     /* device elements whose per-element word is currently nonzero */
     dgx_aux *d_aux = nullptr;
@@ -88,7 +90,7 @@ struct Batch {
     int used() const
     {
         int n = 0;
-        for (int k = 0; k < GROUP; k++) n += req[k] >= 0;
+        for (int k = 0; k < G; k++) n += req[k] >= 0;
         return n;
     }
 };
@@ -118,6 +120,7 @@ hipError_t grow(T **p, long *cap, long need)
 
 struct dgx_server {
     int fp32 = 1;
+    int group = 16; /* shots per slot group */
     long long budget = 0, in_use = 0;
     int K = 100;
     dgx_operators ops;
@@ -195,6 +198,12 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
     dgx_server *sv = new dgx_server();
     sv->fp32 = cfg ? (cfg->precision == DGX_FP64 ? 0 : 1) : 1;
     sv->K = (cfg && cfg->steps_per_block > 0) ? cfg->steps_per_block : 100;
+    sv->group = (cfg && cfg->group_shots) ? cfg->group_shots : 16;
+    if (sv->group != 16 && sv->group != 64) {
+        dgx_set_error("dgx_server_create: group_shots must be 0, 16 or 64");
+        delete sv;
+        return DGX_EARG;
+    }
     sv->ops = *ops;
     make_ops(sv);
     size_t fr = 0, tot = 0;
@@ -289,7 +298,7 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
 extern "C" int dgx_server_submit(dgx_server *sv, const dgx_request *rq, long *id)
 {
     if (!sv || !rq || !id) return DGX_EARG;
-    if (rq->mesh < 0 || rq->mesh >= (int)sv->meshes.size() || rq->nshots < 1 || rq->nshots > GROUP ||
+    if (rq->mesh < 0 || rq->mesh >= (int)sv->meshes.size() || rq->nshots < 1 || rq->nshots > sv->group ||
         rq->nsteps < 1 || rq->nrec < 0 || !rq->src || !rq->wavelet || (rq->nrec > 0 && !rq->rec)) {
         dgx_set_error("dgx_server_submit: invalid request");
         return DGX_EARG;
@@ -329,8 +338,10 @@ static int new_batch(dgx_server *sv, int mesh, int *bi)
     if ((long)sv->batches.size() >= MAX_BATCHES) return DGX_ENOMEM;
     Batch b;
     b.mesh = mesh;
-    b.field_bytes = (long long)5 * m.tri * 6 * GROUP * sv->rsize();
-    for (int k = 0; k < GROUP; k++) {
+    b.G = sv->group;
+    const int G = b.G;
+    b.field_bytes = (long long)5 * m.tri * 6 * G * sv->rsize();
+    for (int k = 0; k < MAXG; k++) {
         b.req[k] = -1;
         b.shot[k] = 0;
         b.h_start[k] = 0;
@@ -341,8 +352,8 @@ static int new_batch(dgx_server *sv, int mesh, int *bi)
     }
     SV_TRY(hipMalloc(&b.fields, b.field_bytes));
     SV_TRY(hipMemsetAsync(b.fields, 0, b.field_bytes, sv->stream));
-    SV_TRY(hipMalloc((void **)&b.d_slot_int, sizeof(int) * 3 * GROUP));
-    SV_TRY(hipMalloc((void **)&b.d_slot_ptr, sizeof(double *) * 2 * GROUP));
+    SV_TRY(hipMalloc((void **)&b.d_slot_int, sizeof(int) * 3 * G));
+    SV_TRY(hipMalloc((void **)&b.d_slot_ptr, sizeof(double *) * 2 * G));
     SV_TRY(hipMalloc((void **)&b.d_aux_of_elem, sizeof(int) * m.tri));
     SV_TRY(hipMemsetAsync(b.d_aux_of_elem, 0, sizeof(int) * m.tri, sv->stream)); /* no aux, no source */
     sv->in_use += b.field_bytes + 4 * m.tri;
@@ -358,7 +369,8 @@ static int refresh_batch(dgx_server *sv, Batch &b)
 {
     const MeshDev &m = sv->meshes[b.mesh];
     std::map<int, std::pair<std::vector<dgx_src_dev>, std::vector<dgx_rec_dev>>> by_elem;
-    for (int k = 0; k < GROUP; k++) {
+    const int G = b.G;
+    for (int k = 0; k < G; k++) {
         if (b.req[k] < 0) continue;
         const Request &r = sv->reqs.at(b.req[k]);
         const dgx_source &sr = r.src[b.shot[k]];
@@ -429,17 +441,17 @@ static int refresh_batch(dgx_server *sv, Batch &b)
     }
     b.aux_elems.clear();
     for (auto &kv : word) b.aux_elems.push_back(kv.first);
-    int si[3 * GROUP];
-    double *sp[2 * GROUP];
-    for (int k = 0; k < GROUP; k++) {
+    int si[3 * MAXG];
+    double *sp[2 * MAXG];
+    for (int k = 0; k < G; k++) {
         si[k] = b.h_start[k];
-        si[GROUP + k] = b.h_len[k];
-        si[2 * GROUP + k] = b.h_nrec[k];
+        si[G + k] = b.h_len[k];
+        si[2 * G + k] = b.h_nrec[k];
         sp[k] = b.h_wav[k];
-        sp[GROUP + k] = b.h_trace[k];
+        sp[G + k] = b.h_trace[k];
     }
-    SV_TRY(hipMemcpyAsync(b.d_slot_int, si, sizeof(si), hipMemcpyHostToDevice, sv->stream));
-    SV_TRY(hipMemcpyAsync(b.d_slot_ptr, sp, sizeof(sp), hipMemcpyHostToDevice, sv->stream));
+    SV_TRY(hipMemcpyAsync(b.d_slot_int, si, sizeof(int) * 3 * G, hipMemcpyHostToDevice, sv->stream));
+    SV_TRY(hipMemcpyAsync(b.d_slot_ptr, sp, sizeof(double *) * 2 * G, hipMemcpyHostToDevice, sv->stream));
     /* the staging copies above read host memory asynchronously: wait before it goes out of scope */
     SV_TRY(hipStreamSynchronize(sv->stream));
     b.dirty = false;
@@ -453,7 +465,8 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
     std::memset(&a, 0, sizeof(a));
     a.op = op;
     R *f = (R *)b.fields;
-    size_t per = (size_t)m.tri * 6 * GROUP;
+    const int G = b.G;
+    size_t per = (size_t)m.tri * 6 * G;
     for (int c = 0; c < 5; c++) a.f[c] = f + c * per;
     a.fstride = m.tri;
     a.nbr = m.nbr;
@@ -470,14 +483,14 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
     a.nsteps = 0x7fffffff;
     a.tri = (int)m.tri;
     a.dt = m.dt;
-    a.spad = GROUP;
-    a.shot_tile = GROUP;
-    a.nshots = GROUP;
+    a.spad = G;
+    a.shot_tile = G;
+    a.nshots = G;
     a.shot_start = b.d_slot_int;
-    a.shot_len = b.d_slot_int + GROUP;
-    a.slot_nrec = b.d_slot_int + 2 * GROUP;
+    a.shot_len = b.d_slot_int + G;
+    a.slot_nrec = b.d_slot_int + 2 * G;
     a.slot_wav = (const double *const *)b.d_slot_ptr;
-    a.slot_trace = b.d_slot_ptr + GROUP;
+    a.slot_trace = b.d_slot_ptr + G;
     a.aux_of_elem = b.d_aux_of_elem;
 }
 
@@ -493,7 +506,8 @@ static int upload_table(dgx_server *sv)
         if (sv->fp32) fill_batch_args(sv, b, sv->op32, *(dgx::StepArgs<float> *)(table.data() + nb * asz));
         else fill_batch_args(sv, b, sv->op64, *(dgx::StepArgs<double> *)(table.data() + nb * asz));
         sched.push_back(tiles);
-        tiles += (int)((sv->meshes[b.mesh].tri + 15) / 16);
+        const int per_tile = 256 / sv->group; /* elements per tile: a workgroup covers 256/G elements x G shots */
+        tiles += (int)((sv->meshes[b.mesh].tri + per_tile - 1) / per_tile);
         nb++;
     }
     sched.push_back(tiles);
@@ -515,22 +529,23 @@ static int admit(dgx_server *sv)
         Request &r = sv->reqs.at(id);
         const MeshDev &m = sv->meshes[r.mesh];
         /* best fit among this mesh's batches */
-        int best = -1, best_free = GROUP + 1;
+        const int G = sv->group;
+        int best = -1, best_free = G + 1;
         for (int bi : m.batches) {
-            int fr = GROUP - sv->batches[bi].used();
+            int fr = G - sv->batches[bi].used();
             if (fr >= r.nshots && fr < best_free) { best = bi; best_free = fr; }
         }
         long long need = r.bytes;
-        if (best < 0) need += (long long)5 * m.tri * 6 * GROUP * sv->rsize() + 4 * m.tri;
+        if (best < 0) need += (long long)5 * m.tri * 6 * G * sv->rsize() + 4 * m.tri;
         if (sv->in_use + need > sv->budget) { still.push_back(id); continue; }
         if (best < 0) {
             int rc = new_batch(sv, r.mesh, &best);
             if (rc) return rc;
         }
         Batch &b = sv->batches[best];
-        unsigned mask = 0;
+        unsigned long long mask = 0;
         int k = 0;
-        for (int slot = 0; slot < GROUP && k < r.nshots; slot++) {
+        for (int slot = 0; slot < G && k < r.nshots; slot++) {
             if (b.req[slot] >= 0) continue;
             double *w = pool_get(sv, r.nsteps);
             double *t = pool_get(sv, (size_t)r.nsteps * 2 * std::max<long>(r.nrec, 1));
@@ -547,13 +562,13 @@ static int admit(dgx_server *sv)
             r.slots.push_back({best, slot});
             r.d_wav.push_back(w);
             r.d_trace.push_back(t);
-            mask |= 1u << slot;
+            mask |= 1ull << slot;
             k++;
         }
         void *f[5];
-        size_t per = (size_t)m.tri * 6 * GROUP * sv->rsize();
+        size_t per = (size_t)m.tri * 6 * G * sv->rsize();
         for (int c = 0; c < 5; c++) f[c] = (char *)b.fields + c * per;
-        SV_TRY(dgx_launch_clear_slots(sv->fp32, f, (long long)m.tri * 6, mask, sv->stream));
+        SV_TRY(dgx_launch_clear_slots(sv->fp32, f, (long long)m.tri * 6, G, mask, sv->stream));
         b.dirty = true;
         sv->table_dirty = true;
         r.state = DGX_REQ_RUNNING;
@@ -611,8 +626,11 @@ static int build_graph(dgx_server *sv)
     hipError_t e = hipSuccess;
     const void *ops = sv->fp32 ? (const void *)&sv->op32 : (const void *)&sv->op64;
     for (int k = 0; k < sv->K && e == hipSuccess; k++) {
-        e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 1, GRID, sv->stream);
-        if (e == hipSuccess) e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 0, GRID, sv->stream);
+        e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 1, sv->group, GRID,
+                             sv->stream);
+        if (e == hipSuccess)
+            e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 0, sv->group, GRID,
+                                 sv->stream);
     }
     if (e == hipSuccess) e = dgx_launch_advance(sv->d_step_base, sv->K, sv->stream);
     hipError_t e2 = hipStreamEndCapture(sv->stream, &g);

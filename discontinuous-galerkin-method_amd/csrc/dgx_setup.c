@@ -10,6 +10,7 @@
 #include "dgx_setup.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -171,6 +172,112 @@ int dgx_read_mesh(const char *ele, const char *node, const char *neigh, long tri
 done:
     free(E); free(G); free(P);
     return rc;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Binary mesh: header + four arrays, moved through a fixed-size buffer.
+ * ---------------------------------------------------------------------------------------------- */
+static const char MESH_MAGIC[8] = {'D', 'G', 'X', 'M', 'E', 'S', 'H', '1'};
+
+static int write_chunked(FILE *f, const void *src, size_t elem, size_t n, int widen_long)
+{
+    /* widen_long: src holds C longs, the file holds int64 */
+    if (!widen_long || sizeof(long) == 8) return fwrite(src, elem, n, f) == n ? 0 : -1;
+    int64_t buf[4096];
+    const long *l = (const long *)src;
+    for (size_t i = 0; i < n; i += 4096) {
+        size_t k = n - i < 4096 ? n - i : 4096;
+        for (size_t j = 0; j < k; j++) buf[j] = (int64_t)l[i + j];
+        if (fwrite(buf, 8, k, f) != k) return -1;
+    }
+    return 0;
+}
+
+static int read_chunked(FILE *f, void *dst, size_t elem, size_t n, int narrow_long)
+{
+    if (!narrow_long || sizeof(long) == 8) return fread(dst, elem, n, f) == n ? 0 : -1;
+    int64_t buf[4096];
+    long *l = (long *)dst;
+    for (size_t i = 0; i < n; i += 4096) {
+        size_t k = n - i < 4096 ? n - i : 4096;
+        if (fread(buf, 8, k, f) != k) return -1;
+        for (size_t j = 0; j < k; j++) l[i + j] = (long)buf[j];
+    }
+    return 0;
+}
+
+int dgx_write_mesh_binary(const char *path, long tri, long point_num, const double *point, const long *triangle,
+                          const long *neighbor, const long *attribute)
+{
+    if (!path || tri <= 0 || point_num <= 0 || !point || !triangle || !neighbor) return DGX_EARG;
+    FILE *f = fopen(path, "wb");
+    if (!f) { dgx_set_error("cannot write %s", path); return DGX_EIO; }
+    int64_t hdr[2] = {tri, point_num};
+    int bad = fwrite(MESH_MAGIC, 1, 8, f) != 8 || fwrite(hdr, 8, 2, f) != 2;
+    bad = bad || write_chunked(f, point, 8, (size_t)point_num * 2, 0);
+    bad = bad || write_chunked(f, triangle, sizeof(long), (size_t)tri * 3, 1);
+    bad = bad || write_chunked(f, neighbor, sizeof(long), (size_t)tri * 3, 1);
+    if (!bad) {
+        if (attribute) {
+            bad = write_chunked(f, attribute, sizeof(long), (size_t)tri, 1);
+        } else {
+            int64_t one[1024];
+            for (int j = 0; j < 1024; j++) one[j] = 1;
+            for (long i = 0; i < tri && !bad; i += 1024) {
+                size_t k = (size_t)(tri - i < 1024 ? tri - i : 1024);
+                bad = fwrite(one, 8, k, f) != k;
+            }
+        }
+    }
+    if (fclose(f) != 0) bad = 1;
+    if (bad) { dgx_set_error("write error on %s", path); return DGX_EIO; }
+    return DGX_OK;
+}
+
+static FILE *open_mesh_binary(const char *path, long *tri, long *point_num)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) { dgx_set_error("cannot read %s", path); return NULL; }
+    char magic[8];
+    int64_t hdr[2];
+    if (fread(magic, 1, 8, f) != 8 || memcmp(magic, MESH_MAGIC, 8) != 0 || fread(hdr, 8, 2, f) != 2 || hdr[0] <= 0 ||
+        hdr[1] <= 0) {
+        dgx_set_error("%s is not a binary mesh file", path);
+        fclose(f);
+        return NULL;
+    }
+    *tri = (long)hdr[0];
+    *point_num = (long)hdr[1];
+    return f;
+}
+
+int dgx_mesh_binary_counts(const char *path, long *tri, long *point_num)
+{
+    if (!path || !tri || !point_num) return DGX_EARG;
+    FILE *f = open_mesh_binary(path, tri, point_num);
+    if (!f) return DGX_EIO;
+    fclose(f);
+    return DGX_OK;
+}
+
+int dgx_read_mesh_binary(const char *path, long tri, long point_num, double *point, long *triangle, long *neighbor,
+                         long *attribute)
+{
+    if (!path || !point || !triangle || !neighbor) return DGX_EARG;
+    long t, np;
+    FILE *f = open_mesh_binary(path, &t, &np);
+    if (!f) return DGX_EIO;
+    if (t != tri || np != point_num) {
+        dgx_set_error("%s holds %ld elements and %ld points, not %ld and %ld", path, t, np, tri, point_num);
+        fclose(f);
+        return DGX_EARG;
+    }
+    int bad = read_chunked(f, point, 8, (size_t)np * 2, 0) || read_chunked(f, triangle, sizeof(long), (size_t)t * 3, 1) ||
+              read_chunked(f, neighbor, sizeof(long), (size_t)t * 3, 1);
+    if (!bad && attribute) bad = read_chunked(f, attribute, sizeof(long), (size_t)t, 1);
+    fclose(f);
+    if (bad) { dgx_set_error("%s is truncated", path); return DGX_EIO; }
+    return DGX_OK;
 }
 
 /* ------------------------------------------------------------------------------------------------

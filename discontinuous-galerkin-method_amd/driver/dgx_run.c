@@ -12,6 +12,7 @@
  * Usage: dgx_run <config file>
  * Config: "key value" lines, '#' comments.  Keys (defaults in brackets):
  *   mesh <prefix>            reads <prefix>.ele, .node, .neigh           [input_files/mesh/model.1]
+ *   mesh_binary <path>       reads a binary mesh (dgx_write_mesh_binary) instead of the text files
  *   operators <dir>          precomputed_matrix directory               [input_files/precomputed_matrix]
  *   output <dir>             output directory (must exist)               [output_files]
  *   steps <n>                time steps                                  [2500, dg.h:4]
@@ -38,7 +39,7 @@
 #define MAXSRC 64
 
 typedef struct {
-    char mesh[1024], ops[1024], out[1024];
+    char mesh[1024], mesh_bin[1024], ops[1024], out[1024];
     long steps, snapshot;
     double f0, pi;
     dgx_sponge_params sp;
@@ -95,6 +96,7 @@ static int read_config(const char *path, config *c)
         long n;
         if (sscanf(line, "%63s", key) != 1) continue;
         if (!strcmp(key, "mesh") && sscanf(line, "%*s %1023s", a) == 1) strcpy(c->mesh, a);
+        else if (!strcmp(key, "mesh_binary") && sscanf(line, "%*s %1023s", a) == 1) strcpy(c->mesh_bin, a);
         else if (!strcmp(key, "operators") && sscanf(line, "%*s %1023s", a) == 1) strcpy(c->ops, a);
         else if (!strcmp(key, "output") && sscanf(line, "%*s %1023s", a) == 1) strcpy(c->out, a);
         else if (!strcmp(key, "steps") && sscanf(line, "%*s %ld", &n) == 1) c->steps = n;
@@ -183,13 +185,23 @@ int main(int argc, char **argv)
     snprintf(ele, sizeof(ele), "%s.ele", c.mesh);
     snprintf(node, sizeof(node), "%s.node", c.mesh);
     snprintf(neigh, sizeof(neigh), "%s.neigh", c.mesh);
-    long tri = dgx_mesh_count(ele), np = dgx_mesh_count(node);
-    if (tri <= 0 || np <= 0) { fprintf(stderr, "cannot read mesh %s: %s\n", c.mesh, dgx_last_error()); return 1; }
+    long tri, np;
+    if (c.mesh_bin[0]) {
+        if (dgx_mesh_binary_counts(c.mesh_bin, &tri, &np)) tri = np = -1;
+    } else {
+        tri = dgx_mesh_count(ele);
+        np = dgx_mesh_count(node);
+    }
+    if (tri <= 0 || np <= 0) {
+        fprintf(stderr, "cannot read mesh %s: %s\n", c.mesh_bin[0] ? c.mesh_bin : c.mesh, dgx_last_error());
+        return 1;
+    }
     printf("Triangle number %ld\nPoint number %ld\n", tri, np);
     double *point = malloc(sizeof(double) * np * 2);
     long *triangle = malloc(sizeof(long) * tri * 3), *neighbor = malloc(sizeof(long) * tri * 3);
     long *attribute = malloc(sizeof(long) * tri);
-    CHECK(dgx_read_mesh(ele, node, neigh, tri, np, point, triangle, neighbor, attribute));
+    if (c.mesh_bin[0]) CHECK(dgx_read_mesh_binary(c.mesh_bin, tri, np, point, triangle, neighbor, attribute));
+    else CHECK(dgx_read_mesh(ele, node, neigh, tri, np, point, triangle, neighbor, attribute));
     dgx_operators ops;
     CHECK(dgx_read_operators(c.ops, &ops));
 

@@ -46,3 +46,33 @@ def write_mesh(prefix: os.PathLike | str, mesh: Mesh) -> tuple[str, str, str]:
 
 def read_binary(path: os.PathLike | str) -> np.ndarray:
     return np.fromfile(path, dtype="<f8")
+
+
+MESH_MAGIC = b"DGXMESH1"
+
+
+def write_mesh_binary(path: os.PathLike | str, mesh: Mesh) -> str:
+    """Binary mesh (dgx_write_mesh_binary's format): magic, int64 tri and point count, float64 points,
+    int64 triangles, .neigh columns and attributes, little-endian."""
+    with open(path, "wb") as f:
+        f.write(MESH_MAGIC)
+        np.array([mesh.tri, mesh.point_num], "<i8").tofile(f)
+        np.ascontiguousarray(mesh.point, "<f8").tofile(f)
+        np.ascontiguousarray(mesh.triangle, "<i8").tofile(f)
+        np.ascontiguousarray(mesh.neighbor, "<i8").tofile(f)
+        np.ascontiguousarray(mesh.attribute, "<i8").tofile(f)
+    return str(path)
+
+
+def read_mesh_binary(path: os.PathLike | str) -> Mesh:
+    with open(path, "rb") as f:
+        if f.read(8) != MESH_MAGIC:
+            raise ValueError(f"{path} is not a binary mesh file")
+        T, P = (int(v) for v in np.fromfile(f, "<i8", 2))
+        point = np.fromfile(f, "<f8", 2 * P).reshape(P, 2)
+        tri = np.fromfile(f, "<i8", 3 * T).reshape(T, 3)
+        nb = np.fromfile(f, "<i8", 3 * T).reshape(T, 3)
+        attr = np.fromfile(f, "<i8", T)
+    if len(attr) != T:
+        raise ValueError(f"{path} is truncated")
+    return Mesh(point, tri, nb, attr)

@@ -18,6 +18,7 @@ OPERATOR_DIR = PKG_DIR / "input_files" / "precomputed_matrix"
 DGX_FP64, DGX_FP32 = 0, 1
 DGX_SRC_FORCE_Z, DGX_SRC_EXPLOSIVE = 0, 1
 DGX_LAYOUT_AUTO, DGX_LAYOUT_ELEMENT, DGX_LAYOUT_SHOT_INNER = 0, 1, 2
+DGX_OK, DGX_EARG, DGX_EMESH, DGX_EHIP, DGX_ENOMEM, DGX_EIO, DGX_ESTATE = 0, -1, -2, -3, -4, -5, -6
 DGX_STEP_AUTO, DGX_STEP_HALF, DGX_STEP_FULL = 0, 1, 2
 
 dptr = C.POINTER(C.c_double)
@@ -54,7 +55,8 @@ class ProblemC(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("precision", C.c_int), ("hbm_budget", C.c_longlong), ("steps_per_block", C.c_int)]
+    _fields_ = [("precision", C.c_int), ("hbm_budget", C.c_longlong), ("steps_per_block", C.c_int),
+                ("group_shots", C.c_int)]
 
 
 class RequestC(C.Structure):
@@ -115,6 +117,9 @@ def load() -> C.CDLL:
         "dgx_save_data": (C.c_int, [C.c_char_p, dptr, C.c_long, C.c_long]),
         "dgx_save_record": (C.c_int, [C.c_char_p, dptr, C.c_long]),
         "dgx_save_binary": (C.c_int, [C.c_char_p, dptr, C.c_long]),
+        "dgx_write_mesh_binary": (C.c_int, [C.c_char_p, C.c_long, C.c_long, dptr, lptr, lptr, lptr]),
+        "dgx_mesh_binary_counts": (C.c_int, [C.c_char_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+        "dgx_read_mesh_binary": (C.c_int, [C.c_char_p, C.c_long, C.c_long, dptr, lptr, lptr, lptr]),
         "dgx_all_node": (None, [C.c_long, dptr, lptr, dptr, dptr]),
         "dgx_sponge": (C.c_long, [C.c_long, dptr, lptr, dptr, C.POINTER(SpongeParams), dptr]),
         "dgx_cfl_dt": (C.c_double, [C.c_long, dptr, lptr, dptr]),

@@ -5,6 +5,8 @@
     python -m dgx.runs serving   [--requests 2000 --rate 20] open-loop request stream on 8 meshes
     python -m dgx.runs carbonate [--shots 256 --steps 8000] 256 shots through the request loop, FP64 on 4
 
+The serving runs take --group (16 or 64 shots per slot group, default 64).
+
 Progress lines go to stderr while a run is going."""
 from __future__ import annotations
 
@@ -116,9 +118,9 @@ def run_slope(a):
             "fp64_reference_checks": checks}
 
 
-def serve_stream(meshes, specs, arrivals, precision=L.DGX_FP32, steps_per_block=100):
+def serve_stream(meshes, specs, arrivals, precision=L.DGX_FP32, steps_per_block=100, group=16):
     from . import serve as SV
-    sv = SV.Server(precision, 0, steps_per_block)
+    sv = SV.Server(precision, 0, steps_per_block, group_shots=group)
     rms = [sv.add_mesh(m, md, sp) for m, md, sp in meshes]
     reqs = [sv.make_request(rms[r["mesh"]], r["xz"], [r["f0"]] * len(r["xz"]), r["nsteps"], r["rec"]) for r in specs]
     log(f"serving {len(reqs)} requests on meshes {[m.tri for m, _, _ in meshes]}")
@@ -130,8 +132,8 @@ def serve_stream(meshes, specs, arrivals, precision=L.DGX_FP32, steps_per_block=
 def run_serving(a):
     ms = configs.serving_meshes()
     arrivals, specs = configs.serving_requests(ms, n=a.requests, rate=a.rate)
-    lat, wall, st, _, _ = serve_stream(ms, specs, arrivals)
-    return {"requests": a.requests, "rate_per_s": a.rate, "meshes": [m.tri for m, _, _ in ms],
+    lat, wall, st, _, _ = serve_stream(ms, specs, arrivals, group=a.group)
+    return {"requests": a.requests, "rate_per_s": a.rate, "meshes": [m.tri for m, _, _ in ms], "group_shots": a.group,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "wall_s": wall, "dof_updates": st["dof_updates"], "dof_updates_per_s": st["dof_updates"] / wall,
             "offered_dof_updates_per_s": float(sum(len(r["xz"]) * r["nsteps"] * ms[r["mesh"]][0].tri * 30
@@ -143,20 +145,20 @@ def run_carbonate(a):
     T = w.mesh.tri
     log(f"carbonate mesh {T} elements")
     st = build_setup(w.mesh, w.model, w.sources[:1], w.receivers, 1, sponge=w.sponge)
-    group = 16
+    per_req = 16  # shots per request
     specs, arrivals = [], []
-    for g in range(0, a.shots, group):
-        xz = np.array([[s.x, s.z] for s in w.sources[g:g + group]])
+    for g in range(0, a.shots, per_req):
+        xz = np.array([[s.x, s.z] for s in w.sources[g:g + per_req]])
         specs.append(dict(mesh=0, xz=xz, f0=30.0, nsteps=a.steps, rec=w.receivers))
         arrivals.append(0.0)
-    lat, wall, stats, res, reqs = serve_stream([(w.mesh, w.model, w.sponge)], specs, np.array(arrivals))
+    lat, wall, stats, res, reqs = serve_stream([(w.mesh, w.model, w.sponge)], specs, np.array(arrivals), group=a.group)
     picks = [0, a.shots // 3, (2 * a.shots) // 3, a.shots - 1]
     errs = []
     for k in picks:
         ((x64, z64), _), _ = single_shot_reference(w, k, st.dt, a.steps)
-        x32, z32 = res[k // group][0][k % group], res[k // group][1][k % group]
+        x32, z32 = res[k // per_req][0][k % per_req], res[k // per_req][1][k % per_req]
         errs.append(trace_errors(x32, z32, x64, z64))
-    return {"workload": w.name, "elements": T, "shots": a.shots, "steps": a.steps, "dt": st.dt,
+    return {"workload": w.name, "elements": T, "shots": a.shots, "steps": a.steps, "dt": st.dt, "group_shots": a.group,
             "served_fp32_dof_updates_per_s": stats["dof_updates"] / wall, "wall_s": wall,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "fp32_vs_fp64_trace_rel_l2_per_checked_shot": errs}
@@ -169,6 +171,7 @@ def main(argv=None):
     ap.add_argument("--shots", type=int, default=0)
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--group", type=int, default=64, help="shots per slot group of the server (16 or 64)")
     a = ap.parse_args(argv)
     defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256)}
     a.steps = a.steps or defaults[a.which][0]

@@ -26,10 +26,11 @@ class ResidentMesh:
 
 class Server:
     def __init__(self, precision: int = L.DGX_FP32, hbm_budget: int = 0, steps_per_block: int = 100,
-                 ops: L.Operators | None = None):
+                 ops: L.Operators | None = None, group_shots: int = 16):
         self.lib = L.load()
         self.ops = ops or L.read_operators()
-        cfg = L.ServerConfig(precision, hbm_budget, steps_per_block)
+        cfg = L.ServerConfig(precision, hbm_budget, steps_per_block, group_shots)
+        self.group_shots = group_shots
         self.handle = C.c_void_p()
         L.check(self.lib.dgx_server_create(C.byref(cfg), C.byref(self.ops), C.byref(self.handle)), "server_create")
         self.meshes: list[ResidentMesh] = []

@@ -5,6 +5,7 @@
  * a single shot).  The server keeps meshes, materials and sponge factors resident in HBM and runs many
  * shots of many requests at once: every request is 1..16 shots on one resident mesh, each shot with its
  * own source, FP64 wavelet samples and start step, all shots of a request sharing one receiver spread.
+ * A request holds 1..group_shots shots (16 or 64, dgx_server_config).
  * Requests are admitted as slots and memory free up (continuous batching), stepped by one persistent
  * kernel per half-step over all active (mesh, slot group) batches, in hipGraph-captured blocks of steps,
  * and retired with their traces copied back.  Every shot computes exactly what a single-shot solve of
@@ -25,13 +26,16 @@ typedef struct {
     int precision;         /* DGX_FP32 (default for serving) or DGX_FP64 */
     long long hbm_budget;  /* bytes for wavefields, wavelets and traces; 0 = 80 % of free HBM */
     int steps_per_block;   /* steps between scheduling points (one graph replay); 0 = 100 */
+    int group_shots;       /* shots per slot group, 16 or 64 (0 = 16): the serving kernel's shot tile.  64
+                            * moves whole 256-byte rows per node (faster); 16 wastes fewer lanes when
+                            * requests are small and meshes many */
 } dgx_server_config;
 
 enum dgx_request_state { DGX_REQ_QUEUED = 0, DGX_REQ_RUNNING = 1, DGX_REQ_DONE = 2 };
 
 typedef struct {
     int mesh;                /* id from dgx_server_add_mesh */
-    int nshots;              /* 1..16 */
+    int nshots;              /* 1..group_shots */
     long nsteps;             /* time steps of every shot */
     const dgx_source *src;   /* nshots sources; src[k].shot and .wavelet are ignored (shot k uses row k) */
     const double *wavelet;   /* [nshots][nsteps] FP64 source samples */

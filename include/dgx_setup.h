@@ -26,6 +26,16 @@ long dgx_mesh_count(const char *path);
 int dgx_read_mesh(const char *ele, const char *node, const char *neigh, long tri, long point_num, double *point,
                   long *triangle, long *neighbor, long *attribute);
 
+/* Binary mesh (no counterpart in the reference, whose text reader cannot hold files over 1e8 bytes,
+ * fun_save_to_file.c:36,130): little-endian, "DGXMESH1", int64 tri, int64 point_num, then float64
+ * point [np*2], int64 triangle [tri*3], int64 neighbor [tri*3] (the .neigh columns, -1 = none), int64
+ * attribute [tri].  Read and written in chunks, so a mesh never needs a second in-memory copy. */
+int dgx_write_mesh_binary(const char *path, long tri, long point_num, const double *point, const long *triangle,
+                          const long *neighbor, const long *attribute);
+int dgx_mesh_binary_counts(const char *path, long *tri, long *point_num);
+int dgx_read_mesh_binary(const char *path, long tri, long point_num, double *point, long *triangle, long *neighbor,
+                         long *attribute);
+
 /* Writers: Save_Data "%0.14f " rows (fun_save_to_file.c:233-255), Save_Record (fun_save_to_file.c:260-281),
  * and a lossless little-endian float64 twin of either. */
 int dgx_save_data(const char *path, const double *m, long rows, long cols);

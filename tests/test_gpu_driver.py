@@ -52,3 +52,32 @@ def test_driver_matches_oracle(tmp_path):
         assert rel_l2(gx, ox[:, j]) <= 1e-12 and rel_l2(gz, oz[:, j]) <= 1e-12, j
     allnode = np.loadtxt(out / "All_node.txt")
     assert allnode.shape == (T * 6, 2)
+
+
+def test_driver_binary_mesh_equals_text_mesh(tmp_path):
+    """mesh_binary holding the arrays the text reader produces gives bit-identical outputs."""
+    assert BIN.exists(), "bin/dgx_run not built"
+    sp = dgx.Sponge.for_domain(2000.0, 2000.0, 100.0)
+    mesh = dgx.structured(30, 30, 2000.0 / 30, diagonal="random", seed=4, sponge_box=sp.box(),
+                          attribute=lambda c: np.where(c[:, 1] > 1500, 1, 3))
+    dgx.formats.write_mesh(tmp_path / "model.1", mesh)
+    lib = L.load()
+    T, P = mesh.tri, mesh.point_num
+    pt, tr, nb, at = np.zeros(P * 2), np.zeros(T * 3, np.int64), np.zeros(T * 3, np.int64), np.zeros(T, np.int64)
+    paths = [str(tmp_path / f"model.1.{x}").encode() for x in ("ele", "node", "neigh")]
+    L.check(lib.dgx_read_mesh(*paths, T, P, L.dp(pt), L.lp(tr), L.lp(nb), L.lp(at)))
+    L.check(lib.dgx_write_mesh_binary(str(tmp_path / "model.dgxm").encode(), T, P, L.dp(pt), L.lp(tr), L.lp(nb),
+                                      L.lp(at)))
+    outs = []
+    for name, line in (("text", f"mesh {tmp_path / 'model.1'}"), ("bin", f"mesh_binary {tmp_path / 'model.dgxm'}")):
+        out = tmp_path / name
+        out.mkdir()
+        (tmp_path / f"{name}.cfg").write_text(
+            f"{line}\noperators {L.OPERATOR_DIR}\noutput {out}\nsteps 120\nsnapshot 60\n"
+            "source 1200 1200 force\nreceiver_line 300 1450 1700 1450 4\n")
+        print(f"case driver {name} mesh", flush=True)
+        r = subprocess.run([str(BIN), str(tmp_path / f"{name}.cfg")], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(out)
+    for f in ["Vx60.bin", "Vz120.bin", "record_x0.bin", "record_z3.bin"]:
+        assert open(outs[0] / f, "rb").read() == open(outs[1] / f, "rb").read(), f

@@ -5,6 +5,9 @@
 * they equal the standalone shot-innermost FP32 solver's traces for the same shots to FP32 rounding;
 * they are within 1e-4 relative L2 of the FP64 oracle;
 * a tight memory budget queues requests instead of failing, and all of them complete.
+
+The first two hold for both slot-group sizes (16 and 64 shots per group, which is also the serving
+kernel's shot tile).
 """
 import numpy as np
 import pytest
@@ -38,15 +41,15 @@ def build(server, rms, spec, k):
     return server.make_request(rms[mi], xz, [f0] * ns, nsteps, rec), xz, rec
 
 
-def new_server(precision=L.DGX_FP32, budget=0, K=50):
+def new_server(precision=L.DGX_FP32, budget=0, K=50, group=16):
     sp, ms = meshes()
-    sv = S.Server(precision, budget, K)
+    sv = S.Server(precision, budget, K, group_shots=group)
     rms = [sv.add_mesh(m, md, sp) for m, md in ms]
     return sv, rms, sp, ms
 
 
-def run_alone(k):
-    sv, rms, _, _ = new_server()
+def run_alone(k, group=16):
+    sv, rms, _, _ = new_server(group=group)
     req, xz, rec = build(sv, rms, REQS[k], k)
     rid = sv.submit(req)
     sv.drain()
@@ -55,8 +58,10 @@ def run_alone(k):
     return out, xz, rec
 
 
-def test_results_identical_alone_and_in_the_mix():
-    sv, rms, _, _ = new_server()
+@pytest.mark.parametrize("group", [16, 64])
+def test_results_identical_alone_and_in_the_mix(group):
+    print(f"case serve mix, group {group}", flush=True)
+    sv, rms, _, _ = new_server(group=group)
     built = [build(sv, rms, spec, k) for k, spec in enumerate(REQS)]
     ids = [sv.submit(built[0][0]), sv.submit(built[1][0])]
     sv.advance(1)
@@ -69,19 +74,21 @@ def test_results_identical_alone_and_in_the_mix():
     mixed = [sv.result(rid, built[k][0]) for k, rid in enumerate(ids)]
     sv.close()
     for k in range(len(REQS)):
-        (ax, az), _, _ = run_alone(k)
+        (ax, az), _, _ = run_alone(k, group)
         assert np.array_equal(mixed[k][0], ax) and np.array_equal(mixed[k][1], az), k
 
 
-def test_request_equals_standalone_solver_and_oracle():
+@pytest.mark.parametrize("group", [16, 64])
+def test_request_equals_standalone_solver_and_oracle(group):
+    print(f"case serve vs standalone and oracle, group {group}", flush=True)
     k = 0
-    (x, z), xz, rec = run_alone(k)
+    (x, z), xz, rec = run_alone(k, group)
     sp, ms = meshes()
     mesh, model = ms[REQS[k][0]]
     nsteps, f0 = REQS[k][2], REQS[k][3]
     srcs = [dgx.SourceSpec(float(a), float(b), L.DGX_SRC_EXPLOSIVE, s, f0) for s, (a, b) in enumerate(xz)]
     st = dgx.build_setup(mesh, model, srcs, rec, nsteps, sponge=sp)
-    solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, 16)
+    solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, group)
     solver.step(nsteps)
     for s in range(len(xz)):
         gx, gz = solver.traces(s)

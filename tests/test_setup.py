@@ -6,6 +6,7 @@ import pytest
 
 import dgx
 import oracle_py as O
+from dgx import formats as F
 from dgx import lib as L
 from helpers import OPDIR, OracleCase, product_setup
 
@@ -89,3 +90,44 @@ def test_vertex_source_and_receiver_follow_reference_rules():
     assert st.sources[0].elem == holders.min() + 1
     holders = np.nonzero((mesh.triangle == 1 + np.argmin(np.hypot(*(mesh.point - [1600, 1600]).T))).any(1))[0]
     assert st.receivers[0].elem == holders.max() + 1
+
+
+def test_binary_mesh_round_trips_and_matches_the_text_reader(tmp_path):
+    """dgx_write_mesh_binary / dgx_read_mesh_binary against the numpy writer/reader of the same format and
+    against the text reader (Mesh_Input) on the same mesh."""
+    lib = L.load()
+    mesh = dgx.mesh.structured(7, 5, 30.0, diagonal="random", seed=5, attribute=lambda c: np.where(c[:, 1] > 60, 1, 3))
+    T, P = mesh.tri, mesh.point_num
+    pt = np.ascontiguousarray(mesh.point, np.float64)
+    tr = np.ascontiguousarray(mesh.triangle, np.int64)
+    nb = np.ascontiguousarray(mesh.neighbor, np.int64)
+    at = np.ascontiguousarray(mesh.attribute, np.int64)
+    c_path = str(tmp_path / "c.dgxm")
+    assert lib.dgx_write_mesh_binary(c_path.encode(), T, P, L.dp(pt), L.lp(tr), L.lp(nb), L.lp(at)) == 0
+    m = F.read_mesh_binary(c_path)
+    assert np.array_equal(m.point, pt) and np.array_equal(m.triangle, tr)
+    assert np.array_equal(m.neighbor, nb) and np.array_equal(m.attribute, at)
+
+    py_path = str(tmp_path / "py.dgxm")
+    F.write_mesh_binary(py_path, mesh)
+    assert open(py_path, "rb").read() == open(c_path, "rb").read()
+    t, p = C.c_long(), C.c_long()
+    assert lib.dgx_mesh_binary_counts(py_path.encode(), C.byref(t), C.byref(p)) == 0 and (t.value, p.value) == (T, P)
+    pt2, tr2, nb2, at2 = np.zeros_like(pt), np.zeros_like(tr), np.zeros_like(nb), np.zeros_like(at)
+    assert lib.dgx_read_mesh_binary(py_path.encode(), T, P, L.dp(pt2), L.lp(tr2), L.lp(nb2), L.lp(at2)) == 0
+    assert np.array_equal(pt2, pt) and np.array_equal(tr2, tr) and np.array_equal(nb2, nb) and np.array_equal(at2, at)
+
+    # the text files of the same mesh read to the same arrays, except for the %16.7e rounding of coordinates
+    ele, node, neigh = F.write_mesh(tmp_path / "m", mesh)
+    pt3, tr3, nb3, at3 = np.zeros_like(pt), np.zeros_like(tr), np.zeros_like(nb), np.zeros_like(at)
+    assert lib.dgx_read_mesh(ele.encode(), node.encode(), neigh.encode(), T, P, L.dp(pt3), L.lp(tr3), L.lp(nb3),
+                             L.lp(at3)) == 0
+    assert np.array_equal(tr3, tr) and np.array_equal(nb3, nb) and np.array_equal(at3, at)
+    assert np.allclose(pt3, pt, rtol=1e-7, atol=0)
+
+    # wrong counts, a foreign file and a truncated file are errors, not reads past the end
+    assert lib.dgx_read_mesh_binary(py_path.encode(), T + 1, P, L.dp(pt2), L.lp(tr2), L.lp(nb2), L.lp(at2)) == L.DGX_EARG
+    assert lib.dgx_mesh_binary_counts(ele.encode(), C.byref(t), C.byref(p)) == L.DGX_EIO
+    trunc = str(tmp_path / "t.dgxm")
+    open(trunc, "wb").write(open(py_path, "rb").read()[:-9])
+    assert lib.dgx_read_mesh_binary(trunc.encode(), T, P, L.dp(pt2), L.lp(tr2), L.lp(nb2), L.lp(at2)) == L.DGX_EIO
