@@ -158,14 +158,13 @@ __device__ __forceinline__ void condition_stress6(const StepArgs<R> &a, unsigned
 }
 
 template <typename R>
-__device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int gstep)
+__device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int lstep)
 {
     R *__restrict__ Vx = a.f[0];
     R *__restrict__ Vz = a.f[1];
     const R *__restrict__ Txx = a.f[2];
     const R *__restrict__ Tzz = a.f[3];
     const R *__restrict__ Txz = a.f[4];
-    const int lstep = shot_step(a, s, gstep);
 
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
@@ -314,14 +313,13 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
 }
 
 template <typename R>
-__device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int gstep)
+__device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int lstep)
 {
     const R *__restrict__ Vx = a.f[0];
     const R *__restrict__ Vz = a.f[1];
     R *__restrict__ Txx = a.f[2];
     R *__restrict__ Tzz = a.f[3];
     R *__restrict__ Txz = a.f[4];
-    const int lstep = shot_step(a, s, gstep);
 
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
@@ -447,7 +445,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
 {
     int q, s;
     if (!my_work<ST>(a, q, s)) return;
-    velocity_body(a.op, a, q, s, *a.step_base + a.step_off);
+    velocity_body(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
 }
 
 template <typename R, int ST>
@@ -455,7 +453,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
 {
     int q, s;
     if (!my_work<ST>(a, q, s)) return;
-    stress_body(a.op, a, q, s, *a.step_base + a.step_off);
+    stress_body(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
 }
 
 template <typename R>
@@ -501,17 +499,23 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const Step
     const int chunk = (total + DGX_NXCD - 1) / DGX_NXCD;
     const int t_end = min(total, (xcd + 1) * chunk);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int b = 0;
+    const int s = lane % G; /* a lane keeps its slot for the whole launch */
+    int b = 0, cur = -1, slot_start = 0, slot_len = 0;
     for (int tile = xcd * chunk + (int)(blockIdx.x >> 3); tile < t_end; tile += per_xcd) {
         while (b + 1 < nb && sched[2 + b + 1] <= tile) b++;
         const StepArgs<R> a = global_args(table[b]);
-        const int q = (tile - sched[2 + b]) * (4 * EW) + wave * EW + lane / G, s = lane % G;
-        if (q >= a.tri) continue;
+        if (b != cur) { /* the lane's slot window, reloaded only when the batch changes */
+            slot_start = a.shot_start[s];
+            slot_len = a.shot_len[s];
+            cur = b;
+        }
+        const int q = (tile - sched[2 + b]) * (4 * EW) + wave * EW + lane / G;
+        const int t = gstep - slot_start;
         /* a slot that is not running (free, or its request has not started / has finished) is skipped:
          * its fields are never read again before the slot is cleared for a new request */
-        if (shot_step(a, s, gstep) < 0) continue;
-        if (VEL) velocity_body(op, a, q, s, gstep);
-        else stress_body(op, a, q, s, gstep);
+        if (q >= a.tri || t < 0 || t >= slot_len) continue;
+        if (VEL) velocity_body(op, a, q, s, t);
+        else stress_body(op, a, q, s, t);
     }
 }
 
@@ -557,6 +561,20 @@ static hipError_t dispatch_b(const dgx::StepArgs<R> &a, hipStream_t st, int whic
     return hipErrorInvalidValue;
 }
 
+/* Workgroups of one serving kernel that are resident on the whole device at once (occupancy x CUs),
+ * rounded down to a multiple of the XCD count: the persistent grid, so that every workgroup runs from the
+ * start and the tiles are shared evenly. */
+template <typename R, bool VEL, int G>
+static int serve_grid()
+{
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dgxb::ks_step<R, VEL, G>, DGX_BLOCK, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        per_cu < 1 || cus < 1)
+        return 0;
+    return (per_cu * cus) / DGX_NXCD * DGX_NXCD;
+}
+
 template <typename R>
 static hipError_t launch_serve(const void *ops, const void *table, const int *sched, const int *step_base,
                                int step_off, int vel, int group, int grid, hipStream_t st)
@@ -576,6 +594,16 @@ static hipError_t launch_serve(const void *ops, const void *table, const int *sc
 }
 
 extern "C" {
+int dgx_serve_grid(int fp32, int vel, int group)
+{
+    if (group != 16 && group != 64) return 0;
+    if (fp32) {
+        if (group == 16) return vel ? serve_grid<float, true, 16>() : serve_grid<float, false, 16>();
+        return vel ? serve_grid<float, true, 64>() : serve_grid<float, false, 64>();
+    }
+    if (group == 16) return vel ? serve_grid<double, true, 16>() : serve_grid<double, false, 16>();
+    return vel ? serve_grid<double, true, 64>() : serve_grid<double, false, 64>();
+}
 hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched, const int *step_base,
                             int step_off, int vel, int group, int grid, hipStream_t st)
 {

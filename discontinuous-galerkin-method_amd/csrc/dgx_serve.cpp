@@ -20,6 +20,7 @@
 extern "C" {
 hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched, const int *step_base,
                             int step_off, int vel, int group, int grid, hipStream_t st);
+int dgx_serve_grid(int fp32, int vel, int group);
 hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, int group, unsigned long long mask,
                                   hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
@@ -38,7 +39,6 @@ hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st
 namespace {
 
 const int MAXG = 64;      /* largest slot group (shots per batch = shot tile of the serving kernel) */
-const int GRID = 2048;    /* persistent grid: 8 XCDs x 256 blocks */
 const int MAX_BATCHES = 4096;
 
 struct MeshDev {
@@ -121,6 +121,7 @@ hipError_t grow(T **p, long *cap, long need)
 struct dgx_server {
     int fp32 = 1;
     int group = 16; /* shots per slot group */
+    int grid[2] = {0, 0}; /* persistent grids of the stress / velocity kernels: resident workgroups */
     long long budget = 0, in_use = 0;
     int K = 100;
     dgx_operators ops;
@@ -206,6 +207,14 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
     }
     sv->ops = *ops;
     make_ops(sv);
+    for (int vel = 0; vel < 2; vel++) {
+        sv->grid[vel] = dgx_serve_grid(sv->fp32, vel, sv->group);
+        if (sv->grid[vel] < DGX_NXCD) {
+            dgx_set_error("dgx_server_create: occupancy query for the serving kernel failed");
+            delete sv;
+            return DGX_EHIP;
+        }
+    }
     size_t fr = 0, tot = 0;
     SV_TRY(hipMemGetInfo(&fr, &tot));
     sv->budget = (cfg && cfg->hbm_budget > 0) ? cfg->hbm_budget : (long long)(0.8 * fr);
@@ -626,11 +635,11 @@ static int build_graph(dgx_server *sv)
     hipError_t e = hipSuccess;
     const void *ops = sv->fp32 ? (const void *)&sv->op32 : (const void *)&sv->op64;
     for (int k = 0; k < sv->K && e == hipSuccess; k++) {
-        e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 1, sv->group, GRID,
-                             sv->stream);
+        e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 1, sv->group,
+                             sv->grid[1], sv->stream);
         if (e == hipSuccess)
-            e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 0, sv->group, GRID,
-                                 sv->stream);
+            e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 0, sv->group,
+                                 sv->grid[0], sv->stream);
     }
     if (e == hipSuccess) e = dgx_launch_advance(sv->d_step_base, sv->K, sv->stream);
     hipError_t e2 = hipStreamEndCapture(sv->stream, &g);
