@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the seeded serving stream to run (0 = skip; 2000 = the whole stream)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-group", type=int, default=64, help="shots per slot group of the server (16 or 64)")
+    ap.add_argument("--serve-group", type=int, default=16,
+                    help="shots per slot group of the server (16 or 64; 16 suits this stream of 1-16-shot requests)")
     ap.add_argument("--copy-probe", type=int, default=1,
                     help="also time a plain device copy (achievable bandwidth, counter calibration)")
     return ap.parse_args()

@@ -69,6 +69,7 @@ struct Request {
 };
 
 struct Batch {
+    bool live = true; /* false: freed (all slots were empty when memory was needed); the entry is reused */
     int mesh = 0;
     void *fields = nullptr;
     long long field_bytes = 0;
@@ -147,6 +148,12 @@ struct dgx_server {
 
     size_t rsize() const { return fp32 ? 4 : 8; }
 };
+
+/* Device bytes of one slot group of mesh m (fields and the per-element word). */
+static long long group_bytes(const dgx_server *sv, const MeshDev &m)
+{
+    return (long long)5 * m.tri * 6 * sv->group * sv->rsize() + 4 * m.tri;
+}
 
 static void make_ops(dgx_server *sv)
 {
@@ -335,16 +342,48 @@ extern "C" int dgx_server_submit(dgx_server *sv, const dgx_request *rq, long *id
     if (rq->nrec) r.rec.assign(rq->rec, rq->rec + rq->nrec);
     r.arrival = rq->arrival;
     r.bytes = (long long)rq->nshots * rq->nsteps * (1 + 2 * rq->nrec) * 8;
+    if (r.bytes + group_bytes(sv, sv->meshes[rq->mesh]) > sv->budget) {
+        dgx_set_error("dgx_server_submit: the request needs %lld bytes with its slot group, more than the %lld-byte "
+                      "budget", r.bytes + group_bytes(sv, sv->meshes[rq->mesh]), sv->budget);
+        sv->next_id--;
+        return DGX_ENOMEM;
+    }
     *id = r.id;
     sv->queue.push_back(r.id);
     sv->reqs.emplace(r.id, std::move(r));
     return DGX_OK;
 }
 
+/* Free every slot group that has no request, so that its memory can go to a request for any mesh.
+ * Called between blocks of steps (the stream is idle); returns the number of groups freed. */
+static int reclaim_empty(dgx_server *sv)
+{
+    int n = 0;
+    for (int bi = 0; bi < (int)sv->batches.size(); bi++) {
+        Batch &b = sv->batches[bi];
+        if (!b.live || b.used() != 0) continue;
+        MeshDev &m = sv->meshes[b.mesh];
+        void *p[] = {b.fields, b.d_slot_int, b.d_slot_ptr, b.d_aux_of_elem, b.d_aux, b.d_src, b.d_rec, b.d_pairs};
+        for (void *x : p)
+            if (x) (void)hipFree(x);
+        sv->in_use -= group_bytes(sv, m);
+        m.batches.erase(std::remove(m.batches.begin(), m.batches.end(), bi), m.batches.end());
+        Batch dead;
+        dead.live = false;
+        sv->batches[bi] = std::move(dead);
+        n++;
+    }
+    if (n) sv->table_dirty = true;
+    return n;
+}
+
 static int new_batch(dgx_server *sv, int mesh, int *bi)
 {
     const MeshDev &m = sv->meshes[mesh];
-    if ((long)sv->batches.size() >= MAX_BATCHES) return DGX_ENOMEM;
+    int slot = -1;
+    for (int k = 0; k < (int)sv->batches.size() && slot < 0; k++)
+        if (!sv->batches[k].live) slot = k;
+    if (slot < 0 && (long)sv->batches.size() >= MAX_BATCHES) return DGX_ENOMEM;
     Batch b;
     b.mesh = mesh;
     b.G = sv->group;
@@ -365,9 +404,14 @@ static int new_batch(dgx_server *sv, int mesh, int *bi)
     SV_TRY(hipMalloc((void **)&b.d_slot_ptr, sizeof(double *) * 2 * G));
     SV_TRY(hipMalloc((void **)&b.d_aux_of_elem, sizeof(int) * m.tri));
     SV_TRY(hipMemsetAsync(b.d_aux_of_elem, 0, sizeof(int) * m.tri, sv->stream)); /* no aux, no source */
-    sv->in_use += b.field_bytes + 4 * m.tri;
-    sv->batches.push_back(std::move(b));
-    *bi = (int)sv->batches.size() - 1;
+    sv->in_use += group_bytes(sv, m);
+    if (slot >= 0) {
+        sv->batches[slot] = std::move(b);
+        *bi = slot;
+    } else {
+        sv->batches.push_back(std::move(b));
+        *bi = (int)sv->batches.size() - 1;
+    }
     sv->meshes[mesh].batches.push_back(*bi);
     sv->table_dirty = true;
     return DGX_OK;
@@ -510,7 +554,7 @@ static int upload_table(dgx_server *sv)
     int nb = 0, tiles = 0;
     size_t asz = sv->fp32 ? sizeof(dgx::StepArgs<float>) : sizeof(dgx::StepArgs<double>);
     for (auto &b : sv->batches) {
-        if (b.used() == 0) continue;
+        if (!b.live || b.used() == 0) continue;
         table.resize((nb + 1) * asz);
         if (sv->fp32) fill_batch_args(sv, b, sv->op32, *(dgx::StepArgs<float> *)(table.data() + nb * asz));
         else fill_batch_args(sv, b, sv->op64, *(dgx::StepArgs<double> *)(table.data() + nb * asz));
@@ -545,7 +589,12 @@ static int admit(dgx_server *sv)
             if (fr >= r.nshots && fr < best_free) { best = bi; best_free = fr; }
         }
         long long need = r.bytes;
-        if (best < 0) need += (long long)5 * m.tri * 6 * G * sv->rsize() + 4 * m.tri;
+        if (best < 0) need += group_bytes(sv, m);
+        if (sv->in_use + need > sv->budget && best < 0 && reclaim_empty(sv) > 0) {
+            /* memory held by idle groups (of any mesh) went back to the budget: look again */
+            sv->queue.push_front(id);
+            continue;
+        }
         if (sv->in_use + need > sv->budget) { still.push_back(id); continue; }
         if (best < 0) {
             int rc = new_batch(sv, r.mesh, &best);
@@ -662,7 +711,7 @@ extern "C" int dgx_server_advance(dgx_server *sv, int nblocks)
         int rc = admit(sv);
         if (rc) return rc;
         for (auto &b : sv->batches)
-            if (b.dirty) {
+            if (b.live && b.dirty) {
                 rc = refresh_batch(sv, b);
                 if (rc) return rc;
             }
@@ -739,7 +788,7 @@ extern "C" int dgx_server_get_stats(const dgx_server *sv, dgx_server_stats *st)
         else if (kv.second.state == DGX_REQ_RUNNING) st->running++;
         else st->done++;
     }
-    st->batches = (long)sv->batches.size();
+    for (const auto &b : sv->batches) st->batches += b.live ? 1 : 0;
     st->bytes_in_use = sv->in_use;
     st->dof_updates = sv->dof_updates;
     st->busy_seconds = sv->busy;

@@ -5,7 +5,8 @@
     python -m dgx.runs serving   [--requests 2000 --rate 20] open-loop request stream on 8 meshes
     python -m dgx.runs carbonate [--shots 256 --steps 8000] 256 shots through the request loop, FP64 on 4
 
-The serving runs take --group (16 or 64 shots per slot group, default 64).
+The serving runs take --group (16 or 64 shots per slot group; default 16 for the request stream of 1-16-shot
+requests over 8 meshes, 64 for carbonate's 16-shot requests on one mesh).
 
 Progress lines go to stderr while a run is going."""
 from __future__ import annotations
@@ -171,11 +172,13 @@ def main(argv=None):
     ap.add_argument("--shots", type=int, default=0)
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=20.0)
-    ap.add_argument("--group", type=int, default=64, help="shots per slot group of the server (16 or 64)")
+    ap.add_argument("--group", type=int, default=0,
+                    help="shots per slot group of the server, 16 or 64 (default: 16 for serving, 64 for carbonate)")
     a = ap.parse_args(argv)
     defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256)}
     a.steps = a.steps or defaults[a.which][0]
     a.shots = a.shots or defaults[a.which][1]
+    a.group = a.group or (64 if a.which == "carbonate" else 16)
     fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate}[a.which]
     print(json.dumps(fn(a)), flush=True)
 

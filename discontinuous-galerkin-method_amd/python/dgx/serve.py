@@ -132,6 +132,9 @@ class Server:
             if s["queued"] == 0 and s["running"] == 0:
                 return
             self.advance(1)
+            t = self.stats()
+            if t["running"] == 0 and t["queued"] == s["queued"] and t["done"] == s["done"]:
+                raise L.DgxError("queued requests cannot be admitted (HBM budget)")
 
 
 def run_open_loop(server: Server, requests: list[dict], arrivals: np.ndarray, keep_results: bool = False,

@@ -120,3 +120,30 @@ def test_tight_budget_queues_and_completes():
     sv.drain()
     assert all(sv.status(i)[0] == 2 for i in ids)
     sv.close()
+
+
+def test_idle_groups_are_reclaimed_for_another_mesh():
+    """With room for one slot group only, a request on a second mesh still runs once the first mesh's
+    group is idle: idle groups are freed when a queued request does not fit."""
+    sp, ms = meshes()
+    T0, T1 = ms[0][0].tri, ms[1][0].tri
+    budget = 5 * max(T0, T1) * 6 * 16 * 4 + 4 * max(T0, T1) + 2_000_000
+    sv = S.Server(L.DGX_FP32, budget, 50)
+    rm0, rm1 = sv.add_mesh(ms[0][0], ms[0][1], sp), sv.add_mesh(ms[1][0], ms[1][1], sp)
+    a = sv.submit(sv.make_request(rm0, [(900.0, 900.0)], [15.0], 100, [[1000.0, 1500.0]]))
+    sv.drain()
+    b = sv.submit(sv.make_request(rm1, [(900.0, 900.0)], [15.0], 100, [[1000.0, 1500.0]]))
+    sv.drain()
+    assert sv.status(a)[0] == 2 and sv.status(b)[0] == 2
+    assert sv.stats()["batches"] == 1
+    sv.close()
+
+
+def test_request_that_can_never_fit_is_rejected():
+    sp, ms = meshes()
+    sv = S.Server(L.DGX_FP32, 1_000_000, 50)  # smaller than one slot group
+    rm0 = sv.add_mesh(ms[0][0], ms[0][1], sp)
+    with pytest.raises(L.DgxError):
+        sv.submit(sv.make_request(rm0, [(900.0, 900.0)], [15.0], 100, [[1000.0, 1500.0]]))
+    assert sv.stats()["queued"] == 0
+    sv.close()
