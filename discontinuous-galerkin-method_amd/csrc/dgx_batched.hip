@@ -445,6 +445,10 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
 {
     int q, s;
     if (!my_work<ST>(a, q, s)) return;
+    /* with 64 shots per tile a wavefront holds one element: saying so turns the element's records, its
+     * neighbour ids and the row offsets into scalar values (measured: faster for the velocity kernel,
+     * slightly slower for the stress kernel, which keeps the per-lane form) */
+    if (ST == 64) q = __builtin_amdgcn_readfirstlane(q);
     velocity_body(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
 }
 
@@ -509,7 +513,8 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const Step
             slot_len = a.shot_len[s];
             cur = b;
         }
-        const int q = (tile - sched[2 + b]) * (4 * EW) + wave * EW + lane / G;
+        int q = (tile - sched[2 + b]) * (4 * EW) + wave * EW + lane / G;
+        if (G == 64 && VEL) q = __builtin_amdgcn_readfirstlane(q); /* one element per wavefront */
         const int t = gstep - slot_start;
         /* a slot that is not running (free, or its request has not started / has finished) is skipped:
          * its fields are never read again before the slot is cleared for a new request */

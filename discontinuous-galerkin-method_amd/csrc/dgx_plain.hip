@@ -24,6 +24,7 @@
  * (OpenMP_Function.c:25-46) is applied to every value as it is loaded, the source increment
  * (OpenMP_Function.c:49-58) right after it, and the receiver pick (OpenMP_Function.c:61-72) is done by
  * the stress kernel from the velocities it already holds.  Each field is read once per half-step.
+ * k_fullstep runs both half-steps in one launch (opt-in; see its comment).
  */
 #include <hip/hip_runtime.h>
 

@@ -3,9 +3,9 @@
  *
  * No counterpart in the reference, which runs one shot per process (DG_leap_Frog_OpenMP.c:397 loops over
  * a single shot).  The server keeps meshes, materials and sponge factors resident in HBM and runs many
- * shots of many requests at once: every request is 1..16 shots on one resident mesh, each shot with its
- * own source, FP64 wavelet samples and start step, all shots of a request sharing one receiver spread.
- * A request holds 1..group_shots shots (16 or 64, dgx_server_config).
+ * shots of many requests at once: every request is 1..group_shots shots (16 or 64, dgx_server_config) on
+ * one resident mesh, each shot with its own source, FP64 wavelet samples and start step, all shots of a
+ * request sharing one receiver spread.
  * Requests are admitted as slots and memory free up (continuous batching), stepped by one persistent
  * kernel per half-step over all active (mesh, slot group) batches, in hipGraph-captured blocks of steps,
  * and retired with their traces copied back.  Every shot computes exactly what a single-shot solve of
@@ -46,7 +46,7 @@ typedef struct {
 
 typedef struct {
     long queued, running, done;
-    long batches;               /* allocated (mesh, 16-slot group) batches */
+    long batches;               /* allocated (mesh, slot group) batches */
     long long bytes_in_use;     /* wavefields + per-request buffers */
     long long dof_updates;      /* useful DOF-updates advanced so far (active shots only) */
     double busy_seconds;        /* wall time spent in dgx_server_advance */
