@@ -509,6 +509,17 @@ int dgx_build_mesh_host(const dgx_problem *p, int shot_inner, long tri_pad, cons
     return DGX_OK;
 }
 
+/* HIP_TRY for dgx_create: a failure frees the partly built solver */
+#define CREATE_TRY(call)                                                                                       \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess) {                                                                                \
+            dgx_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__);          \
+            free_solver(s);                                                                                    \
+            return e_ == hipErrorOutOfMemory ? DGX_ENOMEM : DGX_EHIP;                                          \
+        }                                                                                                      \
+    } while (0)
+
 extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_solver **out)
 {
     if (!p || !ops || !out) { dgx_set_error("dgx_create: null argument"); return DGX_EARG; }
@@ -677,56 +688,60 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         free_solver(s);
         return DGX_EHIP;
     }
-    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    for (auto &ev : s->ev) HIP_TRY(hipEventCreate(&ev));
+    CREATE_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    for (auto &ev : s->ev) CREATE_TRY(hipEventCreate(&ev));
     size_t nreal_fields = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)5 * s->tri * 6 * s->spad
                                                                  : (size_t)5 * s->nshots * s->vrows * 6;
-    HIP_TRY(hipMalloc(&s->fields, nreal_fields * s->real_size()));
-    HIP_TRY(hipMemset(s->fields, 0, nreal_fields * s->real_size()));
+    CREATE_TRY(hipMalloc(&s->fields, nreal_fields * s->real_size()));
+    CREATE_TRY(hipMemset(s->fields, 0, nreal_fields * s->real_size()));
     if (s->full) {
-        HIP_TRY(hipMalloc(&s->fields2, nreal_fields * s->real_size()));
-        HIP_TRY(hipMemset(s->fields2, 0, nreal_fields * s->real_size()));
-        HIP_TRY(upload(&s->tiles, tiles_host));
-        HIP_TRY(upload(&s->ring, ring_host));
-        HIP_TRY(upload(&s->nb2, nb2_host));
+        CREATE_TRY(hipMalloc(&s->fields2, nreal_fields * s->real_size()));
+        CREATE_TRY(hipMemset(s->fields2, 0, nreal_fields * s->real_size()));
+        CREATE_TRY(upload(&s->tiles, tiles_host));
+        CREATE_TRY(upload(&s->ring, ring_host));
+        CREATE_TRY(upload(&s->nb2, nb2_host));
     }
-    HIP_TRY(upload(&s->nbr, nbr_host));
-    HIP_TRY(upload(&s->vtx, vtx_host));
-    HIP_TRY(upload(&s->pt, pt_host));
+    CREATE_TRY(upload(&s->nbr, nbr_host));
+    CREATE_TRY(upload(&s->vtx, vtx_host));
+    CREATE_TRY(upload(&s->pt, pt_host));
     if (s->fp32) {
-        float *m, *a;
-        HIP_TRY(upload(&m, convert<float>(mat_host)));
-        HIP_TRY(upload(&a, convert<float>(absorb_host)));
+        float *m = nullptr, *a = nullptr;
+        hipError_t e = upload(&m, convert<float>(mat_host));
         s->mat = m;
+        CREATE_TRY(e);
+        e = upload(&a, convert<float>(absorb_host));
         s->absorb = a;
+        CREATE_TRY(e);
     } else {
-        double *m, *a;
-        HIP_TRY(upload(&m, mat_host));
-        HIP_TRY(upload(&a, absorb_host));
+        double *m = nullptr, *a = nullptr;
+        hipError_t e = upload(&m, mat_host);
         s->mat = m;
+        CREATE_TRY(e);
+        e = upload(&a, absorb_host);
         s->absorb = a;
+        CREATE_TRY(e);
     }
-    HIP_TRY(upload(&s->aux, aux_host));
+    CREATE_TRY(upload(&s->aux, aux_host));
     {
         /* fused kernels read sources through the aux ranges; the split source kernel reads the list in
          * caller order: keep both, aux-ordered first */
         std::vector<dgx_src_dev> all = src_host;
         all.insert(all.end(), src_list.begin(), src_list.end());
-        HIP_TRY(upload(&s->src, all));
+        CREATE_TRY(upload(&s->src, all));
     }
-    HIP_TRY(upload(&s->rec, rec_host));
+    CREATE_TRY(upload(&s->rec, rec_host));
     if (p->nsrc > 0) {
         std::vector<double> w(p->wavelet, p->wavelet + (size_t)p->nwavelets * p->nsteps);
-        HIP_TRY(upload(&s->wav, w));
+        CREATE_TRY(upload(&s->wav, w));
     }
     size_t ntrace = (size_t)s->nshots * s->nsteps * 2 * s->nrec;
     if (ntrace) {
-        HIP_TRY(hipMalloc((void **)&s->trace, ntrace * sizeof(double)));
-        HIP_TRY(hipMemset(s->trace, 0, ntrace * sizeof(double)));
+        CREATE_TRY(hipMalloc((void **)&s->trace, ntrace * sizeof(double)));
+        CREATE_TRY(hipMemset(s->trace, 0, ntrace * sizeof(double)));
     }
-    HIP_TRY(hipMalloc((void **)&s->step_base, sizeof(int)));
-    HIP_TRY(hipMemset(s->step_base, 0, sizeof(int)));
-    HIP_TRY(hipDeviceSynchronize());
+    CREATE_TRY(hipMalloc((void **)&s->step_base, sizeof(int)));
+    CREATE_TRY(hipMemset(s->step_base, 0, sizeof(int)));
+    CREATE_TRY(hipDeviceSynchronize());
 
     if (s->fp32) fill_args(s, s->a32, ops, p->dt);
     else fill_args(s, s->a64, ops, p->dt);

@@ -398,12 +398,21 @@ static int new_batch(dgx_server *sv, int mesh, int *bi)
         b.h_wav[k] = nullptr;
         b.h_trace[k] = nullptr;
     }
-    SV_TRY(hipMalloc(&b.fields, b.field_bytes));
-    SV_TRY(hipMemsetAsync(b.fields, 0, b.field_bytes, sv->stream));
-    SV_TRY(hipMalloc((void **)&b.d_slot_int, sizeof(int) * 3 * G));
-    SV_TRY(hipMalloc((void **)&b.d_slot_ptr, sizeof(double *) * 2 * G));
-    SV_TRY(hipMalloc((void **)&b.d_aux_of_elem, sizeof(int) * m.tri));
-    SV_TRY(hipMemsetAsync(b.d_aux_of_elem, 0, sizeof(int) * m.tri, sv->stream)); /* no aux, no source */
+    hipError_t e = hipMalloc(&b.fields, b.field_bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(b.fields, 0, b.field_bytes, sv->stream);
+    if (e == hipSuccess) e = hipMalloc((void **)&b.d_slot_int, sizeof(int) * 3 * G);
+    if (e == hipSuccess) e = hipMalloc((void **)&b.d_slot_ptr, sizeof(double *) * 2 * G);
+    if (e == hipSuccess) e = hipMalloc((void **)&b.d_aux_of_elem, sizeof(int) * m.tri);
+    if (e == hipSuccess) e = hipMemsetAsync(b.d_aux_of_elem, 0, sizeof(int) * m.tri, sv->stream); /* no aux, no source */
+    if (e != hipSuccess) {
+        /* nothing of the half-built group is kept */
+        (void)hipStreamSynchronize(sv->stream);
+        void *p[] = {b.fields, b.d_slot_int, b.d_slot_ptr, b.d_aux_of_elem};
+        for (void *x : p)
+            if (x) (void)hipFree(x);
+        dgx_set_error("slot group allocation failed: %s", hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? DGX_ENOMEM : DGX_EHIP;
+    }
     sv->in_use += group_bytes(sv, m);
     if (slot >= 0) {
         sv->batches[slot] = std::move(b);
