@@ -9,12 +9,14 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
 
 #include "dgx.h"
 #include "dgx_kernels.h"
+#include "dgx_p2ops.h"
 
 extern "C" {
 hipError_t dgx_launch_f64(const void *args, int nshots, hipStream_t st, int which);
@@ -269,6 +271,15 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     const double *S[3] = {ops->S1, ops->S2, ops->S3};
     for (int j = 0; j < 3; j++)
         for (int k = 0; k < 18; k++) a.op.L[j * 18 + k] = (R)S[j][k];
+    /* The velocity kernels have a variant with the P2 operators of the shipped operator files compiled
+     * in; it serves this problem when its operators equal that table bit for bit.  The results agree
+     * with the argument variant to rounding (with constants the compiler may fuse a different product
+     * into an FMA).  DGX_GENERIC_OPERATORS=1 forces the variant that reads them from the argument. */
+    static_assert(sizeof(a.op) == 126 * sizeof(R), "Ops<R> is Dr | Ds | L, packed");
+    R table[126];
+    for (int k = 0; k < 126; k++) table[k] = (R)dgx_p2_ops[k];
+    const char *generic = getenv("DGX_GENERIC_OPERATORS");
+    a.p2 = (memcmp(table, &a.op, sizeof table) == 0 && !(generic && strcmp(generic, "1") == 0)) ? 1 : 0;
     R *f = (R *)s->fields, *f2 = (R *)s->fields2;
     size_t per_field = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)s->tri * 6 * s->spad
                                                              : (size_t)s->nshots * s->vrows * 6;
@@ -1066,6 +1077,13 @@ extern "C" int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile,
         if (ms_per_step) ms_per_step[k] = s->tune_ms[k];
     }
     return n;
+}
+
+extern "C" int dgx_compiled_operators(const dgx_solver *s)
+{
+    if (!s) return DGX_EARG;
+    if (s->layout != DGX_LAYOUT_ELEMENT) return 0;
+    return s->fp32 ? s->a32.p2 : s->a64.p2;
 }
 
 extern "C" int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stress)

@@ -100,6 +100,7 @@ struct StepArgs {
     const int *ring;       /* ring entries: element ids, tile after tile */
     int ntile, tper;       /* tiles; tiles per XCD slice */
     int vrow0;             /* velocity row of ring entry 0 (rows below it are elements) */
+    int p2;                /* element layout: op equals the compiled-in P2 table (dgx_p2ops.h) */
 };
 
 } // namespace dgx

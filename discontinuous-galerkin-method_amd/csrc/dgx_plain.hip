@@ -29,10 +29,38 @@
 #include <hip/hip_runtime.h>
 
 #include "dgx_kernels.h"
+#include "dgx_p2ops.h"
 
 namespace dgx {
 
 __device__ __forceinline__ int fidx(int q, int i) { return ((q >> 6) * 6 + i) * 64 + (q & 63); }
+
+/* Reference-element operators of the velocity update.  P2: the P2 Fekete operators of the shipped
+ * operator files, compiled in as immediates (the host selects these kernels only when the caller's
+ * operators equal them bit for bit; results agree with the argument variant to rounding, since the
+ * compiler may fuse a different product into an FMA); otherwise the caller's operators from the kernel
+ * argument.  As
+ * arguments, the 126 FP64 values do not fit the scalar registers and the FP64 velocity kernel spills
+ * about 200 of them to vector-register lanes; compiled in, it spills none.  The stress update keeps the
+ * argument operators: compiled in, its FP64 kernel needs more than the 168 VGPRs of 3 waves per SIMD. */
+template <bool P2, typename R>
+__device__ __forceinline__ R op_dr(const StepArgs<R> &a, int k)
+{
+    if constexpr (P2) return (R)dgx_p2_ops[k];
+    else return a.op.Dr[k];
+}
+template <bool P2, typename R>
+__device__ __forceinline__ R op_ds(const StepArgs<R> &a, int k)
+{
+    if constexpr (P2) return (R)dgx_p2_ops[36 + k];
+    else return a.op.Ds[k];
+}
+template <bool P2, typename R>
+__device__ __forceinline__ R op_l(const StepArgs<R> &a, int k)
+{
+    if constexpr (P2) return (R)dgx_p2_ops[72 + k];
+    else return a.op.L[k];
+}
 
 /* Workgroups are dealt to the 8 XCDs round-robin; give each XCD one contiguous slice of the
  * (Hilbert-ordered) elements so that face neighbours mostly share that XCD's L2. */
@@ -136,7 +164,7 @@ __device__ __forceinline__ void condition_stress6(const StepArgs<R> &a, unsigned
  * kernel, which also keeps the conditioned stresses of its own elements in LDS (when `save`:
  * tsave[(c*6 + i)*DGX_BLOCK + threadIdx.x], c = xx, zz, xz).
  */
-template <typename R, bool FUSED, bool INPLACE>
+template <typename R, bool FUSED, bool INPLACE, bool P2>
 __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int shot, int step, int orow,
                                                 const R *__restrict__ Txx, const R *__restrict__ Tzz,
                                                 const R *__restrict__ Txz, const R *__restrict__ Vxi,
@@ -189,8 +217,8 @@ __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int
         const R uz = rx * txz[n] + rz * tzz[n], wz = sx * txz[n] + sz * tzz[n];
 #pragma unroll
         for (int i = 0; i < 6; i++) {
-            rhx[i] += a.op.Dr[i * 6 + n] * ux + a.op.Ds[i * 6 + n] * wx;
-            rhz[i] += a.op.Dr[i * 6 + n] * uz + a.op.Ds[i * 6 + n] * wz;
+            rhx[i] += op_dr<P2>(a, i * 6 + n) * ux + op_ds<P2>(a, i * 6 + n) * wx;
+            rhz[i] += op_dr<P2>(a, i * 6 + n) * uz + op_ds<P2>(a, i * 6 + n) * wz;
         }
     }
 
@@ -236,8 +264,8 @@ __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int
                 const R gz = ez * dxz - ex * dzz;
 #pragma unroll
                 for (int i = 0; i < 6; i++) {
-                    flx[i] += a.op.L[j * 18 + i * 3 + k] * gx;
-                    flz[i] += a.op.L[j * 18 + i * 3 + k] * gz;
+                    flx[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gx;
+                    flz[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gz;
                 }
             }
         }
@@ -425,7 +453,7 @@ __device__ __forceinline__ void stress_update(const StepArgs<R> &a, int q, int s
 
 /* The element kernels ask for at least 3 wavefronts per SIMD: this caps the FP64 kernels at 168 VGPRs
  * (unbounded they take 170-171, which leaves room for 2), without spilling. */
-template <typename R, bool FUSED>
+template <typename R, bool FUSED, bool P2>
 __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
 {
     const int q = element_of_thread(a.nper);
@@ -434,7 +462,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
     const long long so = (long long)shot * a.fstride * 6;
     const int step = FUSED ? (*a.step_base + a.step_off) : 0;
     R *V0 = a.f[0] + so, *V1 = a.f[1] + so;
-    velocity_update<R, FUSED, true>(a, q, shot, step, q, a.f[2] + so, a.f[3] + so, a.f[4] + so, nullptr, nullptr,
+    velocity_update<R, FUSED, true, P2>(a, q, shot, step, q, a.f[2] + so, a.f[3] + so, a.f[4] + so, nullptr, nullptr,
                                     V0, V1, nullptr, false);
 }
 
@@ -467,7 +495,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_stress(StepArgs<R> a)
  * workgroup, tiles never wait on each other.  Fields move once per step: 5 reads + 5 writes per
  * element-node, against 7 + 8 for the two half-step kernels.
  */
-template <typename R>
+template <typename R, bool P2>
 __global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
 {
     __shared__ R tc[18 * DGX_BLOCK];
@@ -490,7 +518,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
         orow = a.vrow0 + k;
     }
     if (q >= 0)
-        velocity_update<R, true, false>(a, q, shot, step, orow, a.f[2] + so, a.f[3] + so, a.f[4] + so, a.f[0] + so,
+        velocity_update<R, true, false, P2>(a, q, shot, step, orow, a.f[2] + so, a.f[3] + so, a.f[4] + so, a.f[0] + so,
                                  a.f[1] + so, a.g[0] + so, a.g[1] + so, tc, lane < td.y);
     __syncthreads();
     if (lane < td.y)
@@ -610,18 +638,25 @@ __global__ void k_advance(int *step_base, int n)
 /* ---- host-side launchers (C linkage inside the library only) ---- */
 using dgx::StepArgs;
 
-template <typename R>
-static hipError_t launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st, int which)
+template <typename R, bool P2>
+static void launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st, int which)
 {
     dim3 grid(DGX_NXCD * a.nper, nshots), block(DGX_BLOCK);
     switch (which) {
-    case 0: dgx::k_velocity<R, true><<<grid, block, 0, st>>>(a); break;
+    case 0: dgx::k_velocity<R, true, P2><<<grid, block, 0, st>>>(a); break;
     case 1: dgx::k_stress<R, true><<<grid, block, 0, st>>>(a); break;
-    case 2: dgx::k_velocity<R, false><<<grid, block, 0, st>>>(a); break;
+    case 2: dgx::k_velocity<R, false, P2><<<grid, block, 0, st>>>(a); break;
     case 3: dgx::k_stress<R, false><<<grid, block, 0, st>>>(a); break;
     case 4: dgx::k_absorb<R><<<grid, block, 0, st>>>(a); break;
-    case 5: dgx::k_fullstep<R><<<dim3(DGX_NXCD * a.tper, nshots), block, 0, st>>>(a); break;
+    case 5: dgx::k_fullstep<R, P2><<<dim3(DGX_NXCD * a.tper, nshots), block, 0, st>>>(a); break;
     }
+}
+
+template <typename R>
+static hipError_t launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st, int which)
+{
+    if (a.p2) launch_step_kernels<R, true>(a, nshots, st, which);
+    else launch_step_kernels<R, false>(a, nshots, st, which);
     return hipGetLastError();
 }
 

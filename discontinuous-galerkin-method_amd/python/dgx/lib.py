@@ -109,6 +109,7 @@ def load() -> C.CDLL:
         "dgx_kernel_bench": (C.c_int, [vp, C.c_int, C.c_long, dptr]),
         "dgx_copy_probe": (C.c_int, [C.c_long, C.c_long, C.c_int, dptr]),
         "dgx_layout_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), dptr, C.c_int]),
+        "dgx_compiled_operators": (C.c_int, [vp]),
         "dgx_read_ascii": (C.c_long, [C.c_char_p, dptr, C.c_long, C.c_long]),
         "dgx_read_ascii_long": (C.c_long, [C.c_char_p, lptr, C.c_long, C.c_long]),
         "dgx_read_operators": (C.c_int, [C.c_char_p, C.POINTER(Operators)]),

@@ -244,6 +244,12 @@ class Solver:
         n = L.check(self.lib.dgx_layout_info(self.handle, C.byref(lay), C.byref(tile), tiles, ms, 8), "layout_info")
         return lay.value, tile.value, {tiles[k]: ms[k] for k in range(n)}
 
+    @property
+    def compiled_operators(self) -> bool:
+        """True when the element kernels run with the P2 operators compiled in (the problem's operators
+        equal the shipped ones bit for bit and DGX_GENERIC_OPERATORS=1 was not set at creation)."""
+        return bool(L.check(self.lib.dgx_compiled_operators(self.handle), "dgx_compiled_operators"))
+
     def kernel_times(self, n: int) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()
         L.check(self.lib.dgx_kernel_times(self.handle, n, C.byref(a), C.byref(b)), "dgx_kernel_times")

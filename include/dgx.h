@@ -135,6 +135,12 @@ int dgx_copy_probe(long bytes, long reps, int variant, double *ms);
 /* Layout and shot tile in use (after autotuning), and the autotuner's timings: for each candidate
  * tile k, tiles[k] and ms_per_step[k]; returns the number of candidates (0 if none were timed). */
 int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile, int *tiles, double *ms_per_step, int cap);
+/* 1 when the element-layout velocity kernels of this solver run with the P2 Fekete operators of the
+ * shipped operator files compiled in (its operators equal them bit for bit and DGX_GENERIC_OPERATORS=1 is not
+ * set), 0 when they read the operators from their kernel argument or the layout is shot-innermost.
+ * Diagnostic with no counterpart in the reference, which reads its operators from files at start-up
+ * (DG_leap_Frog_OpenMP.c:52-81); the two variants agree to rounding. */
+int dgx_compiled_operators(const dgx_solver *s);
 const char *dgx_last_error(void);
 
 #ifdef __cplusplus

@@ -132,6 +132,42 @@ def test_full_step_kernel_equals_half_step_kernels(precision, tol):
             assert e <= tol, (k, c, e)
 
 
+@pytest.mark.parametrize("precision,tol", [(L.DGX_FP64, 1e-13), (L.DGX_FP32, 1e-5)])
+def test_compiled_operators_equal_argument_operators(precision, tol, monkeypatch):
+    """The velocity kernel with the P2 operators compiled in against the one that reads them from its
+    argument.  Same operator values; with constants the compiler may fuse a different product of a sum
+    into an FMA (x*2 becomes x+x, for one), so the two agree to rounding, not bit for bit: traces and
+    every snapshot within the full-step bar.  A change to one lift entry that survives the cast to the
+    working precision selects the argument variant."""
+    import dataclasses
+    w = two_media_workload(120)
+    st = product_setup(w)
+    print(f"case two_media compiled vs argument operators, precision {precision}", flush=True)
+    s = dgx.Solver(st, precision)
+    assert s.compiled_operators
+    s.close()
+    comp = product_run(st, precision, snap_every=40)
+    monkeypatch.setenv("DGX_GENERIC_OPERATORS", "1")
+    s = dgx.Solver(st, precision)
+    assert not s.compiled_operators
+    s.close()
+    arg = product_run(st, precision, snap_every=40)
+    monkeypatch.delenv("DGX_GENERIC_OPERATORS")
+    ex, ez = rel_l2(comp[0], arg[0]), rel_l2(comp[1], arg[1])
+    print(f"compiled vs argument operators: traces rel L2 Vx {ex:.3e} Vz {ez:.3e}", flush=True)
+    assert ex <= tol and ez <= tol
+    assert sorted(comp[2]) == [40, 80, 120]
+    for k in comp[2]:
+        for c in range(5):
+            e = rel_l2(comp[2][k][c], arg[2][k][c])
+            assert e <= tol, (k, c, e)
+    ops = L.Operators.from_buffer_copy(st.ops)
+    ops.S1[0] *= 1.0 + 2.0 ** -20
+    s = dgx.Solver(dataclasses.replace(st, ops=ops), precision)
+    assert not s.compiled_operators
+    s.close()
+
+
 def test_shots_are_independent():
     w = two_media_workload(200)
     w.sources = [dgx.SourceSpec(1200.0, 1200.0, L.DGX_SRC_FORCE_Z, 0, 15.0),
