@@ -188,9 +188,10 @@ __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int
         if (sizeof(R) == 4) { vxe[i] = Vxl[o]; vze[i] = Vzl[o]; } /* FP32: issue every load up front */
     }
     if (FUSED) {
-        /* per element (one branch each for sponge and sources) in FP32; per node in FP64, where the
-         * per-element form needs more registers than the 168 that keep 3 wavefronts per SIMD */
-        if constexpr (sizeof(R) == 4) {
+        /* per element (one branch each for sponge and sources) in FP32 and with compiled-in operators;
+         * per node in FP64 with argument operators, where the per-element form needs more registers
+         * than the 168 that keep 3 wavefronts per SIMD */
+        if constexpr (sizeof(R) == 4 || P2) {
             condition_stress6(a, code, q, shot, step, txx, tzz, txz);
         } else {
 #pragma unroll
