@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the seeded serving stream to run (0 = skip; 2000 = the whole stream)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=0.0,
+                    help="HBM budget of the serving leg's wavefields and request buffers (0 = 80%% of free HBM)")
     ap.add_argument("--serve-group", type=int, default=16,
                     help="shots per slot group of the server (16 or 64; 16 suits this stream of 1-16-shot requests)")
     ap.add_argument("--copy-probe", type=int, default=1,
@@ -125,7 +127,7 @@ def serving_leg(a, rate64):
     ms = dgx.configs.serving_meshes()
     arrivals, specs = dgx.configs.serving_requests(ms, n=max(a.serve_requests, 1), rate=a.serve_rate)
     arrivals, specs = arrivals[:a.serve_requests], specs[:a.serve_requests]
-    sv = SV.Server(L.DGX_FP32, 0, 100, group_shots=a.serve_group)
+    sv = SV.Server(L.DGX_FP32, int(a.serve_budget_gb * 2**30), 100, group_shots=a.serve_group)
     rms = [sv.add_mesh(m, md, sp) for m, md, sp in ms]
     reqs = [sv.make_request(rms[r["mesh"]], r["xz"], [r["f0"]] * len(r["xz"]), r["nsteps"], r["rec"])
             for r in specs]
@@ -133,11 +135,13 @@ def serving_leg(a, rate64):
     sv.close()
     dof = st["dof_updates"]
     return {"requests": len(reqs), "arrival_rate_per_s": a.serve_rate, "meshes": [m.tri for m, _, _ in ms],
-            "group_shots": a.serve_group,
+            "group_shots": a.serve_group, "hbm_budget_gb": a.serve_budget_gb or "80% of free",
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "wall_s": wall, "dof_updates": dof,
             "value": dof / wall, "unit": "DOF-updates/s", "dtype": "f32",
             "x_vs_plain_fp64": dof / wall / rate64, "busy_s": st["busy_seconds"],
+            "gpu_s": st["gpu_seconds"], "slot_occupancy": dof / max(st["lane_updates"], 1), "moved": st["moved"],
+            "lane_rate": st["lane_updates"] / max(st["gpu_seconds"], 1e-9),
             "sample": f"first {len(reqs)} requests of the seeded stream (2000 in the full stream)"}
 
 

@@ -542,7 +542,47 @@ __global__ void k_clear_slots(R *f0, R *f1, R *f2, R *f3, R *f4, long long n_nod
     }
 }
 
+/* Slot lanes moved between two G-slot groups of one mesh (serving compaction). */
+template <typename R>
+struct MoveArgs {
+    const R *src[5];
+    R *dst[5];
+    int from[64], to[64];
+};
+
+/* dst[c][node*G + to[k]] = src[c][node*G + from[k]] for every field c, node and move k < n: the moved
+ * shots continue bit for bit in their new slots */
+template <typename R>
+__global__ void k_move_slots(MoveArgs<R> m, long long n_nodes, int G, int n)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_nodes * n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long node = t / n;
+        const int k = (int)(t - node * n);
+        const long long s = node * G + m.from[k], d = node * G + m.to[k];
+#pragma unroll
+        for (int c = 0; c < 5; c++) m.dst[c][d] = m.src[c][s];
+    }
+}
+
 } // namespace dgxb
+
+template <typename R>
+static hipError_t launch_move(void *const *src, void *const *dst, long long n_nodes, int group, int n, const int *from,
+                              const int *to, hipStream_t st)
+{
+    dgxb::MoveArgs<R> m;
+    for (int c = 0; c < 5; c++) {
+        m.src[c] = (const R *)src[c];
+        m.dst[c] = (R *)dst[c];
+    }
+    for (int k = 0; k < 64; k++) {
+        m.from[k] = k < n ? from[k] : 0;
+        m.to[k] = k < n ? to[k] : 0;
+    }
+    dgxb::k_move_slots<R><<<2048, 256, 0, st>>>(m, n_nodes, group, n);
+    return hipGetLastError();
+}
 
 template <typename R, int ST>
 static hipError_t launch_b(const dgx::StepArgs<R> &a, hipStream_t st, int which)
@@ -629,6 +669,16 @@ hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, i
     if (fp32) dgxb::k_clear_slots<float><<<1024, 256, 0, st>>>((float *)f[0], (float *)f[1], (float *)f[2], (float *)f[3], (float *)f[4], n_nodes, group, mask);
     else dgxb::k_clear_slots<double><<<1024, 256, 0, st>>>((double *)f[0], (double *)f[1], (double *)f[2], (double *)f[3], (double *)f[4], n_nodes, group, mask);
     return hipGetLastError();
+}
+hipError_t dgx_launch_move_slots(int fp32, void *const *src, void *const *dst, long long n_nodes, int group, int n,
+                                 const int *from, const int *to, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > 64 || n > group) return hipErrorInvalidValue;
+    for (int k = 0; k < n; k++)
+        if (from[k] < 0 || from[k] >= group || to[k] < 0 || to[k] >= group) return hipErrorInvalidValue;
+    return fp32 ? launch_move<float>(src, dst, n_nodes, group, n, from, to, st)
+                : launch_move<double>(src, dst, n_nodes, group, n, from, to, st);
 }
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which)
 {

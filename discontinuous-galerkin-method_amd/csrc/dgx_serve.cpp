@@ -25,6 +25,8 @@ hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, i
                                   hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
 hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st);
+hipError_t dgx_launch_move_slots(int fp32, void *const *src, void *const *dst, long long n_nodes, int group, int n,
+                                 const int *from, const int *to, hipStream_t st);
 }
 
 #define SV_TRY(call)                                                                                           \
@@ -138,7 +140,11 @@ struct dgx_server {
     std::deque<long> queue;
     long next_id = 1;
     long long dof_updates = 0;
+    long long lane_updates = 0; /* slot lanes stepped, idle slots included */
+    long long moved = 0;        /* shots moved between groups by compaction */
     double busy = 0;
+    double gpu = 0;             /* graph execution time between events on the stream */
+    hipEvent_t ev[2] = {nullptr, nullptr};
     long blocks = 0;
     bool table_dirty = true;
     dgx::Ops<double> op64;
@@ -222,17 +228,30 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
             return DGX_EHIP;
         }
     }
+    /* a failure from here on frees what was already created */
+#define CREATE_TRY(call)                                                                                       \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess) {                                                                                \
+            dgx_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__);          \
+            (void)dgx_server_destroy(sv);                                                                      \
+            return DGX_EHIP;                                                                                   \
+        }                                                                                                      \
+    } while (0)
     size_t fr = 0, tot = 0;
-    SV_TRY(hipMemGetInfo(&fr, &tot));
+    CREATE_TRY(hipMemGetInfo(&fr, &tot));
     sv->budget = (cfg && cfg->hbm_budget > 0) ? cfg->hbm_budget : (long long)(0.8 * fr);
-    SV_TRY(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
-    SV_TRY(hipMalloc((void **)&sv->d_step_base, sizeof(int)));
-    SV_TRY(hipMemset(sv->d_step_base, 0, sizeof(int)));
+    CREATE_TRY(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+    CREATE_TRY(hipEventCreate(&sv->ev[0]));
+    CREATE_TRY(hipEventCreate(&sv->ev[1]));
+    CREATE_TRY(hipMalloc((void **)&sv->d_step_base, sizeof(int)));
+    CREATE_TRY(hipMemset(sv->d_step_base, 0, sizeof(int)));
     size_t args = sv->fp32 ? sizeof(dgx::StepArgs<float>) : sizeof(dgx::StepArgs<double>);
-    SV_TRY(hipMalloc(&sv->d_table, args * MAX_BATCHES));
-    SV_TRY(hipMalloc((void **)&sv->d_sched, sizeof(int) * (MAX_BATCHES + 3)));
+    CREATE_TRY(hipMalloc(&sv->d_table, args * MAX_BATCHES));
+    CREATE_TRY(hipMalloc((void **)&sv->d_sched, sizeof(int) * (MAX_BATCHES + 3)));
     int zero[3] = {0, 0, 0};
-    SV_TRY(hipMemcpy(sv->d_sched, zero, sizeof(zero), hipMemcpyHostToDevice));
+    CREATE_TRY(hipMemcpy(sv->d_sched, zero, sizeof(zero), hipMemcpyHostToDevice));
+#undef CREATE_TRY
     *out = sv;
     return DGX_OK;
 }
@@ -240,7 +259,7 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
 extern "C" int dgx_server_destroy(dgx_server *sv)
 {
     if (!sv) return DGX_EARG;
-    (void)hipStreamSynchronize(sv->stream);
+    if (sv->stream) (void)hipStreamSynchronize(sv->stream);
     if (sv->graph) (void)hipGraphExecDestroy(sv->graph);
     for (auto &m : sv->meshes) {
         void *p[] = {m.nbr, m.vtx, m.pt, m.mat, m.absorb};
@@ -260,7 +279,9 @@ extern "C" int dgx_server_destroy(dgx_server *sv)
     void *p[] = {sv->d_step_base, sv->d_table, sv->d_sched};
     for (void *x : p)
         if (x) (void)hipFree(x);
-    (void)hipStreamDestroy(sv->stream);
+    for (hipEvent_t e : sv->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (sv->stream) (void)hipStreamDestroy(sv->stream);
     delete sv;
     return DGX_OK;
 }
@@ -582,61 +603,88 @@ static int upload_table(dgx_server *sv)
     return DGX_OK;
 }
 
+/* A batch entry is free for a new group (reused entry, or the table has room). */
+static bool batch_entry_free(const dgx_server *sv)
+{
+    for (const auto &b : sv->batches)
+        if (!b.live) return true;
+    return (long)sv->batches.size() < MAX_BATCHES;
+}
+
+/*
+ * Admit queued requests in arrival order (later ones may pass one that does not fit yet).  A request's
+ * shots are independent, so they may spread over several groups of its mesh: free slots are taken from
+ * the fullest groups first, and a new group is opened only for the shots left over (at most one, since
+ * a request has at most G shots).  Groups stay full, instead of one partly used group per request.
+ */
 static int admit(dgx_server *sv)
 {
     std::deque<long> still;
+    const int G = sv->group;
     while (!sv->queue.empty()) {
         long id = sv->queue.front();
         sv->queue.pop_front();
         Request &r = sv->reqs.at(id);
         const MeshDev &m = sv->meshes[r.mesh];
-        /* best fit among this mesh's batches */
-        const int G = sv->group;
-        int best = -1, best_free = G + 1;
+        /* this mesh's groups with free slots, fullest first */
+        std::vector<std::pair<int, int>> open; /* (free slots, batch) */
+        int free_total = 0;
         for (int bi : m.batches) {
-            int fr = G - sv->batches[bi].used();
-            if (fr >= r.nshots && fr < best_free) { best = bi; best_free = fr; }
+            const int fr = G - sv->batches[bi].used();
+            if (fr > 0) {
+                open.push_back({fr, bi});
+                free_total += fr;
+            }
         }
-        long long need = r.bytes;
-        if (best < 0) need += group_bytes(sv, m);
-        if (sv->in_use + need > sv->budget && best < 0 && reclaim_empty(sv) > 0) {
-            /* memory held by idle groups (of any mesh) went back to the budget: look again */
+        std::stable_sort(open.begin(), open.end(),
+                         [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first < y.first; });
+        const bool grow_group = free_total < r.nshots;
+        long long need = r.bytes + (grow_group ? group_bytes(sv, m) : 0);
+        bool fits = sv->in_use + need <= sv->budget && (!grow_group || batch_entry_free(sv));
+        if (!fits && grow_group && reclaim_empty(sv) > 0) {
+            /* memory and entries held by idle groups (of any mesh) went back: look again */
             sv->queue.push_front(id);
             continue;
         }
-        if (sv->in_use + need > sv->budget) { still.push_back(id); continue; }
-        if (best < 0) {
-            int rc = new_batch(sv, r.mesh, &best);
+        if (!fits) { still.push_back(id); continue; }
+        if (grow_group) {
+            int bi = -1;
+            int rc = new_batch(sv, r.mesh, &bi);
             if (rc) return rc;
+            open.push_back({G, bi});
         }
-        Batch &b = sv->batches[best];
-        unsigned long long mask = 0;
         int k = 0;
-        for (int slot = 0; slot < G && k < r.nshots; slot++) {
-            if (b.req[slot] >= 0) continue;
-            double *w = pool_get(sv, r.nsteps);
-            double *t = pool_get(sv, (size_t)r.nsteps * 2 * std::max<long>(r.nrec, 1));
-            if (!w || !t) { dgx_set_error("out of device memory for request buffers"); return DGX_ENOMEM; }
-            SV_TRY(hipMemcpyAsync(w, &r.wav[(size_t)k * r.nsteps], sizeof(double) * r.nsteps, hipMemcpyHostToDevice,
-                                  sv->stream));
-            b.req[slot] = id;
-            b.shot[slot] = k;
-            b.h_start[slot] = (int)sv->global_step;
-            b.h_len[slot] = (int)r.nsteps;
-            b.h_nrec[slot] = (int)r.nrec;
-            b.h_wav[slot] = w;
-            b.h_trace[slot] = t;
-            r.slots.push_back({best, slot});
-            r.d_wav.push_back(w);
-            r.d_trace.push_back(t);
-            mask |= 1ull << slot;
-            k++;
+        for (const auto &ob : open) {
+            if (k == r.nshots) break;
+            Batch &b = sv->batches[ob.second];
+            unsigned long long mask = 0;
+            for (int slot = 0; slot < G && k < r.nshots; slot++) {
+                if (b.req[slot] >= 0) continue;
+                double *w = pool_get(sv, r.nsteps);
+                double *t = pool_get(sv, (size_t)r.nsteps * 2 * std::max<long>(r.nrec, 1));
+                if (!w || !t) { dgx_set_error("out of device memory for request buffers"); return DGX_ENOMEM; }
+                SV_TRY(hipMemcpyAsync(w, &r.wav[(size_t)k * r.nsteps], sizeof(double) * r.nsteps,
+                                      hipMemcpyHostToDevice, sv->stream));
+                b.req[slot] = id;
+                b.shot[slot] = k;
+                b.h_start[slot] = (int)sv->global_step;
+                b.h_len[slot] = (int)r.nsteps;
+                b.h_nrec[slot] = (int)r.nrec;
+                b.h_wav[slot] = w;
+                b.h_trace[slot] = t;
+                r.slots.push_back({ob.second, slot});
+                r.d_wav.push_back(w);
+                r.d_trace.push_back(t);
+                mask |= 1ull << slot;
+                k++;
+            }
+            if (!mask) continue;
+            void *f[5];
+            size_t per = (size_t)m.tri * 6 * G * sv->rsize();
+            for (int c = 0; c < 5; c++) f[c] = (char *)b.fields + c * per;
+            SV_TRY(dgx_launch_clear_slots(sv->fp32, f, (long long)m.tri * 6, G, mask, sv->stream));
+            b.dirty = true;
         }
-        void *f[5];
-        size_t per = (size_t)m.tri * 6 * G * sv->rsize();
-        for (int c = 0; c < 5; c++) f[c] = (char *)b.fields + c * per;
-        SV_TRY(dgx_launch_clear_slots(sv->fp32, f, (long long)m.tri * 6, G, mask, sv->stream));
-        b.dirty = true;
         sv->table_dirty = true;
         r.state = DGX_REQ_RUNNING;
         r.start = sv->global_step;
@@ -686,6 +734,83 @@ static int retire(dgx_server *sv, int *completed)
     return DGX_OK;
 }
 
+/*
+ * Compaction.  When the running shots of a mesh fit in fewer of its groups than now hold them (requests
+ * finished and nothing queued filled their slots), the shots of the emptiest groups move into the free
+ * slots of the fullest.  A move copies the shot's field lanes bit for bit and carries its step range,
+ * wavelet and trace buffer along, so the shot continues exactly as it would have in place; the emptied
+ * groups are no longer stepped.  Runs between blocks, after admission.
+ */
+static int compact(dgx_server *sv)
+{
+    const int G = sv->group;
+    for (int mi = 0; mi < (int)sv->meshes.size(); mi++) {
+        const MeshDev &m = sv->meshes[mi];
+        std::vector<std::pair<int, int>> act; /* (used slots, batch) of groups that are stepped */
+        int used = 0;
+        for (int bi : m.batches) {
+            const int u = sv->batches[bi].used();
+            if (u > 0) {
+                act.push_back({u, bi});
+                used += u;
+            }
+        }
+        const int keep = (used + G - 1) / G;
+        if ((int)act.size() <= keep) continue;
+        std::stable_sort(act.begin(), act.end(),
+                         [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first > y.first; });
+        size_t t = 0; /* target group (act[0 .. keep-1]) and its next slot to look at */
+        int ts = 0;
+        const size_t per = (size_t)m.tri * 6 * G * sv->rsize();
+        for (size_t d = keep; d < act.size(); d++) {
+            const int dbi = act[d].second;
+            int s = 0;
+            while (s < G) {
+                /* one launch per (donor, target) pair */
+                while (t < (size_t)keep && ts >= G) { t++; ts = 0; }
+                if (t >= (size_t)keep) break; /* cannot happen: the targets have room for every donor shot */
+                const int tbi = act[t].second;
+                Batch &D = sv->batches[dbi];
+                Batch &T = sv->batches[tbi];
+                int from[MAXG], to[MAXG], n = 0;
+                for (; s < G; s++) {
+                    if (D.req[s] < 0) continue;
+                    while (ts < G && T.req[ts] >= 0) ts++;
+                    if (ts >= G) break;
+                    from[n] = s;
+                    to[n] = ts;
+                    n++;
+                    T.req[ts] = D.req[s];
+                    T.shot[ts] = D.shot[s];
+                    T.h_start[ts] = D.h_start[s];
+                    T.h_len[ts] = D.h_len[s];
+                    T.h_nrec[ts] = D.h_nrec[s];
+                    T.h_wav[ts] = D.h_wav[s];
+                    T.h_trace[ts] = D.h_trace[s];
+                    for (auto &sl : sv->reqs.at(D.req[s]).slots)
+                        if (sl.first == dbi && sl.second == s) sl = {tbi, ts};
+                    D.req[s] = -1;
+                    D.h_len[s] = 0;
+                    D.h_wav[s] = nullptr;
+                    D.h_trace[s] = nullptr;
+                }
+                if (!n) continue;
+                void *src[5], *dst[5];
+                for (int c = 0; c < 5; c++) {
+                    src[c] = (char *)D.fields + c * per;
+                    dst[c] = (char *)T.fields + c * per;
+                }
+                SV_TRY(dgx_launch_move_slots(sv->fp32, src, dst, (long long)m.tri * 6, G, n, from, to, sv->stream));
+                D.dirty = true;
+                T.dirty = true;
+                sv->table_dirty = true;
+                sv->moved += n;
+            }
+        }
+    }
+    return DGX_OK;
+}
+
 static int build_graph(dgx_server *sv)
 {
     hipGraph_t g;
@@ -719,6 +844,8 @@ extern "C" int dgx_server_advance(dgx_server *sv, int nblocks)
     for (int it = 0; it < nblocks; it++) {
         int rc = admit(sv);
         if (rc) return rc;
+        rc = compact(sv);
+        if (rc) return rc;
         for (auto &b : sv->batches)
             if (b.live && b.dirty) {
                 rc = refresh_batch(sv, b);
@@ -738,14 +865,23 @@ extern "C" int dgx_server_advance(dgx_server *sv, int nblocks)
                 dof += (long long)r.nshots * sv->meshes[r.mesh].tri * 30 * std::min<long>(left, sv->K);
             }
         if (!running) break;
+        long long lanes = 0; /* groups with a running slot are stepped (upload_table) */
+        for (const auto &b : sv->batches)
+            if (b.live && b.used() > 0) lanes += (long long)b.G * sv->meshes[b.mesh].tri * 30 * sv->K;
         if (!sv->graph) {
             rc = build_graph(sv);
             if (rc) return rc;
         }
+        SV_TRY(hipEventRecord(sv->ev[0], sv->stream));
         SV_TRY(hipGraphLaunch(sv->graph, sv->stream));
+        SV_TRY(hipEventRecord(sv->ev[1], sv->stream));
         SV_TRY(hipStreamSynchronize(sv->stream));
+        float ms = 0;
+        SV_TRY(hipEventElapsedTime(&ms, sv->ev[0], sv->ev[1]));
+        sv->gpu += ms * 1e-3;
         sv->global_step += sv->K;
         sv->dof_updates += dof;
+        sv->lane_updates += lanes;
         sv->blocks++;
         rc = retire(sv, &completed);
         if (rc) return rc;
@@ -803,5 +939,8 @@ extern "C" int dgx_server_get_stats(const dgx_server *sv, dgx_server_stats *st)
     st->busy_seconds = sv->busy;
     st->blocks = sv->blocks;
     st->global_step = sv->global_step;
+    st->lane_updates = sv->lane_updates;
+    st->moved = sv->moved;
+    st->gpu_seconds = sv->gpu;
     return DGX_OK;
 }

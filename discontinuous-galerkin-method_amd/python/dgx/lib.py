@@ -67,7 +67,8 @@ class RequestC(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("queued", C.c_long), ("running", C.c_long), ("done", C.c_long), ("batches", C.c_long),
                 ("bytes_in_use", C.c_longlong), ("dof_updates", C.c_longlong), ("busy_seconds", C.c_double),
-                ("blocks", C.c_long), ("global_step", C.c_long)]
+                ("blocks", C.c_long), ("global_step", C.c_long), ("lane_updates", C.c_longlong),
+                ("gpu_seconds", C.c_double), ("moved", C.c_longlong)]
 
 
 class SpongeParams(C.Structure):

@@ -52,6 +52,11 @@ typedef struct {
     double busy_seconds;        /* wall time spent in dgx_server_advance */
     long blocks;                /* step blocks run */
     long global_step;
+    long long lane_updates;     /* DOF-updates of every slot lane stepped, idle slots of live groups
+                                 * included: dof_updates / lane_updates is the slot occupancy */
+    double gpu_seconds;         /* execution time of the step blocks on the device (HIP events); the
+                                 * rest of busy_seconds is host work between blocks */
+    long long moved;            /* shots moved into fuller groups of their mesh by compaction */
 } dgx_server_stats;
 
 int dgx_server_create(const dgx_server_config *cfg, const dgx_operators *ops, dgx_server **out);

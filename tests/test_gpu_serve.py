@@ -4,7 +4,9 @@
   and on other meshes;
 * they equal the standalone shot-innermost FP32 solver's traces for the same shots to FP32 rounding;
 * they are within 1e-4 relative L2 of the FP64 oracle;
-* a tight memory budget queues requests instead of failing, and all of them complete.
+* a tight memory budget queues requests instead of failing, and all of them complete;
+* a request's shots may spread over several slot groups of its mesh, and may move between them when
+  groups are compacted, without changing its results.
 
 The first two hold for both slot-group sizes (16 and 64 shots per group, which is also the serving
 kernel's shot tile).
@@ -76,6 +78,60 @@ def test_results_identical_alone_and_in_the_mix(group):
     for k in range(len(REQS)):
         (ax, az), _, _ = run_alone(k, group)
         assert np.array_equal(mixed[k][0], ax) and np.array_equal(mixed[k][1], az), k
+
+
+def test_requests_spread_over_groups():
+    """Three requests of 10, 10 and 12 shots on one mesh with 16-shot groups fill exactly two groups (the
+    second and third request take the free slots of the first group before a new one is opened), and each
+    request's traces equal its traces when it runs alone in a group of its own."""
+    print("case serve spread over groups, group 16", flush=True)
+    specs = [(0, 10, 140, 15.0), (0, 10, 200, 20.0), (0, 12, 170, 12.0)]
+    sv, rms, _, _ = new_server(group=16)
+    built = [build(sv, rms, spec, 10 + k) for k, spec in enumerate(specs)]
+    ids = [sv.submit(b[0]) for b in built]
+    sv.advance(1)
+    st = sv.stats()
+    assert st["running"] == 3 and st["batches"] == 2, st
+    sv.drain()
+    spread = [sv.result(rid, built[k][0]) for k, rid in enumerate(ids)]
+    sv.close()
+    for k, spec in enumerate(specs):
+        sv, rms, _, _ = new_server(group=16)
+        req, _, _ = build(sv, rms, spec, 10 + k)
+        rid = sv.submit(req)
+        sv.drain()
+        ax, az = sv.result(rid, req)
+        sv.close()
+        assert np.array_equal(spread[k][0], ax) and np.array_equal(spread[k][1], az), k
+
+
+def test_compaction_moves_shots_without_changing_results():
+    """16-shot groups, blocks of 50 steps: A (10 shots, 200 steps) fills group 1, B (10 shots, 60 steps)
+    takes the other 6 slots of group 1 and 4 of group 2, C (6 shots, 200 steps) joins group 2.  When B
+    retires, A and C (16 shots) fit one group, so C's 6 shots move into group 1.  Every request's traces
+    equal its traces alone, bit for bit."""
+    print("case serve compaction, group 16", flush=True)
+    specs = [(0, 10, 200, 15.0), (0, 10, 60, 20.0), (0, 6, 200, 12.0)]
+    sv, rms, _, _ = new_server(group=16, K=50)
+    built = [build(sv, rms, spec, 20 + k) for k, spec in enumerate(specs)]
+    ids = [sv.submit(b[0]) for b in built]
+    sv.advance(2)
+    assert sv.stats()["moved"] == 0
+    sv.advance(1)
+    st = sv.stats()
+    assert st["done"] == 1 and st["running"] == 2 and st["moved"] == 6, st
+    sv.drain()
+    out = [sv.result(rid, built[k][0]) for k, rid in enumerate(ids)]
+    assert sv.stats()["moved"] == 6
+    sv.close()
+    for k, spec in enumerate(specs):
+        sv, rms, _, _ = new_server(group=16, K=50)
+        req, _, _ = build(sv, rms, spec, 20 + k)
+        rid = sv.submit(req)
+        sv.drain()
+        ax, az = sv.result(rid, req)
+        sv.close()
+        assert np.array_equal(out[k][0], ax) and np.array_equal(out[k][1], az), k
 
 
 @pytest.mark.parametrize("group", [16, 64])
