@@ -59,6 +59,62 @@ def test_create_rejects_bad_arguments_before_touching_the_gpu():
     assert lib.dgx_create(C.byref(p), C.byref(ops), C.byref(h)) == -2
 
 
+def _with_source(mesh, elem=1, shot=0, wavelet=0, kind=None, **over):
+    src = (L.Source * 1)()
+    src[0].elem, src[0].shot, src[0].wavelet = elem, shot, wavelet
+    src[0].kind = L.DGX_SRC_EXPLOSIVE if kind is None else kind
+    wav = np.zeros(10)
+    fields = dict(nsrc=1, src=C.cast(src, C.POINTER(L.Source)), wavelet=L.dp(wav), nwavelets=1)
+    fields.update(over)
+    p, keep = _problem(mesh, **fields)
+    keep.update(src=src, wav=wav)
+    return p, keep
+
+
+def _with_receiver(mesh, elem):
+    rec = (L.Receiver * 1)()
+    rec[0].elem = elem
+    rec[0].weight[0] = 1.0
+    p, keep = _problem(mesh, nrec=1, rec=C.cast(rec, C.POINTER(L.Receiver)))
+    keep.update(rec=rec)
+    return p, keep
+
+
+def test_create_rejects_invalid_configurations():
+    """Every rejection below happens on the host, with DGX_EARG and a message, before any device work:
+    an empty mesh, a negative step count, an unknown precision, layout or step kernel, the full-step
+    kernel with the shot-innermost layout, a shot tile that is not a power of two from 4 to 64 or that
+    is given for the element layout, and sources or receivers that name a missing element, shot,
+    wavelet or kind."""
+    lib = L.load()
+    mesh = dgx.structured(4, 4, 10.0)
+    ops = L.read_operators()
+    T = mesh.tri
+    cases = {
+        "empty mesh": _problem(mesh, tri=0),
+        "negative steps": _problem(mesh, nsteps=-1),
+        "precision": _problem(mesh, precision=7),
+        "layout": _problem(mesh, layout=9),
+        "step kernel": _problem(mesh, step_kernel=5),
+        "full step, shot-innermost": _problem(mesh, nshots=8, layout=L.DGX_LAYOUT_SHOT_INNER,
+                                              step_kernel=L.DGX_STEP_FULL),
+        "shot tile 12": _problem(mesh, nshots=8, layout=L.DGX_LAYOUT_SHOT_INNER, shot_tile=12),
+        "shot tile, element layout": _problem(mesh, layout=L.DGX_LAYOUT_ELEMENT, shot_tile=16),
+        "source element 0": _with_source(mesh, elem=0),
+        "source element past the mesh": _with_source(mesh, elem=T + 1),
+        "source shot": _with_source(mesh, shot=1),
+        "source wavelet": _with_source(mesh, wavelet=1),
+        "source kind": _with_source(mesh, kind=9),
+        "sources without wavelets": _with_source(mesh, nwavelets=0),
+        "receiver element": _with_receiver(mesh, T + 1),
+    }
+    for name, (p, keep) in cases.items():
+        h = C.c_void_p()
+        assert lib.dgx_create(C.byref(p), C.byref(ops), C.byref(h)) == L.DGX_EARG, name
+        assert not h.value, name
+        assert lib.dgx_last_error(), name
+
+
 def test_step_calls_reject_null_handle():
     lib = L.load()
     assert lib.dgx_step(None, 1) == -1
