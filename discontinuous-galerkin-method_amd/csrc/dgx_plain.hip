@@ -35,14 +35,14 @@ namespace dgx {
 
 __device__ __forceinline__ int fidx(int q, int i) { return ((q >> 6) * 6 + i) * 64 + (q & 63); }
 
-/* Reference-element operators of the velocity update.  P2: the P2 Fekete operators of the shipped
- * operator files, compiled in as immediates (the host selects these kernels only when the caller's
- * operators equal them bit for bit; results agree with the argument variant to rounding, since the
- * compiler may fuse a different product into an FMA); otherwise the caller's operators from the kernel
- * argument.  As
- * arguments, the 126 FP64 values do not fit the scalar registers and the FP64 velocity kernel spills
- * about 200 of them to vector-register lanes; compiled in, it spills none.  The stress update keeps the
- * argument operators: compiled in, its FP64 kernel needs more than the 168 VGPRs of 3 waves per SIMD. */
+/* Reference-element operators.  P2: the P2 Fekete operators of the shipped operator files, compiled in
+ * as immediates (the host selects these kernels only when the caller's operators equal them bit for
+ * bit; results agree with the argument variant to rounding, since the compiler may fuse a different
+ * product into an FMA); otherwise the caller's operators from the kernel argument.  As arguments, the
+ * 126 FP64 values do not fit the scalar registers and the FP64 velocity kernel spills about 200 of them
+ * to vector-register lanes; compiled in, it spills none.  The stress update compiles in only the face
+ * lifts: with all operators compiled in its FP64 kernel needs more than the 168 VGPRs of 3 waves per
+ * SIMD, with the lifts alone it does not and runs faster in both precisions. */
 template <bool P2, typename R>
 __device__ __forceinline__ R op_dr(const StepArgs<R> &a, int k)
 {
@@ -338,7 +338,7 @@ __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int
  * conditioned here; in the full-step kernel (tload != nullptr) they come conditioned from LDS.  Records the
  * receivers of q from the velocities it holds.
  */
-template <typename R, bool FUSED>
+template <typename R, bool FUSED, bool P2>
 __device__ __forceinline__ void stress_update(const StepArgs<R> &a, int q, int shot, int step, int4 nrow,
                                               const R *__restrict__ Vx, const R *__restrict__ Vz, const R *tload,
                                               R *__restrict__ Txx, R *__restrict__ Tzz, R *__restrict__ Txz)
@@ -403,7 +403,7 @@ __device__ __forceinline__ void stress_update(const StepArgs<R> &a, int q, int s
             for (int i = 0; i < 6; i++) {
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    const R l = a.op.L[j * 18 + i * 3 + k];
+                    const R l = op_l<P2>(a, j * 18 + i * 3 + k);
                     fx[i] += l * px[k];
                     fz[i] += l * pz[k];
                     fxz[i] += l * pxz[k];
@@ -467,7 +467,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
                                     V0, V1, nullptr, false);
 }
 
-template <typename R, bool FUSED>
+template <typename R, bool FUSED, bool P2>
 __global__ void __launch_bounds__(DGX_BLOCK, 3) k_stress(StepArgs<R> a)
 {
     const int q = element_of_thread(a.nper);
@@ -475,7 +475,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_stress(StepArgs<R> a)
     const int shot = blockIdx.y;
     const long long so = (long long)shot * a.fstride * 6;
     const int step = FUSED ? (*a.step_base + a.step_off) : 0;
-    stress_update<R, FUSED>(a, q, shot, step, a.nbr[q], a.f[0] + so, a.f[1] + so, nullptr, a.f[2] + so,
+    stress_update<R, FUSED, P2>(a, q, shot, step, a.nbr[q], a.f[0] + so, a.f[1] + so, nullptr, a.f[2] + so,
                             a.f[3] + so, a.f[4] + so);
 }
 
@@ -522,8 +522,9 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
         velocity_update<R, true, false, P2>(a, q, shot, step, orow, a.f[2] + so, a.f[3] + so, a.f[4] + so, a.f[0] + so,
                                  a.f[1] + so, a.g[0] + so, a.g[1] + so, tc, lane < td.y);
     __syncthreads();
+    /* argument operators in the stress phase: with the lifts compiled in, the FP64 kernel spills */
     if (lane < td.y)
-        stress_update<R, true>(a, q, shot, step, a.nb2[q], a.g[0] + so, a.g[1] + so, tc, a.g[2] + so,
+        stress_update<R, true, false>(a, q, shot, step, a.nb2[q], a.g[0] + so, a.g[1] + so, tc, a.g[2] + so,
                                a.g[3] + so, a.g[4] + so);
 }
 
@@ -645,9 +646,9 @@ static void launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st
     dim3 grid(DGX_NXCD * a.nper, nshots), block(DGX_BLOCK);
     switch (which) {
     case 0: dgx::k_velocity<R, true, P2><<<grid, block, 0, st>>>(a); break;
-    case 1: dgx::k_stress<R, true><<<grid, block, 0, st>>>(a); break;
+    case 1: dgx::k_stress<R, true, P2><<<grid, block, 0, st>>>(a); break;
     case 2: dgx::k_velocity<R, false, P2><<<grid, block, 0, st>>>(a); break;
-    case 3: dgx::k_stress<R, false><<<grid, block, 0, st>>>(a); break;
+    case 3: dgx::k_stress<R, false, P2><<<grid, block, 0, st>>>(a); break;
     case 4: dgx::k_absorb<R><<<grid, block, 0, st>>>(a); break;
     case 5: dgx::k_fullstep<R, P2><<<dim3(DGX_NXCD * a.tper, nshots), block, 0, st>>>(a); break;
     }
