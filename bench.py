@@ -33,6 +33,11 @@ from dgx import lib as L  # noqa: E402
 HBM_PEAK = 8.0e12
 
 
+def progress(msg):
+    """One line per finished stage on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,9 +210,13 @@ def main():
     T = w.mesh.tri
     dof = T * 30
 
+    progress(f"{w.name}: {T} elements set up")
     sec64, kt64, x64, z64, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps)
+    progress(f"plain FP64: {sec64 * 1e3 / a.steps:.4f} ms/step")
     sec32, kt32, x32, z32, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps)
+    progress(f"plain FP32: {sec32 * 1e3 / a.steps:.4f} ms/step")
     secf, ktf, xf, zf, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps, step_kernel=L.DGX_STEP_FULL)
+    progress(f"full-step FP64: {secf * 1e3 / a.steps:.4f} ms/step")
     rate64 = dof * a.steps / sec64
     rate32 = dof * a.steps / sec32
     rate64f = dof * a.steps / secf
@@ -261,8 +270,10 @@ def main():
     }
     if a.shots > 0:
         out["batched_fp32"] = batched_leg(a, rate64)
+        progress(f"batched FP32: {out['batched_fp32']['value']:.3e} DOF-updates/s")
     if a.serve_requests > 0:
         out["serving"] = serving_leg(a, rate64)
+        progress(f"serving: {out['serving']['value']:.3e} DOF-updates/s")
     if a.copy_probe:
         import ctypes as C
         nbytes, reps = 1 << 30, 20

@@ -487,41 +487,51 @@ __device__ __forceinline__ StepArgs<R> global_args(const StepArgs<R> &t)
 /*
  * Serving: one launch per half-step over every active (mesh, slot group) batch.  sched = {nbatch,
  * total tiles, tile offset of batch 0, ..., tile offset of batch nbatch}; a tile is 256/G elements x G
- * shots (G = slots per group, 16 or 64: 64/G elements per wavefront).  The grid is fixed (a persistent
- * grid-stride loop), so a captured step graph stays valid while batches are added and retired: only the
- * table and sched change.  The tiles are dealt in 8 contiguous ranges, one per XCD.
+ * shots (G = slots per group, 16 or 64: 64/G elements per wavefront).  One workgroup per tile, and the
+ * tiles are dealt to the 8 XCDs in contiguous ranges swept in order, as in the batched kernels: face
+ * neighbours are stepped close together in time on one XCD and their rows are still in its L2.  (A
+ * persistent grid-stride loop over the same tiles let the workgroups drift apart over their ~200 tiles
+ * each; measured on 640k elements x 64 shots, it fetched 1.28-1.44x the field bytes from HBM against
+ * 1.02-1.04x here, and ran 20 % slower.)  The grid depends on the tile count, so the host recaptures the
+ * step graph when the set of stepped batches changes; the table itself is read from device memory.
+ *
+ * Everything a wavefront needs before its first element load comes through the scalar cache: the batch
+ * (binary search over the tile offsets), its table entry, and which of its slots run at this step
+ * (runmask[step_off * nb + batch], bit s = slot s, written by the host before each block).  The slot's
+ * local step, needed only where a source or receiver sits, is read per lane alongside the element loads.
  */
 template <typename R, bool VEL, int G>
 __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const StepArgs<R> *__restrict__ table,
-                                                        const int *__restrict__ sched, const int *step_base,
-                                                        int step_off)
+                                                        const int *__restrict__ sched,
+                                                        const unsigned long long *__restrict__ runmask,
+                                                        const int *step_base, int step_off, int nb, int total,
+                                                        int nper)
 {
     constexpr int EW = 64 / G; /* elements per wavefront */
-    const int nb = sched[0], total = sched[1];
-    const int gstep = *step_base + step_off;
-    const int xcd = blockIdx.x & (DGX_NXCD - 1), per_xcd = gridDim.x / DGX_NXCD;
-    const int chunk = (total + DGX_NXCD - 1) / DGX_NXCD;
-    const int t_end = min(total, (xcd + 1) * chunk);
+    const int tile = (int)(blockIdx.x & (DGX_NXCD - 1)) * nper + (int)(blockIdx.x >> 3);
+    if (tile >= total) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int s = lane % G; /* a lane keeps its slot for the whole launch */
-    int b = 0, cur = -1, slot_start = 0, slot_len = 0;
-    for (int tile = xcd * chunk + (int)(blockIdx.x >> 3); tile < t_end; tile += per_xcd) {
-        while (b + 1 < nb && sched[2 + b + 1] <= tile) b++;
-        const StepArgs<R> a = global_args(table[b]);
-        if (b != cur) { /* the lane's slot window, reloaded only when the batch changes */
-            slot_start = a.shot_start[s];
-            slot_len = a.shot_len[s];
-            cur = b;
-        }
-        int q = (tile - sched[2 + b]) * (4 * EW) + wave * EW + lane / G;
-        if (G == 64 && VEL) q = __builtin_amdgcn_readfirstlane(q); /* one element per wavefront */
-        const int t = gstep - slot_start;
-        /* a slot that is not running (free, or its request has not started / has finished) is skipped:
-         * its fields are never read again before the slot is cleared for a new request */
-        if (q >= a.tri || t < 0 || t >= slot_len) continue;
-        if (VEL) velocity_body(op, a, q, s, t);
-        else stress_body(op, a, q, s, t);
+    /* batch of the tile: the last batch whose first tile sched[2 + b] is at or below it (the offsets
+     * increase strictly; every batch has at least one tile) */
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sched[2 + mid] <= tile) lo = mid;
+        else hi = mid - 1;
     }
+    const int b = lo;
+    const unsigned long long run = runmask[(long long)step_off * nb + b];
+    const int s = lane % G;
+    /* a slot that is not running (free, or its request has not started / has finished) is skipped: its
+     * fields are never read again before the slot is cleared for a new request */
+    if (!((run >> s) & 1ull)) return;
+    const StepArgs<R> a = global_args(table[b]);
+    int q = (tile - sched[2 + b]) * (4 * EW) + wave * EW + lane / G;
+    if (G == 64 && VEL) q = __builtin_amdgcn_readfirstlane(q); /* one element per wavefront */
+    if (q >= a.tri) return;
+    const int t = *step_base + step_off - a.shot_start[s];
+    if (VEL) velocity_body(op, a, q, s, t);
+    else stress_body(op, a, q, s, t);
 }
 
 /* dst[pairs[k].x] = pairs[k].y */
@@ -606,32 +616,22 @@ static hipError_t dispatch_b(const dgx::StepArgs<R> &a, hipStream_t st, int whic
     return hipErrorInvalidValue;
 }
 
-/* Workgroups of one serving kernel that are resident on the whole device at once (occupancy x CUs),
- * rounded down to a multiple of the XCD count: the persistent grid, so that every workgroup runs from the
- * start and the tiles are shared evenly. */
-template <typename R, bool VEL, int G>
-static int serve_grid()
-{
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dgxb::ks_step<R, VEL, G>, DGX_BLOCK, 0) != hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        per_cu < 1 || cus < 1)
-        return 0;
-    return (per_cu * cus) / DGX_NXCD * DGX_NXCD;
-}
-
 template <typename R>
-static hipError_t launch_serve(const void *ops, const void *table, const int *sched, const int *step_base,
-                               int step_off, int vel, int group, int grid, hipStream_t st)
+static hipError_t launch_serve(const void *ops, const void *table, const int *sched,
+                               const unsigned long long *runmask, const int *step_base, int step_off, int vel,
+                               int group, int nb, int total, hipStream_t st)
 {
     const dgx::Ops<R> &op = *(const dgx::Ops<R> *)ops;
     const dgx::StepArgs<R> *t = (const dgx::StepArgs<R> *)table;
+    if (total <= 0) return hipSuccess;
+    const int nper = (total + DGX_NXCD - 1) / DGX_NXCD; /* tiles per XCD */
+    const int grid = DGX_NXCD * nper;
     if (group == 16) {
-        if (vel) dgxb::ks_step<R, true, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
-        else dgxb::ks_step<R, false, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
+        if (vel) dgxb::ks_step<R, true, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
+        else dgxb::ks_step<R, false, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
     } else if (group == 64) {
-        if (vel) dgxb::ks_step<R, true, 64><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
-        else dgxb::ks_step<R, false, 64><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, step_base, step_off);
+        if (vel) dgxb::ks_step<R, true, 64><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
+        else dgxb::ks_step<R, false, 64><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
     } else {
         return hipErrorInvalidValue;
     }
@@ -639,23 +639,14 @@ static hipError_t launch_serve(const void *ops, const void *table, const int *sc
 }
 
 extern "C" {
-int dgx_serve_grid(int fp32, int vel, int group)
-{
-    if (group != 16 && group != 64) return 0;
-    if (fp32) {
-        if (group == 16) return vel ? serve_grid<float, true, 16>() : serve_grid<float, false, 16>();
-        return vel ? serve_grid<float, true, 64>() : serve_grid<float, false, 64>();
-    }
-    if (group == 16) return vel ? serve_grid<double, true, 16>() : serve_grid<double, false, 16>();
-    return vel ? serve_grid<double, true, 64>() : serve_grid<double, false, 64>();
-}
-hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched, const int *step_base,
-                            int step_off, int vel, int group, int grid, hipStream_t st)
+hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched,
+                            const unsigned long long *runmask, const int *step_base, int step_off, int vel, int group,
+                            int nb, int total, hipStream_t st)
 {
     /* the operators are the same for every mesh: they travel as a kernel argument (scalar registers),
      * not through the batch table */
-    if (fp32) return launch_serve<float>(ops, table, sched, step_base, step_off, vel, group, grid, st);
-    return launch_serve<double>(ops, table, sched, step_base, step_off, vel, group, grid, st);
+    if (fp32) return launch_serve<float>(ops, table, sched, runmask, step_base, step_off, vel, group, nb, total, st);
+    return launch_serve<double>(ops, table, sched, runmask, step_base, step_off, vel, group, nb, total, st);
 }
 hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st)
 {

@@ -1,6 +1,6 @@
 /*
  * dgx_serve.cpp -- request loop (include/dgx_serve.h): resident meshes, slot groups of 16 or 64 shots in the
- * shot-innermost layout, memory-bounded admission, one persistent kernel per half-step over all active
+ * shot-innermost layout, memory-bounded admission, one kernel per half-step over the tiles of all active
  * batches, hipGraph-captured blocks of steps, retirement with trace read-back.
  */
 #include <hip/hip_runtime.h>
@@ -18,9 +18,9 @@
 #include "dgx_serve.h"
 
 extern "C" {
-hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched, const int *step_base,
-                            int step_off, int vel, int group, int grid, hipStream_t st);
-int dgx_serve_grid(int fp32, int vel, int group);
+hipError_t dgx_launch_serve(int fp32, const void *ops, const void *table, const int *sched,
+                            const unsigned long long *runmask, const int *step_base, int step_off, int vel, int group,
+                            int nb, int total, hipStream_t st);
 hipError_t dgx_launch_clear_slots(int fp32, void *const *f, long long n_nodes, int group, unsigned long long mask,
                                   hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
@@ -124,7 +124,9 @@ hipError_t grow(T **p, long *cap, long need)
 struct dgx_server {
     int fp32 = 1;
     int group = 16; /* shots per slot group */
-    int grid[2] = {0, 0}; /* persistent grids of the stress / velocity kernels: resident workgroups */
+    int nb = 0, tiles = 0; /* stepped batches and their tiles (sched); the step graph is built for these */
+    std::vector<int> order; /* batch entry of each stepped batch, in table order */
+    unsigned long long *d_runmask = nullptr; /* [step of block][stepped batch]: slots running at that step */
     long long budget = 0, in_use = 0;
     int K = 100;
     dgx_operators ops;
@@ -220,14 +222,6 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
     }
     sv->ops = *ops;
     make_ops(sv);
-    for (int vel = 0; vel < 2; vel++) {
-        sv->grid[vel] = dgx_serve_grid(sv->fp32, vel, sv->group);
-        if (sv->grid[vel] < DGX_NXCD) {
-            dgx_set_error("dgx_server_create: occupancy query for the serving kernel failed");
-            delete sv;
-            return DGX_EHIP;
-        }
-    }
     /* a failure from here on frees what was already created */
 #define CREATE_TRY(call)                                                                                       \
     do {                                                                                                       \
@@ -249,6 +243,7 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
     size_t args = sv->fp32 ? sizeof(dgx::StepArgs<float>) : sizeof(dgx::StepArgs<double>);
     CREATE_TRY(hipMalloc(&sv->d_table, args * MAX_BATCHES));
     CREATE_TRY(hipMalloc((void **)&sv->d_sched, sizeof(int) * (MAX_BATCHES + 3)));
+    CREATE_TRY(hipMalloc((void **)&sv->d_runmask, sizeof(unsigned long long) * MAX_BATCHES * sv->K));
     int zero[3] = {0, 0, 0};
     CREATE_TRY(hipMemcpy(sv->d_sched, zero, sizeof(zero), hipMemcpyHostToDevice));
 #undef CREATE_TRY
@@ -276,7 +271,7 @@ extern "C" int dgx_server_destroy(dgx_server *sv)
         for (double *p : r.second.d_trace) (void)hipFree(p);
     }
     for (auto &kv : sv->pool) (void)hipFree(kv.second);
-    void *p[] = {sv->d_step_base, sv->d_table, sv->d_sched};
+    void *p[] = {sv->d_step_base, sv->d_table, sv->d_sched, sv->d_runmask};
     for (void *x : p)
         if (x) (void)hipFree(x);
     for (hipEvent_t e : sv->ev)
@@ -583,8 +578,11 @@ static int upload_table(dgx_server *sv)
     std::vector<char> table;
     int nb = 0, tiles = 0;
     size_t asz = sv->fp32 ? sizeof(dgx::StepArgs<float>) : sizeof(dgx::StepArgs<double>);
-    for (auto &b : sv->batches) {
+    sv->order.clear();
+    for (size_t bi = 0; bi < sv->batches.size(); bi++) {
+        const Batch &b = sv->batches[bi];
         if (!b.live || b.used() == 0) continue;
+        sv->order.push_back((int)bi);
         table.resize((nb + 1) * asz);
         if (sv->fp32) fill_batch_args(sv, b, sv->op32, *(dgx::StepArgs<float> *)(table.data() + nb * asz));
         else fill_batch_args(sv, b, sv->op64, *(dgx::StepArgs<double> *)(table.data() + nb * asz));
@@ -596,6 +594,13 @@ static int upload_table(dgx_server *sv)
     sched.push_back(tiles);
     sched[0] = nb;
     sched[1] = tiles;
+    if ((nb != sv->nb || tiles != sv->tiles) && sv->graph) {
+        /* the kernels' grid and batch count are part of the captured graph */
+        (void)hipGraphExecDestroy(sv->graph);
+        sv->graph = nullptr;
+    }
+    sv->nb = nb;
+    sv->tiles = tiles;
     if (nb) SV_TRY(hipMemcpyAsync(sv->d_table, table.data(), table.size(), hipMemcpyHostToDevice, sv->stream));
     SV_TRY(hipMemcpyAsync(sv->d_sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice, sv->stream));
     SV_TRY(hipStreamSynchronize(sv->stream));
@@ -811,6 +816,29 @@ static int compact(dgx_server *sv)
     return DGX_OK;
 }
 
+/* The slots of every stepped batch that run at each step of the next block (bit s: slot s), read by the
+ * serving kernels through the scalar cache instead of a per-lane load of each slot's step window. */
+static int upload_runmask(dgx_server *sv)
+{
+    const int nb = (int)sv->order.size();
+    if (!nb) return DGX_OK;
+    std::vector<unsigned long long> m((size_t)sv->K * nb, 0ull);
+    for (int i = 0; i < nb; i++) {
+        const Batch &b = sv->batches[sv->order[i]];
+        for (int s = 0; s < b.G; s++) {
+            if (b.req[s] < 0 || b.h_len[s] <= 0) continue;
+            const long lo = std::max<long>(b.h_start[s], sv->global_step);
+            const long hi = std::min<long>((long)b.h_start[s] + b.h_len[s], sv->global_step + sv->K);
+            for (long g = lo; g < hi; g++) m[(size_t)(g - sv->global_step) * nb + i] |= 1ull << s;
+        }
+    }
+    SV_TRY(hipMemcpyAsync(sv->d_runmask, m.data(), sizeof(unsigned long long) * m.size(), hipMemcpyHostToDevice,
+                          sv->stream));
+    /* the copy reads host memory asynchronously: wait before it goes out of scope */
+    SV_TRY(hipStreamSynchronize(sv->stream));
+    return DGX_OK;
+}
+
 static int build_graph(dgx_server *sv)
 {
     hipGraph_t g;
@@ -818,11 +846,11 @@ static int build_graph(dgx_server *sv)
     hipError_t e = hipSuccess;
     const void *ops = sv->fp32 ? (const void *)&sv->op32 : (const void *)&sv->op64;
     for (int k = 0; k < sv->K && e == hipSuccess; k++) {
-        e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 1, sv->group,
-                             sv->grid[1], sv->stream);
+        e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_runmask, sv->d_step_base, k, 1,
+                             sv->group, sv->nb, sv->tiles, sv->stream);
         if (e == hipSuccess)
-            e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_step_base, k, 0, sv->group,
-                                 sv->grid[0], sv->stream);
+            e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_runmask, sv->d_step_base, k, 0,
+                                 sv->group, sv->nb, sv->tiles, sv->stream);
     }
     if (e == hipSuccess) e = dgx_launch_advance(sv->d_step_base, sv->K, sv->stream);
     hipError_t e2 = hipStreamEndCapture(sv->stream, &g);
@@ -868,6 +896,8 @@ extern "C" int dgx_server_advance(dgx_server *sv, int nblocks)
         long long lanes = 0; /* groups with a running slot are stepped (upload_table) */
         for (const auto &b : sv->batches)
             if (b.live && b.used() > 0) lanes += (long long)b.G * sv->meshes[b.mesh].tri * 30 * sv->K;
+        rc = upload_runmask(sv);
+        if (rc) return rc;
         if (!sv->graph) {
             rc = build_graph(sv);
             if (rc) return rc;

@@ -6,8 +6,8 @@
  * shots of many requests at once: every request is 1..group_shots shots (16 or 64, dgx_server_config) on
  * one resident mesh, each shot with its own source, FP64 wavelet samples and start step, all shots of a
  * request sharing one receiver spread.
- * Requests are admitted as slots and memory free up (continuous batching), stepped by one persistent
- * kernel per half-step over all active (mesh, slot group) batches, in hipGraph-captured blocks of steps,
+ * Requests are admitted as slots and memory free up (continuous batching), stepped by one kernel per
+ * half-step over the tiles of all active (mesh, slot group) batches, in hipGraph-captured blocks of steps,
  * and retired with their traces copied back.  Every shot computes exactly what a single-shot solve of
  * the same problem on the shot-innermost layout computes; the mix of requests does not change a result.
  */
