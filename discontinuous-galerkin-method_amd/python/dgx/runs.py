@@ -4,6 +4,7 @@
     python -m dgx.runs slope     [--steps 5000 --shots 64]  batched FP32, FP64 reference on 4 shots
     python -m dgx.runs serving   [--requests 2000 --rate 20] open-loop request stream on 8 meshes
     python -m dgx.runs carbonate [--shots 256 --steps 8000] 256 shots through the request loop, FP64 on 4
+    python -m dgx.runs lanes     [--steps 400]              batched vs serving kernels at full slot occupancy
 
 The serving runs take --group (16 or 64 shots per slot group; default 16 for the request stream of 1-16-shot
 requests over 8 meshes, 64 for carbonate's 16-shot requests on one mesh).
@@ -165,9 +166,48 @@ def run_carbonate(a):
             "fp32_vs_fp64_trace_rel_l2_per_checked_shot": errs}
 
 
+def run_lanes(a):
+    """The batched solver and the request loop on the same mesh (the 640k-element gradient model) and the
+    same shots, every slot running: batched at shot tile 64 and 16 (64 shots), batched with 16 shots,
+    and the server with one 64-shot request in a 64-shot group, four 16-shot requests in 16-shot groups
+    and one 16-shot request.  DOF-updates/s per case (the server's from its device time)."""
+    from . import serve as SV
+    w = configs.gradient_8x4km(a.steps + 50)
+    T, W, H = w.mesh.tri, 8000.0, 4000.0
+    out = {"workload": w.name, "elements": T, "steps": a.steps, "dtype": "f32"}
+
+    def shots(n):
+        return [SourceSpec(float(x), H - 500.0, L.DGX_SRC_EXPLOSIVE, k, 20.0)
+                for k, x in enumerate(np.linspace(500.0, W - 500.0, n))]
+    for S, ST in ((64, 64), (64, 16), (16, 16)):
+        st = build_setup(w.mesh, w.model, shots(S), w.receivers, w.nsteps, sponge=w.sponge)
+        s = Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, ST)
+        s.step(10)
+        s.synchronize()
+        t = time.perf_counter()
+        s.step(a.steps)
+        s.synchronize()
+        sec = time.perf_counter() - t
+        s.close()
+        out[f"batched_{S}shots_tile{ST}"] = T * 30 * S * a.steps / sec
+        log(f"batched {S} shots, tile {ST}: {out[f'batched_{S}shots_tile{ST}']:.3e} DOF-updates/s")
+    for G, nreq in ((64, 1), (16, 4), (16, 1)):
+        sv = SV.Server(L.DGX_FP32, 0, 100, group_shots=G)
+        rm = sv.add_mesh(w.mesh, w.model, w.sponge)
+        per = 16 if G == 16 else 64
+        reqs = [sv.make_request(rm, np.stack([np.linspace(500.0 + 10 * r, W - 500.0, per), np.full(per, H - 500.0)], 1),
+                                [20.0] * per, a.steps, w.receivers[:51]) for r in range(nreq)]
+        _, _, st, _ = SV.run_open_loop(sv, reqs, np.zeros(nreq), progress_every=0)
+        sv.close()
+        key = f"serving_group{G}_{nreq}x{per}shots"
+        out[key] = st["dof_updates"] / st["gpu_seconds"]
+        log(f"serving {nreq} x {per} shots in {G}-shot groups: {out[key]:.3e} DOF-updates/s")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["gradient", "slope", "serving", "carbonate"])
+    ap.add_argument("which", choices=["gradient", "slope", "serving", "carbonate", "lanes"])
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--shots", type=int, default=0)
     ap.add_argument("--requests", type=int, default=2000)
@@ -175,11 +215,13 @@ def main(argv=None):
     ap.add_argument("--group", type=int, default=0,
                     help="shots per slot group of the server, 16 or 64 (default: 16 for serving, 64 for carbonate)")
     a = ap.parse_args(argv)
-    defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256)}
+    defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256),
+                "lanes": (400, 0)}
     a.steps = a.steps or defaults[a.which][0]
     a.shots = a.shots or defaults[a.which][1]
     a.group = a.group or (64 if a.which == "carbonate" else 16)
-    fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate}[a.which]
+    fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate,
+          "lanes": run_lanes}[a.which]
     print(json.dumps(fn(a)), flush=True)
 
 
