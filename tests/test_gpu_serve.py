@@ -3,7 +3,8 @@
 * every request's traces are bit-identical alone and in a staggered mix with other requests on the same
   and on other meshes;
 * they equal the standalone shot-innermost FP32 solver's traces for the same shots to FP32 rounding;
-* they are within 1e-4 relative L2 of the FP64 oracle;
+* they are within 1e-4 relative L2 of the FP64 oracle (FP32 serving), and within 1e-12 of it when the
+  server runs in FP64;
 * a tight memory budget queues requests instead of failing, and all of them complete;
 * a request's shots may spread over several slot groups of its mesh, and may move between them when
   groups are compacted, without changing its results.
@@ -50,8 +51,8 @@ def new_server(precision=L.DGX_FP32, budget=0, K=50, group=16):
     return sv, rms, sp, ms
 
 
-def run_alone(k, group=16):
-    sv, rms, _, _ = new_server(group=group)
+def run_alone(k, group=16, precision=L.DGX_FP32):
+    sv, rms, _, _ = new_server(precision=precision, group=group)
     req, xz, rec = build(sv, rms, REQS[k], k)
     rid = sv.submit(req)
     sv.drain()
@@ -159,6 +160,31 @@ def test_request_equals_standalone_solver_and_oracle(group):
         worst = max(worst, rel_l2(x[s], ox), rel_l2(z[s], oz))
     print(f"served fp32 vs fp64 oracle: worst trace rel L2 {worst:.3e}")
     assert worst <= 1e-4
+
+
+@pytest.mark.parametrize("group", [16, 64])
+def test_fp64_serving_matches_oracle(group):
+    """FP64 serving reproduces the oracle to the plain path's parity bar, a request mixed with others."""
+    print(f"case fp64 serve vs oracle, group {group}", flush=True)
+    sv, rms, sp, ms = new_server(precision=L.DGX_FP64, group=group)
+    reqs = [build(sv, rms, REQS[k], k) for k in (0, 1, 3)]
+    ids = [sv.submit(r[0]) for r in reqs[:2]]
+    sv.advance(1)
+    ids.append(sv.submit(reqs[2][0]))  # joins while the others run
+    sv.drain()
+    worst = 0.0
+    for (req, xz, rec), rid, k in zip(reqs, ids, (0, 1, 3)):
+        x, z = sv.result(rid, req)
+        mi, _, nsteps, f0 = REQS[k]
+        mesh, model = ms[mi]
+        srcs = [dgx.SourceSpec(float(a), float(b), L.DGX_SRC_EXPLOSIVE, s, f0) for s, (a, b) in enumerate(xz)]
+        oc = OracleCase(dgx.configs.Workload("serve64", mesh, model, sp, srcs, rec, nsteps, 0))
+        for s in range(len(xz)):
+            ox, oz, _ = oc.run(shot=s)
+            worst = max(worst, rel_l2(x[s], ox), rel_l2(z[s], oz))
+    sv.close()
+    print(f"served fp64 vs oracle: worst trace rel L2 {worst:.3e}")
+    assert worst <= 1e-12
 
 
 def test_tight_budget_queues_and_completes():
