@@ -24,6 +24,8 @@ hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int whic
 hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st);
 hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
+hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double *out, long tri, long spad, long shot,
+                                hipStream_t st);
 hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int variant, hipStream_t st);
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
 hipError_t dgx_launch_batched_f32(const void *args, hipStream_t st, int which);
@@ -142,6 +144,13 @@ struct dgx_solver {
     dgx::StepArgs<double> a64;
     dgx::StepArgs<float> a32;
     std::vector<dgx_src_dev> src_host;
+    /* snapshots (allocated at the first one): the fields of one shot in the caller's element order as
+     * doubles on the device, their pinned host copy, and the stream that moves them while stepping goes on */
+    int *d_perm = nullptr;
+    double *d_snap = nullptr, *h_snap = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+    int snap_mask = 0; /* fields of the snapshot in flight; 0 = none */
 
     const void *args() const { return fp32 ? (const void *)&a32 : (const void *)&a64; }
     size_t real_size() const { return fp32 ? 4 : 8; }
@@ -335,12 +344,17 @@ static void free_solver(dgx_solver *s)
     /* teardown: errors here cannot be acted on */
     for (auto &g : s->graph)
         if (g) (void)hipGraphExecDestroy(g);
+    if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {s->snap_ready, s->snap_done})
+        if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {s->fields, s->fields2, s->nb2, s->tiles, s->ring, s->nbr, s->vtx, s->pt, s->mat, s->absorb,
-                    s->aux, s->src, s->rec, s->wav, s->trace, s->step_base};
+                    s->aux, s->src, s->rec, s->wav, s->trace, s->step_base, s->d_perm, s->d_snap};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (s->h_snap) (void)hipHostFree(s->h_snap);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -952,10 +966,88 @@ static void raw_span(const dgx_solver *s, int c, long shot, size_t *off, size_t 
     }
 }
 
+/* Snapshot buffers, allocated at the first snapshot and kept: the device and pinned host copies hold all
+ * five fields of one shot (tri*6 doubles each). */
+static int snapshot_buffers(dgx_solver *s)
+{
+    if (s->h_snap) return DGX_OK;
+    const size_t n = (size_t)s->tri * 6 * 5;
+    std::vector<int> perm(s->perm.begin(), s->perm.end());
+    HIP_TRY(upload(&s->d_perm, perm));
+    HIP_TRY(hipMalloc((void **)&s->d_snap, n * sizeof(double)));
+    HIP_TRY(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&s->snap_ready, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&s->snap_done, hipEventDisableTiming));
+    HIP_TRY(hipHostMalloc((void **)&s->h_snap, n * sizeof(double), hipHostMallocDefault));
+    return DGX_OK;
+}
+
+extern "C" int dgx_snapshot_begin(dgx_solver *s, long shot, int mask)
+{
+    if (!s || shot < 0 || shot >= s->nshots || mask <= 0 || mask > 31) {
+        dgx_set_error("dgx_snapshot_begin: bad shot or field mask");
+        return DGX_EARG;
+    }
+    if (s->snap_mask) {
+        dgx_set_error("dgx_snapshot_begin: the previous snapshot has not been collected (dgx_snapshot_wait)");
+        return DGX_ESTATE;
+    }
+    int rc = snapshot_buffers(s);
+    if (rc) return rc;
+    const size_t nf = (size_t)s->tri * 6;
+    const long spad = s->layout == DGX_LAYOUT_SHOT_INNER ? s->spad : 0;
+    /* on the solver's stream, after the steps enqueued so far: device-side reorder into the caller's order */
+    for (int c = 0; c < 5; c++) {
+        if (!((mask >> c) & 1)) continue;
+        size_t off, len;
+        raw_span(s, c, shot, &off, &len);
+        HIP_TRY(dgx_launch_unpermute((const char *)s->state() + off, s->fp32, s->d_perm, s->d_snap + c * nf, s->tri,
+                                     spad, shot, s->stream));
+    }
+    HIP_TRY(hipEventRecord(s->snap_ready, s->stream));
+    /* on the copy stream: the download, concurrent with the steps enqueued after this call */
+    HIP_TRY(hipStreamWaitEvent(s->copy_stream, s->snap_ready, 0));
+    for (int c = 0; c < 5; c++)
+        if ((mask >> c) & 1)
+            HIP_TRY(hipMemcpyAsync(s->h_snap + c * nf, s->d_snap + c * nf, nf * sizeof(double), hipMemcpyDeviceToHost,
+                                   s->copy_stream));
+    HIP_TRY(hipEventRecord(s->snap_done, s->copy_stream));
+    s->snap_mask = mask;
+    return DGX_OK;
+}
+
+extern "C" int dgx_snapshot_wait(dgx_solver *s, double *vx, double *vz, double *txx, double *tzz, double *txz)
+{
+    if (!s) return DGX_EARG;
+    if (!s->snap_mask) {
+        dgx_set_error("dgx_snapshot_wait: no snapshot in flight");
+        return DGX_ESTATE;
+    }
+    double *outs[5] = {vx, vz, txx, tzz, txz};
+    const size_t nf = (size_t)s->tri * 6;
+    const int mask = s->snap_mask;
+    s->snap_mask = 0;
+    HIP_TRY(hipEventSynchronize(s->snap_done));
+    for (int c = 0; c < 5; c++)
+        if (((mask >> c) & 1) && outs[c]) std::memcpy(outs[c], s->h_snap + c * nf, nf * sizeof(double));
+    return DGX_OK;
+}
+
 extern "C" int dgx_get_fields(dgx_solver *s, long shot, double *vx, double *vz, double *txx, double *tzz, double *txz)
 {
     if (!s || shot < 0 || shot >= s->nshots) return DGX_EARG;
     double *outs[5] = {vx, vz, txx, tzz, txz};
+    if (!s->snap_mask) {
+        /* no snapshot in flight: the snapshot path (device-side reorder, pinned download) */
+        int mask = 0;
+        for (int c = 0; c < 5; c++)
+            if (outs[c]) mask |= 1 << c;
+        if (!mask) return DGX_OK;
+        int rc = dgx_snapshot_begin(s, shot, mask);
+        if (rc) return rc;
+        return dgx_snapshot_wait(s, vx, vz, txx, tzz, txz);
+    }
+    /* a snapshot of the caller's is in flight and owns the snapshot buffers: copy and reorder on the host */
     HIP_TRY(hipStreamSynchronize(s->stream));
     std::vector<char> raw;
     for (int c = 0; c < 5; c++) {

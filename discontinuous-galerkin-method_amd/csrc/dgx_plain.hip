@@ -630,6 +630,21 @@ __global__ void __launch_bounds__(256) k_read4(const double4 *__restrict__ src, 
     if (threadIdx.x == 0) out[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
+/* Snapshot of one field of one shot in the caller's element order, as doubles: out[6*perm[q] + i] is node i
+ * of device element q, read from the element layout's blocked row or, when `spad` > 0, from the
+ * shot-innermost layout ((q*6 + i)*spad + shot). */
+template <typename R>
+__global__ void __launch_bounds__(256) k_unpermute(const R *__restrict__ f, const int *__restrict__ perm,
+                                                   double *__restrict__ out, int tri, long long spad, long long shot)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= tri) return;
+    double *o = out + 6 * (long long)perm[q];
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+        o[i] = (double)(spad > 0 ? f[((long long)q * 6 + i) * spad + shot] : f[fidx(q, i)]);
+}
+
 __global__ void k_advance(int *step_base, int n)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *step_base += n;
@@ -691,6 +706,14 @@ hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int varian
     if (variant == 0) dgx::k_copy<<<256 * 8, 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
     else if (variant == 1) dgx::k_copy4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double4 *)dst, n32);
     else dgx::k_read4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double *)dst, n32);
+    return hipGetLastError();
+}
+hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double *out, long tri, long spad, long shot,
+                                hipStream_t st)
+{
+    const unsigned blocks = (unsigned)((tri + 255) / 256);
+    if (fp32) dgx::k_unpermute<float><<<blocks, 256, 0, st>>>((const float *)f, perm, out, (int)tri, spad, shot);
+    else dgx::k_unpermute<double><<<blocks, 256, 0, st>>>((const double *)f, perm, out, (int)tri, spad, shot);
     return hipGetLastError();
 }
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st)

@@ -101,6 +101,8 @@ def load() -> C.CDLL:
         "dgx_lu_stress_elastic": (C.c_int, [vp]),
         "dgx_record": (C.c_int, [vp, C.c_long]),
         "dgx_get_fields": (C.c_int, [vp, C.c_long] + [dptr] * 5),
+        "dgx_snapshot_begin": (C.c_int, [vp, C.c_long, C.c_int]),
+        "dgx_snapshot_wait": (C.c_int, [vp] + [dptr] * 5),
         "dgx_set_fields": (C.c_int, [vp, C.c_long] + [dptr] * 5),
         "dgx_get_traces": (C.c_int, [vp, C.c_long, dptr, dptr]),
         "dgx_synchronize": (C.c_int, [vp]),

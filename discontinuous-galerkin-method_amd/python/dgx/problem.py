@@ -217,6 +217,22 @@ class Solver:
         L.check(self.lib.dgx_get_fields(self.handle, shot, *[L.dp(out[c]) for c in range(5)]), "dgx_get_fields")
         return out
 
+    def snapshot_begin(self, shot: int = 0, mask: int = 31):
+        """Start copying the fields in `mask` (bit c: Vx, Vz, Txx, Tzz, Txz) of one shot to the host; the
+        next steps may be enqueued before snapshot_wait collects them."""
+        L.check(self.lib.dgx_snapshot_begin(self.handle, shot, mask), "dgx_snapshot_begin")
+        self._snap_mask = mask
+
+    def snapshot_wait(self, out: np.ndarray | None = None) -> np.ndarray:
+        """[5, Tri*6] fields of the snapshot begun last, into `out` if given (rows outside the snapshot's
+        mask are left as they are: zero in a new array)."""
+        T = self.setup.mesh.tri
+        if out is None:
+            out = np.zeros((5, T * 6), np.float64)
+        assert out.shape == (5, T * 6) and out.dtype == np.float64 and out.flags.c_contiguous
+        L.check(self.lib.dgx_snapshot_wait(self.handle, *[L.dp(out[c]) for c in range(5)]), "dgx_snapshot_wait")
+        return out
+
     def set_fields(self, f: np.ndarray, shot: int = 0):
         f = np.ascontiguousarray(f, np.float64)
         L.check(self.lib.dgx_set_fields(self.handle, shot, *[L.dp(f[c]) for c in range(5)]), "dgx_set_fields")

@@ -10,7 +10,7 @@
  * one.  Plain C types only; every call returns 0 or a negative DGX_E* code, with a message from
  * dgx_last_error().  Caller-owned memory is never retained after a call returns; the solver handle
  * owns all device memory.  One host thread per handle; calls on a handle are stream-ordered and
- * only dgx_get_* / dgx_synchronize block.
+ * only dgx_get_*, dgx_snapshot_wait and dgx_synchronize block.
  */
 #ifndef DGX_H
 #define DGX_H
@@ -80,7 +80,8 @@ typedef struct {
     double dt;
     int precision;         /* enum dgx_precision: arithmetic and storage type of the fields */
     long nshots;           /* S >= 1 independent wavefields on this mesh */
-    long nsteps;           /* trace capacity in steps (steps beyond it are not recorded) */
+    long nsteps;           /* steps of the run: wavelet samples and trace capacity; dgx_step refuses to go
+                            * past it (DGX_ESTATE) */
     long nwavelets;
     const double *wavelet; /* [nwavelets*nsteps] FP64 source samples */
     long nsrc;
@@ -112,8 +113,19 @@ int dgx_lu_velocity(dgx_solver *s);             /* LU_Velocity, OpenMP_Function.
 int dgx_lu_stress_elastic(dgx_solver *s);       /* LU_Stress_Elastic, OpenMP_Function.c:188-306 */
 int dgx_record(dgx_solver *s, long t);          /* Record, OpenMP_Function.c:61-72 (trace slot t) */
 
-/* Fields of one shot in the caller's element order, [tri*6] each (any pointer may be NULL). */
+/* Fields of one shot in the caller's element order, [tri*6] each (any pointer may be NULL): the host
+ * arrays Save_Data writes at the snapshot cadence (DG_leap_Frog_OpenMP.c:433-444).  Blocks until the
+ * steps enqueued so far are done. */
 int dgx_get_fields(dgx_solver *s, long shot, double *vx, double *vz, double *txx, double *tzz, double *txz);
+/* The same as two calls, so that a snapshot leaves the device while the next steps run.  begin takes the
+ * fields in `mask` (bit c: Vx, Vz, Txx, Tzz, Txz for c = 0..4) of one shot at the current step, reorders
+ * them on the device and starts their download into pinned memory without waiting; dgx_step may be called
+ * right after it.  wait blocks until that download is done and copies the fields into the caller's arrays
+ * (pointers of fields outside the mask are ignored).  One snapshot may be in flight per solver: begin
+ * fails (DGX_ESTATE) until the previous one has been collected, and wait fails (DGX_ESTATE) with none.
+ * (The reference writes snapshots synchronously inside its time loop, DG_leap_Frog_OpenMP.c:433-444.) */
+int dgx_snapshot_begin(dgx_solver *s, long shot, int mask);
+int dgx_snapshot_wait(dgx_solver *s, double *vx, double *vz, double *txx, double *tzz, double *txz);
 int dgx_set_fields(dgx_solver *s, long shot, const double *vx, const double *vz, const double *txx,
                    const double *tzz, const double *txz);
 /* Traces of one shot, [nsteps][nrec] like record_x/record_z (DG_leap_Frog_OpenMP.c:166-171). */
