@@ -65,7 +65,7 @@ def run_path(setup, precision, warmup, steps, layout=L.DGX_LAYOUT_AUTO, bench_la
     back-to-back launches between two HIP events on the solver's stream: {"full": ms} for the
     full-step kernel, {"velocity": ms, "stress": ms} for the half-step pair."""
     s = dgx.Solver(setup, precision, layout, step_kernel=step_kernel)
-    info = s.layout_info()
+    info = s.layout_info() + (s.tile_info(),)
     s.step(warmup)
     s.synchronize()
     t = time.perf_counter()
@@ -134,12 +134,13 @@ def batched_leg(a, rate64):
     st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
     T = w.mesh.tri
     secb, ktb, xb, zb, info = run_path(st, L.DGX_FP32, 20, nb, L.DGX_LAYOUT_SHOT_INNER, 20)
+    tinfo = info[3]
     kvb, ksb = ktb["velocity"], ktb["stress"]
     rateb = T * 30 * S * nb / secb
     # FP64 reference on 4 of the shots (same mesh, sources, steps), element layout
     picks = [0, S // 3, (2 * S) // 3, S - 1]
     worst = 0.0
-    s32 = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, info[1])
+    s32 = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, tinfo[0], wg_waves=tinfo[1])
     s32.step(20 + nb)
     for k in picks:
         one = dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge, [dgx.SourceSpec(
@@ -157,8 +158,9 @@ def batched_leg(a, rate64):
     bytes_b = T * 6 * 4 * S
     return {"workload": w.name, "elements": T, "value": rateb, "unit": "DOF-updates/s", "dtype": "f32",
             "shots": S, "steps": nb, "ms_per_step": secb * 1e3 / nb, "x_vs_plain_fp64": rateb / rate64,
-            "trace_rel_l2_vs_fp64_worst_of_4_shots": worst, "layout": "shot-innermost", "shot_tile": info[1],
-            "autotune_ms_per_step": {str(k): v for k, v in info[2].items()},
+            "trace_rel_l2_vs_fp64_worst_of_4_shots": worst, "layout": "shot-innermost", "shot_tile": tinfo[0],
+            "wavefronts_per_workgroup": tinfo[1], "elements_per_workgroup": tinfo[1] * 64 // tinfo[0],
+            "autotune_ms_per_step": {f"tile {t} x {w} wavefronts": v for (t, w), v in tinfo[2].items()},
             "velocity_ms": kvb, "stress_ms": ksb,
             "velocity_GBps": bytes_b * 7 / (kvb * 1e-3) / 1e9, "stress_GBps": bytes_b * 8 / (ksb * 1e-3) / 1e9}
 

@@ -28,6 +28,7 @@ hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double
                                 hipStream_t st);
 hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int variant, hipStream_t st);
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
+int dgx_batched_shape_ok(int shot_tile, int wg_waves);
 hipError_t dgx_launch_batched_f32(const void *args, hipStream_t st, int which);
 }
 
@@ -109,8 +110,9 @@ struct dgx_solver {
     int fp32 = 0;
     int layout = DGX_LAYOUT_ELEMENT;
     int shot_tile = 0;               /* shot-innermost: shots per lane group */
+    int wg_waves = 4;                /* shot-innermost: wavefronts per workgroup */
     long spad = 1;                   /* shot-innermost: padded shot count */
-    std::vector<int> tune_tiles;     /* autotuner candidates and their ms per step */
+    std::vector<int> tune_tiles, tune_waves; /* autotuner candidates and their ms per step */
     std::vector<double> tune_ms;
     long tri = 0, tri_pad = 0, nshots = 1, nsteps = 0, nrec = 0, nsrc = 0;
     long step = 0;          /* steps completed */
@@ -326,6 +328,7 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     a.spad = (int)s->spad;
     a.nshots = (int)s->nshots;
     a.shot_tile = s->shot_tile;
+    a.wg_waves = s->wg_waves;
     a.shot_start = nullptr;
     a.shot_len = nullptr;
 }
@@ -333,7 +336,7 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
 /* Blocks per XCD slice for the current layout and shot tile. */
 static int blocks_per_xcd(const dgx_solver *s)
 {
-    long per_block = (s->layout == DGX_LAYOUT_SHOT_INNER) ? 4L * (64 / s->shot_tile) : DGX_BLOCK;
+    long per_block = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (long)s->wg_waves * (64 / s->shot_tile) : DGX_BLOCK;
     long nblk = (s->tri + per_block - 1) / per_block;
     return (int)((nblk + DGX_NXCD - 1) / DGX_NXCD);
 }
@@ -365,26 +368,30 @@ static std::vector<R> convert(const std::vector<double> &v)
     return std::vector<R>(v.begin(), v.end());
 }
 
-static void set_tile(dgx_solver *s, int tile)
+static void set_tile(dgx_solver *s, int tile, int waves)
 {
     s->shot_tile = tile;
+    s->wg_waves = waves;
     s->nper = blocks_per_xcd(s);
     s->a64.shot_tile = s->a32.shot_tile = tile;
+    s->a64.wg_waves = s->a32.wg_waves = waves;
     s->a64.nper = s->a32.nper = s->nper;
 }
 
-/* Shot-innermost autotuner: time a few real steps with every shot tile that divides the padded shot
- * count (the tile fixes elements per workgroup, 4*64/tile, and shots per lane group), keep the
- * fastest, then restore the initial state (zero fields and traces, step 0). */
-static int autotune(dgx_solver *s)
+/* Shot-innermost autotuner: time a few real steps with every tile shape the kernels are built for whose
+ * shot tile divides the padded shot count (shots per lane group, and wavefronts per workgroup, which
+ * with it fix the elements per workgroup), keep the fastest, then restore the initial state (zero
+ * fields and traces, step 0).  `fixed_waves` > 0 tunes the shot tile alone. */
+static int autotune(dgx_solver *s, int fixed_waves)
 {
     const int cands[] = {4, 8, 16, 32, 64};
     const int nt = (int)std::min<long>(3, s->nsteps);
-    int best = 0;
+    int best = 0, best_waves = 4;
     double best_ms = 1e300;
-    for (int tile : cands) {
-        if (s->spad % tile) continue;
-        set_tile(s, tile);
+    for (int tile : cands)
+    for (int waves : {2, 4, 8}) {
+        if (s->spad % tile || (fixed_waves && waves != fixed_waves) || !dgx_batched_shape_ok(tile, waves)) continue;
+        set_tile(s, tile, waves);
         for (int rep = 0; rep < 2; rep++) { /* first pass warms up */
             HIP_TRY(hipEventRecord(s->ev[0], s->stream));
             for (int k = 0; k < nt; k++) {
@@ -398,12 +405,18 @@ static int autotune(dgx_solver *s)
             HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
             if (rep == 1) {
                 s->tune_tiles.push_back(tile);
+                s->tune_waves.push_back(waves);
                 s->tune_ms.push_back(ms / nt);
-                if (ms < best_ms) { best_ms = ms; best = tile; }
+                if (ms < best_ms) { best_ms = ms; best = tile; best_waves = waves; }
             }
         }
     }
-    set_tile(s, best);
+    if (!best) {
+        dgx_set_error("dgx_create: no shot tile with %d wavefronts per workgroup divides the %ld padded shots",
+                      fixed_waves, s->spad);
+        return DGX_EARG;
+    }
+    set_tile(s, best, best_waves);
     size_t rs = s->real_size();
     HIP_TRY(hipMemsetAsync(s->fields, 0, (size_t)5 * s->tri * 6 * s->spad * rs, s->stream));
     if (s->trace)
@@ -593,12 +606,19 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
             free_solver(s);
             return DGX_EARG;
         }
+        const int w = p->wg_waves;
+        if (w != 0 && (t ? !dgx_batched_shape_ok(t, w) : (w != 2 && w != 4 && w != 8))) {
+            dgx_set_error("dgx_create: wg_waves must be 0 (autotune), 2, 4 or 8, and 4 with shot tiles 4 and 8");
+            free_solver(s);
+            return DGX_EARG;
+        }
         const long unit = t ? t : 8;
         s->spad = (p->nshots + unit - 1) / unit * unit;
         s->shot_tile = t ? t : 8;
+        s->wg_waves = w ? w : 4;
         s->tri_pad = T; /* shot-innermost fields are element-major: no block padding */
-    } else if (p->shot_tile) {
-        dgx_set_error("dgx_create: shot_tile applies to the shot-innermost layout only");
+    } else if (p->shot_tile || p->wg_waves) {
+        dgx_set_error("dgx_create: shot_tile and wg_waves apply to the shot-innermost layout only");
         free_solver(s);
         return DGX_EARG;
     }
@@ -771,7 +791,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     if (s->fp32) fill_args(s, s->a32, ops, p->dt);
     else fill_args(s, s->a64, ops, p->dt);
     if (s->layout == DGX_LAYOUT_SHOT_INNER && p->shot_tile == 0 && s->nsteps > 0) {
-        int rc = autotune(s);
+        int rc = autotune(s, p->wg_waves);
         if (rc) { free_solver(s); return rc; }
     }
     *out = s;
@@ -1166,6 +1186,21 @@ extern "C" int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile,
     int n = (int)s->tune_tiles.size();
     for (int k = 0; k < n && k < cap; k++) {
         if (tiles) tiles[k] = s->tune_tiles[k];
+        if (ms_per_step) ms_per_step[k] = s->tune_ms[k];
+    }
+    return n;
+}
+
+extern "C" int dgx_tile_info(const dgx_solver *s, int *shot_tile, int *wg_waves, int *tiles, int *waves,
+                             double *ms_per_step, int cap)
+{
+    if (!s) return DGX_EARG;
+    if (shot_tile) *shot_tile = s->layout == DGX_LAYOUT_SHOT_INNER ? s->shot_tile : 0;
+    if (wg_waves) *wg_waves = s->layout == DGX_LAYOUT_SHOT_INNER ? s->wg_waves : 0;
+    int n = (int)s->tune_tiles.size();
+    for (int k = 0; k < n && k < cap; k++) {
+        if (tiles) tiles[k] = s->tune_tiles[k];
+        if (waves) waves[k] = s->tune_waves[k];
         if (ms_per_step) ms_per_step[k] = s->tune_ms[k];
     }
     return n;

@@ -2,7 +2,8 @@
  * dgx_batched.hip -- shot-innermost kernels: many shots (independent wavefields) on one mesh.
  *
  * Layout: value (element q, node i, shot s) of a field lives at (q*6 + i)*S + s, S the padded shot
- * count.  A wavefront covers EW = 64/ST elements x ST consecutive shots (lane = e_sub*ST + shot), so
+ * count.  A wavefront covers EW = 64/ST elements x ST consecutive shots (lane = e_sub*ST + shot), and a
+ * workgroup NW wavefronts (NW*EW elements; shot tile and NW are picked by the autotuner at create), so
  * every load of one node of one field for one element is ST consecutive values, and so is every face
  * gather from a neighbour: the irregular part of the access pattern is amortised over ST shots.  The
  * per-element data (neighbours, code word, vertices, material) is identical across a lane group and is
@@ -66,15 +67,15 @@ __device__ __forceinline__ Geom geometry(const double2 *pt, int4 v)
     return g;
 }
 
-/* Element and shot of this lane; false when outside the mesh. */
-template <int ST, typename R>
+/* Element and shot of this lane (workgroups of NW wavefronts); false when outside the mesh. */
+template <int ST, int NW, typename R>
 __device__ __forceinline__ bool my_work(const StepArgs<R> &a, int &q, int &s)
 {
     constexpr int EW = 64 / ST;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = blockIdx.x;
     const int tile = (b & (DGX_NXCD - 1)) * a.nper + (b >> 3); /* XCD-contiguous element tiles */
-    q = (tile * 4 + wave) * EW + lane / ST;
+    q = (tile * NW + wave) * EW + lane / ST;
     s = blockIdx.y * ST + (lane % ST);
     return q < a.tri;
 }
@@ -440,11 +441,11 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
     }
 }
 
-template <typename R, int ST>
-__global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
+template <typename R, int ST, int NW>
+__global__ void __launch_bounds__(64 * NW) kb_velocity(StepArgs<R> a)
 {
     int q, s;
-    if (!my_work<ST>(a, q, s)) return;
+    if (!my_work<ST, NW>(a, q, s)) return;
     /* with 64 shots per tile a wavefront holds one element: saying so turns the element's records, its
      * neighbour ids and the row offsets into scalar values (measured: faster for the velocity kernel,
      * slightly slower for the stress kernel, which keeps the per-lane form) */
@@ -452,11 +453,11 @@ __global__ void __launch_bounds__(DGX_BLOCK) kb_velocity(StepArgs<R> a)
     velocity_body(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
 }
 
-template <typename R, int ST>
-__global__ void __launch_bounds__(DGX_BLOCK) kb_stress(StepArgs<R> a)
+template <typename R, int ST, int NW>
+__global__ void __launch_bounds__(64 * NW) kb_stress(StepArgs<R> a)
 {
     int q, s;
-    if (!my_work<ST>(a, q, s)) return;
+    if (!my_work<ST, NW>(a, q, s)) return;
     stress_body(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
 }
 
@@ -594,24 +595,37 @@ static hipError_t launch_move(void *const *src, void *const *dst, long long n_no
     return hipGetLastError();
 }
 
-template <typename R, int ST>
+template <typename R, int ST, int NW>
 static hipError_t launch_b(const dgx::StepArgs<R> &a, hipStream_t st, int which)
 {
-    dim3 grid(DGX_NXCD * a.nper, a.spad / ST), block(DGX_BLOCK);
-    if (which == 0) dgxb::kb_velocity<R, ST><<<grid, block, 0, st>>>(a);
-    else dgxb::kb_stress<R, ST><<<grid, block, 0, st>>>(a);
+    dim3 grid(DGX_NXCD * a.nper, a.spad / ST), block(64 * NW);
+    if (which == 0) dgxb::kb_velocity<R, ST, NW><<<grid, block, 0, st>>>(a);
+    else dgxb::kb_stress<R, ST, NW><<<grid, block, 0, st>>>(a);
     return hipGetLastError();
+}
+
+/* The tile shapes the shot-innermost kernels are built for: shot tiles 4 and 8 with 4 wavefronts per
+ * workgroup, and 16, 32, 64 with 2, 4 or 8 (dgx_batched_shape_ok lists the same set for the host). */
+template <typename R, int ST>
+static hipError_t launch_bw(const dgx::StepArgs<R> &a, hipStream_t st, int which)
+{
+    switch (a.wg_waves) {
+    case 2: return launch_b<R, ST, 2>(a, st, which);
+    case 4: return launch_b<R, ST, 4>(a, st, which);
+    case 8: return launch_b<R, ST, 8>(a, st, which);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <typename R>
 static hipError_t dispatch_b(const dgx::StepArgs<R> &a, hipStream_t st, int which)
 {
     switch (a.shot_tile) {
-    case 4: return launch_b<R, 4>(a, st, which);
-    case 8: return launch_b<R, 8>(a, st, which);
-    case 16: return launch_b<R, 16>(a, st, which);
-    case 32: return launch_b<R, 32>(a, st, which);
-    case 64: return launch_b<R, 64>(a, st, which);
+    case 4: return a.wg_waves == 4 ? launch_b<R, 4, 4>(a, st, which) : hipErrorInvalidValue;
+    case 8: return a.wg_waves == 4 ? launch_b<R, 8, 4>(a, st, which) : hipErrorInvalidValue;
+    case 16: return launch_bw<R, 16>(a, st, which);
+    case 32: return launch_bw<R, 32>(a, st, which);
+    case 64: return launch_bw<R, 64>(a, st, which);
     }
     return hipErrorInvalidValue;
 }
@@ -670,6 +684,12 @@ hipError_t dgx_launch_move_slots(int fp32, void *const *src, void *const *dst, l
         if (from[k] < 0 || from[k] >= group || to[k] < 0 || to[k] >= group) return hipErrorInvalidValue;
     return fp32 ? launch_move<float>(src, dst, n_nodes, group, n, from, to, st)
                 : launch_move<double>(src, dst, n_nodes, group, n, from, to, st);
+}
+int dgx_batched_shape_ok(int shot_tile, int wg_waves)
+{
+    if (shot_tile == 4 || shot_tile == 8) return wg_waves == 4;
+    if (shot_tile == 16 || shot_tile == 32 || shot_tile == 64) return wg_waves == 2 || wg_waves == 4 || wg_waves == 8;
+    return 0;
 }
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which)
 {

@@ -51,7 +51,7 @@ class ProblemC(C.Structure):
                 ("dt", C.c_double), ("precision", C.c_int), ("nshots", C.c_long), ("nsteps", C.c_long),
                 ("nwavelets", C.c_long), ("wavelet", dptr), ("nsrc", C.c_long), ("src", C.POINTER(Source)),
                 ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("layout", C.c_int), ("shot_tile", C.c_int),
-                ("step_kernel", C.c_int)]
+                ("step_kernel", C.c_int), ("wg_waves", C.c_int)]
 
 
 class ServerConfig(C.Structure):
@@ -112,6 +112,7 @@ def load() -> C.CDLL:
         "dgx_kernel_bench": (C.c_int, [vp, C.c_int, C.c_long, dptr]),
         "dgx_copy_probe": (C.c_int, [C.c_long, C.c_long, C.c_int, dptr]),
         "dgx_layout_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), dptr, C.c_int]),
+        "dgx_tile_info": (C.c_int, [vp] + [C.POINTER(C.c_int)] * 4 + [dptr, C.c_int]),
         "dgx_compiled_operators": (C.c_int, [vp]),
         "dgx_read_ascii": (C.c_long, [C.c_char_p, dptr, C.c_long, C.c_long]),
         "dgx_read_ascii_long": (C.c_long, [C.c_char_p, lptr, C.c_long, C.c_long]),

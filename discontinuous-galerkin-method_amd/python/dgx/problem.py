@@ -149,7 +149,7 @@ class Solver:
     """Owns one dgx_solver handle (device-resident fields for ``nshots`` shots on one mesh)."""
 
     def __init__(self, setup: Setup, precision: int = L.DGX_FP64, layout: int = L.DGX_LAYOUT_AUTO,
-                 shot_tile: int = 0, step_kernel: int = L.DGX_STEP_AUTO):
+                 shot_tile: int = 0, step_kernel: int = L.DGX_STEP_AUTO, wg_waves: int = 0):
         self.lib = L.load()
         self.setup = setup
         m = setup.mesh
@@ -169,7 +169,7 @@ class Solver:
         p = L.ProblemC(m.tri, m.point_num, L.dp(pt), L.lp(tr), L.lp(nb), L.dp(la), L.dp(mi), L.dp(de),
                        L.dp(ab) if ab is not None else L.dptr(), setup.dt, precision, setup.nshots, setup.nsteps,
                        wv.shape[0], L.dp(wv), len(setup.sources), srcs, len(setup.receivers), recs, layout,
-                       shot_tile, step_kernel)
+                       shot_tile, step_kernel, wg_waves)
         self.handle = C.c_void_p()
         L.check(self.lib.dgx_create(C.byref(p), C.byref(setup.ops), C.byref(self.handle)), "dgx_create")
         self.precision = precision
@@ -253,12 +253,22 @@ class Solver:
         return ms.value
 
     def layout_info(self):
-        """(layout, shot_tile, {tile: ms per step} from the autotuner)."""
+        """(layout, shot_tile, {tile: best ms per step over the workgroup sizes} from the autotuner)."""
         lay, tile = C.c_int(), C.c_int()
-        tiles = (C.c_int * 8)()
-        ms = (C.c_double * 8)()
-        n = L.check(self.lib.dgx_layout_info(self.handle, C.byref(lay), C.byref(tile), tiles, ms, 8), "layout_info")
-        return lay.value, tile.value, {tiles[k]: ms[k] for k in range(n)}
+        tiles = (C.c_int * 32)()
+        ms = (C.c_double * 32)()
+        n = L.check(self.lib.dgx_layout_info(self.handle, C.byref(lay), C.byref(tile), tiles, ms, 32), "layout_info")
+        best = {}
+        for k in range(min(n, 32)):
+            best[tiles[k]] = min(best.get(tiles[k], float("inf")), ms[k])
+        return lay.value, tile.value, best
+
+    def tile_info(self):
+        """(shot_tile, wavefronts per workgroup, {(tile, waves): ms per step} from the autotuner)."""
+        tile, waves = C.c_int(), C.c_int()
+        tl, wv, ms = (C.c_int * 32)(), (C.c_int * 32)(), (C.c_double * 32)()
+        n = L.check(self.lib.dgx_tile_info(self.handle, C.byref(tile), C.byref(waves), tl, wv, ms, 32), "tile_info")
+        return tile.value, waves.value, {(tl[k], wv[k]): ms[k] for k in range(n)}
 
     @property
     def compiled_operators(self) -> bool:

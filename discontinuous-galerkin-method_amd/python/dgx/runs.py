@@ -98,7 +98,7 @@ def run_slope(a):
     log(f"slope mesh {w.mesh.tri} elements")
     st = build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
     s32, sec32 = timed_run(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER)
-    info = s32.layout_info()
+    tinfo = s32.tile_info()
     picks = [0, a.shots // 3, (2 * a.shots) // 3, a.shots - 1]
     checks, sec64 = [], 0.0
     for k in picks:
@@ -115,7 +115,7 @@ def run_slope(a):
     s32.close()
     dof = w.mesh.tri * 30 * a.shots * a.steps
     return {"workload": w.name, "elements": w.mesh.tri, "shots": a.shots, "steps": a.steps, "dt": st.dt,
-            "batched_fp32_dof_updates_per_s": dof / sec32, "shot_tile": info[1],
+            "batched_fp32_dof_updates_per_s": dof / sec32, "shot_tile": tinfo[0], "wavefronts_per_workgroup": tinfo[1],
             "plain_fp64_dof_updates_per_s": w.mesh.tri * 30 * a.steps * len(picks) / sec64,
             "fp64_reference_checks": checks}
 

@@ -91,6 +91,9 @@ typedef struct {
     int layout;            /* enum dgx_layout */
     int shot_tile;         /* shots per lane group in the shot-innermost layout; 0 = autotune at create */
     int step_kernel;       /* enum dgx_step_kernel */
+    int wg_waves;          /* wavefronts per workgroup of the shot-innermost kernels: 2, 4 or 8 with shot
+                            * tiles 16-64, 4 with tiles 4 and 8.  0: autotuned with the shot tile when
+                            * shot_tile is 0, else 4.  Given with shot_tile 0, only the tile is tuned. */
 } dgx_problem;
 
 typedef struct dgx_solver dgx_solver;
@@ -147,6 +150,12 @@ int dgx_copy_probe(long bytes, long reps, int variant, double *ms);
 /* Layout and shot tile in use (after autotuning), and the autotuner's timings: for each candidate
  * tile k, tiles[k] and ms_per_step[k]; returns the number of candidates (0 if none were timed). */
 int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile, int *tiles, double *ms_per_step, int cap);
+/* The tile shape in use: shot tile and wavefronts per workgroup (elements per workgroup =
+ * wg_waves * 64 / shot_tile), and the autotuner's table: candidate k is (tiles[k], waves[k]) at
+ * ms_per_step[k].  Returns the number of candidates timed.  No counterpart in the reference, which has
+ * one shot per run. */
+int dgx_tile_info(const dgx_solver *s, int *shot_tile, int *wg_waves, int *tiles, int *waves, double *ms_per_step,
+                  int cap);
 /* 1 when the element-layout velocity kernels of this solver run with the P2 Fekete operators of the
  * shipped operator files compiled in (its operators equal them bit for bit and DGX_GENERIC_OPERATORS=1 is not
  * set), 0 when they read the operators from their kernel argument or the layout is shot-innermost.

@@ -84,7 +84,8 @@ def test_create_rejects_invalid_configurations():
     """Every rejection below happens on the host, with DGX_EARG and a message, before any device work:
     an empty mesh, a negative step count, an unknown precision, layout or step kernel, the full-step
     kernel with the shot-innermost layout, a shot tile that is not a power of two from 4 to 64 or that
-    is given for the element layout, and sources or receivers that name a missing element, shot,
+    is given for the element layout, a workgroup size the kernels are not built for (or given for the
+    element layout), and sources or receivers that name a missing element, shot,
     wavelet or kind."""
     lib = L.load()
     mesh = dgx.structured(4, 4, 10.0)
@@ -100,6 +101,11 @@ def test_create_rejects_invalid_configurations():
                                               step_kernel=L.DGX_STEP_FULL),
         "shot tile 12": _problem(mesh, nshots=8, layout=L.DGX_LAYOUT_SHOT_INNER, shot_tile=12),
         "shot tile, element layout": _problem(mesh, layout=L.DGX_LAYOUT_ELEMENT, shot_tile=16),
+        "wavefronts per workgroup 3": _problem(mesh, nshots=16, layout=L.DGX_LAYOUT_SHOT_INNER, shot_tile=16,
+                                               wg_waves=3),
+        "2 wavefronts with shot tile 8": _problem(mesh, nshots=8, layout=L.DGX_LAYOUT_SHOT_INNER, shot_tile=8,
+                                                  wg_waves=2),
+        "wavefronts per workgroup, element layout": _problem(mesh, layout=L.DGX_LAYOUT_ELEMENT, wg_waves=4),
         "source element 0": _with_source(mesh, elem=0),
         "source element past the mesh": _with_source(mesh, elem=T + 1),
         "source shot": _with_source(mesh, shot=1),

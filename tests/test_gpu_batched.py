@@ -67,17 +67,35 @@ def test_batched_fp32_against_fp64_oracle():
     assert worst <= 1e-4
 
 
+def test_workgroup_size_does_not_change_results():
+    """The wavefronts per workgroup only regroup elements into workgroups: bit-identical traces."""
+    w = batched_workload(16, 100)
+    st = product_setup(w)
+    ref = None
+    for waves in (2, 4, 8):
+        s = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, 16, wg_waves=waves)
+        assert s.tile_info()[:2] == (16, waves)
+        s.step(w.nsteps)
+        tr = np.stack([np.stack(s.traces(k)) for k in (0, 15)])
+        s.close()
+        if ref is None:
+            ref = tr
+        assert np.array_equal(tr, ref), waves
+
+
 def test_autotuner_restores_initial_state():
     w = batched_workload(32, 120)
     st = product_setup(w)
     auto = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, 0)
-    lay, tile, table = auto.layout_info()
-    print("autotune table (ms/step):", table, "chosen", tile)
-    assert lay == L.DGX_LAYOUT_SHOT_INNER and tile in table and len(table) >= 3
+    lay, tile, _ = auto.layout_info()
+    tile2, waves, table = auto.tile_info()
+    print("autotune table (ms/step):", table, "chosen", tile, waves)
+    # shot tiles 4 and 8 with 4 wavefronts, 16 and 32 with 2, 4 and 8 (32 padded shots: no tile 64)
+    assert lay == L.DGX_LAYOUT_SHOT_INNER and tile == tile2 and (tile, waves) in table and len(table) == 8
     assert auto.current_step == 0
     assert not np.any(auto.fields(7))
     auto.step(w.nsteps)
-    fixed = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, tile)
+    fixed = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, tile, wg_waves=waves)
     fixed.step(w.nsteps)
     for k in (0, 31):
         assert np.array_equal(auto.traces(k)[0], fixed.traces(k)[0])
