@@ -187,6 +187,36 @@ def test_fp64_serving_matches_oracle(group):
     assert worst <= 1e-12
 
 
+def test_many_slot_groups():
+    """80 requests of 16 shots on one mesh fill 80 slot groups (80 batches in one launch): every sampled
+    request equals the same request served alone, bit for bit."""
+    print("case many slot groups", flush=True)
+    sp, ms = meshes()
+
+    def req(sv, rms, k):
+        rng = np.random.default_rng(500 + k)
+        xz = np.stack([rng.uniform(300, 1700, 16), rng.uniform(600, 1400, 16)], 1)
+        rec = np.stack([np.linspace(200, 1800, 5), np.full(5, 1450.0)], 1)
+        return sv.make_request(rms[1], xz, [20.0] * 16, 40 + (k % 7) * 10, rec)
+
+    sv, rms, _, _ = new_server(group=16, K=25)
+    reqs = [req(sv, rms, k) for k in range(80)]
+    ids = [sv.submit(r) for r in reqs]
+    sv.advance(1)
+    assert sv.stats()["running"] == 80
+    sv.drain()
+    mixed = {k: sv.result(ids[k], reqs[k]) for k in (0, 41, 63, 64, 79)}
+    sv.close()
+    for k, (x, z) in mixed.items():
+        sv1, rms1, _, _ = new_server(group=16, K=25)
+        r = req(sv1, rms1, k)
+        rid = sv1.submit(r)
+        sv1.drain()
+        x1, z1 = sv1.result(rid, r)
+        sv1.close()
+        assert np.array_equal(x, x1) and np.array_equal(z, z1), k
+
+
 def test_tight_budget_queues_and_completes():
     sp, ms = meshes()
     T0 = ms[0][0].tri
