@@ -990,15 +990,21 @@ static void raw_span(const dgx_solver *s, int c, long shot, size_t *off, size_t 
  * five fields of one shot (tri*6 doubles each). */
 static int snapshot_buffers(dgx_solver *s)
 {
-    if (s->h_snap) return DGX_OK;
+    /* each piece once, so that a call after a failed allocation neither leaks nor repeats the others */
     const size_t n = (size_t)s->tri * 6 * 5;
-    std::vector<int> perm(s->perm.begin(), s->perm.end());
-    HIP_TRY(upload(&s->d_perm, perm));
-    HIP_TRY(hipMalloc((void **)&s->d_snap, n * sizeof(double)));
-    HIP_TRY(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&s->snap_ready, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&s->snap_done, hipEventDisableTiming));
-    HIP_TRY(hipHostMalloc((void **)&s->h_snap, n * sizeof(double), hipHostMallocDefault));
+    if (!s->d_perm) {
+        std::vector<int> perm(s->perm.begin(), s->perm.end());
+        int *d = nullptr;
+        const hipError_t e = upload(&d, perm);
+        if (e != hipSuccess && d) (void)hipFree(d);
+        HIP_TRY(e);
+        s->d_perm = d;
+    }
+    if (!s->d_snap) HIP_TRY(hipMalloc((void **)&s->d_snap, n * sizeof(double)));
+    if (!s->copy_stream) HIP_TRY(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    if (!s->snap_ready) HIP_TRY(hipEventCreateWithFlags(&s->snap_ready, hipEventDisableTiming));
+    if (!s->snap_done) HIP_TRY(hipEventCreateWithFlags(&s->snap_done, hipEventDisableTiming));
+    if (!s->h_snap) HIP_TRY(hipHostMalloc((void **)&s->h_snap, n * sizeof(double), hipHostMallocDefault));
     return DGX_OK;
 }
 
