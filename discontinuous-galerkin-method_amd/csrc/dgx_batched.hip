@@ -488,7 +488,7 @@ __device__ __forceinline__ StepArgs<R> global_args(const StepArgs<R> &t)
 /*
  * Serving: one launch per half-step over every active (mesh, slot group) batch.  sched = {nbatch,
  * total tiles, tile offset of batch 0, ..., tile offset of batch nbatch}; a tile is 256/G elements x G
- * shots (G = slots per group, 16 or 64: 64/G elements per wavefront).  One workgroup per tile, and the
+ * shots (G = slots per group, 16, 32 or 64: 64/G elements per wavefront).  One workgroup per tile, and the
  * tiles are dealt to the 8 XCDs in contiguous ranges swept in order, as in the batched kernels: face
  * neighbours are stepped close together in time on one XCD and their rows are still in its L2.  (A
  * persistent grid-stride loop over the same tiles let the workgroups drift apart over their ~200 tiles
@@ -643,6 +643,9 @@ static hipError_t launch_serve(const void *ops, const void *table, const int *sc
     if (group == 16) {
         if (vel) dgxb::ks_step<R, true, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
         else dgxb::ks_step<R, false, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
+    } else if (group == 32) {
+        if (vel) dgxb::ks_step<R, true, 32><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
+        else dgxb::ks_step<R, false, 32><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
     } else if (group == 64) {
         if (vel) dgxb::ks_step<R, true, 64><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
         else dgxb::ks_step<R, false, 64><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);

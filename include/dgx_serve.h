@@ -3,7 +3,7 @@
  *
  * No counterpart in the reference, which runs one shot per process (DG_leap_Frog_OpenMP.c:397 loops over
  * a single shot).  The server keeps meshes, materials and sponge factors resident in HBM and runs many
- * shots of many requests at once: every request is 1..group_shots shots (16 or 64, dgx_server_config) on
+ * shots of many requests at once: every request is 1..group_shots shots (16, 32 or 64, dgx_server_config) on
  * one resident mesh, each shot with its own source, FP64 wavelet samples and start step, all shots of a
  * request sharing one receiver spread.
  * Requests are admitted as slots and memory free up (continuous batching), stepped by one kernel per
@@ -26,9 +26,10 @@ typedef struct {
     int precision;         /* DGX_FP32 (default for serving) or DGX_FP64 */
     long long hbm_budget;  /* bytes for wavefields, wavelets and traces; 0 = 80 % of free HBM */
     int steps_per_block;   /* steps between scheduling points (one graph replay); 0 = 100 */
-    int group_shots;       /* shots per slot group, 16 or 64 (0 = 16): the serving kernel's shot tile.  64
-                            * moves whole 256-byte rows per node (faster); 16 wastes fewer lanes when
-                            * requests are small and meshes many */
+    int group_shots;       /* shots per slot group, 16, 32 or 64 (0 = 16): the serving kernel's shot tile.
+                            * A node row of a group is 4*G bytes in FP32: 32 and 64 move whole 128-byte
+                            * lines (faster per lane); 16 wastes fewer lanes when requests are small and
+                            * meshes many */
 } dgx_server_config;
 
 enum dgx_request_state { DGX_REQ_QUEUED = 0, DGX_REQ_RUNNING = 1, DGX_REQ_DONE = 2 };
