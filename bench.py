@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU time budget of the reference step-loop sample (the first two timed steps overestimate the per-step time, so the sample runs somewhat shorter)")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batched FP32 leg (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--serve-requests", type=int, default=24,
