@@ -24,6 +24,9 @@
  * (OpenMP_Function.c:25-46) is applied to every value as it is loaded, the source increment
  * (OpenMP_Function.c:49-58) right after it, and the receiver pick (OpenMP_Function.c:61-72) is done by
  * the stress kernel from the velocities it already holds.  Each field is read once per half-step.
+ * The half-step kernels stage the values their workgroup's 256 elements hold (conditioned stresses for the
+ * velocity update, new velocities for the stress update) in LDS, and read the face neighbours that lie in
+ * the workgroup from there; only the others are gathered from global memory.
  * k_fullstep runs both half-steps in one launch (opt-in; see its comment).
  */
 #include <hip/hip_runtime.h>
@@ -333,6 +336,186 @@ __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int
 }
 
 /*
+ * Velocity half-step with the workgroup's own stresses staged in LDS.  Each thread loads and conditions
+ * its element's stresses and writes them to `wsh` ((c*6 + i)*DGX_BLOCK + thread, c = xx, zz, xz); face
+ * neighbours inside the workgroup's 256 elements (about 91 % of interior faces in the locality order) are
+ * then read from there after one barrier, already conditioned by their own thread, and only the others
+ * are gathered from global memory, issued before the barrier.  Every thread of the workgroup reaches the
+ * barrier: threads past the last element load element 0 and return after it.  The values and the
+ * arithmetic are those of velocity_update (bit-identical results measured in FP64 and FP32).
+ */
+template <typename R, bool FUSED, bool P2>
+__device__ __forceinline__ void velocity_update_staged(const StepArgs<R> &a, int q, int q0, bool valid, int shot,
+                                                       int step, const R *__restrict__ Txx,
+                                                       const R *__restrict__ Tzz, const R *__restrict__ Txz,
+                                                       R *__restrict__ Vx, R *__restrict__ Vz, R *wsh)
+{
+    const int qs = valid ? q : 0;
+    const int4 nb = a.nbr[qs];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[qs]);
+
+    R txx[6], tzz[6], txz[6], vxe[6], vze[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(qs, i);
+        txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
+        if (sizeof(R) == 4) { vxe[i] = Vx[o]; vze[i] = Vz[o]; }
+    }
+    /* face gathers from outside the workgroup, issued before the barrier */
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+    bool inwg[3];
+    R gxx[3][3], gzz[3][3], gxz[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int ee = nbs[j] >= 0 ? nbs[j] : qs;
+        inwg[j] = (unsigned)(ee - q0) < (unsigned)DGX_BLOCK;
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            gxx[j][k] = 0; gzz[j][k] = 0; gxz[j][k] = 0;
+            if (!inwg[j]) {
+                const int o = fidx(ee, nb_node(m, rev, k));
+                gxx[j][k] = Txx[o]; gzz[j][k] = Tzz[o]; gxz[j][k] = Txz[o];
+            }
+        }
+    }
+    if (FUSED) {
+        if constexpr (sizeof(R) == 4 || P2) {
+            condition_stress6(a, code, qs, shot, step, txx, tzz, txz);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; i++) condition_stress(a, code, qs, i, shot, step, txx[i], tzz[i], txz[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        wsh[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = txx[i];
+        wsh[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = tzz[i];
+        wsh[(2 * 6 + i) * DGX_BLOCK + threadIdx.x] = txz[i];
+    }
+
+    const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
+    R rhx[6], rhz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { rhx[i] = 0; rhz[i] = 0; }
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        const R ux = rx * txx[n] + rz * txz[n], wx = sx * txx[n] + sz * txz[n];
+        const R uz = rx * txz[n] + rz * tzz[n], wz = sx * txz[n] + sz * tzz[n];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            rhx[i] += op_dr<P2>(a, i * 6 + n) * ux + op_ds<P2>(a, i * 6 + n) * wx;
+            rhz[i] += op_dr<P2>(a, i * 6 + n) * uz + op_ds<P2>(a, i * 6 + n) * wz;
+        }
+    }
+
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (!inwg[j]) continue;
+        const int ee = nbs[j] >= 0 ? nbs[j] : qs;
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+        const int r = ee - q0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int nn = nb_node(m, rev, k);
+            gxx[j][k] = wsh[(0 * 6 + nn) * DGX_BLOCK + r];
+            gzz[j][k] = wsh[(1 * 6 + nn) * DGX_BLOCK + r];
+            gxz[j][k] = wsh[(2 * 6 + nn) * DGX_BLOCK + r];
+        }
+    }
+    if (!valid) return;
+    if (FUSED && (code & (DGX_CODE_NBSPECIAL(0) | DGX_CODE_NBSPECIAL(1) | DGX_CODE_NBSPECIAL(2)))) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            /* neighbours staged in LDS were conditioned by their own thread */
+            if (!(code & DGX_CODE_NBSPECIAL(j)) || nbs[j] < 0 || inwg[j]) continue;
+            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+            const unsigned ncode = (unsigned)a.nbr[nbs[j]].w;
+            for (int k = 0; k < 3; k++)
+                condition_stress(a, ncode, nbs[j], nb_node(m, rev, k), shot, step, gxx[j][k], gzz[j][k], gxz[j][k]);
+        }
+    }
+    R flx[6], flz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const bool has = nbs[j] >= 0;
+        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(j, k);
+            R dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
+            if (has) { dxx += gxx[j][k]; dzz += gzz[j][k]; dxz += gxz[j][k]; }
+            const R gx = ez * dxx - ex * dxz;
+            const R gz = ez * dxz - ex * dzz;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                flx[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gx;
+                flz[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gz;
+            }
+        }
+    }
+
+    const R invD = (R)g.invD;
+    const R c = a.rho_inv[q] * (R)a.dt;
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if constexpr (sizeof(R) == 4) {
+        if (FUSED) {
+            if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    const R f = a.absorb[fidx(q, i)];
+                    vxe[i] *= f;
+                    vze[i] *= f;
+                }
+            }
+            if (ax) {
+                const dgx_aux x = a.aux[ax - 1];
+                for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+                    const dgx_src_dev &src = a.src[s];
+                    if (src.shot != shot || src.kind != 0) continue;
+                    const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
+#pragma unroll
+                    for (int i = 0; i < 6; i++) vze[i] += (R)(src.fun[i] * w);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const int oo = fidx(q, i);
+            Vx[oo] = (rhx[i] + flx[i] * invD) * c + vxe[i];
+            Vz[oo] = (rhz[i] + flz[i] * invD) * c + vze[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const int o = fidx(q, i);
+            R vx = Vx[o], vz = Vz[o];
+            if (FUSED) {
+                if (code & DGX_CODE_SPONGE) {
+                    const R f = a.absorb[o];
+                    vx *= f;
+                    vz *= f;
+                }
+                if (ax) {
+                    const dgx_aux x = a.aux[ax - 1];
+                    for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+                        const dgx_src_dev &src = a.src[s];
+                        if (src.shot == shot && src.kind == 0)
+                            vz += (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + step]);
+                    }
+                }
+            }
+            Vx[o] = (rhx[i] + flx[i] * invD) * c + vx;
+            Vz[o] = (rhz[i] + flz[i] * invD) * c + vz;
+        }
+    }
+}
+
+/*
  * Stress update of element q from the new velocities V (own values at row q, face neighbour j's at row
  * nrow.j; nrow.w is the element's code word).  In place, own stresses come from Txx, Tzz, Txz and are
  * conditioned here; in the full-step kernel (tload != nullptr) they come conditioned from LDS.  Records the
@@ -452,31 +635,174 @@ __device__ __forceinline__ void stress_update(const StepArgs<R> &a, int q, int s
     }
 }
 
+/*
+ * Stress half-step with the workgroup's new velocities staged in LDS (`wsh`, (c*6 + i)*DGX_BLOCK + thread,
+ * c = x, z), as velocity_update_staged does for the stresses: face neighbours inside the workgroup are read
+ * from LDS after one barrier, the others are gathered from global memory before it.  The values and the
+ * arithmetic are those of stress_update (in place); results agree with it to rounding (in FP64 the compiler
+ * contracts a few products into FMAs differently: measured differences are at 1e-82 absolute on traces of
+ * order 1e-8).
+ */
+template <typename R, bool FUSED, bool P2>
+__device__ __forceinline__ void stress_update_staged(const StepArgs<R> &a, int q, int q0, bool valid, int shot,
+                                                     int step, const R *__restrict__ Vx, const R *__restrict__ Vz,
+                                                     R *__restrict__ Txx, R *__restrict__ Tzz, R *__restrict__ Txz,
+                                                     R *wsh)
+{
+    const int qs = valid ? q : 0;
+    const int4 nb = a.nbr[qs];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[qs]);
+
+    R vx[6], vz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(qs, i);
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+    bool inwg[3];
+    R gvx[3][3], gvz[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int ee = nbs[j] >= 0 ? nbs[j] : qs;
+        inwg[j] = (unsigned)(ee - q0) < (unsigned)DGX_BLOCK;
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            gvx[j][k] = 0; gvz[j][k] = 0;
+            if (!inwg[j]) {
+                const int o = fidx(ee, nb_node(m, rev, k));
+                gvx[j][k] = Vx[o]; gvz[j][k] = Vz[o];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        wsh[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = vx[i];
+        wsh[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = vz[i];
+    }
+    const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
+    R exx[6], ezz[6], exz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        R rvx = 0, svx = 0, rvz = 0, svz = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            rvx += a.op.Dr[i * 6 + n] * vx[n];
+            svx += a.op.Ds[i * 6 + n] * vx[n];
+            rvz += a.op.Dr[i * 6 + n] * vz[n];
+            svz += a.op.Ds[i * 6 + n] * vz[n];
+        }
+        exx[i] = rx * rvx + sx * svx;
+        ezz[i] = rz * rvz + sz * svz;
+        exz[i] = rx * rvz + sx * svz + rz * rvx + sz * svx;
+    }
+
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (!inwg[j]) continue;
+        const int ee = nbs[j] >= 0 ? nbs[j] : qs;
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+        const int r = ee - q0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int nn = nb_node(m, rev, k);
+            gvx[j][k] = wsh[(0 * 6 + nn) * DGX_BLOCK + r];
+            gvz[j][k] = wsh[(1 * 6 + nn) * DGX_BLOCK + r];
+        }
+    }
+    if (!valid) return;
+
+    R fx[6], fz[6], fxz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const bool has = nbs[j] >= 0;
+        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+        R px[3], pz[3], pxz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(j, k);
+            R dvx = -vx[on], dvz = -vz[on];
+            if (has) { dvx += gvx[j][k]; dvz += gvz[j][k]; }
+            px[k] = ez * dvx;
+            pz[k] = -ex * dvz;
+            pxz[k] = ez * dvz - ex * dvx;
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const R l = op_l<P2>(a, j * 18 + i * 3 + k);
+                fx[i] += l * px[k];
+                fz[i] += l * pz[k];
+                fxz[i] += l * pxz[k];
+            }
+        }
+    }
+
+    const R invD = (R)g.invD;
+    const R lam = a.lam[q], mu = a.mu[q], l2m = lam + 2 * mu, dt = (R)a.dt;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        R txx = Txx[o], tzz = Tzz[o], txz = Txz[o];
+        if (FUSED) condition_stress(a, code, q, i, shot, step, txx, tzz, txz);
+        const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
+        Txx[o] = (l2m * axx + lam * azz) * dt + txx;
+        Tzz[o] = (lam * axx + l2m * azz) * dt + tzz;
+        Txz[o] = (mu * axz) * dt + txz;
+    }
+
+    if (FUSED) {
+        const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+        if (ax && step < a.nsteps) {
+            const dgx_aux x = a.aux[ax - 1];
+            for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
+                const dgx_rec_dev &rc = a.rec[r];
+                double sx_ = 0, sz_ = 0;
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    sx_ += rc.w[i] * (double)vx[i];
+                    sz_ += rc.w[i] * (double)vz[i];
+                }
+                double *t = a.trace + ((long long)shot * a.nsteps + step) * 2 * a.nrec;
+                t[rc.index] = sx_;
+                t[a.nrec + rc.index] = sz_;
+            }
+        }
+    }
+}
+
 /* The element kernels ask for at least 3 wavefronts per SIMD: this caps the FP64 kernels at 168 VGPRs
  * (unbounded they take 170-171, which leaves room for 2), without spilling. */
 template <typename R, bool FUSED, bool P2>
 __global__ void __launch_bounds__(DGX_BLOCK, 3) k_velocity(StepArgs<R> a)
 {
+    __shared__ R wsh[18 * DGX_BLOCK];
     const int q = element_of_thread(a.nper);
-    if (q >= a.tri) return;
+    const int q0 = q - (int)threadIdx.x;
     const int shot = blockIdx.y;
     const long long so = (long long)shot * a.fstride * 6;
     const int step = FUSED ? (*a.step_base + a.step_off) : 0;
-    R *V0 = a.f[0] + so, *V1 = a.f[1] + so;
-    velocity_update<R, FUSED, true, P2>(a, q, shot, step, q, a.f[2] + so, a.f[3] + so, a.f[4] + so, nullptr, nullptr,
-                                    V0, V1, nullptr, false);
+    velocity_update_staged<R, FUSED, P2>(a, q, q0, q < a.tri, shot, step, a.f[2] + so, a.f[3] + so, a.f[4] + so,
+                                         a.f[0] + so, a.f[1] + so, wsh);
 }
 
 template <typename R, bool FUSED, bool P2>
 __global__ void __launch_bounds__(DGX_BLOCK, 3) k_stress(StepArgs<R> a)
 {
+    __shared__ R wsh[12 * DGX_BLOCK];
     const int q = element_of_thread(a.nper);
-    if (q >= a.tri) return;
+    const int q0 = q - (int)threadIdx.x;
     const int shot = blockIdx.y;
     const long long so = (long long)shot * a.fstride * 6;
     const int step = FUSED ? (*a.step_base + a.step_off) : 0;
-    stress_update<R, FUSED, P2>(a, q, shot, step, a.nbr[q], a.f[0] + so, a.f[1] + so, nullptr, a.f[2] + so,
-                            a.f[3] + so, a.f[4] + so);
+    stress_update_staged<R, FUSED, P2>(a, q, q0, q < a.tri, shot, step, a.f[0] + so, a.f[1] + so, a.f[2] + so,
+                                       a.f[3] + so, a.f[4] + so, wsh);
 }
 
 /*
