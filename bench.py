@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU time budget of the reference step-loop sample (the first two timed steps overestimate the per-step time, so the sample runs somewhat shorter)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU time of the reference step-loop sample (steps sized from a short timed run)")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batched FP32 leg (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--serve-requests", type=int, default=24,
@@ -221,8 +222,9 @@ def cpu_baseline(w, setup, budget_s):
             run.run(n)
             return time.perf_counter() - t
         kind = "port"
-    t1 = timed(2)
-    n = max(3, min(2000, int(budget_s / max(t1 / 2, 1e-6))))
+    timed(2)  # warm-up: first touches of the arrays, thread start
+    t1 = timed(6)
+    n = max(3, min(3000, int(budget_s / max(t1 / 6, 1e-6))))
     sec = timed(n)
     rate = lay.tri * 30 * n / sec
     model = "unknown CPU"
