@@ -161,242 +161,75 @@ __device__ __forceinline__ void condition_stress6(const StepArgs<R> &a, unsigned
     }
 }
 
-/*
- * Velocity update of element q (shot `shot`) from stresses T and velocities Vi, written to row `orow`
- * of Vo.  In place (Vi == Vo, orow == q) for the half-step kernel; out of place for the full-step
- * kernel, which also keeps the conditioned stresses of its own elements in LDS (when `save`:
- * tsave[(c*6 + i)*DGX_BLOCK + threadIdx.x], c = xx, zz, xz).
- */
-template <typename R, bool FUSED, bool INPLACE, bool P2>
-__device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int shot, int step, int orow,
-                                                const R *__restrict__ Txx, const R *__restrict__ Tzz,
-                                                const R *__restrict__ Txz, const R *__restrict__ Vxi,
-                                                const R *__restrict__ Vzi, R *__restrict__ Vxo,
-                                                R *__restrict__ Vzo, R *tsave, bool save)
+/* ---- arithmetic shared by the half-step and full-step updates ---- */
+
+/* Own stresses of element q conditioned (sponge, explosive source) as the reference leaves them before
+ * LU_Velocity reads them: per element (one branch each for sponge and sources) in FP32 and with
+ * compiled-in operators; per node in FP64 with argument operators, where the per-element form needs more
+ * registers than the 168 that keep 3 wavefronts per SIMD. */
+template <typename R, bool P2>
+__device__ __forceinline__ void condition_own_stresses(const StepArgs<R> &a, unsigned code, int q, int shot, int step,
+                                                       R (&txx)[6], R (&tzz)[6], R (&txz)[6])
 {
-    /* in place, the velocities are read through the output pointers (Vxi, Vzi unused) */
-    const R *Vxl = INPLACE ? Vxo : Vxi;
-    const R *Vzl = INPLACE ? Vzo : Vzi;
-    const int4 nb = a.nbr[q];
-    const unsigned code = (unsigned)nb.w;
-    const Geom g = geometry(a.pt, a.vtx[q]);
-
-    /* stresses, conditioned (sponge, explosive source) as the reference leaves them before
-     * LU_Velocity reads them */
-    R txx[6], tzz[6], txz[6], vxe[6], vze[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int o = fidx(q, i);
-        txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
-        if (sizeof(R) == 4) { vxe[i] = Vxl[o]; vze[i] = Vzl[o]; } /* FP32: issue every load up front */
-    }
-    if (FUSED) {
-        /* per element (one branch each for sponge and sources) in FP32 and with compiled-in operators;
-         * per node in FP64 with argument operators, where the per-element form needs more registers
-         * than the 168 that keep 3 wavefronts per SIMD */
-        if constexpr (sizeof(R) == 4 || P2) {
-            condition_stress6(a, code, q, shot, step, txx, tzz, txz);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, shot, step, txx[i], tzz[i], txz[i]);
-        }
-    }
-    if (save) {
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            tsave[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = txx[i];
-            tsave[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = tzz[i];
-            tsave[(2 * 6 + i) * DGX_BLOCK + threadIdx.x] = txz[i];
-        }
-    }
-
-    /* volume term: Dr(rx u + rz w) + Ds(sx u + sz w) */
-    const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
-    R rhx[6], rhz[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) { rhx[i] = 0; rhz[i] = 0; }
-#pragma unroll
-    for (int n = 0; n < 6; n++) {
-        const R ux = rx * txx[n] + rz * txz[n], wx = sx * txx[n] + sz * txz[n];
-        const R uz = rx * txz[n] + rz * tzz[n], wz = sx * txz[n] + sz * tzz[n];
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            rhx[i] += op_dr<P2>(a, i * 6 + n) * ux + op_ds<P2>(a, i * 6 + n) * wx;
-            rhz[i] += op_dr<P2>(a, i * 6 + n) * uz + op_ds<P2>(a, i * 6 + n) * wz;
-        }
-    }
-
-    /* surface term, scaled by 1/D at the end */
-    R flx[6], flz[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
-    {
-        const int nbs[3] = {nb.x, nb.y, nb.z};
-        /* all face gathers first (boundary faces read the own element and drop it), so that the 27
-         * loads are in flight together */
-        R gxx[3][3], gzz[3][3], gxz[3][3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int ee = nbs[j] >= 0 ? nbs[j] : q;
-            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int o = fidx(ee, nb_node(m, rev, k));
-                gxx[j][k] = Txx[o]; gzz[j][k] = Tzz[o]; gxz[j][k] = Txz[o];
-            }
-        }
-        if (FUSED && (code & (DGX_CODE_NBSPECIAL(0) | DGX_CODE_NBSPECIAL(1) | DGX_CODE_NBSPECIAL(2)))) {
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                if (!(code & DGX_CODE_NBSPECIAL(j)) || nbs[j] < 0) continue;
-                const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-                const unsigned ncode = (unsigned)a.nbr[nbs[j]].w;
-                for (int k = 0; k < 3; k++)
-                    condition_stress(a, ncode, nbs[j], nb_node(m, rev, k), shot, step, gxx[j][k], gzz[j][k], gxz[j][k]);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const bool has = nbs[j] >= 0;
-            const R ex = (R)g.ex[j], ez = (R)g.ez[j];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int on = face_node(j, k);
-                R dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
-                if (has) { dxx += gxx[j][k]; dzz += gzz[j][k]; dxz += gxz[j][k]; }
-                const R gx = ez * dxx - ex * dxz;
-                const R gz = ez * dxz - ex * dzz;
-#pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    flx[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gx;
-                    flz[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gz;
-                }
-            }
-        }
-    }
-
-    /* V = (sponge(V) + source) + dt/rho * rhs */
-    const R invD = (R)g.invD;
-    const R c = a.rho_inv[q] * (R)a.dt;
-    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
-    if constexpr (sizeof(R) == 4) {
-        /* FP32: velocities loaded up front, conditioned per element */
-        if (FUSED) {
-            if (code & DGX_CODE_SPONGE) {
-#pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    const R f = a.absorb[fidx(q, i)];
-                    vxe[i] *= f;
-                    vze[i] *= f;
-                }
-            }
-            if (ax) {
-                const dgx_aux x = a.aux[ax - 1];
-                for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
-                    const dgx_src_dev &src = a.src[s];
-                    if (src.shot != shot || src.kind != 0) continue;
-                    const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
-#pragma unroll
-                    for (int i = 0; i < 6; i++) vze[i] += (R)(src.fun[i] * w);
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            const int oo = INPLACE ? fidx(q, i) : fidx(orow, i);
-            Vxo[oo] = (rhx[i] + flx[i] * invD) * c + vxe[i];
-            Vzo[oo] = (rhz[i] + flz[i] * invD) * c + vze[i];
-        }
+    if constexpr (sizeof(R) == 4 || P2) {
+        condition_stress6(a, code, q, shot, step, txx, tzz, txz);
     } else {
-        /* FP64: velocities loaded late (registers for occupancy), conditioned per node */
 #pragma unroll
-        for (int i = 0; i < 6; i++) {
-            const int o = fidx(q, i);
-            R vx = Vxl[o], vz = Vzl[o];
-            if (FUSED) {
-                if (code & DGX_CODE_SPONGE) {
-                    const R f = a.absorb[o];
-                    vx *= f;
-                    vz *= f;
-                }
-                if (ax) {
-                    const dgx_aux x = a.aux[ax - 1];
-                    for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
-                        const dgx_src_dev &src = a.src[s];
-                        if (src.shot == shot && src.kind == 0)
-                            vz += (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + step]);
-                    }
-                }
-            }
-            const int oo = INPLACE ? o : fidx(orow, i);
-            Vxo[oo] = (rhx[i] + flx[i] * invD) * c + vx;
-            Vzo[oo] = (rhz[i] + flz[i] * invD) * c + vz;
-        }
+        for (int i = 0; i < 6; i++) condition_stress(a, code, q, i, shot, step, txx[i], tzz[i], txz[i]);
     }
 }
 
-/*
- * Velocity half-step with the workgroup's own stresses staged in LDS.  Each thread loads and conditions
- * its element's stresses and writes them to `wsh` ((c*6 + i)*DGX_BLOCK + thread, c = xx, zz, xz); face
- * neighbours inside the workgroup's 256 elements (about 91 % of interior faces in the locality order) are
- * then read from there after one barrier, already conditioned by their own thread, and only the others
- * are gathered from global memory, issued before the barrier.  Every thread of the workgroup reaches the
- * barrier: threads past the last element load element 0 and return after it.  The values and the
- * arithmetic are those of velocity_update (bit-identical results measured in FP64 and FP32).
- */
-template <typename R, bool FUSED, bool P2>
-__device__ __forceinline__ void velocity_update_staged(const StepArgs<R> &a, int q, int q0, bool valid, int shot,
-                                                       int step, const R *__restrict__ Txx,
-                                                       const R *__restrict__ Tzz, const R *__restrict__ Txz,
-                                                       R *__restrict__ Vx, R *__restrict__ Vz, R *wsh)
+/* Neighbour stresses facing own face j, position k, from global memory.  STAGED: only for the faces whose
+ * element lies outside the workgroup's elements q0 .. q0+DGX_BLOCK-1 (local[j] false; the others are read
+ * from LDS later and their slots start at 0); otherwise for every face (local[j] = false).  A boundary
+ * face reads the own element and its value is dropped, so that all loads are issued together. */
+template <bool STAGED, typename R>
+__device__ __forceinline__ void gather_stresses(unsigned code, const int (&nbs)[3], int q, int q0, bool (&local)[3],
+                                                const R *Txx, const R *Tzz,
+                                                const R *Txz, R (&gxx)[3][3], R (&gzz)[3][3],
+                                                R (&gxz)[3][3])
 {
-    const int qs = valid ? q : 0;
-    const int4 nb = a.nbr[qs];
-    const unsigned code = (unsigned)nb.w;
-    const Geom g = geometry(a.pt, a.vtx[qs]);
-
-    R txx[6], tzz[6], txz[6], vxe[6], vze[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int o = fidx(qs, i);
-        txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
-        if (sizeof(R) == 4) { vxe[i] = Vx[o]; vze[i] = Vz[o]; }
-    }
-    /* face gathers from outside the workgroup, issued before the barrier */
-    const int nbs[3] = {nb.x, nb.y, nb.z};
-    bool inwg[3];
-    R gxx[3][3], gzz[3][3], gxz[3][3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        const int ee = nbs[j] >= 0 ? nbs[j] : qs;
-        inwg[j] = (unsigned)(ee - q0) < (unsigned)DGX_BLOCK;
+        const int ee = nbs[j] >= 0 ? nbs[j] : q;
+        local[j] = STAGED && (unsigned)(ee - q0) < (unsigned)DGX_BLOCK;
         const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             gxx[j][k] = 0; gzz[j][k] = 0; gxz[j][k] = 0;
-            if (!inwg[j]) {
+            if (!local[j]) {
                 const int o = fidx(ee, nb_node(m, rev, k));
                 gxx[j][k] = Txx[o]; gzz[j][k] = Tzz[o]; gxz[j][k] = Txz[o];
             }
         }
     }
-    if (FUSED) {
-        if constexpr (sizeof(R) == 4 || P2) {
-            condition_stress6(a, code, qs, shot, step, txx, tzz, txz);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 6; i++) condition_stress(a, code, qs, i, shot, step, txx[i], tzz[i], txz[i]);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        wsh[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = txx[i];
-        wsh[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = tzz[i];
-        wsh[(2 * 6 + i) * DGX_BLOCK + threadIdx.x] = txz[i];
-    }
+}
 
+/* Sponge and explosive source applied to the gathered stresses of neighbours that carry either (the code
+ * word's NBSPECIAL bits); faces with local[j] true hold values their own thread conditioned already. */
+template <typename R>
+__device__ __forceinline__ void condition_neighbour_stresses(const StepArgs<R> &a, unsigned code, const int (&nbs)[3],
+                                                             const bool (&local)[3], int shot, int step,
+                                                             R (&gxx)[3][3], R (&gzz)[3][3], R (&gxz)[3][3])
+{
+    if (!(code & (DGX_CODE_NBSPECIAL(0) | DGX_CODE_NBSPECIAL(1) | DGX_CODE_NBSPECIAL(2)))) return;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (!(code & DGX_CODE_NBSPECIAL(j)) || nbs[j] < 0 || local[j]) continue;
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+        const unsigned ncode = (unsigned)a.nbr[nbs[j]].w;
+        for (int k = 0; k < 3; k++)
+            condition_stress(a, ncode, nbs[j], nb_node(m, rev, k), shot, step, gxx[j][k], gzz[j][k], gxz[j][k]);
+    }
+}
+
+/* Velocity volume term: rhx = Dr(rx Txx + rz Txz) + Ds(sx Txx + sz Txz), rhz = Dr(rx Txz + rz Tzz) +
+ * Ds(sx Txz + sz Tzz). */
+template <typename R, bool P2>
+__device__ __forceinline__ void velocity_volume(const StepArgs<R> &a, const Geom &g, const R (&txx)[6],
+                                                const R (&tzz)[6], const R (&txz)[6], R (&rhx)[6], R (&rhz)[6])
+{
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
-    R rhx[6], rhz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) { rhx[i] = 0; rhz[i] = 0; }
 #pragma unroll
@@ -409,35 +242,16 @@ __device__ __forceinline__ void velocity_update_staged(const StepArgs<R> &a, int
             rhz[i] += op_dr<P2>(a, i * 6 + n) * uz + op_ds<P2>(a, i * 6 + n) * wz;
         }
     }
+}
 
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-        if (!inwg[j]) continue;
-        const int ee = nbs[j] >= 0 ? nbs[j] : qs;
-        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-        const int r = ee - q0;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int nn = nb_node(m, rev, k);
-            gxx[j][k] = wsh[(0 * 6 + nn) * DGX_BLOCK + r];
-            gzz[j][k] = wsh[(1 * 6 + nn) * DGX_BLOCK + r];
-            gxz[j][k] = wsh[(2 * 6 + nn) * DGX_BLOCK + r];
-        }
-    }
-    if (!valid) return;
-    if (FUSED && (code & (DGX_CODE_NBSPECIAL(0) | DGX_CODE_NBSPECIAL(1) | DGX_CODE_NBSPECIAL(2)))) {
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            /* neighbours staged in LDS were conditioned by their own thread */
-            if (!(code & DGX_CODE_NBSPECIAL(j)) || nbs[j] < 0 || inwg[j]) continue;
-            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-            const unsigned ncode = (unsigned)a.nbr[nbs[j]].w;
-            for (int k = 0; k < 3; k++)
-                condition_stress(a, ncode, nbs[j], nb_node(m, rev, k), shot, step, gxx[j][k], gzz[j][k], gxz[j][k]);
-        }
-    }
-    R flx[6], flz[6];
+/* Velocity surface term before its 1/D scale: sum_j S_j (ez_j dTxx_j - ex_j dTxz_j) and
+ * sum_j S_j (ez_j dTxz_j - ex_j dTzz_j), the jump dT = T_nb - T_own (T_nb = 0 on the outer boundary). */
+template <typename R, bool P2>
+__device__ __forceinline__ void velocity_flux(const StepArgs<R> &a, const Geom &g, const int (&nbs)[3],
+                                              const R (&txx)[6], const R (&tzz)[6], const R (&txz)[6],
+                                              const R (&gxx)[3][3], const R (&gzz)[3][3], const R (&gxz)[3][3],
+                                              R (&flx)[6], R (&flz)[6])
+{
 #pragma unroll
     for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
 #pragma unroll
@@ -458,232 +272,86 @@ __device__ __forceinline__ void velocity_update_staged(const StepArgs<R> &a, int
             }
         }
     }
+}
 
-    const R invD = (R)g.invD;
-    const R c = a.rho_inv[q] * (R)a.dt;
+/* Sponge and force source applied to the 6 velocity nodes of element q (FP32 path: one branch each per
+ * element). */
+template <typename R>
+__device__ __forceinline__ void condition_velocity6(const StepArgs<R> &a, unsigned code, int q, int shot, int step,
+                                                    R (&vx)[6], R (&vz)[6])
+{
+    if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const R f = a.absorb[fidx(q, i)];
+            vx[i] *= f;
+            vz[i] *= f;
+        }
+    }
     const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
-    if constexpr (sizeof(R) == 4) {
-        if (FUSED) {
-            if (code & DGX_CODE_SPONGE) {
+    if (ax) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+            const dgx_src_dev &src = a.src[s];
+            if (src.shot != shot || src.kind != 0) continue;
+            const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
 #pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    const R f = a.absorb[fidx(q, i)];
-                    vxe[i] *= f;
-                    vze[i] *= f;
-                }
-            }
-            if (ax) {
-                const dgx_aux x = a.aux[ax - 1];
-                for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
-                    const dgx_src_dev &src = a.src[s];
-                    if (src.shot != shot || src.kind != 0) continue;
-                    const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
-#pragma unroll
-                    for (int i = 0; i < 6; i++) vze[i] += (R)(src.fun[i] * w);
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            const int oo = fidx(q, i);
-            Vx[oo] = (rhx[i] + flx[i] * invD) * c + vxe[i];
-            Vz[oo] = (rhz[i] + flz[i] * invD) * c + vze[i];
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-            const int o = fidx(q, i);
-            R vx = Vx[o], vz = Vz[o];
-            if (FUSED) {
-                if (code & DGX_CODE_SPONGE) {
-                    const R f = a.absorb[o];
-                    vx *= f;
-                    vz *= f;
-                }
-                if (ax) {
-                    const dgx_aux x = a.aux[ax - 1];
-                    for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
-                        const dgx_src_dev &src = a.src[s];
-                        if (src.shot == shot && src.kind == 0)
-                            vz += (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + step]);
-                    }
-                }
-            }
-            Vx[o] = (rhx[i] + flx[i] * invD) * c + vx;
-            Vz[o] = (rhz[i] + flz[i] * invD) * c + vz;
+            for (int i = 0; i < 6; i++) vz[i] += (R)(src.fun[i] * w);
         }
     }
 }
 
-/*
- * Stress update of element q from the new velocities V (own values at row q, face neighbour j's at row
- * nrow.j; nrow.w is the element's code word).  In place, own stresses come from Txx, Tzz, Txz and are
- * conditioned here; in the full-step kernel (tload != nullptr) they come conditioned from LDS.  Records the
- * receivers of q from the velocities it holds.
- */
-template <typename R, bool FUSED, bool P2>
-__device__ __forceinline__ void stress_update(const StepArgs<R> &a, int q, int shot, int step, int4 nrow,
-                                              const R *__restrict__ Vx, const R *__restrict__ Vz, const R *tload,
-                                              R *__restrict__ Txx, R *__restrict__ Tzz, R *__restrict__ Txz)
+/* Sponge and force source applied to velocity node i of element q, at offset o = fidx(q, i) (FP64 path,
+ * per node). */
+template <typename R>
+__device__ __forceinline__ void condition_velocity(const StepArgs<R> &a, unsigned code, int o, int i, int shot,
+                                                   int step, R &vx, R &vz)
 {
-    const unsigned code = (unsigned)nrow.w;
-    const Geom g = geometry(a.pt, a.vtx[q]);
-
-    R vx[6], vz[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int o = fidx(q, i);
-        vx[i] = Vx[o]; vz[i] = Vz[o];
+    if (code & DGX_CODE_SPONGE) {
+        const R f = a.absorb[o];
+        vx *= f;
+        vz *= f;
     }
-    const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
-    R exx[6], ezz[6], exz[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        R rvx = 0, svx = 0, rvz = 0, svz = 0;
-#pragma unroll
-        for (int n = 0; n < 6; n++) {
-            rvx += a.op.Dr[i * 6 + n] * vx[n];
-            svx += a.op.Ds[i * 6 + n] * vx[n];
-            rvz += a.op.Dr[i * 6 + n] * vz[n];
-            svz += a.op.Ds[i * 6 + n] * vz[n];
-        }
-        exx[i] = rx * rvx + sx * svx;
-        ezz[i] = rz * rvz + sz * svz;
-        exz[i] = rx * rvz + sx * svz + rz * rvx + sz * svx;
-    }
-
-    R fx[6], fz[6], fxz[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
-    {
-        const int nbs[3] = {nrow.x, nrow.y, nrow.z};
-        R gvx[3][3], gvz[3][3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int ee = nbs[j] >= 0 ? nbs[j] : q;
-            const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int o = fidx(ee, nb_node(m, rev, k));
-                gvx[j][k] = Vx[o]; gvz[j][k] = Vz[o];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const bool has = nbs[j] >= 0;
-            const R ex = (R)g.ex[j], ez = (R)g.ez[j];
-            R px[3], pz[3], pxz[3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int on = face_node(j, k);
-                R dvx = -vx[on], dvz = -vz[on];
-                if (has) { dvx += gvx[j][k]; dvz += gvz[j][k]; }
-                px[k] = ez * dvx;
-                pz[k] = -ex * dvz;
-                pxz[k] = ez * dvz - ex * dvx;
-            }
-#pragma unroll
-            for (int i = 0; i < 6; i++) {
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const R l = op_l<P2>(a, j * 18 + i * 3 + k);
-                    fx[i] += l * px[k];
-                    fz[i] += l * pz[k];
-                    fxz[i] += l * pxz[k];
-                }
-            }
-        }
-    }
-
-    const R invD = (R)g.invD;
-    const R lam = a.lam[q], mu = a.mu[q], l2m = lam + 2 * mu, dt = (R)a.dt;
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int o = fidx(q, i);
-        R txx, tzz, txz;
-        if (tload) {
-            txx = tload[(0 * 6 + i) * DGX_BLOCK + threadIdx.x];
-            tzz = tload[(1 * 6 + i) * DGX_BLOCK + threadIdx.x];
-            txz = tload[(2 * 6 + i) * DGX_BLOCK + threadIdx.x];
-        } else {
-            txx = Txx[o]; tzz = Tzz[o]; txz = Txz[o];
-            if (FUSED) condition_stress(a, code, q, i, shot, step, txx, tzz, txz);
-        }
-        const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
-        Txx[o] = (l2m * axx + lam * azz) * dt + txx;
-        Tzz[o] = (lam * axx + l2m * azz) * dt + tzz;
-        Txz[o] = (mu * axz) * dt + txz;
-    }
-
-    if (FUSED) {
-        const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
-        if (ax && step < a.nsteps) {
-            const dgx_aux x = a.aux[ax - 1];
-            for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
-                const dgx_rec_dev &rc = a.rec[r];
-                double sx_ = 0, sz_ = 0;
-#pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    sx_ += rc.w[i] * (double)vx[i];
-                    sz_ += rc.w[i] * (double)vz[i];
-                }
-                double *t = a.trace + ((long long)shot * a.nsteps + step) * 2 * a.nrec;
-                t[rc.index] = sx_;
-                t[a.nrec + rc.index] = sz_;
-            }
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (ax) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+            const dgx_src_dev &src = a.src[s];
+            if (src.shot == shot && src.kind == 0)
+                vz += (R)(src.fun[i] * a.wav[(long long)src.wavelet * a.nsteps + step]);
         }
     }
 }
 
-/*
- * Stress half-step with the workgroup's new velocities staged in LDS (`wsh`, (c*6 + i)*DGX_BLOCK + thread,
- * c = x, z), as velocity_update_staged does for the stresses: face neighbours inside the workgroup are read
- * from LDS after one barrier, the others are gathered from global memory before it.  The values and the
- * arithmetic are those of stress_update (in place); results agree with it to rounding (in FP64 the compiler
- * contracts a few products into FMAs differently: measured differences are at 1e-82 absolute on traces of
- * order 1e-8).
- */
-template <typename R, bool FUSED, bool P2>
-__device__ __forceinline__ void stress_update_staged(const StepArgs<R> &a, int q, int q0, bool valid, int shot,
-                                                     int step, const R *__restrict__ Vx, const R *__restrict__ Vz,
-                                                     R *__restrict__ Txx, R *__restrict__ Tzz, R *__restrict__ Txz,
-                                                     R *wsh)
+/* Neighbour velocities facing own face j, position k, from global memory (as gather_stresses). */
+template <bool STAGED, typename R>
+__device__ __forceinline__ void gather_velocities(unsigned code, const int (&nbs)[3], int q, int q0, bool (&local)[3],
+                                                  const R *Vx, const R *Vz,
+                                                  R (&gvx)[3][3], R (&gvz)[3][3])
 {
-    const int qs = valid ? q : 0;
-    const int4 nb = a.nbr[qs];
-    const unsigned code = (unsigned)nb.w;
-    const Geom g = geometry(a.pt, a.vtx[qs]);
-
-    R vx[6], vz[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int o = fidx(qs, i);
-        vx[i] = Vx[o]; vz[i] = Vz[o];
-    }
-    const int nbs[3] = {nb.x, nb.y, nb.z};
-    bool inwg[3];
-    R gvx[3][3], gvz[3][3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        const int ee = nbs[j] >= 0 ? nbs[j] : qs;
-        inwg[j] = (unsigned)(ee - q0) < (unsigned)DGX_BLOCK;
+        const int ee = nbs[j] >= 0 ? nbs[j] : q;
+        local[j] = STAGED && (unsigned)(ee - q0) < (unsigned)DGX_BLOCK;
         const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             gvx[j][k] = 0; gvz[j][k] = 0;
-            if (!inwg[j]) {
+            if (!local[j]) {
                 const int o = fidx(ee, nb_node(m, rev, k));
                 gvx[j][k] = Vx[o]; gvz[j][k] = Vz[o];
             }
         }
     }
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        wsh[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = vx[i];
-        wsh[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = vz[i];
-    }
+}
+
+/* Strain-rate volume terms: exx = Dx Vx, ezz = Dz Vz, exz = Dx Vz + Dz Vx with Dx = rx Dr + sx Ds and
+ * Dz = rz Dr + sz Ds (operators from the kernel argument in both variants). */
+template <typename R>
+__device__ __forceinline__ void strain_volume(const StepArgs<R> &a, const Geom &g, const R (&vx)[6],
+                                              const R (&vz)[6], R (&exx)[6], R (&ezz)[6], R (&exz)[6])
+{
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
-    R exx[6], ezz[6], exz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         R rvx = 0, svx = 0, rvz = 0, svz = 0;
@@ -698,24 +366,15 @@ __device__ __forceinline__ void stress_update_staged(const StepArgs<R> &a, int q
         ezz[i] = rz * rvz + sz * svz;
         exz[i] = rx * rvz + sx * svz + rz * rvx + sz * svx;
     }
+}
 
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-        if (!inwg[j]) continue;
-        const int ee = nbs[j] >= 0 ? nbs[j] : qs;
-        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
-        const int r = ee - q0;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int nn = nb_node(m, rev, k);
-            gvx[j][k] = wsh[(0 * 6 + nn) * DGX_BLOCK + r];
-            gvz[j][k] = wsh[(1 * 6 + nn) * DGX_BLOCK + r];
-        }
-    }
-    if (!valid) return;
-
-    R fx[6], fz[6], fxz[6];
+/* Strain-rate surface terms before their 1/D scale: sum_j S_j(ez_j dVx_j), sum_j S_j(-ex_j dVz_j) and
+ * sum_j S_j(ez_j dVz_j - ex_j dVx_j). */
+template <typename R, bool P2>
+__device__ __forceinline__ void stress_flux(const StepArgs<R> &a, const Geom &g, const int (&nbs)[3],
+                                            const R (&vx)[6], const R (&vz)[6], const R (&gvx)[3][3],
+                                            const R (&gvz)[3][3], R (&fx)[6], R (&fz)[6], R (&fxz)[6])
+{
 #pragma unroll
     for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
 #pragma unroll
@@ -743,38 +402,293 @@ __device__ __forceinline__ void stress_update_staged(const StepArgs<R> &a, int q
             }
         }
     }
+}
 
+/* Txx += dt[(l+2m) exx + l ezz], Tzz += dt[l exx + (l+2m) ezz], Txz += dt m exz for the 6 nodes of element
+ * q (strain rates = volume + surface/D).  The old stresses come conditioned from `tload` (LDS,
+ * (c*6 + i)*DGX_BLOCK + thread) when it is given, else from T, conditioned here when FUSED. */
+template <typename R, bool FUSED>
+__device__ __forceinline__ void stress_store(const StepArgs<R> &a, unsigned code, int q, int shot, int step,
+                                             const Geom &g, const R (&exx)[6], const R (&ezz)[6],
+                                             const R (&exz)[6], const R (&fx)[6], const R (&fz)[6],
+                                             const R (&fxz)[6], const R *tload, R *Txx,
+                                             R *Tzz, R *Txz)
+{
     const R invD = (R)g.invD;
     const R lam = a.lam[q], mu = a.mu[q], l2m = lam + 2 * mu, dt = (R)a.dt;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const int o = fidx(q, i);
-        R txx = Txx[o], tzz = Tzz[o], txz = Txz[o];
-        if (FUSED) condition_stress(a, code, q, i, shot, step, txx, tzz, txz);
+        R txx, tzz, txz;
+        if (tload) {
+            txx = tload[(0 * 6 + i) * DGX_BLOCK + threadIdx.x];
+            tzz = tload[(1 * 6 + i) * DGX_BLOCK + threadIdx.x];
+            txz = tload[(2 * 6 + i) * DGX_BLOCK + threadIdx.x];
+        } else {
+            txx = Txx[o]; tzz = Tzz[o]; txz = Txz[o];
+            if (FUSED) condition_stress(a, code, q, i, shot, step, txx, tzz, txz);
+        }
         const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
         Txx[o] = (l2m * axx + lam * azz) * dt + txx;
         Tzz[o] = (lam * axx + l2m * azz) * dt + tzz;
         Txz[o] = (mu * axz) * dt + txz;
     }
+}
 
-    if (FUSED) {
-        const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
-        if (ax && step < a.nsteps) {
-            const dgx_aux x = a.aux[ax - 1];
-            for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
-                const dgx_rec_dev &rc = a.rec[r];
-                double sx_ = 0, sz_ = 0;
+/* Receivers of the element (OpenMP_Function.c:61-72 with per-node weights) from the new velocities it holds:
+ * trace slot `step` of shot `shot`. */
+template <typename R>
+__device__ __forceinline__ void record_receivers(const StepArgs<R> &a, unsigned code, int shot, int step,
+                                                 const R (&vx)[6], const R (&vz)[6])
+{
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (!ax || step >= a.nsteps) return;
+    const dgx_aux x = a.aux[ax - 1];
+    for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
+        const dgx_rec_dev &rc = a.rec[r];
+        double sx_ = 0, sz_ = 0;
 #pragma unroll
-                for (int i = 0; i < 6; i++) {
-                    sx_ += rc.w[i] * (double)vx[i];
-                    sz_ += rc.w[i] * (double)vz[i];
-                }
-                double *t = a.trace + ((long long)shot * a.nsteps + step) * 2 * a.nrec;
-                t[rc.index] = sx_;
-                t[a.nrec + rc.index] = sz_;
-            }
+        for (int i = 0; i < 6; i++) {
+            sx_ += rc.w[i] * (double)vx[i];
+            sz_ += rc.w[i] * (double)vz[i];
+        }
+        double *t = a.trace + ((long long)shot * a.nsteps + step) * 2 * a.nrec;
+        t[rc.index] = sx_;
+        t[a.nrec + rc.index] = sz_;
+    }
+}
+
+/* ---- the updates ---- */
+
+/*
+ * Velocity update of element q (full-step kernel, fused): stresses T and velocities Vi in, the new velocities
+ * written to row `orow` of Vo; when `save`, the conditioned stresses of the element are also kept in LDS
+ * (tsave[(c*6 + i)*DGX_BLOCK + threadIdx.x], c = xx, zz, xz) for the stress phase.
+ */
+template <typename R, bool P2>
+__device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int shot, int step, int orow,
+                                                const R *__restrict__ Txx, const R *__restrict__ Tzz,
+                                                const R *__restrict__ Txz, const R *__restrict__ Vxi,
+                                                const R *__restrict__ Vzi, R *__restrict__ Vxo,
+                                                R *__restrict__ Vzo, R *tsave, bool save)
+{
+    const int4 nb = a.nbr[q];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[q]);
+
+    R txx[6], tzz[6], txz[6], vxe[6], vze[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
+        if (sizeof(R) == 4) { vxe[i] = Vxi[o]; vze[i] = Vzi[o]; } /* FP32: issue every load up front */
+    }
+    condition_own_stresses<R, P2>(a, code, q, shot, step, txx, tzz, txz);
+    if (save) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            tsave[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = txx[i];
+            tsave[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = tzz[i];
+            tsave[(2 * 6 + i) * DGX_BLOCK + threadIdx.x] = txz[i];
         }
     }
+    R rhx[6], rhz[6];
+    velocity_volume<R, P2>(a, g, txx, tzz, txz, rhx, rhz);
+
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+    bool global[3];
+    R gxx[3][3], gzz[3][3], gxz[3][3];
+    gather_stresses<false>(code, nbs, q, 0, global, Txx, Tzz, Txz, gxx, gzz, gxz);
+    condition_neighbour_stresses(a, code, nbs, global, shot, step, gxx, gzz, gxz);
+    R flx[6], flz[6];
+    velocity_flux<R, P2>(a, g, nbs, txx, tzz, txz, gxx, gzz, gxz, flx, flz);
+    /* V = (sponge(V) + force source) + dt/rho * (volume + surface/D), read at row q of Vi, written at row orow
+     * of Vo.  FP32: velocities loaded up front, conditioned per element; FP64: loaded late (registers for
+     * occupancy), conditioned per node. */
+    const R invD = (R)g.invD;
+    const R c = a.rho_inv[q] * (R)a.dt;
+    if constexpr (sizeof(R) == 4) condition_velocity6(a, code, q, shot, step, vxe, vze);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        R vx, vz;
+        if constexpr (sizeof(R) == 4) {
+            vx = vxe[i]; vz = vze[i];
+        } else {
+            const int o = fidx(q, i);
+            vx = Vxi[o]; vz = Vzi[o];
+            condition_velocity(a, code, o, i, shot, step, vx, vz);
+        }
+        const int oo = fidx(orow, i);
+        Vxo[oo] = (rhx[i] + flx[i] * invD) * c + vx;
+        Vzo[oo] = (rhz[i] + flz[i] * invD) * c + vz;
+    }
+}
+
+/*
+ * Velocity half-step with the workgroup's own stresses staged in LDS.  Each thread loads and conditions
+ * its element's stresses and writes them to `wsh` ((c*6 + i)*DGX_BLOCK + thread, c = xx, zz, xz); face
+ * neighbours inside the workgroup's 256 elements (about 91 % of interior faces in the locality order) are
+ * then read from there after one barrier, already conditioned by their own thread, and only the others
+ * are gathered from global memory, issued before the barrier.  Every thread of the workgroup reaches the
+ * barrier: threads past the last element load element 0 and return after it.  In place: the velocities
+ * are read and written at row q of Vx, Vz.
+ */
+template <typename R, bool FUSED, bool P2>
+__device__ __forceinline__ void velocity_update_staged(const StepArgs<R> &a, int q, int q0, bool valid, int shot,
+                                                       int step, const R *__restrict__ Txx,
+                                                       const R *__restrict__ Tzz, const R *__restrict__ Txz,
+                                                       R *__restrict__ Vx, R *__restrict__ Vz, R *wsh)
+{
+    const int qs = valid ? q : 0;
+    const int4 nb = a.nbr[qs];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[qs]);
+
+    R txx[6], tzz[6], txz[6], vxe[6], vze[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(qs, i);
+        txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
+        if (sizeof(R) == 4) { vxe[i] = Vx[o]; vze[i] = Vz[o]; }
+    }
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+    bool inwg[3];
+    R gxx[3][3], gzz[3][3], gxz[3][3];
+    gather_stresses<true>(code, nbs, qs, q0, inwg, Txx, Tzz, Txz, gxx, gzz, gxz);
+    if (FUSED) condition_own_stresses<R, P2>(a, code, qs, shot, step, txx, tzz, txz);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        wsh[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = txx[i];
+        wsh[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = tzz[i];
+        wsh[(2 * 6 + i) * DGX_BLOCK + threadIdx.x] = txz[i];
+    }
+    R rhx[6], rhz[6];
+    velocity_volume<R, P2>(a, g, txx, tzz, txz, rhx, rhz);
+
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (!inwg[j]) continue;
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+        const int r = (nbs[j] >= 0 ? nbs[j] : qs) - q0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int nn = nb_node(m, rev, k);
+            gxx[j][k] = wsh[(0 * 6 + nn) * DGX_BLOCK + r];
+            gzz[j][k] = wsh[(1 * 6 + nn) * DGX_BLOCK + r];
+            gxz[j][k] = wsh[(2 * 6 + nn) * DGX_BLOCK + r];
+        }
+    }
+    if (!valid) return;
+    if (FUSED) condition_neighbour_stresses(a, code, nbs, inwg, shot, step, gxx, gzz, gxz);
+    R flx[6], flz[6];
+    velocity_flux<R, P2>(a, g, nbs, txx, tzz, txz, gxx, gzz, gxz, flx, flz);
+    /* V = (sponge(V) + force source) + dt/rho * (volume + surface/D), in place; loads and conditioning as in
+     * velocity_update */
+    const R invD = (R)g.invD;
+    const R c = a.rho_inv[q] * (R)a.dt;
+    if constexpr (sizeof(R) == 4) {
+        if (FUSED) condition_velocity6(a, code, q, shot, step, vxe, vze);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        R vx, vz;
+        if constexpr (sizeof(R) == 4) {
+            vx = vxe[i]; vz = vze[i];
+        } else {
+            vx = Vx[o]; vz = Vz[o];
+            if (FUSED) condition_velocity(a, code, o, i, shot, step, vx, vz);
+        }
+        Vx[o] = (rhx[i] + flx[i] * invD) * c + vx;
+        Vz[o] = (rhz[i] + flz[i] * invD) * c + vz;
+    }
+}
+
+/*
+ * Stress update of element q in the full-step kernel: new velocities V (own values at row q, face neighbour
+ * j's at row nrow.j; nrow.w is the element's code word), the element's conditioned stresses from LDS
+ * (tload), the new stresses written to row q of T.  Records the receivers of q from the velocities it holds.
+ */
+template <typename R>
+__device__ __forceinline__ void stress_update(const StepArgs<R> &a, int q, int shot, int step, int4 nrow,
+                                              const R *__restrict__ Vx, const R *__restrict__ Vz, const R *tload,
+                                              R *__restrict__ Txx, R *__restrict__ Tzz, R *__restrict__ Txz)
+{
+    const unsigned code = (unsigned)nrow.w;
+    const Geom g = geometry(a.pt, a.vtx[q]);
+
+    R vx[6], vz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(q, i);
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    R exx[6], ezz[6], exz[6];
+    strain_volume(a, g, vx, vz, exx, ezz, exz);
+
+    const int nbs[3] = {nrow.x, nrow.y, nrow.z};
+    bool global[3];
+    R gvx[3][3], gvz[3][3];
+    gather_velocities<false>(code, nbs, q, 0, global, Vx, Vz, gvx, gvz);
+    R fx[6], fz[6], fxz[6];
+    stress_flux<R, false>(a, g, nbs, vx, vz, gvx, gvz, fx, fz, fxz);
+    stress_store<R, true>(a, code, q, shot, step, g, exx, ezz, exz, fx, fz, fxz, tload, Txx, Tzz, Txz);
+    record_receivers(a, code, shot, step, vx, vz);
+}
+
+/*
+ * Stress half-step with the workgroup's new velocities staged in LDS (`wsh`, (c*6 + i)*DGX_BLOCK + thread,
+ * c = x, z), as velocity_update_staged does for the stresses: face neighbours inside the workgroup are read
+ * from LDS after one barrier, the others are gathered from global memory before it.  In place on T.
+ */
+template <typename R, bool FUSED, bool P2>
+__device__ __forceinline__ void stress_update_staged(const StepArgs<R> &a, int q, int q0, bool valid, int shot,
+                                                     int step, const R *__restrict__ Vx, const R *__restrict__ Vz,
+                                                     R *__restrict__ Txx, R *__restrict__ Tzz, R *__restrict__ Txz,
+                                                     R *wsh)
+{
+    const int qs = valid ? q : 0;
+    const int4 nb = a.nbr[qs];
+    const unsigned code = (unsigned)nb.w;
+    const Geom g = geometry(a.pt, a.vtx[qs]);
+
+    R vx[6], vz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int o = fidx(qs, i);
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+    bool inwg[3];
+    R gvx[3][3], gvz[3][3];
+    gather_velocities<true>(code, nbs, qs, q0, inwg, Vx, Vz, gvx, gvz);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        wsh[(0 * 6 + i) * DGX_BLOCK + threadIdx.x] = vx[i];
+        wsh[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = vz[i];
+    }
+    R exx[6], ezz[6], exz[6];
+    strain_volume(a, g, vx, vz, exx, ezz, exz);
+
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (!inwg[j]) continue;
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+        const int r = (nbs[j] >= 0 ? nbs[j] : qs) - q0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int nn = nb_node(m, rev, k);
+            gvx[j][k] = wsh[(0 * 6 + nn) * DGX_BLOCK + r];
+            gvz[j][k] = wsh[(1 * 6 + nn) * DGX_BLOCK + r];
+        }
+    }
+    if (!valid) return;
+    R fx[6], fz[6], fxz[6];
+    stress_flux<R, P2>(a, g, nbs, vx, vz, gvx, gvz, fx, fz, fxz);
+    stress_store<R, FUSED>(a, code, q, shot, step, g, exx, ezz, exz, fx, fz, fxz, nullptr, Txx, Tzz, Txz);
+    if (FUSED) record_receivers(a, code, shot, step, vx, vz);
 }
 
 /* The element kernels ask for at least 3 wavefronts per SIMD: this caps the FP64 kernels at 168 VGPRs
@@ -845,12 +759,12 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
         orow = a.vrow0 + k;
     }
     if (q >= 0)
-        velocity_update<R, true, false, P2>(a, q, shot, step, orow, a.f[2] + so, a.f[3] + so, a.f[4] + so, a.f[0] + so,
+        velocity_update<R, P2>(a, q, shot, step, orow, a.f[2] + so, a.f[3] + so, a.f[4] + so, a.f[0] + so,
                                  a.f[1] + so, a.g[0] + so, a.g[1] + so, tc, lane < td.y);
     __syncthreads();
     /* argument operators in the stress phase: with the lifts compiled in, the FP64 kernel spills */
     if (lane < td.y)
-        stress_update<R, true, false>(a, q, shot, step, a.nb2[q], a.g[0] + so, a.g[1] + so, tc, a.g[2] + so,
+        stress_update<R>(a, q, shot, step, a.nb2[q], a.g[0] + so, a.g[1] + so, tc, a.g[2] + so,
                                a.g[3] + so, a.g[4] + so);
 }
 
