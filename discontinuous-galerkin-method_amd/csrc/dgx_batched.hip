@@ -80,10 +80,15 @@ __device__ __forceinline__ bool my_work(const StepArgs<R> &a, int &q, int &s)
     return q < a.tri;
 }
 
-template <typename R>
+/* Offset of value (element q, node i, shot s).  Rows of spad shots; with L > 0 each node row interleaves 2^L
+ * consecutive elements, ((q >> L)*6 + i)*2^L + (q mod 2^L), so that the 64 lanes of a wavefront that holds
+ * 2^L elements x spad shots (spad = 64 >> L) read and write one contiguous 256-byte (FP32) span per node.
+ * L = 0 (one element per row) for the batched solver; the serving groups use L = log2(64/G). */
+template <int L, typename R>
 __device__ __forceinline__ long long bidx(const StepArgs<R> &a, int q, int i, int s)
 {
-    return ((long long)q * 6 + i) * a.spad + s;
+    if constexpr (L == 0) return ((long long)q * 6 + i) * a.spad + s;
+    else return ((((long long)(q >> L) * 6 + i) << L) + (q & ((1 << L) - 1))) * a.spad + s;
 }
 
 /* local step of shot s at the launch's global step; -1 when the shot is not running */
@@ -158,7 +163,7 @@ __device__ __forceinline__ void condition_stress6(const StepArgs<R> &a, unsigned
     }
 }
 
-template <typename R>
+template <typename R, int L = 0>
 __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int lstep)
 {
     R *__restrict__ Vx = a.f[0];
@@ -177,7 +182,7 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
     R txx[6], tzz[6], txz[6], vxe[6], vze[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const long long o = bidx(a, q, i, s);
+        const long long o = bidx<L>(a, q, i, s);
         txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
         if (sizeof(R) == 4) { vxe[i] = Vx[o]; vze[i] = Vz[o]; }
     }
@@ -220,7 +225,7 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
             const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const long long o = bidx(a, ee, nb_node(m, rev, k), s);
+                const long long o = bidx<L>(a, ee, nb_node(m, rev, k), s);
                 gxx[j][k] = Txx[o]; gzz[j][k] = Tzz[o]; gxz[j][k] = Txz[o];
             }
         }
@@ -284,7 +289,7 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
         }
 #pragma unroll
         for (int i = 0; i < 6; i++) {
-            const long long o = bidx(a, q, i, s);
+            const long long o = bidx<L>(a, q, i, s);
             Vx[o] = (rhx[i] + flx[i] * invD) * c + vxe[i];
             Vz[o] = (rhz[i] + flz[i] * invD) * c + vze[i];
         }
@@ -292,7 +297,7 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
     }
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const long long o = bidx(a, q, i, s);
+        const long long o = bidx<L>(a, q, i, s);
         R vx, vz;
         if (sizeof(R) == 4) { vx = vxe[i]; vz = vze[i]; }
         else { vx = Vx[o]; vz = Vz[o]; }
@@ -313,7 +318,7 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
     }
 }
 
-template <typename R>
+template <typename R, int L = 0>
 __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> &a, int q, int s, int lstep)
 {
     const R *__restrict__ Vx = a.f[0];
@@ -329,7 +334,7 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
     R vx[6], vz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const long long o = bidx(a, q, i, s);
+        const long long o = bidx<L>(a, q, i, s);
         vx[i] = Vx[o]; vz[i] = Vz[o];
     }
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
@@ -360,7 +365,7 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
             const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const long long o = bidx(a, ee, nb_node(m, rev, k), s);
+                const long long o = bidx<L>(a, ee, nb_node(m, rev, k), s);
                 gvx[j][k] = Vx[o]; gvz[j][k] = Vz[o];
             }
         }
@@ -397,13 +402,13 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
         R txx[6], tzz[6], txz[6];
 #pragma unroll
         for (int i = 0; i < 6; i++) {
-            const long long o = bidx(a, q, i, s);
+            const long long o = bidx<L>(a, q, i, s);
             txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
         }
         condition_stress6(a, code, q, s, lstep, txx, tzz, txz);
 #pragma unroll
         for (int i = 0; i < 6; i++) {
-            const long long o = bidx(a, q, i, s);
+            const long long o = bidx<L>(a, q, i, s);
             const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
             Txx[o] = (l2m * axx + lam * azz) * dt + txx[i];
             Tzz[o] = (lam * axx + l2m * azz) * dt + tzz[i];
@@ -412,7 +417,7 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
     } else
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const long long o = bidx(a, q, i, s);
+        const long long o = bidx<L>(a, q, i, s);
         R txx = Txx[o], tzz = Tzz[o], txz = Txz[o];
         condition_stress(a, code, q, i, s, lstep, txx, tzz, txz);
         const R axx = exx[i] + fx[i] * invD, azz = ezz[i] + fz[i] * invD, axz = exz[i] + fxz[i] * invD;
@@ -488,7 +493,8 @@ __device__ __forceinline__ StepArgs<R> global_args(const StepArgs<R> &t)
 /*
  * Serving: one launch per half-step over every active (mesh, slot group) batch.  sched = {nbatch,
  * total tiles, tile offset of batch 0, ..., tile offset of batch nbatch}; a tile is 256/G elements x G
- * shots (G = slots per group, 16, 32 or 64: 64/G elements per wavefront).  One workgroup per tile, and the
+ * shots (G = slots per group, 8, 16, 32 or 64: 64/G elements per wavefront, stored interleaved in the
+ * group's node rows so that each node load and store of a wavefront is one contiguous span; bidx).  One workgroup per tile, and the
  * tiles are dealt to the 8 XCDs in contiguous ranges swept in order, as in the batched kernels: face
  * neighbours are stepped close together in time on one XCD and their rows are still in its L2.  (A
  * persistent grid-stride loop over the same tiles let the workgroups drift apart over their ~200 tiles
@@ -508,7 +514,9 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const Step
                                                         const int *step_base, int step_off, int nb, int total,
                                                         int nper)
 {
-    constexpr int EW = 64 / G; /* elements per wavefront */
+    constexpr int EW = 64 / G; /* elements per wavefront, interleaved in one node row of the group */
+    constexpr int LG = G == 64 ? 0 : G == 32 ? 1 : G == 16 ? 2 : 3;
+    static_assert((1 << LG) == EW, "slot groups of 8, 16, 32 or 64 shots");
     const int tile = (int)(blockIdx.x & (DGX_NXCD - 1)) * nper + (int)(blockIdx.x >> 3);
     if (tile >= total) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -531,8 +539,8 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const Step
     if (G == 64 && VEL) q = __builtin_amdgcn_readfirstlane(q); /* one element per wavefront */
     if (q >= a.tri) return;
     const int t = *step_base + step_off - a.shot_start[s];
-    if (VEL) velocity_body(op, a, q, s, t);
-    else stress_body(op, a, q, s, t);
+    if (VEL) velocity_body<R, LG>(op, a, q, s, t);
+    else stress_body<R, LG>(op, a, q, s, t);
 }
 
 /* dst[pairs[k].x] = pairs[k].y */
@@ -640,7 +648,10 @@ static hipError_t launch_serve(const void *ops, const void *table, const int *sc
     if (total <= 0) return hipSuccess;
     const int nper = (total + DGX_NXCD - 1) / DGX_NXCD; /* tiles per XCD */
     const int grid = DGX_NXCD * nper;
-    if (group == 16) {
+    if (group == 8) {
+        if (vel) dgxb::ks_step<R, true, 8><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
+        else dgxb::ks_step<R, false, 8><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
+    } else if (group == 16) {
         if (vel) dgxb::ks_step<R, true, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
         else dgxb::ks_step<R, false, 16><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
     } else if (group == 32) {

@@ -1,5 +1,5 @@
 /*
- * dgx_serve.cpp -- request loop (include/dgx_serve.h): resident meshes, slot groups of 16, 32 or 64 shots in the
+ * dgx_serve.cpp -- request loop (include/dgx_serve.h): resident meshes, slot groups of 8, 16, 32 or 64 shots in the
  * shot-innermost layout, memory-bounded admission, one kernel per half-step over the tiles of all active
  * batches, hipGraph-captured blocks of steps, retirement with trace read-back.
  */
@@ -157,10 +157,21 @@ struct dgx_server {
     size_t rsize() const { return fp32 ? 4 : 8; }
 };
 
+/* Slot groups that hold n shots. */
+static int groups_for(const dgx_server *sv, int n) { return (n + sv->group - 1) / sv->group; }
+
+/* Node rows of G lanes per field in a slot group of mesh m: the serving kernel interleaves 64/G elements
+ * per row, so the element count is rounded up to a multiple of 64/G. */
+static long long group_rows(const dgx_server *sv, const MeshDev &m)
+{
+    const long long per = 64 / sv->group;
+    return (m.tri + per - 1) / per * per * 6;
+}
+
 /* Device bytes of one slot group of mesh m (fields and the per-element word). */
 static long long group_bytes(const dgx_server *sv, const MeshDev &m)
 {
-    return (long long)5 * m.tri * 6 * sv->group * sv->rsize() + 4 * m.tri;
+    return (long long)5 * group_rows(sv, m) * sv->group * sv->rsize() + 4 * m.tri;
 }
 
 static void make_ops(dgx_server *sv)
@@ -215,8 +226,8 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
     sv->fp32 = cfg ? (cfg->precision == DGX_FP64 ? 0 : 1) : 1;
     sv->K = (cfg && cfg->steps_per_block > 0) ? cfg->steps_per_block : 100;
     sv->group = (cfg && cfg->group_shots) ? cfg->group_shots : 16;
-    if (sv->group != 16 && sv->group != 32 && sv->group != 64) {
-        dgx_set_error("dgx_server_create: group_shots must be 0, 16, 32 or 64");
+    if (sv->group != 8 && sv->group != 16 && sv->group != 32 && sv->group != 64) {
+        dgx_set_error("dgx_server_create: group_shots must be 0, 8, 16, 32 or 64");
         delete sv;
         return DGX_EARG;
     }
@@ -330,7 +341,7 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
 extern "C" int dgx_server_submit(dgx_server *sv, const dgx_request *rq, long *id)
 {
     if (!sv || !rq || !id) return DGX_EARG;
-    if (rq->mesh < 0 || rq->mesh >= (int)sv->meshes.size() || rq->nshots < 1 || rq->nshots > sv->group ||
+    if (rq->mesh < 0 || rq->mesh >= (int)sv->meshes.size() || rq->nshots < 1 || rq->nshots > DGX_SERVE_MAX_SHOTS ||
         rq->nsteps < 1 || rq->nrec < 0 || !rq->src || !rq->wavelet || (rq->nrec > 0 && !rq->rec)) {
         dgx_set_error("dgx_server_submit: invalid request");
         return DGX_EARG;
@@ -358,9 +369,10 @@ extern "C" int dgx_server_submit(dgx_server *sv, const dgx_request *rq, long *id
     if (rq->nrec) r.rec.assign(rq->rec, rq->rec + rq->nrec);
     r.arrival = rq->arrival;
     r.bytes = (long long)rq->nshots * rq->nsteps * (1 + 2 * rq->nrec) * 8;
-    if (r.bytes + group_bytes(sv, sv->meshes[rq->mesh]) > sv->budget) {
-        dgx_set_error("dgx_server_submit: the request needs %lld bytes with its slot group, more than the %lld-byte "
-                      "budget", r.bytes + group_bytes(sv, sv->meshes[rq->mesh]), sv->budget);
+    const long long alone = r.bytes + groups_for(sv, rq->nshots) * group_bytes(sv, sv->meshes[rq->mesh]);
+    if (alone > sv->budget) {
+        dgx_set_error("dgx_server_submit: the request needs %lld bytes with its slot groups, more than the %lld-byte "
+                      "budget", alone, sv->budget);
         sv->next_id--;
         return DGX_ENOMEM;
     }
@@ -404,7 +416,7 @@ static int new_batch(dgx_server *sv, int mesh, int *bi)
     b.mesh = mesh;
     b.G = sv->group;
     const int G = b.G;
-    b.field_bytes = (long long)5 * m.tri * 6 * G * sv->rsize();
+    b.field_bytes = (long long)5 * group_rows(sv, m) * G * sv->rsize();
     for (int k = 0; k < MAXG; k++) {
         b.req[k] = -1;
         b.shot[k] = 0;
@@ -544,7 +556,7 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
     a.op = op;
     R *f = (R *)b.fields;
     const int G = b.G;
-    size_t per = (size_t)m.tri * 6 * G;
+    size_t per = (size_t)group_rows(sv, m) * G;
     for (int c = 0; c < 5; c++) a.f[c] = f + c * per;
     a.fstride = m.tri;
     a.nbr = m.nbr;
@@ -608,19 +620,20 @@ static int upload_table(dgx_server *sv)
     return DGX_OK;
 }
 
-/* A batch entry is free for a new group (reused entry, or the table has room). */
-static bool batch_entry_free(const dgx_server *sv)
+/* Batch entries free for new groups (reused entries, and the table's room). */
+static long batch_entries_free(const dgx_server *sv)
 {
+    long n = MAX_BATCHES - (long)sv->batches.size();
     for (const auto &b : sv->batches)
-        if (!b.live) return true;
-    return (long)sv->batches.size() < MAX_BATCHES;
+        if (!b.live) n++;
+    return n;
 }
 
 /*
  * Admit queued requests in arrival order (later ones may pass one that does not fit yet).  A request's
  * shots are independent, so they may spread over several groups of its mesh: free slots are taken from
- * the fullest groups first, and a new group is opened only for the shots left over (at most one, since
- * a request has at most G shots).  Groups stay full, instead of one partly used group per request.
+ * the fullest groups first, and new groups are opened only for the shots left over.  Groups stay full,
+ * instead of one partly used group per request.
  */
 static int admit(dgx_server *sv)
 {
@@ -643,16 +656,16 @@ static int admit(dgx_server *sv)
         }
         std::stable_sort(open.begin(), open.end(),
                          [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first < y.first; });
-        const bool grow_group = free_total < r.nshots;
-        long long need = r.bytes + (grow_group ? group_bytes(sv, m) : 0);
-        bool fits = sv->in_use + need <= sv->budget && (!grow_group || batch_entry_free(sv));
-        if (!fits && grow_group && reclaim_empty(sv) > 0) {
+        const int grow = groups_for(sv, std::max(0, r.nshots - free_total)); /* new groups needed */
+        long long need = r.bytes + grow * group_bytes(sv, m);
+        bool fits = sv->in_use + need <= sv->budget && batch_entries_free(sv) >= grow;
+        if (!fits && grow > 0 && reclaim_empty(sv) > 0) {
             /* memory and entries held by idle groups (of any mesh) went back: look again */
             sv->queue.push_front(id);
             continue;
         }
         if (!fits) { still.push_back(id); continue; }
-        if (grow_group) {
+        for (int g = 0; g < grow; g++) {
             int bi = -1;
             int rc = new_batch(sv, r.mesh, &bi);
             if (rc) return rc;
@@ -685,9 +698,9 @@ static int admit(dgx_server *sv)
             }
             if (!mask) continue;
             void *f[5];
-            size_t per = (size_t)m.tri * 6 * G * sv->rsize();
+            size_t per = (size_t)group_rows(sv, m) * G * sv->rsize();
             for (int c = 0; c < 5; c++) f[c] = (char *)b.fields + c * per;
-            SV_TRY(dgx_launch_clear_slots(sv->fp32, f, (long long)m.tri * 6, G, mask, sv->stream));
+            SV_TRY(dgx_launch_clear_slots(sv->fp32, f, group_rows(sv, m), G, mask, sv->stream));
             b.dirty = true;
         }
         sv->table_dirty = true;
@@ -766,7 +779,7 @@ static int compact(dgx_server *sv)
                          [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first > y.first; });
         size_t t = 0; /* target group (act[0 .. keep-1]) and its next slot to look at */
         int ts = 0;
-        const size_t per = (size_t)m.tri * 6 * G * sv->rsize();
+        const size_t per = (size_t)group_rows(sv, m) * G * sv->rsize();
         for (size_t d = keep; d < act.size(); d++) {
             const int dbi = act[d].second;
             int s = 0;
@@ -805,7 +818,7 @@ static int compact(dgx_server *sv)
                     src[c] = (char *)D.fields + c * per;
                     dst[c] = (char *)T.fields + c * per;
                 }
-                SV_TRY(dgx_launch_move_slots(sv->fp32, src, dst, (long long)m.tri * 6, G, n, from, to, sv->stream));
+                SV_TRY(dgx_launch_move_slots(sv->fp32, src, dst, group_rows(sv, m), G, n, from, to, sv->stream));
                 D.dirty = true;
                 T.dirty = true;
                 sv->table_dirty = true;

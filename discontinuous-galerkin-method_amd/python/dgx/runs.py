@@ -6,7 +6,7 @@
     python -m dgx.runs carbonate [--shots 256 --steps 8000] 256 shots through the request loop, FP64 on 4
     python -m dgx.runs lanes     [--steps 400]              batched vs serving kernels at full slot occupancy
 
-The serving runs take --group (16, 32 or 64 shots per slot group; default 16 for the request stream of 1-16-shot
+The serving runs take --group (8, 16, 32 or 64 shots per slot group; default 8 for the request stream of 1-16-shot
 requests over 8 meshes, 64 for carbonate's 16-shot requests on one mesh).
 
 Progress lines go to stderr while a run is going."""
@@ -179,7 +179,7 @@ def run_lanes(a):
     def shots(n):
         return [SourceSpec(float(x), H - 500.0, L.DGX_SRC_EXPLOSIVE, k, 20.0)
                 for k, x in enumerate(np.linspace(500.0, W - 500.0, n))]
-    for S, ST in ((64, 64), (64, 16), (16, 16)):
+    for S, ST in ((64, 64), (64, 16), (16, 16), (8, 8)):
         st = build_setup(w.mesh, w.model, shots(S), w.receivers, w.nsteps, sponge=w.sponge)
         s = Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, ST)
         s.step(10)
@@ -191,10 +191,10 @@ def run_lanes(a):
         s.close()
         out[f"batched_{S}shots_tile{ST}"] = T * 30 * S * a.steps / sec
         log(f"batched {S} shots, tile {ST}: {out[f'batched_{S}shots_tile{ST}']:.3e} DOF-updates/s")
-    for G, nreq in ((64, 1), (16, 4), (16, 1)):
+    for G, nreq in ((64, 1), (16, 4), (16, 1), (8, 8), (8, 1)):
         sv = SV.Server(L.DGX_FP32, 0, 100, group_shots=G)
         rm = sv.add_mesh(w.mesh, w.model, w.sponge)
-        per = 16 if G == 16 else 64
+        per = G if G < 64 else 64
         reqs = [sv.make_request(rm, np.stack([np.linspace(500.0 + 10 * r, W - 500.0, per), np.full(per, H - 500.0)], 1),
                                 [20.0] * per, a.steps, w.receivers[:51]) for r in range(nreq)]
         _, _, st, _ = SV.run_open_loop(sv, reqs, np.zeros(nreq), progress_every=0)
@@ -213,13 +213,13 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--group", type=int, default=0,
-                    help="shots per slot group of the server, 16, 32 or 64 (default: 16 for serving, 64 for carbonate)")
+                    help="shots per slot group of the server, 8, 16, 32 or 64 (default: 8 for serving, 64 for carbonate)")
     a = ap.parse_args(argv)
     defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256),
                 "lanes": (400, 0)}
     a.steps = a.steps or defaults[a.which][0]
     a.shots = a.shots or defaults[a.which][1]
-    a.group = a.group or (64 if a.which == "carbonate" else 16)
+    a.group = a.group or (64 if a.which == "carbonate" else 8)
     fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate,
           "lanes": run_lanes}[a.which]
     print(json.dumps(fn(a)), flush=True)

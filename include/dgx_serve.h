@@ -3,9 +3,10 @@
  *
  * No counterpart in the reference, which runs one shot per process (DG_leap_Frog_OpenMP.c:397 loops over
  * a single shot).  The server keeps meshes, materials and sponge factors resident in HBM and runs many
- * shots of many requests at once: every request is 1..group_shots shots (16, 32 or 64, dgx_server_config) on
- * one resident mesh, each shot with its own source, FP64 wavelet samples and start step, all shots of a
- * request sharing one receiver spread.
+ * shots of many requests at once: every request is 1..DGX_SERVE_MAX_SHOTS shots on one resident mesh, each
+ * shot with its own source, FP64 wavelet samples and start step, all shots of a request sharing one
+ * receiver spread.  Shots run in slot groups of group_shots lanes (8, 16, 32 or 64, dgx_server_config); a
+ * request's shots may spread over several groups of its mesh.
  * Requests are admitted as slots and memory free up (continuous batching), stepped by one kernel per
  * half-step over the tiles of all active (mesh, slot group) batches, in hipGraph-captured blocks of steps,
  * and retired with their traces copied back.  Every shot computes exactly what a single-shot solve of
@@ -22,21 +23,23 @@ extern "C" {
 
 typedef struct dgx_server dgx_server;
 
+#define DGX_SERVE_MAX_SHOTS 64 /* shots per request */
+
 typedef struct {
     int precision;         /* DGX_FP32 (default for serving) or DGX_FP64 */
     long long hbm_budget;  /* bytes for wavefields, wavelets and traces; 0 = 80 % of free HBM */
     int steps_per_block;   /* steps between scheduling points (one graph replay); 0 = 100 */
-    int group_shots;       /* shots per slot group, 16, 32 or 64 (0 = 16): the serving kernel's shot tile.
-                            * A node row of a group is 4*G bytes in FP32: 32 and 64 move whole 128-byte
-                            * lines (faster per lane); 16 wastes fewer lanes when requests are small and
-                            * meshes many */
+    int group_shots;       /* shots per slot group, 8, 16, 32 or 64 (0 = 16): the serving kernel's shot
+                            * tile.  Each node row of a group holds 64/G elements x G shots, so every G
+                            * moves whole lines; 64 is fastest per lane when groups fill, 8 and 16 waste
+                            * fewer lanes when requests are small and meshes many */
 } dgx_server_config;
 
 enum dgx_request_state { DGX_REQ_QUEUED = 0, DGX_REQ_RUNNING = 1, DGX_REQ_DONE = 2 };
 
 typedef struct {
     int mesh;                /* id from dgx_server_add_mesh */
-    int nshots;              /* 1..group_shots */
+    int nshots;              /* 1..DGX_SERVE_MAX_SHOTS */
     long nsteps;             /* time steps of every shot */
     const dgx_source *src;   /* nshots sources; src[k].shot and .wavelet are ignored (shot k uses row k) */
     const double *wavelet;   /* [nshots][nsteps] FP64 source samples */

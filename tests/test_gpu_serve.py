@@ -6,10 +6,10 @@
 * they are within 1e-4 relative L2 of the FP64 oracle (FP32 serving), and within 1e-12 of it when the
   server runs in FP64;
 * a tight memory budget queues requests instead of failing, and all of them complete;
-* a request's shots may spread over several slot groups of its mesh, and may move between them when
-  groups are compacted, without changing its results.
+* a request's shots may spread over several slot groups of its mesh (also when it has more shots than one
+  group holds), and may move between them when groups are compacted, without changing its results.
 
-The first two hold for every slot-group size (16, 32 and 64 shots per group, which is also the serving
+The first two hold for every slot-group size (8, 16, 32 and 64 shots per group, which is also the serving
 kernel's shot tile).
 """
 import numpy as np
@@ -61,7 +61,7 @@ def run_alone(k, group=16, precision=L.DGX_FP32):
     return out, xz, rec
 
 
-@pytest.mark.parametrize("group", [16, 32, 64])
+@pytest.mark.parametrize("group", [8, 16, 32, 64])
 def test_results_identical_alone_and_in_the_mix(group):
     print(f"case serve mix, group {group}", flush=True)
     sv, rms, _, _ = new_server(group=group)
@@ -106,6 +106,32 @@ def test_requests_spread_over_groups():
         assert np.array_equal(spread[k][0], ax) and np.array_equal(spread[k][1], az), k
 
 
+def test_request_larger_than_a_group():
+    """With 8-shot groups, a 20-shot request opens three groups at once, and each of its shots equals the same
+    shot served as a request of its own, bit for bit.  Requests above DGX_SERVE_MAX_SHOTS are refused."""
+    print("case serve request over three groups, group 8", flush=True)
+    spec = (0, 20, 150, 15.0)
+    sv, rms, _, _ = new_server(group=8)
+    req, xz, rec = build(sv, rms, spec, 30)
+    rid = sv.submit(req)
+    sv.advance(1)
+    st = sv.stats()
+    assert st["running"] == 1 and st["batches"] == 3, st
+    sv.drain()
+    x, z = sv.result(rid, req)
+    with pytest.raises(L.DgxError):
+        sv.submit(sv.make_request(rms[0], [(900.0, 900.0)] * 65, [15.0] * 65, 10, rec))
+    sv.close()
+    sv, rms, _, _ = new_server(group=8)
+    singles = [sv.make_request(rms[0], xz[s:s + 1], [spec[3]], spec[2], rec) for s in range(spec[1])]
+    ids = [sv.submit(r) for r in singles]
+    sv.drain()
+    for s in range(spec[1]):
+        sx, sz = sv.result(ids[s], singles[s])
+        assert np.array_equal(x[s], sx[0]) and np.array_equal(z[s], sz[0]), s
+    sv.close()
+
+
 def test_compaction_moves_shots_without_changing_results():
     """16-shot groups, blocks of 50 steps: A (10 shots, 200 steps) fills group 1, B (10 shots, 60 steps)
     takes the other 6 slots of group 1 and 4 of group 2, C (6 shots, 200 steps) joins group 2.  When B
@@ -135,7 +161,7 @@ def test_compaction_moves_shots_without_changing_results():
         assert np.array_equal(out[k][0], ax) and np.array_equal(out[k][1], az), k
 
 
-@pytest.mark.parametrize("group", [16, 32, 64])
+@pytest.mark.parametrize("group", [8, 16, 32, 64])
 def test_request_equals_standalone_solver_and_oracle(group):
     print(f"case serve vs standalone and oracle, group {group}", flush=True)
     k = 0
@@ -162,7 +188,7 @@ def test_request_equals_standalone_solver_and_oracle(group):
     assert worst <= 1e-4
 
 
-@pytest.mark.parametrize("group", [16, 32, 64])
+@pytest.mark.parametrize("group", [8, 16, 32, 64])
 def test_fp64_serving_matches_oracle(group):
     """FP64 serving reproduces the oracle to the plain path's parity bar, a request mixed with others."""
     print(f"case fp64 serve vs oracle, group {group}", flush=True)
