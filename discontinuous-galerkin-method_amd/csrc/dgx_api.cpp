@@ -24,7 +24,7 @@ hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int whic
 hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st);
 hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
-hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double *out, long tri, long spad, long shot,
+hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double *out, long tri, long spad, int il, long shot,
                                 hipStream_t st);
 hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int variant, hipStream_t st);
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
@@ -111,6 +111,7 @@ struct dgx_solver {
     int layout = DGX_LAYOUT_ELEMENT;
     int shot_tile = 0;               /* shot-innermost: shots per lane group */
     int wg_waves = 4;                /* shot-innermost: wavefronts per workgroup */
+    int il = 0;                      /* shot-innermost: log2 of the elements interleaved per node row */
     long spad = 1;                   /* shot-innermost: padded shot count */
     std::vector<int> tune_tiles, tune_waves; /* autotuner candidates and their ms per step */
     std::vector<double> tune_ms;
@@ -292,7 +293,7 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     const char *generic = getenv("DGX_GENERIC_OPERATORS");
     a.p2 = (memcmp(table, &a.op, sizeof table) == 0 && !(generic && strcmp(generic, "1") == 0)) ? 1 : 0;
     R *f = (R *)s->fields, *f2 = (R *)s->fields2;
-    size_t per_field = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)s->tri * 6 * s->spad
+    size_t per_field = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)s->tri_pad * 6 * s->spad
                                                              : (size_t)s->nshots * s->vrows * 6;
     for (int c = 0; c < 5; c++) {
         a.f[c] = f + c * per_field;
@@ -329,6 +330,7 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     a.nshots = (int)s->nshots;
     a.shot_tile = s->shot_tile;
     a.wg_waves = s->wg_waves;
+    a.il = s->il;
     a.shot_start = nullptr;
     a.shot_len = nullptr;
 }
@@ -376,6 +378,10 @@ static void set_tile(dgx_solver *s, int tile, int waves)
     s->a64.shot_tile = s->a32.shot_tile = tile;
     s->a64.wg_waves = s->a32.wg_waves = waves;
     s->a64.nper = s->a32.nper = s->nper;
+    /* a tile that holds every (padded) shot of the row interleaves the 64/tile elements of a wavefront in
+     * each node row, so that a wavefront's node loads are one contiguous span */
+    s->il = (s->layout == DGX_LAYOUT_SHOT_INNER && tile == s->spad && tile < 64) ? __builtin_ctz(64 / tile) : 0;
+    s->a64.il = s->a32.il = s->il;
 }
 
 /* Shot-innermost autotuner: time a few real steps with every tile shape the kernels are built for whose
@@ -418,7 +424,7 @@ static int autotune(dgx_solver *s, int fixed_waves)
     }
     set_tile(s, best, best_waves);
     size_t rs = s->real_size();
-    HIP_TRY(hipMemsetAsync(s->fields, 0, (size_t)5 * s->tri * 6 * s->spad * rs, s->stream));
+    HIP_TRY(hipMemsetAsync(s->fields, 0, (size_t)5 * s->tri_pad * 6 * s->spad * rs, s->stream));
     if (s->trace)
         HIP_TRY(hipMemsetAsync(s->trace, 0, (size_t)s->nshots * s->nsteps * 2 * s->nrec * sizeof(double), s->stream));
     HIP_TRY(hipMemsetAsync(s->step_base, 0, sizeof(int), s->stream));
@@ -616,7 +622,8 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         s->spad = (p->nshots + unit - 1) / unit * unit;
         s->shot_tile = t ? t : 8;
         s->wg_waves = w ? w : 4;
-        s->tri_pad = T; /* shot-innermost fields are element-major: no block padding */
+        s->tri_pad = (T + 15) / 16 * 16; /* element-major; rows of interleaved elements need up to 16 */
+        s->il = (s->shot_tile == s->spad && s->shot_tile < 64) ? __builtin_ctz(64 / s->shot_tile) : 0;
     } else if (p->shot_tile || p->wg_waves) {
         dgx_set_error("dgx_create: shot_tile and wg_waves apply to the shot-innermost layout only");
         free_solver(s);
@@ -735,7 +742,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     CREATE_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (auto &ev : s->ev) CREATE_TRY(hipEventCreate(&ev));
-    size_t nreal_fields = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)5 * s->tri * 6 * s->spad
+    size_t nreal_fields = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)5 * s->tri_pad * 6 * s->spad
                                                                  : (size_t)5 * s->nshots * s->vrows * 6;
     CREATE_TRY(hipMalloc(&s->fields, nreal_fields * s->real_size()));
     CREATE_TRY(hipMemset(s->fields, 0, nreal_fields * s->real_size()));
@@ -949,7 +956,10 @@ extern "C" int dgx_record(dgx_solver *s, long t)
  * that shot's array; for the shot-innermost layout it is the whole field and the shot is added. */
 static inline long long raw_index(const dgx_solver *s, long q, int i, long shot)
 {
-    if (s->layout == DGX_LAYOUT_SHOT_INNER) return ((long long)q * 6 + i) * s->spad + shot;
+    if (s->layout == DGX_LAYOUT_SHOT_INNER) {
+        const long long L = s->il;
+        return ((((long long)(q >> L) * 6 + i) << L) + (q & ((1 << L) - 1))) * s->spad + shot;
+    }
     return fidx(q, i);
 }
 
@@ -978,7 +988,7 @@ static void raw_span(const dgx_solver *s, int c, long shot, size_t *off, size_t 
 {
     size_t rs = s->real_size();
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
-        *len = (size_t)s->tri * 6 * s->spad * rs;
+        *len = (size_t)s->tri_pad * 6 * s->spad * rs;
         *off = (size_t)c * *len;
     } else {
         *len = (size_t)s->tri_pad * 6 * rs;
@@ -1028,7 +1038,7 @@ extern "C" int dgx_snapshot_begin(dgx_solver *s, long shot, int mask)
         size_t off, len;
         raw_span(s, c, shot, &off, &len);
         HIP_TRY(dgx_launch_unpermute((const char *)s->state() + off, s->fp32, s->d_perm, s->d_snap + c * nf, s->tri,
-                                     spad, shot, s->stream));
+                                     spad, s->il, shot, s->stream));
     }
     HIP_TRY(hipEventRecord(s->snap_ready, s->stream));
     /* on the copy stream: the download, concurrent with the steps enqueued after this call */

@@ -446,24 +446,28 @@ __device__ __forceinline__ void stress_body(const Ops<R> &op, const StepArgs<R> 
     }
 }
 
-template <typename R, int ST, int NW>
+/* IL: the tile holds every shot of the row (ST == spad < 64), whose node rows interleave the 64/ST elements
+ * of a wavefront (bidx with L = log2(64/ST)) */
+template <typename R, int ST, int NW, bool IL>
 __global__ void __launch_bounds__(64 * NW) kb_velocity(StepArgs<R> a)
 {
+    constexpr int L = IL ? __builtin_ctz(64 / ST) : 0;
     int q, s;
     if (!my_work<ST, NW>(a, q, s)) return;
     /* with 64 shots per tile a wavefront holds one element: saying so turns the element's records, its
      * neighbour ids and the row offsets into scalar values (measured: faster for the velocity kernel,
      * slightly slower for the stress kernel, which keeps the per-lane form) */
     if (ST == 64) q = __builtin_amdgcn_readfirstlane(q);
-    velocity_body(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
+    velocity_body<R, L>(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
 }
 
-template <typename R, int ST, int NW>
+template <typename R, int ST, int NW, bool IL>
 __global__ void __launch_bounds__(64 * NW) kb_stress(StepArgs<R> a)
 {
+    constexpr int L = IL ? __builtin_ctz(64 / ST) : 0;
     int q, s;
     if (!my_work<ST, NW>(a, q, s)) return;
-    stress_body(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
+    stress_body<R, L>(a.op, a, q, s, shot_step(a, s, *a.step_base + a.step_off));
 }
 
 template <typename R>
@@ -607,8 +611,17 @@ template <typename R, int ST, int NW>
 static hipError_t launch_b(const dgx::StepArgs<R> &a, hipStream_t st, int which)
 {
     dim3 grid(DGX_NXCD * a.nper, a.spad / ST), block(64 * NW);
-    if (which == 0) dgxb::kb_velocity<R, ST, NW><<<grid, block, 0, st>>>(a);
-    else dgxb::kb_stress<R, ST, NW><<<grid, block, 0, st>>>(a);
+    if constexpr (ST < 64) {
+        if (a.il) {
+            if (a.spad != ST || (1 << a.il) != 64 / ST) return hipErrorInvalidValue;
+            if (which == 0) dgxb::kb_velocity<R, ST, NW, true><<<grid, block, 0, st>>>(a);
+            else dgxb::kb_stress<R, ST, NW, true><<<grid, block, 0, st>>>(a);
+            return hipGetLastError();
+        }
+    }
+    if (a.il) return hipErrorInvalidValue;
+    if (which == 0) dgxb::kb_velocity<R, ST, NW, false><<<grid, block, 0, st>>>(a);
+    else dgxb::kb_stress<R, ST, NW, false><<<grid, block, 0, st>>>(a);
     return hipGetLastError();
 }
 

@@ -83,6 +83,7 @@ struct StepArgs {
     int nshots;            /* real shots; padded lanes s >= nshots never record */
     int shot_tile;         /* shots per lane group (ST); a wavefront covers 64/ST elements */
     int wg_waves;          /* wavefronts per workgroup of the batched kernels (2, 4 or 8) */
+    int il;                /* shot-innermost rows interleave 2^il elements (0: one element per row) */
     const int *shot_start; /* optional per-shot first global step (NULL: all shots start at 0) */
     const int *shot_len;   /* optional per-shot step count */
     /* serving only (NULL otherwise): sources/receivers change as requests come and go, so they are

@@ -872,17 +872,20 @@ __global__ void __launch_bounds__(256) k_read4(const double4 *__restrict__ src, 
 
 /* Snapshot of one field of one shot in the caller's element order, as doubles: out[6*perm[q] + i] is node i
  * of device element q, read from the element layout's blocked row or, when `spad` > 0, from the
- * shot-innermost layout ((q*6 + i)*spad + shot). */
+ * shot-innermost layout (node rows of spad shots interleaving 2^il elements:
+ * ((((q >> il)*6 + i) << il) + q mod 2^il)*spad + shot). */
 template <typename R>
 __global__ void __launch_bounds__(256) k_unpermute(const R *__restrict__ f, const int *__restrict__ perm,
-                                                   double *__restrict__ out, int tri, long long spad, long long shot)
+                                                   double *__restrict__ out, int tri, long long spad, int il,
+                                                   long long shot)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= tri) return;
     double *o = out + 6 * (long long)perm[q];
+    const long long row0 = (long long)(q >> il) * 6, sub = q & ((1 << il) - 1);
 #pragma unroll
     for (int i = 0; i < 6; i++)
-        o[i] = (double)(spad > 0 ? f[((long long)q * 6 + i) * spad + shot] : f[fidx(q, i)]);
+        o[i] = (double)(spad > 0 ? f[(((row0 + i) << il) + sub) * spad + shot] : f[fidx(q, i)]);
 }
 
 __global__ void k_advance(int *step_base, int n)
@@ -948,12 +951,12 @@ hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int varian
     else dgx::k_read4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double *)dst, n32);
     return hipGetLastError();
 }
-hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double *out, long tri, long spad, long shot,
-                                hipStream_t st)
+hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double *out, long tri, long spad, int il,
+                                long shot, hipStream_t st)
 {
     const unsigned blocks = (unsigned)((tri + 255) / 256);
-    if (fp32) dgx::k_unpermute<float><<<blocks, 256, 0, st>>>((const float *)f, perm, out, (int)tri, spad, shot);
-    else dgx::k_unpermute<double><<<blocks, 256, 0, st>>>((const double *)f, perm, out, (int)tri, spad, shot);
+    if (fp32) dgx::k_unpermute<float><<<blocks, 256, 0, st>>>((const float *)f, perm, out, (int)tri, spad, il, shot);
+    else dgx::k_unpermute<double><<<blocks, 256, 0, st>>>((const double *)f, perm, out, (int)tri, spad, il, shot);
     return hipGetLastError();
 }
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st)

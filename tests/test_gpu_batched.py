@@ -52,6 +52,32 @@ def test_batched_fp64_equals_single_shot_runs(tile):
         assert rel_l2(f[c], g[c]) <= 1e-13, c
 
 
+@pytest.mark.parametrize("nshots,tile", [(8, 8), (16, 16), (32, 32)])
+def test_interleaved_rows_equal_one_element_rows(nshots, tile):
+    """A shot tile that holds every shot of a row stores the 64/tile elements of a wavefront interleaved in
+    each node row; half that tile keeps one element per row.  FP64 traces and fields of the two agree to
+    rounding, and fields written with set_fields read back unchanged."""
+    print(f"case interleaved rows, {nshots} shots, tile {tile}", flush=True)
+    w = batched_workload(nshots, 120)
+    st = product_setup(w)
+    runs = []
+    for t in (tile, tile // 2):
+        s = dgx.Solver(st, L.DGX_FP64, L.DGX_LAYOUT_SHOT_INNER, t)
+        s.step(w.nsteps)
+        runs.append(([s.traces(k) for k in range(nshots)], [s.fields(k) for k in (0, nshots - 1)]))
+        if t == tile:
+            f = s.fields(1) * 0.5 + 1.0
+            s.set_fields(f, 1)
+            assert np.array_equal(s.fields(1), f)
+            assert np.array_equal(s.fields(0), runs[-1][1][0])
+        s.close()
+    (ta, fa), (tb, fb) = runs
+    for k in range(nshots):
+        assert rel_l2(ta[k][0], tb[k][0]) <= 1e-13 and rel_l2(ta[k][1], tb[k][1]) <= 1e-13, k
+    for x, y in zip(fa, fb):
+        assert np.abs(x - y).max() <= 1e-13 * max(np.abs(y).max(), 1e-300)
+
+
 def test_batched_fp32_against_fp64_oracle():
     w = batched_workload(16, 300)
     oc = OracleCase(w)
