@@ -668,7 +668,14 @@ static int admit(dgx_server *sv)
         for (int g = 0; g < grow; g++) {
             int bi = -1;
             int rc = new_batch(sv, r.mesh, &bi);
-            if (rc) return rc;
+            if (rc) {
+                /* nothing of this request is placed yet: it and the requests deferred or not yet looked at
+                 * stay queued, in order (a group opened for it stays empty until reclaimed) */
+                still.push_back(id);
+                for (long rest : sv->queue) still.push_back(rest);
+                sv->queue = still;
+                return rc;
+            }
             open.push_back({G, bi});
         }
         int k = 0;
