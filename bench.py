@@ -53,8 +53,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-budget-gb", type=float, default=0.0,
                     help="HBM budget of the serving leg's wavefields and request buffers (0 = 80%% of free HBM)")
-    ap.add_argument("--serve-group", type=int, default=8,
-                    help="shots per slot group of the server (8, 16, 32 or 64; 8 suits this stream of 1-16-shot requests)")
+    ap.add_argument("--serve-group", type=int, default=4,
+                    help="shots per slot group of the server (4, 8, 16, 32 or 64; 4 suits this stream of 1-16-shot requests)")
     ap.add_argument("--copy-probe", type=int, default=1,
                     help="also time a plain device copy (achievable bandwidth, counter calibration)")
     return ap.parse_args()

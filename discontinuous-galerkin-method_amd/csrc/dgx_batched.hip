@@ -497,7 +497,7 @@ __device__ __forceinline__ StepArgs<R> global_args(const StepArgs<R> &t)
 /*
  * Serving: one launch per half-step over every active (mesh, slot group) batch.  sched = {nbatch,
  * total tiles, tile offset of batch 0, ..., tile offset of batch nbatch}; a tile is 256/G elements x G
- * shots (G = slots per group, 8, 16, 32 or 64: 64/G elements per wavefront, stored interleaved in the
+ * shots (G = slots per group, 4, 8, 16, 32 or 64: 64/G elements per wavefront, stored interleaved in the
  * group's node rows so that each node load and store of a wavefront is one contiguous span; bidx).  One workgroup per tile, and the
  * tiles are dealt to the 8 XCDs in contiguous ranges swept in order, as in the batched kernels: face
  * neighbours are stepped close together in time on one XCD and their rows are still in its L2.  (A
@@ -519,8 +519,8 @@ __global__ void __launch_bounds__(DGX_BLOCK) ks_step(const Ops<R> op, const Step
                                                         int nper)
 {
     constexpr int EW = 64 / G; /* elements per wavefront, interleaved in one node row of the group */
-    constexpr int LG = G == 64 ? 0 : G == 32 ? 1 : G == 16 ? 2 : 3;
-    static_assert((1 << LG) == EW, "slot groups of 8, 16, 32 or 64 shots");
+    constexpr int LG = G == 64 ? 0 : G == 32 ? 1 : G == 16 ? 2 : G == 8 ? 3 : 4;
+    static_assert((1 << LG) == EW, "slot groups of 4, 8, 16, 32 or 64 shots");
     const int tile = (int)(blockIdx.x & (DGX_NXCD - 1)) * nper + (int)(blockIdx.x >> 3);
     if (tile >= total) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -661,7 +661,10 @@ static hipError_t launch_serve(const void *ops, const void *table, const int *sc
     if (total <= 0) return hipSuccess;
     const int nper = (total + DGX_NXCD - 1) / DGX_NXCD; /* tiles per XCD */
     const int grid = DGX_NXCD * nper;
-    if (group == 8) {
+    if (group == 4) {
+        if (vel) dgxb::ks_step<R, true, 4><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
+        else dgxb::ks_step<R, false, 4><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
+    } else if (group == 8) {
         if (vel) dgxb::ks_step<R, true, 8><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
         else dgxb::ks_step<R, false, 8><<<grid, DGX_BLOCK, 0, st>>>(op, t, sched, runmask, step_base, step_off, nb, total, nper);
     } else if (group == 16) {

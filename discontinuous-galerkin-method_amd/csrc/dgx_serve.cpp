@@ -1,5 +1,5 @@
 /*
- * dgx_serve.cpp -- request loop (include/dgx_serve.h): resident meshes, slot groups of 8, 16, 32 or 64 shots in the
+ * dgx_serve.cpp -- request loop (include/dgx_serve.h): resident meshes, slot groups of 4, 8, 16, 32 or 64 shots in the
  * shot-innermost layout, memory-bounded admission, one kernel per half-step over the tiles of all active
  * batches, hipGraph-captured blocks of steps, retirement with trace read-back.
  */
@@ -226,8 +226,8 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
     sv->fp32 = cfg ? (cfg->precision == DGX_FP64 ? 0 : 1) : 1;
     sv->K = (cfg && cfg->steps_per_block > 0) ? cfg->steps_per_block : 100;
     sv->group = (cfg && cfg->group_shots) ? cfg->group_shots : 16;
-    if (sv->group != 8 && sv->group != 16 && sv->group != 32 && sv->group != 64) {
-        dgx_set_error("dgx_server_create: group_shots must be 0, 8, 16, 32 or 64");
+    if (sv->group != 4 && sv->group != 8 && sv->group != 16 && sv->group != 32 && sv->group != 64) {
+        dgx_set_error("dgx_server_create: group_shots must be 0, 4, 8, 16, 32 or 64");
         delete sv;
         return DGX_EARG;
     }

@@ -5,7 +5,7 @@
  * a single shot).  The server keeps meshes, materials and sponge factors resident in HBM and runs many
  * shots of many requests at once: every request is 1..DGX_SERVE_MAX_SHOTS shots on one resident mesh, each
  * shot with its own source, FP64 wavelet samples and start step, all shots of a request sharing one
- * receiver spread.  Shots run in slot groups of group_shots lanes (8, 16, 32 or 64, dgx_server_config); a
+ * receiver spread.  Shots run in slot groups of group_shots lanes (4, 8, 16, 32 or 64, dgx_server_config); a
  * request's shots may spread over several groups of its mesh.
  * Requests are admitted as slots and memory free up (continuous batching), stepped by one kernel per
  * half-step over the tiles of all active (mesh, slot group) batches, in hipGraph-captured blocks of steps,
@@ -29,9 +29,9 @@ typedef struct {
     int precision;         /* DGX_FP32 (default for serving) or DGX_FP64 */
     long long hbm_budget;  /* bytes for wavefields, wavelets and traces; 0 = 80 % of free HBM */
     int steps_per_block;   /* steps between scheduling points (one graph replay); 0 = 100 */
-    int group_shots;       /* shots per slot group, 8, 16, 32 or 64 (0 = 16): the serving kernel's shot
-                            * tile.  Each node row of a group holds 64/G elements x G shots, so every G
-                            * moves whole lines; 64 is fastest per lane when groups fill, 8 and 16 waste
+    int group_shots;       /* shots per slot group, 4, 8, 16, 32 or 64 (0 = 16): the serving kernel's
+                            * shot tile.  Each node row of a group holds 64/G elements x G shots, so every
+                            * G moves whole lines; 64 is fastest per lane when groups fill, 4-16 waste
                             * fewer lanes when requests are small and meshes many */
 } dgx_server_config;
 

@@ -9,7 +9,7 @@
 * a request's shots may spread over several slot groups of its mesh (also when it has more shots than one
   group holds), and may move between them when groups are compacted, without changing its results.
 
-The first two hold for every slot-group size (8, 16, 32 and 64 shots per group, which is also the serving
+The first two hold for every slot-group size (4, 8, 16, 32 and 64 shots per group, which is also the serving
 kernel's shot tile).
 """
 import numpy as np
@@ -61,7 +61,7 @@ def run_alone(k, group=16, precision=L.DGX_FP32):
     return out, xz, rec
 
 
-@pytest.mark.parametrize("group", [8, 16, 32, 64])
+@pytest.mark.parametrize("group", [4, 8, 16, 32, 64])
 def test_results_identical_alone_and_in_the_mix(group):
     print(f"case serve mix, group {group}", flush=True)
     sv, rms, _, _ = new_server(group=group)
@@ -161,7 +161,7 @@ def test_compaction_moves_shots_without_changing_results():
         assert np.array_equal(out[k][0], ax) and np.array_equal(out[k][1], az), k
 
 
-@pytest.mark.parametrize("group", [8, 16, 32, 64])
+@pytest.mark.parametrize("group", [4, 8, 16, 32, 64])
 def test_request_equals_standalone_solver_and_oracle(group):
     print(f"case serve vs standalone and oracle, group {group}", flush=True)
     k = 0
@@ -188,7 +188,7 @@ def test_request_equals_standalone_solver_and_oracle(group):
     assert worst <= 1e-4
 
 
-@pytest.mark.parametrize("group", [8, 16, 32, 64])
+@pytest.mark.parametrize("group", [4, 8, 16, 32, 64])
 def test_fp64_serving_matches_oracle(group):
     """FP64 serving reproduces the oracle to the plain path's parity bar, a request mixed with others."""
     print(f"case fp64 serve vs oracle, group {group}", flush=True)
