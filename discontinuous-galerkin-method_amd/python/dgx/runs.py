@@ -6,7 +6,7 @@
     python -m dgx.runs carbonate [--shots 256 --steps 8000] 256 shots through the request loop, FP64 on 4
     python -m dgx.runs lanes     [--steps 400]              batched vs serving kernels at full slot occupancy
 
-The serving runs take --group (4, 8, 16, 32 or 64 shots per slot group; default 4 for the request stream of 1-16-shot
+The serving runs take --group (4, 8, 16, 32 or 64 shots per slot group; default 8 for the request stream of 1-16-shot
 requests over 8 meshes, 64 for carbonate's 16-shot requests on one mesh).
 
 Progress lines go to stderr while a run is going."""
@@ -213,13 +213,13 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--group", type=int, default=0,
-                    help="shots per slot group of the server, 4, 8, 16, 32 or 64 (default: 4 for serving, 64 for carbonate)")
+                    help="shots per slot group of the server, 4, 8, 16, 32 or 64 (default: 8 for serving, 64 for carbonate)")
     a = ap.parse_args(argv)
     defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256),
                 "lanes": (400, 0)}
     a.steps = a.steps or defaults[a.which][0]
     a.shots = a.shots or defaults[a.which][1]
-    a.group = a.group or (64 if a.which == "carbonate" else 4)
+    a.group = a.group or (64 if a.which == "carbonate" else 8)
     fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate,
           "lanes": run_lanes}[a.which]
     print(json.dumps(fn(a)), flush=True)
