@@ -15,10 +15,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dgx_device.h"
 #include "dgx_kernels.h"
 
 namespace dgxb {
 
+using dgx::face_node;
+using dgx::Geom;
+using dgx::geometry;
+using dgx::nb_node;
 using dgx::Ops;
 using dgx::StepArgs;
 
@@ -31,40 +36,6 @@ template <typename T>
 __device__ __forceinline__ T *as_global(T *p)
 {
     return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
-}
-
-__device__ __forceinline__ constexpr int face_node(int j, int k)
-{
-    return (j == 0) ? (k == 0 ? 5 : k == 1 ? 3 : 0) : (j == 1) ? k : (k == 0 ? 5 : k == 1 ? 4 : 2);
-}
-
-__device__ __forceinline__ int nb_node(unsigned m, unsigned rev, int k)
-{
-    const unsigned long long tab = 0x245210035ull;
-    const int kk = rev ? 2 - k : k;
-    return (int)((tab >> (4 * (3 * m + kk))) & 15);
-}
-
-struct Geom {
-    double rx, rz, sx, sz, invD;
-    double ex[3], ez[3];
-};
-
-__device__ __forceinline__ Geom geometry(const double2 *pt, int4 v)
-{
-    const double2 p0 = pt[v.x], p1 = pt[v.y], p2 = pt[v.z];
-    Geom g;
-    g.ex[0] = p1.x - p0.x; g.ez[0] = p1.y - p0.y;
-    g.ex[1] = p2.x - p1.x; g.ez[1] = p2.y - p1.y;
-    g.ex[2] = p0.x - p2.x; g.ez[2] = p0.y - p2.y;
-    const double D = g.ex[0] * (p2.y - p0.y) - (p2.x - p0.x) * g.ez[0];
-    g.invD = 1.0 / D;
-    const double t = 2.0 * g.invD;
-    g.rx = (p2.y - p0.y) * t;
-    g.rz = -(p2.x - p0.x) * t;
-    g.sx = -g.ez[0] * t;
-    g.sz = g.ex[0] * t;
-    return g;
 }
 
 /* Element and shot of this lane (workgroups of NW wavefronts); false when outside the mesh. */

@@ -4,11 +4,8 @@
  * One thread advances one element of one shot.  The reference-element operators Dr = Vr*inverV,
  * Ds = Vs*inverV and the face lifts S1..S3 arrive as kernel arguments (uniform, so they sit in
  * scalar registers / the scalar cache).  Geometric factors are recomputed in FP64 from the three
- * vertex coordinates; with D = 2*area = 4J (J as in fun_DG.c:447):
- *   rx = 2(z2-z0)/D, rz = -2(x2-x0)/D, sx = -2(z1-z0)/D, sz = 2(x1-x0)/D     (fun_DG.c:1157-1165)
- *   face j with counter-clockwise edge vector (ex_j, ez_j):
- *   0.5 * n_j * (L_j/2) / J = (ez_j, -ex_j) / D                              (fun_DG.c:1118-1131)
- * so the reference's per-element Dx/Dz/Q arrays (fun_DG.c:1091-1192) are never materialised.
+ * vertex coordinates (dgx_device.h), so the reference's per-element Dx/Dz/Q arrays (fun_DG.c:1091-1192)
+ * are never materialised.
  *
  * Velocity (OpenMP_Function.c:75-185), with the material factored out and the Dx/Dz products
  * regrouped as Dr(rx u + rz w) + Ds(sx u + sz w):
@@ -31,6 +28,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include "dgx_device.h"
 #include "dgx_kernels.h"
 #include "dgx_p2ops.h"
 
@@ -71,42 +69,6 @@ __device__ __forceinline__ int element_of_thread(int nper)
 {
     const int b = blockIdx.x;
     return ((b & (DGX_NXCD - 1)) * nper + (b >> 3)) * DGX_BLOCK + threadIdx.x;
-}
-
-/* Local node of face j, position k, in the reference's face order (ibe, fun_DG.c:1048-1056). */
-__device__ __forceinline__ constexpr int face_node(int j, int k)
-{
-    return (j == 0) ? (k == 0 ? 5 : k == 1 ? 3 : 0) : (j == 1) ? k : (k == 0 ? 5 : k == 1 ? 4 : 2);
-}
-
-/* Neighbour's node facing own face position k, given the neighbour's face m and orientation. */
-__device__ __forceinline__ int nb_node(unsigned m, unsigned rev, int k)
-{
-    const unsigned long long tab = 0x245210035ull; /* nibbles: 5,3,0, 0,1,2, 5,4,2 (low first) */
-    const int kk = rev ? 2 - k : k;
-    return (int)((tab >> (4 * (3 * m + kk))) & 15);
-}
-
-struct Geom {
-    double rx, rz, sx, sz, invD;
-    double ex[3], ez[3];
-};
-
-__device__ __forceinline__ Geom geometry(const double2 *pt, int4 v)
-{
-    const double2 p0 = pt[v.x], p1 = pt[v.y], p2 = pt[v.z];
-    Geom g;
-    g.ex[0] = p1.x - p0.x; g.ez[0] = p1.y - p0.y;
-    g.ex[1] = p2.x - p1.x; g.ez[1] = p2.y - p1.y;
-    g.ex[2] = p0.x - p2.x; g.ez[2] = p0.y - p2.y;
-    const double D = g.ex[0] * (p2.y - p0.y) - (p2.x - p0.x) * g.ez[0];
-    g.invD = 1.0 / D;
-    const double t = 2.0 * g.invD;
-    g.rx = (p2.y - p0.y) * t;
-    g.rz = -(p2.x - p0.x) * t;
-    g.sx = -g.ez[0] * t;
-    g.sz = g.ex[0] * t;
-    return g;
 }
 
 /* Sponge and explosive source applied to a stress value of element q at node i (fused step). */

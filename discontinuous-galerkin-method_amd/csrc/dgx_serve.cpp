@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -43,20 +44,35 @@ namespace {
 const int MAXG = 64;      /* largest slot group (shots per batch = shot tile of the serving kernel) */
 const int MAX_BATCHES = 4096;
 
+/* A material model on a resident mesh: 1/rho | lambda | mu per element in device order, and the time step
+ * its requests run with.  Model 0 of every mesh is the one given at dgx_server_add_mesh and stays resident;
+ * the others come with requests, are shared by every request with the same arrays and dt, and are on the
+ * device while one of their requests runs. */
+struct ModelDev {
+    bool used = false;
+    uint64_t hash = 0;
+    std::vector<double> mat; /* host copy, 3 * tri (empty for model 0, whose copy lives on the device only) */
+    double dt = 0;
+    long refs = 0;           /* requests queued or running with this model */
+    long live = 0;           /* running requests */
+    void *d_mat = nullptr;
+};
+
 struct MeshDev {
     long tri = 0;
-    double dt = 0;
     std::vector<long> inv; /* caller element -> device element */
     std::vector<int4> nbr_host; /* device-order neighbours (finding the faces next to a source) */
     int4 *nbr = nullptr, *vtx = nullptr;
     double2 *pt = nullptr;
-    void *mat = nullptr, *absorb = nullptr;
+    void *absorb = nullptr;
+    std::vector<ModelDev> models;
     std::vector<int> batches;
 };
 
 struct Request {
     long id = 0;
     int mesh = 0, nshots = 0;
+    int model = 0; /* index into the mesh's models */
     long nsteps = 0, nrec = 0;
     std::vector<dgx_source> src;
     std::vector<double> wav;
@@ -73,6 +89,7 @@ struct Request {
 struct Batch {
     bool live = true; /* false: freed (all slots were empty when memory was needed); the entry is reused */
     int mesh = 0;
+    int model = 0;    /* every shot of a group runs on one material model of its mesh */
     void *fields = nullptr;
     long long field_bytes = 0;
     int G = 16;                     /* slots */
@@ -153,6 +170,7 @@ struct dgx_server {
     dgx::Ops<float> op32;
     /* memory pool of per-request device buffers, by size */
     std::multimap<size_t, double *> pool;
+    long fail_alloc_at = 0, allocs = 0; /* test hook: DGX_SERVE_FAIL_ALLOC (pool_get) */
 
     size_t rsize() const { return fp32 ? 4 : 8; }
 };
@@ -196,6 +214,13 @@ static void make_ops(dgx_server *sv)
         }
 }
 
+/* Free every pooled request buffer (the pool lies outside the budget and bytes_in_use). */
+static void pool_release(dgx_server *sv)
+{
+    for (auto &kv : sv->pool) (void)hipFree(kv.second);
+    sv->pool.clear();
+}
+
 static double *pool_get(dgx_server *sv, size_t n)
 {
     auto it = sv->pool.lower_bound(n);
@@ -204,9 +229,19 @@ static double *pool_get(dgx_server *sv, size_t n)
         sv->pool.erase(it);
         return p;
     }
+    /* DGX_SERVE_FAIL_ALLOC=k (read at create): the k-th device allocation of a request buffer fails, as if
+     * HBM were exhausted, so that tests can drive admission through its failure path */
+    if (sv->fail_alloc_at > 0 && ++sv->allocs == sv->fail_alloc_at) return nullptr;
     double *p = nullptr;
-    if (hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) return nullptr;
-    return p;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(double);
+    if (hipMalloc((void **)&p, bytes) == hipSuccess) return p;
+    /* pooled buffers of other sizes may hold the memory: give them back and try once more */
+    (void)hipGetLastError();
+    pool_release(sv);
+    p = nullptr;
+    if (hipMalloc((void **)&p, bytes) == hipSuccess) return p;
+    (void)hipGetLastError();
+    return nullptr;
 }
 
 static void pool_put(dgx_server *sv, double *p, size_t n) { sv->pool.emplace(n, p); }
@@ -233,6 +268,7 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
     }
     sv->ops = *ops;
     make_ops(sv);
+    if (const char *f = getenv("DGX_SERVE_FAIL_ALLOC")) sv->fail_alloc_at = atol(f);
     /* a failure from here on frees what was already created */
 #define CREATE_TRY(call)                                                                                       \
     do {                                                                                                       \
@@ -268,9 +304,11 @@ extern "C" int dgx_server_destroy(dgx_server *sv)
     if (sv->stream) (void)hipStreamSynchronize(sv->stream);
     if (sv->graph) (void)hipGraphExecDestroy(sv->graph);
     for (auto &m : sv->meshes) {
-        void *p[] = {m.nbr, m.vtx, m.pt, m.mat, m.absorb};
+        void *p[] = {m.nbr, m.vtx, m.pt, m.absorb};
         for (void *x : p)
             if (x) (void)hipFree(x);
+        for (auto &md : m.models)
+            if (md.d_mat) (void)hipFree(md.d_mat);
     }
     for (auto &b : sv->batches) {
         void *p[] = {b.fields, b.d_slot_int, b.d_slot_ptr, b.d_aux_of_elem, b.d_aux, b.d_src, b.d_rec, b.d_pairs};
@@ -281,7 +319,7 @@ extern "C" int dgx_server_destroy(dgx_server *sv)
         for (double *p : r.second.d_wav) (void)hipFree(p);
         for (double *p : r.second.d_trace) (void)hipFree(p);
     }
-    for (auto &kv : sv->pool) (void)hipFree(kv.second);
+    pool_release(sv);
     void *p[] = {sv->d_step_base, sv->d_table, sv->d_sched, sv->d_runmask};
     for (void *x : p)
         if (x) (void)hipFree(x);
@@ -295,9 +333,42 @@ extern "C" int dgx_server_destroy(dgx_server *sv)
 template <typename T>
 static hipError_t upload_vec(T **dst, const std::vector<T> &v)
 {
+    *dst = nullptr;
     hipError_t e = hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(v.size(), 1));
     if (e != hipSuccess || v.empty()) return e;
     return hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+}
+
+/* doubles to the device in the server's precision */
+static hipError_t upload_reals(const dgx_server *sv, void **dst, const std::vector<double> &v)
+{
+    if (sv->fp32) {
+        std::vector<float> f(v.begin(), v.end());
+        float *d = nullptr;
+        hipError_t e = upload_vec(&d, f);
+        *dst = d;
+        return e;
+    }
+    double *d = nullptr;
+    hipError_t e = upload_vec(&d, v);
+    *dst = d;
+    return e;
+}
+
+/* Bytes of one material model on the device. */
+static long long model_bytes(const dgx_server *sv, const MeshDev &m) { return 3LL * m.tri * (long long)sv->rsize(); }
+
+/* 64-bit FNV-1a over the model's arrays and time step. */
+static uint64_t model_hash(const std::vector<double> &mat, double dt)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t n) {
+        const unsigned char *c = (const unsigned char *)p;
+        for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ull;
+    };
+    mix(mat.data(), mat.size() * sizeof(double));
+    mix(&dt, sizeof dt);
+    return h;
 }
 
 extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *mesh_id)
@@ -313,26 +384,27 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     if (rc) return rc;
     MeshDev m;
     m.tri = p->tri;
-    m.dt = p->dt;
     m.inv = mh.inv;
     m.nbr_host.assign(mh.nbr.begin(), mh.nbr.begin() + mh.tri);
-    SV_TRY(upload_vec(&m.nbr, mh.nbr));
-    SV_TRY(upload_vec(&m.vtx, mh.vtx));
-    SV_TRY(upload_vec(&m.pt, mh.pt));
-    if (sv->fp32) {
-        std::vector<float> mat(mh.mat.begin(), mh.mat.end()), ab(mh.absorb.begin(), mh.absorb.end());
-        float *a, *b;
-        SV_TRY(upload_vec(&a, mat));
-        SV_TRY(upload_vec(&b, ab));
-        m.mat = a;
-        m.absorb = b;
-    } else {
-        double *a, *b;
-        SV_TRY(upload_vec(&a, mh.mat));
-        SV_TRY(upload_vec(&b, mh.absorb));
-        m.mat = a;
-        m.absorb = b;
-    }
+    ModelDev md;
+    md.used = true;
+    md.dt = p->dt;
+    md.refs = 1; /* never released */
+    /* a failure frees what this call allocated */
+    auto fail = [&](hipError_t e) {
+        void *q[] = {m.nbr, m.vtx, m.pt, m.absorb, md.d_mat};
+        for (void *x : q)
+            if (x) (void)hipFree(x);
+        dgx_set_error("dgx_server_add_mesh: %s", hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? DGX_ENOMEM : DGX_EHIP;
+    };
+    hipError_t e = upload_vec(&m.nbr, mh.nbr);
+    if (e == hipSuccess) e = upload_vec(&m.vtx, mh.vtx);
+    if (e == hipSuccess) e = upload_vec(&m.pt, mh.pt);
+    if (e == hipSuccess) e = upload_reals(sv, &md.d_mat, mh.mat);
+    if (e == hipSuccess) e = upload_reals(sv, &m.absorb, mh.absorb);
+    if (e != hipSuccess) return fail(e);
+    m.models.push_back(std::move(md));
     sv->meshes.push_back(std::move(m));
     *mesh_id = (int)sv->meshes.size() - 1;
     return DGX_OK;
@@ -358,9 +430,49 @@ extern "C" int dgx_server_submit(dgx_server *sv, const dgx_request *rq, long *id
             dgx_set_error("dgx_server_submit: receiver %ld invalid", k);
             return DGX_EARG;
         }
+    MeshDev &md = sv->meshes[rq->mesh];
+    int model = 0;
+    if (rq->lambda || rq->miu || rq->den) {
+        if (!rq->lambda || !rq->miu || !rq->den || !(rq->dt > 0)) {
+            dgx_set_error("dgx_server_submit: a material model needs lambda, miu, den and dt > 0");
+            return DGX_EARG;
+        }
+        /* 1/rho | lambda | mu in device order, as dgx_build_mesh_host lays out the mesh's own model */
+        std::vector<double> mat(3 * T);
+        for (long e = 0; e < T; e++) {
+            if (!(rq->den[e] > 0)) {
+                dgx_set_error("dgx_server_submit: element %ld: density must be positive", e + 1);
+                return DGX_EARG;
+            }
+            const long q = md.inv[e];
+            mat[q] = 1 / rq->den[e];
+            mat[T + q] = rq->lambda[e];
+            mat[2 * T + q] = rq->miu[e];
+        }
+        const uint64_t h = model_hash(mat, rq->dt);
+        model = -1;
+        for (int k = 1; k < (int)md.models.size() && model < 0; k++)
+            if (md.models[k].used && md.models[k].hash == h && md.models[k].dt == rq->dt && md.models[k].mat == mat)
+                model = k;
+        if (model < 0) {
+            for (int k = 1; k < (int)md.models.size() && model < 0; k++)
+                if (!md.models[k].used) model = k;
+            if (model < 0) {
+                model = (int)md.models.size();
+                md.models.emplace_back();
+            }
+            ModelDev &nm = md.models[model];
+            nm = ModelDev();
+            nm.used = true;
+            nm.hash = h;
+            nm.mat = std::move(mat);
+            nm.dt = rq->dt;
+        }
+    }
     Request r;
     r.id = sv->next_id++;
     r.mesh = rq->mesh;
+    r.model = model;
     r.nshots = rq->nshots;
     r.nsteps = rq->nsteps;
     r.nrec = rq->nrec;
@@ -369,13 +481,16 @@ extern "C" int dgx_server_submit(dgx_server *sv, const dgx_request *rq, long *id
     if (rq->nrec) r.rec.assign(rq->rec, rq->rec + rq->nrec);
     r.arrival = rq->arrival;
     r.bytes = (long long)rq->nshots * rq->nsteps * (1 + 2 * rq->nrec) * 8;
-    const long long alone = r.bytes + groups_for(sv, rq->nshots) * group_bytes(sv, sv->meshes[rq->mesh]);
+    const long long alone = r.bytes + groups_for(sv, rq->nshots) * group_bytes(sv, md) +
+                            (model ? model_bytes(sv, md) : 0);
     if (alone > sv->budget) {
         dgx_set_error("dgx_server_submit: the request needs %lld bytes with its slot groups, more than the %lld-byte "
                       "budget", alone, sv->budget);
         sv->next_id--;
+        if (model && md.models[model].refs == 0) md.models[model] = ModelDev(); /* created for it: drop */
         return DGX_ENOMEM;
     }
+    md.models[model].refs++;
     *id = r.id;
     sv->queue.push_back(r.id);
     sv->reqs.emplace(r.id, std::move(r));
@@ -405,7 +520,7 @@ static int reclaim_empty(dgx_server *sv)
     return n;
 }
 
-static int new_batch(dgx_server *sv, int mesh, int *bi)
+static int new_batch(dgx_server *sv, int mesh, int model, int *bi)
 {
     const MeshDev &m = sv->meshes[mesh];
     int slot = -1;
@@ -414,6 +529,7 @@ static int new_batch(dgx_server *sv, int mesh, int *bi)
     if (slot < 0 && (long)sv->batches.size() >= MAX_BATCHES) return DGX_ENOMEM;
     Batch b;
     b.mesh = mesh;
+    b.model = model;
     b.G = sv->group;
     const int G = b.G;
     b.field_bytes = (long long)5 * group_rows(sv, m) * G * sv->rsize();
@@ -562,7 +678,8 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
     a.nbr = m.nbr;
     a.vtx = m.vtx;
     a.pt = m.pt;
-    const R *mat = (const R *)m.mat;
+    const ModelDev &md = m.models[b.model];
+    const R *mat = (const R *)md.d_mat;
     a.rho_inv = mat;
     a.lam = mat + m.tri;
     a.mu = mat + 2 * m.tri;
@@ -572,7 +689,7 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
     a.rec = b.d_rec;
     a.nsteps = 0x7fffffff;
     a.tri = (int)m.tri;
-    a.dt = m.dt;
+    a.dt = md.dt;
     a.spad = G;
     a.shot_tile = G;
     a.nshots = G;
@@ -643,11 +760,14 @@ static int admit(dgx_server *sv)
         long id = sv->queue.front();
         sv->queue.pop_front();
         Request &r = sv->reqs.at(id);
-        const MeshDev &m = sv->meshes[r.mesh];
-        /* this mesh's groups with free slots, fullest first */
+        MeshDev &m = sv->meshes[r.mesh];
+        ModelDev &md = m.models[r.model];
+        /* this mesh's groups of the request's material model with free slots, fullest first */
         std::vector<std::pair<int, int>> open; /* (free slots, batch) */
         int free_total = 0;
         for (int bi : m.batches) {
+            /* an empty group of another model of the mesh is memory for any of them */
+            if (sv->batches[bi].model != r.model && sv->batches[bi].used() > 0) continue;
             const int fr = G - sv->batches[bi].used();
             if (fr > 0) {
                 open.push_back({fr, bi});
@@ -657,7 +777,8 @@ static int admit(dgx_server *sv)
         std::stable_sort(open.begin(), open.end(),
                          [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first < y.first; });
         const int grow = groups_for(sv, std::max(0, r.nshots - free_total)); /* new groups needed */
-        long long need = r.bytes + grow * group_bytes(sv, m);
+        const long long mbytes = md.d_mat ? 0 : model_bytes(sv, m); /* the model comes to the device with it */
+        long long need = r.bytes + grow * group_bytes(sv, m) + mbytes;
         bool fits = sv->in_use + need <= sv->budget && batch_entries_free(sv) >= grow;
         if (!fits && grow > 0 && reclaim_empty(sv) > 0) {
             /* memory and entries held by idle groups (of any mesh) went back: look again */
@@ -665,31 +786,65 @@ static int admit(dgx_server *sv)
             continue;
         }
         if (!fits) { still.push_back(id); continue; }
+        /* nothing of this request is placed until every step that can fail has succeeded; on a failure it and
+         * the requests deferred or not yet looked at stay queued, in order, and its buffers go back to the pool
+         * (a group opened for it stays empty until reclaimed) */
+        auto requeue = [&](int rc) {
+            still.push_back(id);
+            for (long rest : sv->queue) still.push_back(rest);
+            sv->queue = still;
+            return rc;
+        };
+        if (!md.d_mat) {
+            void *d = nullptr;
+            if (upload_reals(sv, &d, md.mat) != hipSuccess) {
+                if (d) (void)hipFree(d);
+                (void)hipGetLastError();
+                dgx_set_error("upload of the material model of request %ld failed", id);
+                return requeue(DGX_ENOMEM);
+            }
+            md.d_mat = d;
+            sv->in_use += mbytes;
+        }
         for (int g = 0; g < grow; g++) {
             int bi = -1;
-            int rc = new_batch(sv, r.mesh, &bi);
-            if (rc) {
-                /* nothing of this request is placed yet: it and the requests deferred or not yet looked at
-                 * stay queued, in order (a group opened for it stays empty until reclaimed) */
-                still.push_back(id);
-                for (long rest : sv->queue) still.push_back(rest);
-                sv->queue = still;
-                return rc;
-            }
+            int rc = new_batch(sv, r.mesh, r.model, &bi);
+            if (rc) return requeue(rc);
             open.push_back({G, bi});
+        }
+        /* every shot's wavelet and trace buffer, and the wavelet uploads */
+        const size_t nw = (size_t)r.nsteps, nt = (size_t)r.nsteps * 2 * std::max<long>(r.nrec, 1);
+        std::vector<double *> ws, ts;
+        int rc = DGX_OK;
+        for (int k = 0; k < r.nshots && rc == DGX_OK; k++) {
+            double *w = pool_get(sv, nw);
+            if (w) ws.push_back(w);
+            double *t = w ? pool_get(sv, nt) : nullptr;
+            if (t) ts.push_back(t);
+            if (!w || !t) {
+                dgx_set_error("out of device memory for the buffers of request %ld", id);
+                rc = DGX_ENOMEM;
+            } else if (hipMemcpyAsync(w, &r.wav[(size_t)k * nw], sizeof(double) * nw, hipMemcpyHostToDevice,
+                                      sv->stream) != hipSuccess) {
+                dgx_set_error("wavelet upload of request %ld failed", id);
+                rc = DGX_EHIP;
+            }
+        }
+        if (rc != DGX_OK) {
+            (void)hipStreamSynchronize(sv->stream);
+            for (double *w : ws) pool_put(sv, w, nw);
+            for (double *t : ts) pool_put(sv, t, nt);
+            return requeue(rc);
         }
         int k = 0;
         for (const auto &ob : open) {
             if (k == r.nshots) break;
             Batch &b = sv->batches[ob.second];
+            b.model = r.model;
             unsigned long long mask = 0;
             for (int slot = 0; slot < G && k < r.nshots; slot++) {
                 if (b.req[slot] >= 0) continue;
-                double *w = pool_get(sv, r.nsteps);
-                double *t = pool_get(sv, (size_t)r.nsteps * 2 * std::max<long>(r.nrec, 1));
-                if (!w || !t) { dgx_set_error("out of device memory for request buffers"); return DGX_ENOMEM; }
-                SV_TRY(hipMemcpyAsync(w, &r.wav[(size_t)k * r.nsteps], sizeof(double) * r.nsteps,
-                                      hipMemcpyHostToDevice, sv->stream));
+                double *w = ws[k], *t = ts[k];
                 b.req[slot] = id;
                 b.shot[slot] = k;
                 b.h_start[slot] = (int)sv->global_step;
@@ -711,6 +866,7 @@ static int admit(dgx_server *sv)
             b.dirty = true;
         }
         sv->table_dirty = true;
+        md.live++;
         r.state = DGX_REQ_RUNNING;
         r.start = sv->global_step;
         r.t_admit = now_s();
@@ -750,6 +906,18 @@ static int retire(dgx_server *sv, int *completed)
         }
         r.d_wav.clear();
         r.d_trace.clear();
+        MeshDev &m = sv->meshes[r.mesh];
+        ModelDev &md = m.models[r.model];
+        md.live--;
+        if (r.model && md.live == 0 && md.d_mat) {
+            /* its groups hold no shot now (reclaimed when memory is needed); none is stepped again before a
+             * request of this model re-uploads it */
+            SV_TRY(hipStreamSynchronize(sv->stream));
+            SV_TRY(hipFree(md.d_mat));
+            md.d_mat = nullptr;
+            sv->in_use -= model_bytes(sv, m);
+        }
+        if (--md.refs == 0 && r.model) md = ModelDev();
         r.state = DGX_REQ_DONE;
         r.t_done = now_s();
         sv->in_use -= r.bytes;
@@ -769,11 +937,13 @@ static int retire(dgx_server *sv, int *completed)
 static int compact(dgx_server *sv)
 {
     const int G = sv->group;
-    for (int mi = 0; mi < (int)sv->meshes.size(); mi++) {
+    for (int mi = 0; mi < (int)sv->meshes.size(); mi++)
+    for (int mo = 0; mo < (int)sv->meshes[mi].models.size(); mo++) {
         const MeshDev &m = sv->meshes[mi];
-        std::vector<std::pair<int, int>> act; /* (used slots, batch) of groups that are stepped */
+        std::vector<std::pair<int, int>> act; /* (used slots, batch) of this model's groups that are stepped */
         int used = 0;
         for (int bi : m.batches) {
+            if (sv->batches[bi].model != mo) continue;
             const int u = sv->batches[bi].used();
             if (u > 0) {
                 act.push_back({u, bi});

@@ -61,7 +61,8 @@ class ServerConfig(C.Structure):
 
 class RequestC(C.Structure):
     _fields_ = [("mesh", C.c_int), ("nshots", C.c_int), ("nsteps", C.c_long), ("src", C.POINTER(Source)),
-                ("wavelet", dptr), ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("arrival", C.c_double)]
+                ("wavelet", dptr), ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("arrival", C.c_double),
+                ("lam", dptr), ("miu", dptr), ("den", dptr), ("dt", C.c_double)]
 
 
 class ServerStats(C.Structure):

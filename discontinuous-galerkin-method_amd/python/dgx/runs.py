@@ -4,7 +4,6 @@
     python -m dgx.runs slope     [--steps 5000 --shots 64]  batched FP32, FP64 reference on 4 shots
     python -m dgx.runs serving   [--requests 2000 --rate 20] open-loop request stream on 8 meshes
     python -m dgx.runs carbonate [--shots 256 --steps 8000] 256 shots through the request loop, FP64 on 4
-    python -m dgx.runs lanes     [--steps 400]              batched vs serving kernels at full slot occupancy
 
 The serving runs take --group (4, 8, 16, 32 or 64 shots per slot group; default 8 for the request stream of 1-16-shot
 requests over 8 meshes, 64 for carbonate's 16-shot requests on one mesh).
@@ -20,27 +19,9 @@ import time
 import numpy as np
 
 from . import configs
+from .accuracy import snapshot_errors, trace_errors
 from . import lib as L
 from .problem import SourceSpec, Solver, build_setup
-
-
-def rel(u, v):
-    u, v = np.asarray(u, np.float64), np.asarray(v, np.float64)
-    if not (np.all(np.isfinite(u)) and np.all(np.isfinite(v))):
-        return float("nan")
-    nv = np.linalg.norm(v)
-    return float(np.linalg.norm(u - v) / nv) if nv > 0 else 0.0
-
-
-def trace_errors(x32, z32, x64, z64, floor=1e-6):
-    """Relative L2 error over all traces, and the worst single trace among those whose FP64 energy is
-    above `floor` times the largest one (near-silent traces have no meaningful relative error)."""
-    allerr = max(rel(x32, x64), rel(z32, z64))
-    norms = np.sqrt((x64 ** 2).sum(0) + (z64 ** 2).sum(0))
-    keep = norms > floor * norms.max() if norms.size and norms.max() > 0 else np.zeros(0, bool)
-    worst = max([max(rel(x32[:, j], x64[:, j]), rel(z32[:, j], z64[:, j])) for j in np.nonzero(keep)[0]],
-                default=None)
-    return allerr, worst, int(keep.sum())
 
 
 def log(msg):
@@ -85,12 +66,10 @@ def run_gradient(a):
     x32, z32 = s32.traces(0)
     f32 = s32.fields(0)
     s32.close()
-    allerr, worst, nkept = trace_errors(x32, z32, x64, z64)
     dof = w.mesh.tri * 30 * a.steps
     return {"workload": w.name, "steps": a.steps, "dt": st.dt, "fp64_dof_updates_per_s": dof / sec64,
-            "fp32_dof_updates_per_s": dof / sec32, "fp32_trace_rel_l2_all": allerr,
-            "fp32_trace_rel_l2_worst_single_trace": worst, "traces_above_1e-6_of_max": nkept,
-            "fp32_snapshot_rel_l2": [rel(f32[c], f64[c]) for c in range(5)]}
+            "fp32_dof_updates_per_s": dof / sec32, "fp32_traces": trace_errors(x32, z32, x64, z64),
+            "fp32_snapshot": snapshot_errors(f32, f64)}
 
 
 def run_slope(a):
@@ -106,11 +85,10 @@ def run_slope(a):
         sec64 += t
         x32, z32 = s32.traces(k)
         f32 = s32.fields(k)
-        allerr, worst, nkept = trace_errors(x32, z32, x64, z64)
-        checks.append({"shot": k, "trace_rel_l2_all": allerr, "worst_single_trace": worst, "traces_kept": nkept,
-                       "snapshot_rel_l2": [rel(f32[c], f64[c]) for c in range(5)],
-                       "fp64_finite": bool(np.all(np.isfinite(f64))), "fp32_finite": bool(np.all(np.isfinite(f32))),
-                       "fp64_max_abs_v": float(np.nanmax(np.abs(f64[:2]))), "fp32_max_abs_v": float(np.nanmax(np.abs(f32[:2])))})
+        checks.append({"shot": k, "traces": trace_errors(x32, z32, x64, z64),
+                       "traces_floor_1e-4": trace_errors(x32, z32, x64, z64, floor=1e-4),
+                       "snapshot": snapshot_errors(f32, f64),
+                       "fp64_finite": bool(np.all(np.isfinite(f64))), "fp32_finite": bool(np.all(np.isfinite(f32)))})
         log(f"  shot {k}: {checks[-1]}")
     s32.close()
     dof = w.mesh.tri * 30 * a.shots * a.steps
@@ -159,55 +137,17 @@ def run_carbonate(a):
     for k in picks:
         ((x64, z64), _), _ = single_shot_reference(w, k, st.dt, a.steps)
         x32, z32 = res[k // per_req][0][k % per_req], res[k // per_req][1][k % per_req]
-        errs.append(trace_errors(x32, z32, x64, z64))
+        errs.append({"shot": k, "floor_1e-2": trace_errors(x32, z32, x64, z64),
+                     "floor_1e-4": trace_errors(x32, z32, x64, z64, floor=1e-4)})
     return {"workload": w.name, "elements": T, "shots": a.shots, "steps": a.steps, "dt": st.dt, "group_shots": a.group,
             "served_fp32_dof_updates_per_s": stats["dof_updates"] / wall, "wall_s": wall,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "fp32_vs_fp64_trace_rel_l2_per_checked_shot": errs}
 
 
-def run_lanes(a):
-    """The batched solver and the request loop on the same mesh (the 640k-element gradient model) and the
-    same shots, every slot running: batched at shot tile 64 and 16 (64 shots), batched with 16 shots,
-    and the server with one 64-shot request in a 64-shot group, four 16-shot requests in 16-shot groups
-    and one 16-shot request.  DOF-updates/s per case (the server's from its device time)."""
-    from . import serve as SV
-    w = configs.gradient_8x4km(a.steps + 50)
-    T, W, H = w.mesh.tri, 8000.0, 4000.0
-    out = {"workload": w.name, "elements": T, "steps": a.steps, "dtype": "f32"}
-
-    def shots(n):
-        return [SourceSpec(float(x), H - 500.0, L.DGX_SRC_EXPLOSIVE, k, 20.0)
-                for k, x in enumerate(np.linspace(500.0, W - 500.0, n))]
-    for S, ST in ((64, 64), (64, 16), (16, 16), (8, 8)):
-        st = build_setup(w.mesh, w.model, shots(S), w.receivers, w.nsteps, sponge=w.sponge)
-        s = Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, ST)
-        s.step(10)
-        s.synchronize()
-        t = time.perf_counter()
-        s.step(a.steps)
-        s.synchronize()
-        sec = time.perf_counter() - t
-        s.close()
-        out[f"batched_{S}shots_tile{ST}"] = T * 30 * S * a.steps / sec
-        log(f"batched {S} shots, tile {ST}: {out[f'batched_{S}shots_tile{ST}']:.3e} DOF-updates/s")
-    for G, nreq in ((64, 1), (16, 4), (16, 1), (8, 8), (8, 1)):
-        sv = SV.Server(L.DGX_FP32, 0, 100, group_shots=G)
-        rm = sv.add_mesh(w.mesh, w.model, w.sponge)
-        per = G if G < 64 else 64
-        reqs = [sv.make_request(rm, np.stack([np.linspace(500.0 + 10 * r, W - 500.0, per), np.full(per, H - 500.0)], 1),
-                                [20.0] * per, a.steps, w.receivers[:51]) for r in range(nreq)]
-        _, _, st, _ = SV.run_open_loop(sv, reqs, np.zeros(nreq), progress_every=0)
-        sv.close()
-        key = f"serving_group{G}_{nreq}x{per}shots"
-        out[key] = st["dof_updates"] / st["gpu_seconds"]
-        log(f"serving {nreq} x {per} shots in {G}-shot groups: {out[key]:.3e} DOF-updates/s")
-    return out
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["gradient", "slope", "serving", "carbonate", "lanes"])
+    ap.add_argument("which", choices=["gradient", "slope", "serving", "carbonate"])
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--shots", type=int, default=0)
     ap.add_argument("--requests", type=int, default=2000)
@@ -215,13 +155,11 @@ def main(argv=None):
     ap.add_argument("--group", type=int, default=0,
                     help="shots per slot group of the server, 4, 8, 16, 32 or 64 (default: 8 for serving, 64 for carbonate)")
     a = ap.parse_args(argv)
-    defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256),
-                "lanes": (400, 0)}
+    defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256)}
     a.steps = a.steps or defaults[a.which][0]
     a.shots = a.shots or defaults[a.which][1]
     a.group = a.group or (64 if a.which == "carbonate" else 8)
-    fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate,
-          "lanes": run_lanes}[a.which]
+    fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate}[a.which]
     print(json.dumps(fn(a)), flush=True)
 
 

@@ -76,26 +76,38 @@ class Server:
         return rm
 
     def make_request(self, rm: ResidentMesh, shots_xz, f0s, nsteps: int, receivers_xz, kind=L.DGX_SRC_EXPLOSIVE,
-                     pi: float = PI_REFERENCE):
-        """Locate sources and receivers and build the FP64 wavelet rows (one Ricker per shot)."""
+                     pi: float = PI_REFERENCE, model: Model | None = None, dt: float | None = None):
+        """Locate sources and receivers and build the FP64 wavelet rows (one Ricker per shot).  `model`: the
+        request's own material model (per element, caller order) instead of the mesh's; its dt is the CFL
+        step of that model unless `dt` is given."""
         n = len(shots_xz)
+        if model is not None and dt is None:
+            pt = np.ascontiguousarray(rm.mesh.point, np.float64)
+            tr = np.ascontiguousarray(rm.mesh.triangle, np.int64)
+            dt = self.lib.dgx_cfl_dt(rm.mesh.tri, L.dp(pt), L.lp(tr), L.dp(np.ascontiguousarray(model.vp, np.float64)))
+        step = rm.dt if model is None else float(dt)
         srcs = (L.Source * n)()
         wav = np.zeros((n, nsteps))
         for k, ((x, z), f0) in enumerate(zip(shots_xz, f0s)):
             L.check(self.lib.dgx_locate_source(rm.locator, C.byref(self.ops), float(x), float(z), kind, k, k,
                                                C.byref(srcs[k])), "locate_source")
-            t0 = self.lib.dgx_ricker_delay(rm.dt, float(f0), pi)
-            L.check(self.lib.dgx_wavelet(nsteps, rm.dt, t0, float(f0), pi, L.dp(wav[k])), "wavelet")
+            t0 = self.lib.dgx_ricker_delay(step, float(f0), pi)
+            L.check(self.lib.dgx_wavelet(nsteps, step, t0, float(f0), pi, L.dp(wav[k])), "wavelet")
         rxz = np.ascontiguousarray(np.asarray(receivers_xz, np.float64).reshape(-1, 2))
         recs = (L.Receiver * max(1, len(rxz)))()
         if len(rxz):
             L.check(self.lib.dgx_locate_receivers(rm.locator, len(rxz), L.dp(rxz), recs), "locate_receivers")
-        return dict(mesh=rm.mesh_id, srcs=srcs, wav=wav, recs=recs, nrec=len(rxz), nsteps=nsteps, nshots=n)
+        req = dict(mesh=rm.mesh_id, srcs=srcs, wav=wav, recs=recs, nrec=len(rxz), nsteps=nsteps, nshots=n, dt=step)
+        if model is not None:
+            req["model"] = [np.ascontiguousarray(a, np.float64) for a in (model.lam, model.miu, model.den)]
+        return req
 
     def submit(self, req: dict, arrival: float | None = None) -> int:
         arrival = self.now() if arrival is None else arrival
+        mats = req.get("model")
         r = L.RequestC(req["mesh"], req["nshots"], req["nsteps"], req["srcs"], L.dp(req["wav"]), req["nrec"],
-                       req["recs"], arrival)
+                       req["recs"], arrival, *([L.dp(a) for a in mats] if mats else [L.dptr()] * 3),
+                       req["dt"] if mats else 0.0)
         rid = C.c_long()
         L.check(self.lib.dgx_server_submit(self.handle, C.byref(r), C.byref(rid)), "server_submit")
         return rid.value

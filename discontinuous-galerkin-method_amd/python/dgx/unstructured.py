@@ -40,7 +40,7 @@ def _dist_to_polyline(pts, line):
 
 def triangulate(points: np.ndarray, *, sponge_box=None, attribute=None, min_area_frac: float = 1e-6) -> Mesh:
     pts = np.ascontiguousarray(points, np.float64)
-    tri = Delaunay(pts, qhull_options="QJ Qbb Qc" if False else "Qbb Qc").simplices.astype(np.int64)
+    tri = Delaunay(pts, qhull_options="Qbb Qc").simplices.astype(np.int64)
     v = pts[tri]
     a2 = (v[:, 1, 0] - v[:, 0, 0]) * (v[:, 2, 1] - v[:, 0, 1]) - (v[:, 2, 0] - v[:, 0, 0]) * (v[:, 1, 1] - v[:, 0, 1])
     flip = a2 < 0

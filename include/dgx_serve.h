@@ -46,6 +46,14 @@ typedef struct {
     long nrec;
     const dgx_receiver *rec; /* receivers shared by the request's shots (1-based caller element ids) */
     double arrival;          /* caller's clock (seconds) at which the request arrived, for latency */
+    /* The request's material model, per element of the mesh in the caller's numbering (the arrays the
+     * reference fills per element at DG_leap_Frog_OpenMP.c:236-265), and the time step its wavelet samples
+     * were taken at (the reference's Calculate_Time_Step, fun_DG.c:179-199, from this model's Vp).  All NULL:
+     * the mesh's own model and dt from dgx_server_add_mesh.  Requests with equal arrays and dt share one
+     * resident copy, which is on the device while one of them runs and counts against the HBM budget; the
+     * shots of one slot group all run on one model. */
+    const double *lambda, *miu, *den;
+    double dt;
 } dgx_request;
 
 typedef struct {

@@ -746,21 +746,29 @@ void orc_step(const orc_problem *p, long step, double *Vx, double *Vz, double *T
                   p->node_neighbor, p->neighbor, p->edge_flag);
 }
 
+/* Record (OpenMP_Function.c:61-72): rec_x[k] = Vx at receiver k, for every receiver of the problem.  [EXT] the
+ * value is sum_i weight[i] * V[elem][i] (P2 interpolation); a vertex receiver has one unit weight and reduces
+ * to the reference's nodal pick Vx[6(e-1) + n - 1]. */
+void orc_record(const orc_problem *p, const double *Vx, const double *Vz, double *rec_x, double *rec_z)
+{
+    for (long k = 0; k < p->nrec; k++) {
+        const orc_receiver *r = &p->rec[k];
+        const double *vx = &Vx[(r->elem - 1) * NP], *vz = &Vz[(r->elem - 1) * NP];
+        double ax = 0, az = 0;
+        for (int i = 0; i < NP; i++) {
+            ax += r->weight[i] * vx[i];
+            az += r->weight[i] * vz[i];
+        }
+        if (rec_x) rec_x[k] = ax;
+        if (rec_z) rec_z[k] = az;
+    }
+}
+
 void orc_run(const orc_problem *p, long step0, long nsteps, double *Vx, double *Vz, double *Txx, double *Tzz,
              double *Txz, double *rec_x, double *rec_z)
 {
     for (long t = 0; t < nsteps; t++) {
         orc_step(p, step0 + t, Vx, Vz, Txx, Tzz, Txz);
-        for (long k = 0; k < p->nrec; k++) {
-            const orc_receiver *r = &p->rec[k];
-            const double *vx = &Vx[(r->elem - 1) * NP], *vz = &Vz[(r->elem - 1) * NP];
-            double ax = 0, az = 0;
-            for (int i = 0; i < NP; i++) {
-                ax += r->weight[i] * vx[i];
-                az += r->weight[i] * vz[i];
-            }
-            if (rec_x) rec_x[t * p->nrec + k] = ax;
-            if (rec_z) rec_z[t * p->nrec + k] = az;
-        }
+        orc_record(p, Vx, Vz, rec_x ? rec_x + t * p->nrec : NULL, rec_z ? rec_z + t * p->nrec : NULL);
     }
 }

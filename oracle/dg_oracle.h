@@ -105,6 +105,7 @@ typedef struct {
 } orc_problem;
 
 /* Advance fields by nsteps starting at step index `step0`; traces rec_x/rec_z are [nsteps][nrec]. */
+void orc_record(const orc_problem *p, const double *Vx, const double *Vz, double *rec_x, double *rec_z);
 void orc_run(const orc_problem *p, long step0, long nsteps, double *Vx, double *Vz, double *Txx, double *Tzz,
              double *Txz, double *rec_x, double *rec_z);
 

@@ -49,7 +49,6 @@ class OrcProblem(C.Structure):
 
 
 _orc = None
-_ref = None
 
 
 def build():
@@ -85,25 +84,34 @@ def oracle() -> C.CDLL:
     return _orc
 
 
-def reference_available() -> bool:
-    return REF_SO.exists()
+def _ref_path(variant: str = "") -> pathlib.Path:
+    """oracle/_ref/libref.so (-O2 -fopenmp), or libref_O0.so for variant "O0" (the README's flag-less build)."""
+    return REF_SO if not variant else REF_SO.with_name(f"libref_{variant}.so")
 
 
-def reference() -> C.CDLL:
-    global _ref
-    if _ref is None:
-        if not REF_SO.exists():
-            raise FileNotFoundError(f"{REF_SO} not built (needs the reference checkout; see oracle/Makefile)")
-        _ref = C.CDLL(str(REF_SO))
-        _ref.ref_pi.restype = C.c_double
-        _ref.ref_macro.restype = C.c_long
-        _ref.ref_case.restype = C.c_int
-        _ref.ref_time_steps.restype = C.c_double
+def reference_available(variant: str = "") -> bool:
+    return _ref_path(variant).exists()
+
+
+_refs = {}
+
+
+def reference(variant: str = "") -> C.CDLL:
+    if variant not in _refs:
+        path = _ref_path(variant)
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (needs the reference checkout; see oracle/Makefile)")
+        r = C.CDLL(str(path))
+        r.ref_pi.restype = C.c_double
+        r.ref_macro.restype = C.c_long
+        r.ref_case.restype = C.c_int
+        r.ref_time_steps.restype = C.c_double
         for n in ("Calculate_Time_Step", "InDiameter_Min", "Ricker_Source", "Ricker_Delay_Time"):
-            getattr(_ref, n).restype = C.c_double
-        _ref.Ricker_Source.argtypes = [C.c_long, C.c_double, C.c_double]
-        _ref.Ricker_Delay_Time.argtypes = [C.c_double]
-    return _ref
+            getattr(r, n).restype = C.c_double
+        r.Ricker_Source.argtypes = [C.c_long, C.c_double, C.c_double]
+        r.Ricker_Delay_Time.argtypes = [C.c_double]
+        _refs[variant] = r
+    return _refs[variant]
 
 
 # ------------------------------------------------------------------------------------------------
@@ -271,6 +279,13 @@ class Run:
                             lp(lay.node_neighbor), lp(lay.neighbor), lp(lay.edge_flag), len(sources), self._src,
                             dp(self._wav), len(receivers), self._rec)
         self.step_index = 0
+
+    def record(self, f: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """The receivers' values (Record, OpenMP_Function.c:61-72, with interpolation weights) of fields f."""
+        rx, rz = np.zeros(max(1, self.nrec)), np.zeros(max(1, self.nrec))
+        f = np.ascontiguousarray(f, np.float64)
+        oracle().orc_record(C.byref(self.p), dp(f[0]), dp(f[1]), dp(rx), dp(rz))
+        return rx[:self.nrec], rz[:self.nrec]
 
     def run(self, n: int) -> tuple[np.ndarray, np.ndarray]:
         rx = np.zeros((n, max(1, self.nrec)))

@@ -1,0 +1,213 @@
+"""The measured workloads on their own meshes, at full size, on the GPU.
+
+Each test builds the workload's mesh and model with the same seeded generator the measurements use and
+checks the solver on it at a reduced step count:
+
+* gradient 8 km x 4 km, 640k jittered triangles: plain FP64 against the oracle (the reference's loops,
+  OpenMP_Function.c:25-306) to 1e-12 on traces and all five fields; FP32 against the oracle to 1e-4 on the
+  trace set and on every trace above the amplitude floor;
+* slope 6 km x 3 km, ~1.2M Delaunay triangles, water over three dipping layers, 64 FP32 shots in the
+  shot-innermost layout: sampled shots against FP64 single-shot GPU runs to 1e-4; one shot FP64 against the
+  oracle to 1e-12;
+* the eight resident serving meshes (20k - 2M triangles, random layered models): short requests on several
+  meshes are bit-identical alone and in the mix; one request served in FP64 matches the oracle to 1e-12;
+* carbonate 4 km x 3 km, ~2.4M triangles refined to 1 m around 300 vugs and 50 fractures: one shot FP64
+  against the oracle to 1e-12; shots served in FP32 against FP64 runs to 1e-4.
+
+Receiver sets include a cross through the source, so that traces carry the direct wave within the reduced
+step counts (the workloads' surface receivers see it only after thousands of steps).
+"""
+import numpy as np
+import pytest
+
+import dgx
+import dgx.serve as S
+from dgx import lib as L
+from dgx.accuracy import snapshot_errors, trace_errors
+from helpers import OracleCase, product_run, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-4
+FLOOR = 1e-2  # per-trace bar applies to traces whose FP64 norm is at least this fraction of the largest
+
+
+def cross(x, z, half=200.0, step=10.0):
+    """Receivers on a horizontal and a vertical line through (x, z)."""
+    t = np.arange(-half, half + step / 2, step)
+    return np.concatenate([np.stack([x + t, np.full(t.size, z)], 1), np.stack([np.full(t.size, x), z + t], 1)])
+
+
+def with_receivers(w, rec, nsteps):
+    return dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge, w.sources, rec, nsteps, 0)
+
+
+def check_fp32_traces(x, z, xr, zr, what):
+    e = trace_errors(x, z, xr, zr, floor=FLOOR)
+    print(f"{what}: {e}", flush=True)
+    assert e["all"] is not None and e["all"] <= FP32_TOL
+    assert e["worst"] is not None and e["worst"] <= FP32_TOL
+    return e
+
+
+@pytest.fixture(scope="module")
+def gradient():
+    w = dgx.configs.gradient_8x4km(1200)
+    src = w.sources[0]
+    rec = np.concatenate([w.receivers, cross(src.x, src.z)])
+    w = with_receivers(w, rec, 1200)
+    print(f"case {w.name}: {w.mesh.tri} elements, {len(rec)} receivers, 1200 steps", flush=True)
+    oc = OracleCase(w)
+    ox, oz, osnap = oc.run(snap_every=600)
+    return w, oc, ox, oz, osnap
+
+
+@pytest.mark.timeout(600)
+def test_gradient_640k_fp64_against_oracle(gradient):
+    w, oc, ox, oz, osnap = gradient
+    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
+    assert st.dt == oc.dt
+    gx, gz, gsnap = product_run(st, L.DGX_FP64, snap_every=600)
+    ex, ez = rel_l2(gx, ox), rel_l2(gz, oz)
+    print(f"gradient fp64 traces rel L2: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
+    assert ex <= FP64_TOL and ez <= FP64_TOL
+    for k, f in osnap.items():
+        for c in range(5):
+            e = rel_l2(gsnap[k][c], f[c])
+            assert e <= FP64_TOL, (k, c, e)
+
+
+@pytest.mark.timeout(600)
+def test_gradient_640k_fp32_against_oracle(gradient):
+    w, oc, ox, oz, osnap = gradient
+    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
+    gx, gz, gsnap = product_run(st, L.DGX_FP32, snap_every=600)
+    check_fp32_traces(gx, gz, ox, oz, "gradient fp32 vs oracle")
+    print("gradient fp32 snapshots:", snapshot_errors(gsnap[1200], osnap[1200]), flush=True)
+
+
+@pytest.mark.timeout(900)
+def test_slope_1200k_64_shots():
+    w = dgx.configs.slope_6x3km(1500, nshots=64)
+    print(f"case {w.name}: {w.mesh.tri} elements, 64 shots, 1500 steps", flush=True)
+    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
+    b = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER)
+    b.step(w.nsteps)
+    for k in (0, 21, 42, 63):
+        s = w.sources[k]
+        one = dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge,
+                                   [dgx.SourceSpec(s.x, s.z, s.kind, 0, s.f0)], w.receivers, w.nsteps, 0)
+        st1 = dgx.build_setup(one.mesh, one.model, one.sources, one.receivers, one.nsteps, sponge=one.sponge, dt=st.dt)
+        x64, z64, _ = product_run(st1, L.DGX_FP64)
+        x32, z32 = b.traces(k)
+        check_fp32_traces(x32, z32, x64, z64, f"slope shot {k} batched fp32 vs fp64")
+    b.close()
+    # one shot, FP64, against the oracle
+    s = w.sources[21]
+    one = dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge, [dgx.SourceSpec(s.x, s.z, s.kind, 0, s.f0)],
+                               np.concatenate([w.receivers[::10], cross(s.x, s.z - 60.0, 50.0)]), 150, 0)
+    oc = OracleCase(one)
+    ox, oz, osnap = oc.run(snap_every=150)
+    st1 = dgx.build_setup(one.mesh, one.model, one.sources, one.receivers, one.nsteps, sponge=one.sponge)
+    assert st1.dt == oc.dt
+    gx, gz, gsnap = product_run(st1, L.DGX_FP64, snap_every=150)
+    ex, ez = rel_l2(gx, ox), rel_l2(gz, oz)
+    print(f"slope shot 21 fp64 vs oracle: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
+    assert ex <= FP64_TOL and ez <= FP64_TOL
+    for c in range(5):
+        assert rel_l2(gsnap[150][c], osnap[150][c]) <= FP64_TOL, c
+
+
+@pytest.mark.timeout(900)
+def test_serving_eight_meshes():
+    ms = dgx.configs.serving_meshes()
+    print(f"case serving meshes {[m.tri for m, _, _ in ms]}", flush=True)
+    # (mesh, shots, steps, f0): one or more requests on the smallest, middle and largest meshes
+    specs = [(0, 2, 260, 20.0), (7, 3, 200, 12.0), (4, 1, 230, 25.0), (7, 1, 150, 30.0), (2, 4, 180, 15.0),
+             (5, 2, 210, 18.0)]
+
+    def make(sv, rms, k, spec):
+        mi, ns, nsteps, f0 = spec
+        mesh = ms[mi][0]
+        W, H = mesh.point[:, 0].max(), mesh.point[:, 1].max()
+        rng = np.random.default_rng(900 + k)
+        xz = np.stack([rng.uniform(150, W - 150, ns), H - rng.uniform(100, min(600, H - 200), ns)], 1)
+        rec = np.concatenate([np.stack([np.linspace(120.0, W - 120.0, 21), np.full(21, H - 5.0)], 1),
+                              cross(xz[0, 0], xz[0, 1], 60.0)])
+        return sv.make_request(rms[mi], xz, [f0] * ns, nsteps, rec), xz, rec
+
+    sv = S.Server(L.DGX_FP32, 0, 50, group_shots=4)
+    rms = [sv.add_mesh(m, md, sp) for m, md, sp in ms]
+    built = [make(sv, rms, k, spec) for k, spec in enumerate(specs)]
+    ids = [sv.submit(b[0]) for b in built[:3]]
+    sv.advance(1)
+    ids += [sv.submit(b[0]) for b in built[3:]]
+    sv.drain()
+    mixed = [sv.result(rid, built[k][0]) for k, rid in enumerate(ids)]
+    sv.close()
+    for k in (0, 1, 3, 5):
+        sv = S.Server(L.DGX_FP32, 0, 50, group_shots=4)
+        mi = specs[k][0]
+        rm = sv.add_mesh(*ms[mi])
+        rms1 = {mi: rm}
+        req, _, _ = make(sv, rms1, k, specs[k])
+        rid = sv.submit(req)
+        sv.drain()
+        ax, az = sv.result(rid, req)
+        sv.close()
+        assert np.array_equal(mixed[k][0], ax) and np.array_equal(mixed[k][1], az), k
+    # one request served in FP64 against the oracle (120k-element mesh)
+    sv = S.Server(L.DGX_FP64, 0, 50, group_shots=4)
+    rms = {2: sv.add_mesh(*ms[2])}
+    req, xz, rec = make(sv, rms, 4, (2, 2, 180, 15.0))
+    rid = sv.submit(req)
+    sv.drain()
+    x, z = sv.result(rid, req)
+    sv.close()
+    mesh, model, sp = ms[2]
+    srcs = [dgx.SourceSpec(float(a), float(b), L.DGX_SRC_EXPLOSIVE, s, 15.0) for s, (a, b) in enumerate(xz)]
+    oc = OracleCase(dgx.configs.Workload("serving_120k", mesh, model, sp, srcs, rec, 180, 0))
+    worst = 0.0
+    for s in range(len(xz)):
+        ox, oz, _ = oc.run(shot=s)
+        worst = max(worst, rel_l2(x[s], ox), rel_l2(z[s], oz))
+    print(f"serving 120k fp64 vs oracle: worst trace rel L2 {worst:.3e}", flush=True)
+    assert worst <= FP64_TOL
+
+
+@pytest.mark.timeout(900)
+def test_carbonate_2400k():
+    w = dgx.configs.carbonate_4x3km(1000, nshots=256)
+    print(f"case {w.name}: {w.mesh.tri} elements", flush=True)
+    # one shot FP64 against the oracle, 100 steps
+    s = w.sources[128]
+    one = dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge, [dgx.SourceSpec(s.x, s.z, s.kind, 0, s.f0)],
+                               np.concatenate([w.receivers[::8], cross(s.x, s.z, 20.0, 2.0)]), 100, 0)
+    oc = OracleCase(one)
+    ox, oz, osnap = oc.run(snap_every=100)
+    st1 = dgx.build_setup(one.mesh, one.model, one.sources, one.receivers, 100, sponge=one.sponge)
+    assert st1.dt == oc.dt
+    gx, gz, gsnap = product_run(st1, L.DGX_FP64, snap_every=100)
+    ex, ez = rel_l2(gx, ox), rel_l2(gz, oz)
+    print(f"carbonate fp64 vs oracle: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
+    assert ex <= FP64_TOL and ez <= FP64_TOL
+    for c in range(5):
+        assert rel_l2(gsnap[100][c], osnap[100][c]) <= FP64_TOL, c
+    # four shots served in FP32 (one request, 1000 steps) against FP64 single-shot runs
+    picks = [0, 85, 170, 255]
+    sv = S.Server(L.DGX_FP32, 0, 100, group_shots=4)
+    rm = sv.add_mesh(w.mesh, w.model, w.sponge)
+    xz = np.array([[w.sources[k].x, w.sources[k].z] for k in picks])
+    rec = np.concatenate([w.receivers] + [cross(a, b, 30.0, 3.0) for a, b in xz])
+    req = sv.make_request(rm, xz, [30.0] * 4, 1000, rec)
+    rid = sv.submit(req)
+    sv.drain()
+    x32, z32 = sv.result(rid, req)
+    sv.close()
+    for i, k in enumerate(picks):
+        s = w.sources[k]
+        st = dgx.build_setup(w.mesh, w.model, [dgx.SourceSpec(s.x, s.z, s.kind, 0, s.f0)], rec, 1000,
+                             sponge=w.sponge, dt=rm.dt)
+        x64, z64, _ = product_run(st, L.DGX_FP64)
+        check_fp32_traces(x32[i], z32[i], x64, z64, f"carbonate shot {k} served fp32 vs fp64")

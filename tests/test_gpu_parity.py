@@ -183,3 +183,38 @@ def test_shots_are_independent():
         x, z = both.traces(shot)
         assert np.array_equal(x, alone[0]) and np.array_equal(z, alone[1]), shot
     both.close()
+
+
+def test_split_record_and_explosive_source_match_oracle():
+    """dgx_record (Record, OpenMP_Function.c:61-72, with interpolation weights) on a random state against the
+    oracle's orc_record; dgx_source_kernel with an explosive source (fun*w into Txx and Tzz) against the
+    oracle's orc_source_kernel applied to both stresses, as orc_step does."""
+    import ctypes as C
+    w = two_media_workload(10)
+    w.sources = [dgx.SourceSpec(800.0, 1000.0, L.DGX_SRC_EXPLOSIVE, 0, 15.0)]
+    w.receivers = np.array([[1600.0, 1600.0], [333.3, 1211.7], [1000.0, 500.0], [1742.0, 1890.0]])
+    oc = OracleCase(w)
+    st = product_setup(w)
+    rng = np.random.default_rng(21)
+    T = w.mesh.tri
+    f0 = rng.standard_normal((5, T * 6)) * np.array([1, 1, 1e6, 1e6, 1e6])[:, None]
+    s = dgx.Solver(st, L.DGX_FP64)
+    s.set_fields(f0)
+    s.record(4)
+    x, z = s.traces(0)
+    run = O.Run(oc.lay, oc.dt, [], oc.wav[:1], oc.rec)
+    ox, oz = run.record(f0)
+    # the same six products summed in the same order; the device may fuse them into FMAs
+    assert rel_l2(x[4], ox) <= 1e-15 and rel_l2(z[4], oz) <= 1e-15
+    assert not np.any(x[:4]) and not np.any(x[5:])
+    e, kind, fun, _, _ = oc.src[0]
+    assert kind == L.DGX_SRC_EXPLOSIVE
+    s.source_kernel(-0.61)
+    ref = f0.copy()
+    for c in (2, 3):
+        O.oracle().orc_source_kernel(O.dp(ref[c]), C.c_long(e), O.dp(np.ascontiguousarray(fun)), C.c_double(-0.61))
+    g = s.fields()
+    assert np.array_equal(g[[0, 1, 4]], ref[[0, 1, 4]])
+    for c in (2, 3):
+        assert rel_l2(g[c], ref[c]) <= 1e-15, c
+    s.close()
