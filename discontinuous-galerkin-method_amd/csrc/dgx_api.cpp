@@ -56,8 +56,10 @@ extern "C" const char *dgx_last_error(void) { return g_err; }
 
 namespace {
 
-enum { K_VEL_FUSED = 0, K_STRESS_FUSED = 1, K_VEL = 2, K_STRESS = 3, K_ABSORB = 4, K_FULL = 5 };
-const int kGraphSteps = 32; /* steps captured per graph replay */
+enum { K_VEL_FUSED = 0, K_STRESS_FUSED = 1, K_VEL = 2, K_STRESS = 3, K_ABSORB = 4, K_FULL = 5, K_BLOCK = 6 };
+const int kGraphSteps = 32;    /* steps captured per graph replay (half-step and full-step kernels) */
+const int kGraphLaunches = 16; /* launches captured per graph replay of the blocked kernel (even: the graph
+                                * ends on the field copy it starts from) */
 
 template <typename T>
 hipError_t upload(T **dst, const std::vector<T> &src)
@@ -124,6 +126,11 @@ struct dgx_solver {
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; /* step graphs, by which field copy holds the state */
     int full = 0;           /* one full-step kernel per step (element layout) */
+    int blocked = 0;        /* blocked kernel: bkmax steps per launch (element layout) */
+    int bkmax = 0, bthreads = 0;
+    long bown = 0, blanes = 0; /* own elements and lanes over all tiles of the blocked kernel */
+    int *blevels = nullptr, *bgid = nullptr;
+    int4 *blnb = nullptr;
     int cur = 0;            /* full step: copy of the fields holding the current state */
     long vrows = 0;         /* element layout: rows per field and shot (elements, then ring rows) */
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -230,6 +237,95 @@ static void build_tiles(long tri, const std::vector<int4> &nbr, long vrow0, std:
     }
 }
 
+/*
+ * Tiles of the blocked kernel: runs of consecutive elements (locality order), each grown by bisection on its
+ * length to the longest run that fits `cap` threads together with every element within 2K face hops of it.
+ * Levels 0 .. 2K-1 are updated by the kernel; level 2K (ghosts) only supplies its stresses to the first
+ * velocity update of level 2K-1.  tiles[t] = {first, own, first lane record, lanes};
+ * levels[t*DGX_BLOCK_LEVELS + d] = end lane of level d, entries past level 2K equal to the lane count.  Lane
+ * records: the run's elements in order, then each level in increasing element id; gid = the element, lnb =
+ * {lane of face j's neighbour, code}, -1 on the boundary (ghost lanes: -1 everywhere, never read).  Every
+ * face neighbour of a level below 2K is in the tile.  Returns false if one element and its levels do not fit.
+ */
+bool dgx_build_block_tiles(long tri, const std::vector<int4> &nbr, int K, int cap, std::vector<int4> &tiles,
+                           std::vector<int> &levels, std::vector<int> &gid, std::vector<int4> &lnb)
+{
+    const int depth = 2 * K; /* the ghost level */
+    std::vector<int> stamp(tri, 0), slot(tri, -1);
+    int cur = 0;
+    std::vector<int> front, next;
+    std::vector<std::vector<int>> lv(depth + 1);
+    /* lanes of the run [q0, q0 + n) and its levels, or cap + 1 once they exceed cap */
+    auto lanes_of = [&](long q0, long n, bool keep) -> long {
+        cur++;
+        long total = n;
+        front.clear();
+        for (long q = q0; q < q0 + n; q++) {
+            stamp[q] = cur;
+            front.push_back((int)q);
+        }
+        for (int d = 1; d <= depth; d++) {
+            next.clear();
+            for (int e : front) {
+                const int nq[3] = {nbr[e].x, nbr[e].y, nbr[e].z};
+                for (int x : nq)
+                    if (x >= 0 && stamp[x] != cur) {
+                        stamp[x] = cur;
+                        next.push_back(x);
+                    }
+            }
+            total += (long)next.size();
+            if (total > cap) return cap + 1;
+            if (keep) {
+                lv[d] = next;
+                std::sort(lv[d].begin(), lv[d].end());
+            }
+            front.swap(next);
+        }
+        return total;
+    };
+    long q0 = 0, prev = 1;
+    while (q0 < tri) {
+        long lo = 1, hi = std::min<long>(cap, tri - q0);
+        if (lanes_of(q0, 1, false) > cap) return false;
+        if (prev <= hi) { /* tiles along the order have similar sizes: start from the previous one */
+            if (lanes_of(q0, prev, false) <= cap) lo = prev;
+            else hi = prev - 1;
+        }
+        while (lo < hi) {
+            const long mid = (lo + hi + 1) / 2;
+            if (lanes_of(q0, mid, false) <= cap) lo = mid;
+            else hi = mid - 1;
+        }
+        const long n = lo;
+        const long total = lanes_of(q0, n, true);
+        const int base = (int)gid.size();
+        for (long q = q0; q < q0 + n; q++) gid.push_back((int)q);
+        int lev[DGX_BLOCK_LEVELS];
+        lev[0] = (int)n;
+        for (int d = 1; d <= depth; d++) {
+            gid.insert(gid.end(), lv[d].begin(), lv[d].end());
+            lev[d] = lev[d - 1] + (int)lv[d].size();
+        }
+        for (int d = depth + 1; d < DGX_BLOCK_LEVELS; d++) lev[d] = (int)total;
+        levels.insert(levels.end(), lev, lev + DGX_BLOCK_LEVELS);
+        for (int l = 0; l < (int)total; l++) slot[gid[base + l]] = l;
+        for (int l = 0; l < (int)total; l++) {
+            const int e = gid[base + l];
+            int r[3] = {-1, -1, -1};
+            if (l < lev[depth - 1]) {
+                const int nq[3] = {nbr[e].x, nbr[e].y, nbr[e].z};
+                for (int j = 0; j < 3; j++) r[j] = nq[j] < 0 ? -1 : slot[nq[j]];
+            }
+            lnb.push_back(int4{r[0], r[1], r[2], nbr[e].w});
+        }
+        tiles.push_back(int4{(int)q0, (int)n, base, (int)total});
+        q0 += n;
+        prev = n;
+    }
+    return true;
+}
+
 static int launch(dgx_solver *s, int which, int step_off)
 {
     hipError_t e;
@@ -249,6 +345,10 @@ static int launch(dgx_solver *s, int which, int step_off)
     }
     if (which == K_FULL && !s->full) {
         dgx_set_error("the full-step kernel is not enabled for this solver (dgx_problem.step_kernel)");
+        return DGX_ESTATE;
+    }
+    if (which == K_BLOCK && !s->blocked) {
+        dgx_set_error("the blocked kernel is not enabled for this solver (dgx_problem.step_kernel)");
         return DGX_ESTATE;
     }
     if (s->fp32) {
@@ -301,6 +401,12 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     }
     a.fstride = s->vrows;
     a.nb2 = s->nb2;
+    a.blevels = s->blevels;
+    a.bgid = s->bgid;
+    a.blnb = s->blnb;
+    a.bkmax = s->bkmax;
+    a.bthreads = s->bthreads;
+    a.bsteps = s->bkmax;
     a.tiles = s->tiles;
     a.ring = s->ring;
     a.ntile = (int)s->ntile;
@@ -355,7 +461,8 @@ static void free_solver(dgx_solver *s)
     for (hipEvent_t e : {s->snap_ready, s->snap_done})
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {s->fields, s->fields2, s->nb2, s->tiles, s->ring, s->nbr, s->vtx, s->pt, s->mat, s->absorb,
-                    s->aux, s->src, s->rec, s->wav, s->trace, s->step_base, s->d_perm, s->d_snap};
+                    s->aux, s->src, s->rec, s->wav, s->trace, s->step_base, s->d_perm, s->d_snap,
+                    s->blevels, s->bgid, s->blnb};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_snap) (void)hipHostFree(s->h_snap);
@@ -594,17 +701,26 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         return DGX_EARG;
     }
     s->layout = (p->layout == DGX_LAYOUT_AUTO) ? (p->nshots >= 8 ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT) : p->layout;
-    if (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF && p->step_kernel != DGX_STEP_FULL) {
+    if (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF && p->step_kernel != DGX_STEP_FULL &&
+        p->step_kernel != DGX_STEP_BLOCKED) {
         dgx_set_error("dgx_create: unknown step_kernel %d", p->step_kernel);
         free_solver(s);
         return DGX_EARG;
     }
-    if (p->step_kernel == DGX_STEP_FULL && s->layout != DGX_LAYOUT_ELEMENT) {
-        dgx_set_error("dgx_create: the full-step kernel needs the element layout");
+    if ((p->step_kernel == DGX_STEP_FULL || p->step_kernel == DGX_STEP_BLOCKED) && s->layout != DGX_LAYOUT_ELEMENT) {
+        dgx_set_error("dgx_create: the full-step and blocked kernels need the element layout");
+        free_solver(s);
+        return DGX_EARG;
+    }
+    if (p->block_steps < 0 || p->block_steps > DGX_BLOCK_KMAX ||
+        (p->block_steps && p->step_kernel != DGX_STEP_BLOCKED)) {
+        dgx_set_error("dgx_create: block_steps must be 0 or 1..%d, with step_kernel DGX_STEP_BLOCKED", DGX_BLOCK_KMAX);
         free_solver(s);
         return DGX_EARG;
     }
     s->full = p->step_kernel == DGX_STEP_FULL;
+    s->blocked = p->step_kernel == DGX_STEP_BLOCKED;
+    s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : 2) : 0;
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         const int t = p->shot_tile;
         if (t != 0 && t != 4 && t != 8 && t != 16 && t != 32 && t != 64) {
@@ -733,6 +849,28 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         if (s->vrows * 6 >= (1L << 31)) { dgx_set_error("mesh too large for 32-bit row indices"); free_solver(s); return DGX_EARG; }
     }
 
+    /* ---- blocked-kernel tiles (code words carry their source/receiver bits by now) ---- */
+    std::vector<int4> btiles_host, blnb_host;
+    std::vector<int> blevels_host, bgid_host;
+    if (s->blocked) {
+        /* threads per workgroup of k_block: 512 (FP32: two workgroups per CU, so that one's loads overlap the
+         * other's arithmetic; measured 1.2-1.3x faster than one 1024-thread workgroup per CU despite the larger
+         * rings); DGX_BLOCK_THREADS=1024 selects 1024-thread FP32 tiles */
+        int cap = 512;
+        if (const char *bt = getenv("DGX_BLOCK_THREADS"))
+            if (s->fp32 && atoi(bt) == 1024) cap = 1024;
+        s->bthreads = cap;
+        if (!dgx_build_block_tiles(T, nbr_host, s->bkmax, cap, btiles_host, blevels_host, bgid_host, blnb_host)) {
+            dgx_set_error("dgx_create: an element and its %d levels of neighbours exceed %d threads", 2 * s->bkmax,
+                          cap);
+            free_solver(s);
+            return DGX_EMESH;
+        }
+        s->ntile = (long)btiles_host.size();
+        s->bown = T;
+        s->blanes = (long)bgid_host.size();
+    }
+
     /* ---- device allocations (all host-side validation is done above) ---- */
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -746,12 +884,20 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
                                                                  : (size_t)5 * s->nshots * s->vrows * 6;
     CREATE_TRY(hipMalloc(&s->fields, nreal_fields * s->real_size()));
     CREATE_TRY(hipMemset(s->fields, 0, nreal_fields * s->real_size()));
-    if (s->full) {
+    if (s->full || s->blocked) {
         CREATE_TRY(hipMalloc(&s->fields2, nreal_fields * s->real_size()));
         CREATE_TRY(hipMemset(s->fields2, 0, nreal_fields * s->real_size()));
+    }
+    if (s->full) {
         CREATE_TRY(upload(&s->tiles, tiles_host));
         CREATE_TRY(upload(&s->ring, ring_host));
         CREATE_TRY(upload(&s->nb2, nb2_host));
+    }
+    if (s->blocked) {
+        CREATE_TRY(upload(&s->tiles, btiles_host));
+        CREATE_TRY(upload(&s->blevels, blevels_host));
+        CREATE_TRY(upload(&s->bgid, bgid_host));
+        CREATE_TRY(upload(&s->blnb, blnb_host));
     }
     CREATE_TRY(upload(&s->nbr, nbr_host));
     CREATE_TRY(upload(&s->vtx, vtx_host));
@@ -827,6 +973,27 @@ extern "C" int dgx_full_step_tiling(const dgx_problem *p, long *ntile, long *nri
     return DGX_OK;
 }
 
+extern "C" int dgx_block_tiling(const dgx_problem *p, int K, int cap, long *ntile, long *nlanes, int4 *tiles,
+                                int *levels, int *gid, int4 *lnb, int4 *nbr)
+{
+    if (!p || p->tri <= 0 || K < 1 || K > DGX_BLOCK_KMAX || cap < 1) return DGX_EARG;
+    const long nblk = (p->tri + DGX_BLOCK - 1) / DGX_BLOCK;
+    const long tri_pad = (nblk + DGX_NXCD - 1) / DGX_NXCD * DGX_NXCD * DGX_BLOCK;
+    dgx_mesh_host mh;
+    if (int rc = dgx_build_mesh_host(p, 0, tri_pad, nullptr, &mh)) return rc;
+    std::vector<int4> t, ln;
+    std::vector<int> lv, g;
+    if (!dgx_build_block_tiles(p->tri, mh.nbr, K, cap, t, lv, g, ln)) return DGX_EMESH;
+    if (ntile) *ntile = (long)t.size();
+    if (nlanes) *nlanes = (long)g.size();
+    if (tiles) std::copy(t.begin(), t.end(), tiles);
+    if (levels) std::copy(lv.begin(), lv.end(), levels);
+    if (gid) std::copy(g.begin(), g.end(), gid);
+    if (lnb) std::copy(ln.begin(), ln.end(), lnb);
+    if (nbr) std::copy(mh.nbr.begin(), mh.nbr.begin() + p->tri, nbr);
+    return DGX_OK;
+}
+
 extern "C" int dgx_destroy(dgx_solver *s)
 {
     if (!s) return DGX_EARG;
@@ -853,26 +1020,47 @@ static int step_eager(dgx_solver *s)
     return DGX_OK;
 }
 
+/* One launch of the blocked kernel advancing nst steps from the step after s->step. */
+static int launch_block(dgx_solver *s, int step_off, int nst)
+{
+    s->a32.bsteps = s->a64.bsteps = nst;
+    const int rc = launch(s, K_BLOCK, step_off);
+    s->a32.bsteps = s->a64.bsteps = s->bkmax;
+    return rc;
+}
+
+/* Steps per graph replay: 32 for the half-step and full-step kernels, kGraphLaunches launches of bkmax steps
+ * for the blocked kernel. */
+static long graph_steps(const dgx_solver *s) { return s->blocked ? (long)kGraphLaunches * s->bkmax : kGraphSteps; }
+
 static int build_graph(dgx_solver *s)
 {
     /* K steps with offsets 0..K-1 from the device counter, then advance the counter by K.  A full-step
      * graph alternates the two copies; K is even, so it ends on the copy it starts from. */
-    static_assert(kGraphSteps % 2 == 0, "a full-step graph must end on the copy it starts from");
+    static_assert(kGraphSteps % 2 == 0 && kGraphLaunches % 2 == 0,
+                  "full-step and blocked graphs must end on the copy they start from");
     hipGraph_t g;
     const int c0 = s->cur;
     HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = DGX_OK;
-    for (int k = 0; k < kGraphSteps && rc == DGX_OK; k++) {
-        if (s->full) {
-            rc = launch(s, K_FULL, k);
+    if (s->blocked) {
+        for (int k = 0; k < kGraphLaunches && rc == DGX_OK; k++) {
+            rc = launch_block(s, k * s->bkmax, s->bkmax);
             swap_copies(s);
-        } else {
-            rc = launch(s, K_VEL_FUSED, k);
-            if (rc == DGX_OK) rc = launch(s, K_STRESS_FUSED, k);
+        }
+    } else {
+        for (int k = 0; k < kGraphSteps && rc == DGX_OK; k++) {
+            if (s->full) {
+                rc = launch(s, K_FULL, k);
+                swap_copies(s);
+            } else {
+                rc = launch(s, K_VEL_FUSED, k);
+                if (rc == DGX_OK) rc = launch(s, K_STRESS_FUSED, k);
+            }
         }
     }
     while (s->cur != c0) swap_copies(s); /* only after a failed launch */
-    if (rc == DGX_OK && dgx_launch_advance(s->step_base, kGraphSteps, s->stream) != hipSuccess) rc = DGX_EHIP;
+    if (rc == DGX_OK && dgx_launch_advance(s->step_base, (int)graph_steps(s), s->stream) != hipSuccess) rc = DGX_EHIP;
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (rc != DGX_OK) return rc;
     if (e != hipSuccess) { dgx_set_error("graph capture failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
@@ -889,8 +1077,9 @@ extern "C" int dgx_step(dgx_solver *s, long n)
         dgx_set_error("dgx_step: %ld + %ld steps exceed the capacity of %ld", s->step, n, s->nsteps);
         return DGX_ESTATE;
     }
+    const long gs = graph_steps(s);
     while (n > 0) {
-        if (n >= kGraphSteps) {
+        if (n >= gs) {
             /* a replay works on steps *step_base + 0..K-1: bring the device counter up to date first */
             if (s->step != s->base_host) {
                 if (dgx_launch_advance(s->step_base, (int)(s->step - s->base_host), s->stream) != hipSuccess) {
@@ -904,9 +1093,18 @@ extern "C" int dgx_step(dgx_solver *s, long n)
                 if (rc) return rc;
             }
             HIP_TRY(hipGraphLaunch(s->graph[s->cur], s->stream));
-            s->step += kGraphSteps;
-            s->base_host += kGraphSteps;
-            n -= kGraphSteps;
+            s->step += gs;
+            s->base_host += gs;
+            n -= gs;
+            continue;
+        }
+        if (s->blocked) {
+            const int nst = (int)std::min<long>(n, s->bkmax);
+            int rc = launch_block(s, (int)(s->step - s->base_host), nst);
+            if (rc) return rc;
+            swap_copies(s);
+            s->step += nst;
+            n -= nst;
             continue;
         }
         int rc = step_eager(s);
@@ -1149,9 +1347,10 @@ extern "C" int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms)
     /* n back-to-back launches of one fused kernel between two events: the kernel's own duration
      * without per-launch event overhead.  The launches repeat the current step, so the fields are
      * no longer a valid state afterwards. */
-    if (!s || n <= 0 || which < 0 || which > 2 || s->step >= s->nsteps) return DGX_EARG;
+    if (!s || n <= 0 || which < 0 || which > 3 || s->step >= s->nsteps) return DGX_EARG;
+    if (which == 3 && s->step + s->bkmax > s->nsteps) return DGX_EARG;
     int off = (int)(s->step - s->base_host);
-    int k = which == 0 ? K_VEL_FUSED : which == 1 ? K_STRESS_FUSED : K_FULL;
+    int k = which == 0 ? K_VEL_FUSED : which == 1 ? K_STRESS_FUSED : which == 2 ? K_FULL : K_BLOCK;
     int rc = launch(s, k, off); /* warm */
     if (rc) return rc;
     HIP_TRY(hipEventRecord(s->ev[0], s->stream));
@@ -1220,6 +1419,16 @@ extern "C" int dgx_tile_info(const dgx_solver *s, int *shot_tile, int *wg_waves,
         if (ms_per_step) ms_per_step[k] = s->tune_ms[k];
     }
     return n;
+}
+
+extern "C" int dgx_block_info(const dgx_solver *s, int *steps_per_launch, long *ntile, long *own, long *lanes)
+{
+    if (!s) return DGX_EARG;
+    if (steps_per_launch) *steps_per_launch = s->bkmax;
+    if (ntile) *ntile = s->blocked ? s->ntile : 0;
+    if (own) *own = s->bown;
+    if (lanes) *lanes = s->blanes;
+    return DGX_OK;
 }
 
 extern "C" int dgx_compiled_operators(const dgx_solver *s)

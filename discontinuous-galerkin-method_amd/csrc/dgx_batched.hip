@@ -20,6 +20,7 @@
 
 namespace dgxb {
 
+using dgx::as_global;
 using dgx::face_node;
 using dgx::Geom;
 using dgx::geometry;
@@ -27,16 +28,6 @@ using dgx::nb_node;
 using dgx::Ops;
 using dgx::StepArgs;
 
-/* Pointers read from a device-side table (the serving batch table, the per-slot wavelet and trace
- * tables) are generic, so every access through them would be a flat memory instruction.  They all point
- * to device memory (hipMalloc), where a global pointer has the same value as the generic one: rebuilding
- * each from its address as a global pointer lets the compiler use global instructions for every access
- * derived from it. */
-template <typename T>
-__device__ __forceinline__ T *as_global(T *p)
-{
-    return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
-}
 
 /* Element and shot of this lane (workgroups of NW wavefronts); false when outside the mesh. */
 template <int ST, int NW, typename R>

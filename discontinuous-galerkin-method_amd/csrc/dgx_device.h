@@ -12,8 +12,19 @@
 #define DGX_DEVICE_H
 
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace dgx {
+
+/* Pointers read from a device-side table (the serving batch table, the per-slot wavelet and trace tables) are
+ * generic, so every access through them would be a flat memory instruction.  They all point to device memory
+ * (hipMalloc), where a global pointer has the same value as the generic one: rebuilding each from its address
+ * as a global pointer lets the compiler use global instructions for every access derived from it. */
+template <typename T>
+__device__ __forceinline__ T *as_global(T *p)
+{
+    return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
 
 /* Local node of face j, position k, in the reference's face order (ibe, fun_DG.c:1048-1056). */
 __device__ __forceinline__ constexpr int face_node(int j, int k)
@@ -49,6 +60,24 @@ __device__ __forceinline__ Geom geometry(const double2 *pt, int4 v)
     g.sx = -g.ez[0] * t;
     g.sz = g.ex[0] * t;
     return g;
+}
+
+/* The geometric factors rounded once to the working precision R (the element kernels convert each factor
+ * with the same cast where they use it), for kernels that keep them in registers over several steps. */
+template <typename R>
+struct GeomR {
+    R rx, rz, sx, sz, invD;
+    R ex[3], ez[3];
+};
+
+template <typename R>
+__device__ __forceinline__ GeomR<R> geometry_r(const Geom &g)
+{
+    GeomR<R> r;
+    r.rx = (R)g.rx; r.rz = (R)g.rz; r.sx = (R)g.sx; r.sz = (R)g.sz; r.invD = (R)g.invD;
+#pragma unroll
+    for (int j = 0; j < 3; j++) { r.ex[j] = (R)g.ex[j]; r.ez[j] = (R)g.ez[j]; }
+    return r;
 }
 
 } // namespace dgx

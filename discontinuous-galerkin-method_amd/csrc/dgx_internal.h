@@ -34,4 +34,11 @@ extern "C" void dgx_set_error(const char *fmt, ...);
 extern "C" int dgx_full_step_tiling(const dgx_problem *p, long *ntile, long *nring, long *vrow0, long *vrows,
                                     int4 *tiles, int *ring, int4 *nb2, int4 *nbr);
 
+/* Host-only view of the blocked kernel's tiling of a mesh (element layout, K steps per launch, `cap` threads
+ * per workgroup): counts when the array pointers are NULL, else also tiles [ntile], levels
+ * [ntile * DGX_BLOCK_LEVELS], gid and lnb [nlanes] and the device-order neighbour records nbr [tri].  For tests
+ * of the tiling's invariants; not part of the C-ABI. */
+extern "C" int dgx_block_tiling(const dgx_problem *p, int K, int cap, long *ntile, long *nlanes, int4 *tiles,
+                                int *levels, int *gid, int4 *lnb, int4 *nbr);
+
 #endif

@@ -12,6 +12,8 @@
 #include <stdint.h>
 
 #define DGX_BLOCK 256          /* threads (= elements) per workgroup in the plain kernels */
+#define DGX_BLOCK_LEVELS 8     /* level table entries per tile of the blocked kernel (2 * steps + 1 <= 8) */
+#define DGX_BLOCK_KMAX 3       /* most steps per launch of the blocked kernel */
 #define DGX_NXCD 8             /* MI355X accelerator complex dies; workgroups are dealt round-robin */
 
 /* Per-element code word (the .w of the int4 neighbour record):
@@ -103,6 +105,14 @@ struct StepArgs {
     int ntile, tper;       /* tiles; tiles per XCD slice */
     int vrow0;             /* velocity row of ring entry 0 (rows below it are elements) */
     int p2;                /* element layout: op equals the compiled-in P2 table (dgx_p2ops.h) */
+    /* blocked kernel only (element layout, several steps per launch; tiles/ntile/tper above describe its
+     * tiles {first own element, own elements, first lane record, lanes}) */
+    const int *blevels;    /* per tile DGX_BLOCK_LEVELS ints: end lane of each level (hops from the own elements) */
+    const int *bgid;       /* per lane record: the element */
+    const int4 *blnb;      /* per lane record: {lane of face j's neighbour (j = 0..2), code}; -1 boundary */
+    int bkmax;             /* steps the tiles' levels allow per launch */
+    int bthreads;          /* threads per workgroup (lanes per tile at most) */
+    int bsteps;            /* steps of this launch (1..bkmax) */
 };
 
 } // namespace dgx

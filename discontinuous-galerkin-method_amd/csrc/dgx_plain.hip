@@ -187,8 +187,8 @@ __device__ __forceinline__ void condition_neighbour_stresses(const StepArgs<R> &
 
 /* Velocity volume term: rhx = Dr(rx Txx + rz Txz) + Ds(sx Txx + sz Txz), rhz = Dr(rx Txz + rz Tzz) +
  * Ds(sx Txz + sz Tzz). */
-template <typename R, bool P2>
-__device__ __forceinline__ void velocity_volume(const StepArgs<R> &a, const Geom &g, const R (&txx)[6],
+template <typename R, bool P2, typename G = Geom>
+__device__ __forceinline__ void velocity_volume(const StepArgs<R> &a, const G &g, const R (&txx)[6],
                                                 const R (&tzz)[6], const R (&txz)[6], R (&rhx)[6], R (&rhz)[6])
 {
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
@@ -206,10 +206,33 @@ __device__ __forceinline__ void velocity_volume(const StepArgs<R> &a, const Geom
     }
 }
 
+/* Face j of the velocity surface term: own stresses at the face's nodes (o*[k] at face_node(j, k)) and the
+ * neighbour's facing them (n*[k]; ignored on the outer boundary, has false), accumulated into flx, flz. */
+template <typename R, bool P2, typename G = Geom>
+__device__ __forceinline__ void velocity_flux_face(const StepArgs<R> &a, const G &g, int j, bool has,
+                                                   const R (&oxx)[3], const R (&ozz)[3], const R (&oxz)[3],
+                                                   const R (&nxx)[3], const R (&nzz)[3], const R (&nxz)[3],
+                                                   R (&flx)[6], R (&flz)[6])
+{
+    const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        R dxx = -oxx[k], dzz = -ozz[k], dxz = -oxz[k];
+        if (has) { dxx += nxx[k]; dzz += nzz[k]; dxz += nxz[k]; }
+        const R gx = ez * dxx - ex * dxz;
+        const R gz = ez * dxz - ex * dzz;
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            flx[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gx;
+            flz[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gz;
+        }
+    }
+}
+
 /* Velocity surface term before its 1/D scale: sum_j S_j (ez_j dTxx_j - ex_j dTxz_j) and
  * sum_j S_j (ez_j dTxz_j - ex_j dTzz_j), the jump dT = T_nb - T_own (T_nb = 0 on the outer boundary). */
-template <typename R, bool P2>
-__device__ __forceinline__ void velocity_flux(const StepArgs<R> &a, const Geom &g, const int (&nbs)[3],
+template <typename R, bool P2, typename G = Geom>
+__device__ __forceinline__ void velocity_flux(const StepArgs<R> &a, const G &g, const int (&nbs)[3],
                                               const R (&txx)[6], const R (&tzz)[6], const R (&txz)[6],
                                               const R (&gxx)[3][3], const R (&gzz)[3][3], const R (&gxz)[3][3],
                                               R (&flx)[6], R (&flz)[6])
@@ -218,21 +241,13 @@ __device__ __forceinline__ void velocity_flux(const StepArgs<R> &a, const Geom &
     for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        const bool has = nbs[j] >= 0;
-        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+        R oxx[3], ozz[3], oxz[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int on = face_node(j, k);
-            R dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
-            if (has) { dxx += gxx[j][k]; dzz += gzz[j][k]; dxz += gxz[j][k]; }
-            const R gx = ez * dxx - ex * dxz;
-            const R gz = ez * dxz - ex * dzz;
-#pragma unroll
-            for (int i = 0; i < 6; i++) {
-                flx[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gx;
-                flz[i] += op_l<P2>(a, j * 18 + i * 3 + k) * gz;
-            }
+            oxx[k] = txx[on]; ozz[k] = tzz[on]; oxz[k] = txz[on];
         }
+        velocity_flux_face<R, P2>(a, g, j, nbs[j] >= 0, oxx, ozz, oxz, gxx[j], gzz[j], gxz[j], flx, flz);
     }
 }
 
@@ -309,8 +324,8 @@ __device__ __forceinline__ void gather_velocities(unsigned code, const int (&nbs
 
 /* Strain-rate volume terms: exx = Dx Vx, ezz = Dz Vz, exz = Dx Vz + Dz Vx with Dx = rx Dr + sx Ds and
  * Dz = rz Dr + sz Ds (operators from the kernel argument in both variants). */
-template <typename R>
-__device__ __forceinline__ void strain_volume(const StepArgs<R> &a, const Geom &g, const R (&vx)[6],
+template <typename R, typename G = Geom, bool P2 = false>
+__device__ __forceinline__ void strain_volume(const StepArgs<R> &a, const G &g, const R (&vx)[6],
                                               const R (&vz)[6], R (&exx)[6], R (&ezz)[6], R (&exz)[6])
 {
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
@@ -319,10 +334,10 @@ __device__ __forceinline__ void strain_volume(const StepArgs<R> &a, const Geom &
         R rvx = 0, svx = 0, rvz = 0, svz = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            rvx += a.op.Dr[i * 6 + n] * vx[n];
-            svx += a.op.Ds[i * 6 + n] * vx[n];
-            rvz += a.op.Dr[i * 6 + n] * vz[n];
-            svz += a.op.Ds[i * 6 + n] * vz[n];
+            rvx += op_dr<P2>(a, i * 6 + n) * vx[n];
+            svx += op_ds<P2>(a, i * 6 + n) * vx[n];
+            rvz += op_dr<P2>(a, i * 6 + n) * vz[n];
+            svz += op_ds<P2>(a, i * 6 + n) * vz[n];
         }
         exx[i] = rx * rvx + sx * svx;
         ezz[i] = rz * rvz + sz * svz;
@@ -332,38 +347,45 @@ __device__ __forceinline__ void strain_volume(const StepArgs<R> &a, const Geom &
 
 /* Strain-rate surface terms before their 1/D scale: sum_j S_j(ez_j dVx_j), sum_j S_j(-ex_j dVz_j) and
  * sum_j S_j(ez_j dVz_j - ex_j dVx_j). */
-template <typename R, bool P2>
-__device__ __forceinline__ void stress_flux(const StepArgs<R> &a, const Geom &g, const int (&nbs)[3],
+/* Face j of the strain-rate surface terms (own velocities vx, vz; the neighbour's facing face j's nodes in
+ * gvx[k], gvz[k], ignored on the outer boundary), accumulated into fx, fz, fxz. */
+template <typename R, bool P2, typename G = Geom>
+__device__ __forceinline__ void stress_flux_face(const StepArgs<R> &a, const G &g, int j, bool has, const R (&vx)[6],
+                                                 const R (&vz)[6], const R (&gvx)[3], const R (&gvz)[3], R (&fx)[6],
+                                                 R (&fz)[6], R (&fxz)[6])
+{
+    const R ex = (R)g.ex[j], ez = (R)g.ez[j];
+    R px[3], pz[3], pxz[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int on = face_node(j, k);
+        R dvx = -vx[on], dvz = -vz[on];
+        if (has) { dvx += gvx[k]; dvz += gvz[k]; }
+        px[k] = ez * dvx;
+        pz[k] = -ex * dvz;
+        pxz[k] = ez * dvz - ex * dvx;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const R l = op_l<P2>(a, j * 18 + i * 3 + k);
+            fx[i] += l * px[k];
+            fz[i] += l * pz[k];
+            fxz[i] += l * pxz[k];
+        }
+    }
+}
+
+template <typename R, bool P2, typename G = Geom>
+__device__ __forceinline__ void stress_flux(const StepArgs<R> &a, const G &g, const int (&nbs)[3],
                                             const R (&vx)[6], const R (&vz)[6], const R (&gvx)[3][3],
                                             const R (&gvz)[3][3], R (&fx)[6], R (&fz)[6], R (&fxz)[6])
 {
 #pragma unroll
     for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        const bool has = nbs[j] >= 0;
-        const R ex = (R)g.ex[j], ez = (R)g.ez[j];
-        R px[3], pz[3], pxz[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int on = face_node(j, k);
-            R dvx = -vx[on], dvz = -vz[on];
-            if (has) { dvx += gvx[j][k]; dvz += gvz[j][k]; }
-            px[k] = ez * dvx;
-            pz[k] = -ex * dvz;
-            pxz[k] = ez * dvz - ex * dvx;
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const R l = op_l<P2>(a, j * 18 + i * 3 + k);
-                fx[i] += l * px[k];
-                fz[i] += l * pz[k];
-                fxz[i] += l * pxz[k];
-            }
-        }
-    }
+    for (int j = 0; j < 3; j++) stress_flux_face<R, P2>(a, g, j, nbs[j] >= 0, vx, vz, gvx[j], gvz[j], fx, fz, fxz);
 }
 
 /* Txx += dt[(l+2m) exx + l ezz], Tzz += dt[l exx + (l+2m) ezz], Txz += dt m exz for the 6 nodes of element
@@ -730,6 +752,298 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
                                a.g[3] + so, a.g[4] + so);
 }
 
+/* Wavelet sample of a source at local step `lstep`: the solver's wavelet table row src.wavelet, or in a
+ * serving batch (SERVE) the source's slot row. */
+template <typename R, bool SERVE>
+__device__ __forceinline__ double block_wavelet(const StepArgs<R> &a, const dgx_src_dev &src, int lstep)
+{
+    if constexpr (SERVE) return as_global(a.slot_wav[src.shot])[lstep];
+    else return a.wav[(long long)src.wavelet * a.nsteps + lstep];
+}
+
+/* Sponge factors (ab, loaded once per launch) and the element's sources (aux record ax, 0 = none) at local
+ * step `lstep` applied to its stresses and, with VEL, its velocities: Absorb_Boundary then source_kernel
+ * (OpenMP_Function.c:25-58); explosive sources add to Txx and Tzz, force sources to Vz. */
+template <typename R, bool VEL, bool SERVE>
+__device__ __forceinline__ void condition_block(const StepArgs<R> &a, unsigned code, unsigned ax, int shot, int lstep,
+                                                const R (&ab)[6], R (&vx)[6], R (&vz)[6], R (&txx)[6],
+                                                R (&tzz)[6], R (&txz)[6])
+{
+    if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            txx[i] *= ab[i]; tzz[i] *= ab[i]; txz[i] *= ab[i];
+            if (VEL) { vx[i] *= ab[i]; vz[i] *= ab[i]; }
+        }
+    }
+    if (ax) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int k = x.src_begin; k < x.src_begin + x.src_count; k++) {
+            const dgx_src_dev &src = a.src[k];
+            if (src.shot != shot || (!VEL && src.kind != 1)) continue;
+            const double w = block_wavelet<R, SERVE>(a, src, lstep);
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const R inc = (R)(src.fun[i] * w);
+                if (src.kind == 1) { txx[i] += inc; tzz[i] += inc; }
+                else if (VEL) vz[i] += inc;
+            }
+        }
+    }
+}
+
+/* Receivers of the element (aux record ax) from the velocities it holds (Record, OpenMP_Function.c:61-72, with
+ * interpolation weights): trace slot lstep of the solver's shot, or of the serving slot's trace buffer. */
+template <typename R, bool SERVE>
+__device__ __forceinline__ void record_block(const StepArgs<R> &a, unsigned ax, int shot, int lstep, const R (&vx)[6],
+                                             const R (&vz)[6])
+{
+    if (!ax || (!SERVE && lstep >= a.nsteps)) return;
+    const dgx_aux x = a.aux[ax - 1];
+    for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
+        const dgx_rec_dev &rc = a.rec[r];
+        if (SERVE && rc.pad0 != shot) continue;
+        double sx_ = 0, sz_ = 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            sx_ += rc.w[i] * (double)vx[i];
+            sz_ += rc.w[i] * (double)vz[i];
+        }
+        const int nrec = SERVE ? a.slot_nrec[shot] : a.nrec;
+        double *t = SERVE ? as_global(a.slot_trace[shot]) + (long long)lstep * 2 * nrec
+                          : a.trace + ((long long)shot * a.nsteps + lstep) * 2 * nrec;
+        t[rc.index] = sx_;
+        t[nrec + rc.index] = sz_;
+    }
+}
+
+/*
+ * Several whole leap-frog steps of one tile (element layout), with the fields held on chip between them.
+ * Tile t holds `own` consecutive elements in the locality order (level 0) and every element within 2*bkmax
+ * face hops of them (level d = d hops), one element per thread.  Each thread reads its element's fields from f
+ * once; nst steps follow (global steps step0.., local steps lstep0.. for the wavelet and traces), step j being:
+ *   every element conditions its values (sponge, sources: Absorb_Boundary and source_kernel,
+ *   OpenMP_Function.c:25-58) and stages its stresses in LDS; barrier;
+ *   elements of level <= 2 nst - 1 - 2j update their velocities (LU_Velocity, :75-185), face neighbours read
+ *   from LDS, and stage them; barrier;
+ *   elements of level <= 2 nst - 2 - 2j update their stresses (LU_Stress_Elastic, :188-306); level 0 records
+ *   its receivers (Record, :61-72);
+ * then the own elements' fields go to g.  Level 2*bkmax (ghosts) holds only stresses, conditioned at the
+ * first step, for the first velocity update of the level inside it.  The outer levels are recomputed from the
+ * same inputs with the same code as their owning tile's, so tiles never wait on each other; f is only read
+ * and every row of g is written by one workgroup.  Per launch each field is read once for every thread's
+ * element and written once for every own element, against 7 reads and writes per step and element for the
+ * half-step kernel pair.  Every thread of the workgroup reaches every barrier.
+ */
+template <typename R, int NT, bool P2, bool SERVE>
+__device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], R *const gout[5], int t, int shot,
+                                           int nst, int lstep0, R *sts, R *svl)
+{
+    const int4 td = a.tiles[t];
+    const int l = threadIdx.x;
+    const int ghost = 2 * a.bkmax;
+    const long long so = (long long)shot * a.fstride * 6;
+    int lev = ghost + 1, q = 0; /* lev > ghost: no element (threads past the tile's lanes) */
+    int4 nb = make_int4(-1, -1, -1, 0);
+    if (l < td.w) {
+        const int *le = a.blevels + (long long)t * DGX_BLOCK_LEVELS;
+        lev = 0;
+        while (lev < ghost && l >= le[lev]) lev++;
+        q = a.bgid[td.z + l];
+        nb = a.blnb[td.z + l];
+    }
+    const unsigned code = (unsigned)nb.w;
+    unsigned ax = 0; /* aux record + 1 of the element's sources and receivers */
+    if (lev <= ghost) ax = SERVE ? ((unsigned)a.aux_of_elem[q] & DGX_DYN_AUX_MASK) : (code >> DGX_CODE_AUX_SHIFT);
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+    const bool upd = lev < ghost; /* element updated by this tile (ghosts only supply stresses) */
+
+    /* the stresses live in registers from their update to the next step's conditioning and in LDS (sts) in
+     * between; the velocities stay in registers */
+    R vx[6], vz[6], txx[6], tzz[6], txz[6], ab[6];
+    GeomR<R> g;
+    R rho_inv = 0, lam = 0, mu = 0;
+    if (lev <= ghost) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const int o = fidx(q, i);
+            txx[i] = f[2][so + o]; tzz[i] = f[3][so + o]; txz[i] = f[4][so + o];
+            ab[i] = (code & DGX_CODE_SPONGE) ? a.absorb[o] : (R)1;
+        }
+    }
+    if (upd) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const int o = fidx(q, i);
+            vx[i] = f[0][so + o]; vz[i] = f[1][so + o];
+        }
+        g = geometry_r<R>(geometry(a.pt, a.vtx[q]));
+        rho_inv = a.rho_inv[q];
+        lam = a.lam[q];
+        mu = a.mu[q];
+    }
+    for (int j = 0; j < nst; j++) {
+        const int lstep = lstep0 + j;
+        if (upd || (lev == ghost && j == 0)) {
+            if (upd) condition_block<R, true, SERVE>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz);
+            else condition_block<R, false, SERVE>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz);
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                sts[(0 * 6 + i) * NT + l] = txx[i];
+                sts[(1 * 6 + i) * NT + l] = tzz[i];
+                sts[(2 * 6 + i) * NT + l] = txz[i];
+            }
+        }
+        __syncthreads();
+        if (lev <= 2 * nst - 1 - 2 * j) {
+            R rhx[6], rhz[6], flx[6], flz[6];
+            velocity_volume<R, P2>(a, g, txx, tzz, txz, rhx, rhz);
+#pragma unroll
+            for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
+#pragma unroll
+            for (int fc = 0; fc < 3; fc++) {
+                const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
+                const int s = nbs[fc];
+                R oxx[3], ozz[3], oxz[3], nxx[3], nzz[3], nxz[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int on = face_node(fc, k), nn = nb_node(m, rev, k);
+                    oxx[k] = txx[on]; ozz[k] = tzz[on]; oxz[k] = txz[on];
+                    nxx[k] = 0; nzz[k] = 0; nxz[k] = 0;
+                    if (s >= 0) {
+                        nxx[k] = sts[(0 * 6 + nn) * NT + s];
+                        nzz[k] = sts[(1 * 6 + nn) * NT + s];
+                        nxz[k] = sts[(2 * 6 + nn) * NT + s];
+                    }
+                }
+                velocity_flux_face<R, P2>(a, g, fc, s >= 0, oxx, ozz, oxz, nxx, nzz, nxz, flx, flz);
+            }
+            const R c = rho_inv * (R)a.dt;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                vx[i] = (rhx[i] + flx[i] * g.invD) * c + vx[i];
+                vz[i] = (rhz[i] + flz[i] * g.invD) * c + vz[i];
+            }
+        }
+        if (upd) {
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                svl[(0 * 6 + i) * NT + l] = vx[i];
+                svl[(1 * 6 + i) * NT + l] = vz[i];
+            }
+        }
+        __syncthreads();
+        if (lev <= 2 * nst - 2 - 2 * j) {
+            R exx[6], ezz[6], exz[6], fx[6], fz[6], fxz[6];
+            strain_volume<R, GeomR<R>, P2>(a, g, vx, vz, exx, ezz, exz);
+#pragma unroll
+            for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
+#pragma unroll
+            for (int fc = 0; fc < 3; fc++) {
+                const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
+                const int s = nbs[fc];
+                R gvx[3], gvz[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int nn = nb_node(m, rev, k);
+                    gvx[k] = 0; gvz[k] = 0;
+                    if (s >= 0) {
+                        gvx[k] = svl[(0 * 6 + nn) * NT + s];
+                        gvz[k] = svl[(1 * 6 + nn) * NT + s];
+                    }
+                }
+                stress_flux_face<R, P2>(a, g, fc, s >= 0, vx, vz, gvx, gvz, fx, fz, fxz);
+            }
+            const R l2m = lam + 2 * mu, dt = (R)a.dt;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                const R axx = exx[i] + fx[i] * g.invD, azz = ezz[i] + fz[i] * g.invD, axz = exz[i] + fxz[i] * g.invD;
+                txx[i] = (l2m * axx + lam * azz) * dt + txx[i];
+                tzz[i] = (lam * axx + l2m * azz) * dt + tzz[i];
+                txz[i] = (mu * axz) * dt + txz[i];
+            }
+            if (lev == 0) record_block<R, SERVE>(a, ax, shot, lstep, vx, vz);
+        }
+    }
+    if (lev == 0) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const int o = fidx(q, i);
+            gout[0][so + o] = vx[i]; gout[1][so + o] = vz[i];
+            gout[2][so + o] = txx[i]; gout[3][so + o] = tzz[i]; gout[4][so + o] = txz[i];
+        }
+    }
+}
+
+/* The solver's blocked kernel: workgroup (b, shot) runs tile (b mod 8) * tper + b / 8 (XCD-contiguous runs of
+ * tiles) of that shot for bsteps steps.  MINW: at least that many wavefronts per SIMD (512-thread tiles in
+ * FP32: 4, two workgroups per CU). */
+template <typename R, int NT, int MINW, bool P2>
+__global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
+{
+    __shared__ R sts[18 * NT]; /* conditioned stresses, (c*6 + i)*NT + lane */
+    __shared__ R svl[12 * NT]; /* new velocities */
+    const int b = blockIdx.x;
+    const int t = (b & (DGX_NXCD - 1)) * a.tper + (b >> 3);
+    if (t >= a.ntile) return; /* uniform over the workgroup */
+    const int step0 = *a.step_base + a.step_off;
+    block_tile<R, NT, P2, false>(a, a.f, a.g, t, blockIdx.y, a.bsteps, step0, sts, svl);
+}
+
+/*
+ * Serving with single-shot slots: one launch advances up to bkmax steps of every active slot (a batch of one
+ * shot in the element layout) over the concatenated tiles of all batches: sched = {nbatch, total tiles, tile
+ * offset of batch 0, ..., of batch nbatch}.  Workgroup b takes tile (b mod 8) * nper + b / 8 of that list, its
+ * batch by a scalar binary search; the batch's slot runs min(bkmax, steps left) steps from its local step.
+ * Launch `launch` of a block reads copy (launch & 1) of the batch's fields and writes the other.
+ */
+template <typename R, int NT, int MINW, bool P2>
+__global__ void __launch_bounds__(NT, MINW) ks_block(const StepArgs<R> *__restrict__ table,
+                                                    const int *__restrict__ sched, const int *step_base,
+                                                    int step_off, int launch, int nb, int total, int nper)
+{
+    __shared__ R sts[18 * NT];
+    __shared__ R svl[12 * NT];
+    const int tile = (int)(blockIdx.x & (DGX_NXCD - 1)) * nper + (int)(blockIdx.x >> 3);
+    if (tile >= total) return;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sched[2 + mid] <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    /* the batch's arguments, with the pointers the tile reads made global (device memory) */
+    StepArgs<R> a = table[lo];
+    a.tiles = as_global(a.tiles);
+    a.blevels = as_global(a.blevels);
+    a.bgid = as_global(a.bgid);
+    a.blnb = as_global(a.blnb);
+    a.vtx = as_global(a.vtx);
+    a.pt = as_global(a.pt);
+    a.rho_inv = as_global(a.rho_inv);
+    a.lam = as_global(a.lam);
+    a.mu = as_global(a.mu);
+    a.absorb = as_global(a.absorb);
+    a.aux = as_global(a.aux);
+    a.src = as_global(a.src);
+    a.rec = as_global(a.rec);
+    a.aux_of_elem = as_global(a.aux_of_elem);
+    a.slot_wav = as_global(a.slot_wav);
+    a.slot_trace = as_global(a.slot_trace);
+    a.slot_nrec = as_global(a.slot_nrec);
+    const int lstep0 = *step_base + step_off - as_global(a.shot_start)[0];
+    const int left = as_global(a.shot_len)[0] - lstep0;
+    const int nst = left < a.bsteps ? left : a.bsteps;
+    if (lstep0 < 0 || nst <= 0) return; /* slot not running in this launch: uniform over the workgroup */
+    R *f[5], *gout[5];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        f[c] = as_global((launch & 1) ? a.g[c] : a.f[c]);
+        gout[c] = as_global((launch & 1) ? a.f[c] : a.g[c]);
+    }
+    block_tile<R, NT, P2, true>(a, f, gout, tile - sched[2 + lo], 0, nst, lstep0, sts, svl);
+}
+
 /* ---- split entry points (unfused reference step functions) ---- */
 
 template <typename R>
@@ -871,6 +1185,16 @@ static void launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st
     case 3: dgx::k_stress<R, false, P2><<<grid, block, 0, st>>>(a); break;
     case 4: dgx::k_absorb<R><<<grid, block, 0, st>>>(a); break;
     case 5: dgx::k_fullstep<R, P2><<<dim3(DGX_NXCD * a.tper, nshots), block, 0, st>>>(a); break;
+    case 6:
+        /* one element per thread, a.bthreads threads (the tiles are built for that count): FP32 1024 (one
+         * workgroup per CU) or 512 (two), FP64 512 */
+        if constexpr (sizeof(R) == 4) {
+            if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 1024, 0, st>>>(a);
+            else dgx::k_block<R, 512, 4, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
+        } else {
+            dgx::k_block<R, 512, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
+        }
+        break;
     }
 }
 
@@ -919,6 +1243,24 @@ hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double
     const unsigned blocks = (unsigned)((tri + 255) / 256);
     if (fp32) dgx::k_unpermute<float><<<blocks, 256, 0, st>>>((const float *)f, perm, out, (int)tri, spad, il, shot);
     else dgx::k_unpermute<double><<<blocks, 256, 0, st>>>((const double *)f, perm, out, (int)tri, spad, il, shot);
+    return hipGetLastError();
+}
+/* Serving launch of the blocked kernel over the tiles of nb single-shot batches (ks_block). */
+hipError_t dgx_launch_serve_block(int fp32, int p2, const void *table, const int *sched, const int *step_base,
+                                  int step_off, int launch, int nb, int total, hipStream_t st)
+{
+    if (total <= 0) return hipSuccess;
+    const int nper = (total + DGX_NXCD - 1) / DGX_NXCD;
+    const dim3 grid(DGX_NXCD * nper);
+    if (fp32) {
+        const auto *t = (const dgx::StepArgs<float> *)table;
+        if (p2) dgx::ks_block<float, 512, 4, true><<<grid, 512, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
+        else dgx::ks_block<float, 512, 4, false><<<grid, 512, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
+    } else {
+        const auto *t = (const dgx::StepArgs<double> *)table;
+        if (p2) dgx::ks_block<double, 512, 1, true><<<grid, 512, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
+        else dgx::ks_block<double, 512, 1, false><<<grid, 512, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
+    }
     return hipGetLastError();
 }
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st)

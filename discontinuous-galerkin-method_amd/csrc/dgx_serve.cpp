@@ -16,6 +16,7 @@
 #include "dgx.h"
 #include "dgx_internal.h"
 #include "dgx_kernels.h"
+#include "dgx_p2ops.h"
 #include "dgx_serve.h"
 
 extern "C" {
@@ -28,7 +29,12 @@ hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
 hipError_t dgx_launch_scatter(int *dst, const int2 *pairs, int n, hipStream_t st);
 hipError_t dgx_launch_move_slots(int fp32, void *const *src, void *const *dst, long long n_nodes, int group, int n,
                                  const int *from, const int *to, hipStream_t st);
+hipError_t dgx_launch_serve_block(int fp32, int p2, const void *table, const int *sched, const int *step_base,
+                                  int step_off, int launch, int nb, int total, hipStream_t st);
 }
+/* the blocked kernel's tiling (dgx_api.cpp) */
+bool dgx_build_block_tiles(long tri, const std::vector<int4> &nbr, int K, int cap, std::vector<int4> &tiles,
+                           std::vector<int> &levels, std::vector<int> &gid, std::vector<int4> &lnb);
 
 #define SV_TRY(call)                                                                                           \
     do {                                                                                                       \
@@ -43,6 +49,8 @@ namespace {
 
 const int MAXG = 64;      /* largest slot group (shots per batch = shot tile of the serving kernel) */
 const int MAX_BATCHES = 4096;
+const int kServeBlockSteps = 2;   /* single-shot slots: steps per launch of the blocked kernel */
+const int kServeBlockThreads = 512; /* and threads per workgroup (lanes per tile) */
 
 /* A material model on a resident mesh: 1/rho | lambda | mu per element in device order, and the time step
  * its requests run with.  Model 0 of every mesh is the one given at dgx_server_add_mesh and stays resident;
@@ -60,6 +68,11 @@ struct ModelDev {
 
 struct MeshDev {
     long tri = 0;
+    long tri_pad = 0;      /* single-shot slots: element rows of a field (a multiple of 64) */
+    /* single-shot slots: the blocked kernel's tiling of the mesh */
+    int4 *btiles = nullptr, *blnb = nullptr;
+    int *blevels = nullptr, *bgid = nullptr;
+    long ntile = 0;
     std::vector<long> inv; /* caller element -> device element */
     std::vector<int4> nbr_host; /* device-order neighbours (finding the faces next to a source) */
     int4 *nbr = nullptr, *vtx = nullptr;
@@ -140,7 +153,8 @@ hipError_t grow(T **p, long *cap, long need)
 
 struct dgx_server {
     int fp32 = 1;
-    int group = 16; /* shots per slot group */
+    int group = 16; /* shots per slot group; 1: single-shot slots stepped by the blocked kernel */
+    int p2 = 0;     /* the operators equal the compiled-in P2 table (the blocked kernel's constants) */
     int nb = 0, tiles = 0; /* stepped batches and their tiles (sched); the step graph is built for these */
     std::vector<int> order; /* batch entry of each stepped batch, in table order */
     unsigned long long *d_runmask = nullptr; /* [step of block][stepped batch]: slots running at that step */
@@ -182,6 +196,7 @@ static int groups_for(const dgx_server *sv, int n) { return (n + sv->group - 1) 
  * per row, so the element count is rounded up to a multiple of 64/G. */
 static long long group_rows(const dgx_server *sv, const MeshDev &m)
 {
+    if (sv->group == 1) return m.tri_pad * 6; /* element layout, 64-element blocks */
     const long long per = 64 / sv->group;
     return (m.tri + per - 1) / per * per * 6;
 }
@@ -189,7 +204,8 @@ static long long group_rows(const dgx_server *sv, const MeshDev &m)
 /* Device bytes of one slot group of mesh m (fields and the per-element word). */
 static long long group_bytes(const dgx_server *sv, const MeshDev &m)
 {
-    return (long long)5 * group_rows(sv, m) * sv->group * sv->rsize() + 4 * m.tri;
+    const int copies = sv->group == 1 ? 2 : 1; /* single-shot slots alternate between two copies */
+    return (long long)copies * 5 * group_rows(sv, m) * sv->group * sv->rsize() + 4 * m.tri;
 }
 
 static void make_ops(dgx_server *sv)
@@ -261,13 +277,30 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
     sv->fp32 = cfg ? (cfg->precision == DGX_FP64 ? 0 : 1) : 1;
     sv->K = (cfg && cfg->steps_per_block > 0) ? cfg->steps_per_block : 100;
     sv->group = (cfg && cfg->group_shots) ? cfg->group_shots : 16;
-    if (sv->group != 4 && sv->group != 8 && sv->group != 16 && sv->group != 32 && sv->group != 64) {
-        dgx_set_error("dgx_server_create: group_shots must be 0, 4, 8, 16, 32 or 64");
+    if (sv->group != 1 && sv->group != 4 && sv->group != 8 && sv->group != 16 && sv->group != 32 && sv->group != 64) {
+        dgx_set_error("dgx_server_create: group_shots must be 0, 1, 4, 8, 16, 32 or 64");
+        delete sv;
+        return DGX_EARG;
+    }
+    if (sv->group == 1 && sv->K % (2 * kServeBlockSteps)) {
+        dgx_set_error("dgx_server_create: with single-shot slots steps_per_block must be a multiple of %d",
+                      2 * kServeBlockSteps);
         delete sv;
         return DGX_EARG;
     }
     sv->ops = *ops;
     make_ops(sv);
+    {
+        /* the blocked kernel compiles in the shipped P2 operators: usable when these equal them bit for bit */
+        bool eq = true;
+        const double *o64 = &sv->op64.Dr[0];
+        for (int k = 0; k < 126 && eq; k++) {
+            const double v = o64[k];
+            eq = sv->fp32 ? ((float)v == (float)dgx_p2_ops[k]) : (v == dgx_p2_ops[k]);
+        }
+        const char *generic = getenv("DGX_GENERIC_OPERATORS");
+        sv->p2 = eq && !(generic && strcmp(generic, "1") == 0);
+    }
     if (const char *f = getenv("DGX_SERVE_FAIL_ALLOC")) sv->fail_alloc_at = atol(f);
     /* a failure from here on frees what was already created */
 #define CREATE_TRY(call)                                                                                       \
@@ -304,7 +337,7 @@ extern "C" int dgx_server_destroy(dgx_server *sv)
     if (sv->stream) (void)hipStreamSynchronize(sv->stream);
     if (sv->graph) (void)hipGraphExecDestroy(sv->graph);
     for (auto &m : sv->meshes) {
-        void *p[] = {m.nbr, m.vtx, m.pt, m.absorb};
+        void *p[] = {m.nbr, m.vtx, m.pt, m.absorb, m.btiles, m.blnb, m.blevels, m.bgid};
         for (void *x : p)
             if (x) (void)hipFree(x);
         for (auto &md : m.models)
@@ -380,10 +413,21 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
         return DGX_EARG;
     }
     dgx_mesh_host mh;
-    int rc = dgx_build_mesh_host(p, 1, p->tri, nullptr, &mh);
+    /* single-shot slots: the element layout (64-element blocks), whose sponge factors are blocked the same way */
+    const long tri_pad = sv->group == 1 ? (p->tri + 63) / 64 * 64 : p->tri;
+    int rc = dgx_build_mesh_host(p, sv->group == 1 ? 0 : 1, tri_pad, nullptr, &mh);
     if (rc) return rc;
+    std::vector<int4> bt, bl;
+    std::vector<int> blv, bg;
+    if (sv->group == 1 &&
+        !dgx_build_block_tiles(p->tri, mh.nbr, kServeBlockSteps, kServeBlockThreads, bt, blv, bg, bl)) {
+        dgx_set_error("dgx_server_add_mesh: an element and its neighbour levels exceed %d threads", kServeBlockThreads);
+        return DGX_EMESH;
+    }
     MeshDev m;
     m.tri = p->tri;
+    m.tri_pad = tri_pad;
+    m.ntile = (long)bt.size();
     m.inv = mh.inv;
     m.nbr_host.assign(mh.nbr.begin(), mh.nbr.begin() + mh.tri);
     ModelDev md;
@@ -392,7 +436,7 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     md.refs = 1; /* never released */
     /* a failure frees what this call allocated */
     auto fail = [&](hipError_t e) {
-        void *q[] = {m.nbr, m.vtx, m.pt, m.absorb, md.d_mat};
+        void *q[] = {m.nbr, m.vtx, m.pt, m.absorb, md.d_mat, m.btiles, m.blnb, m.blevels, m.bgid};
         for (void *x : q)
             if (x) (void)hipFree(x);
         dgx_set_error("dgx_server_add_mesh: %s", hipGetErrorString(e));
@@ -401,8 +445,18 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     hipError_t e = upload_vec(&m.nbr, mh.nbr);
     if (e == hipSuccess) e = upload_vec(&m.vtx, mh.vtx);
     if (e == hipSuccess) e = upload_vec(&m.pt, mh.pt);
-    if (e == hipSuccess) e = upload_reals(sv, &md.d_mat, mh.mat);
+    /* every model of the mesh is stored 1/rho | lambda | mu with stride tri (dgx_build_mesh_host pads to tri_pad) */
+    std::vector<double> mat0(3 * p->tri);
+    for (int c = 0; c < 3; c++)
+        std::copy(mh.mat.begin() + c * tri_pad, mh.mat.begin() + c * tri_pad + p->tri, mat0.begin() + c * p->tri);
+    if (e == hipSuccess) e = upload_reals(sv, &md.d_mat, mat0);
     if (e == hipSuccess) e = upload_reals(sv, &m.absorb, mh.absorb);
+    if (e == hipSuccess && sv->group == 1) {
+        e = upload_vec(&m.btiles, bt);
+        if (e == hipSuccess) e = upload_vec(&m.blnb, bl);
+        if (e == hipSuccess) e = upload_vec(&m.blevels, blv);
+        if (e == hipSuccess) e = upload_vec(&m.bgid, bg);
+    }
     if (e != hipSuccess) return fail(e);
     m.models.push_back(std::move(md));
     sv->meshes.push_back(std::move(m));
@@ -532,7 +586,7 @@ static int new_batch(dgx_server *sv, int mesh, int model, int *bi)
     b.model = model;
     b.G = sv->group;
     const int G = b.G;
-    b.field_bytes = (long long)5 * group_rows(sv, m) * G * sv->rsize();
+    b.field_bytes = (long long)(G == 1 ? 2 : 1) * 5 * group_rows(sv, m) * G * sv->rsize();
     for (int k = 0; k < MAXG; k++) {
         b.req[k] = -1;
         b.shot[k] = 0;
@@ -675,6 +729,18 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
     size_t per = (size_t)group_rows(sv, m) * G;
     for (int c = 0; c < 5; c++) a.f[c] = f + c * per;
     a.fstride = m.tri;
+    if (G == 1) { /* blocked kernel: copy A, copy B (the kernel alternates them by launch), the mesh's tiling */
+        for (int c = 0; c < 5; c++) a.g[c] = f + (5 + c) * per;
+        a.fstride = 0;
+        a.tiles = m.btiles;
+        a.blevels = m.blevels;
+        a.bgid = m.bgid;
+        a.blnb = m.blnb;
+        a.ntile = (int)m.ntile;
+        a.bkmax = kServeBlockSteps;
+        a.bsteps = kServeBlockSteps;
+        a.bthreads = kServeBlockThreads;
+    }
     a.nbr = m.nbr;
     a.vtx = m.vtx;
     a.pt = m.pt;
@@ -716,8 +782,12 @@ static int upload_table(dgx_server *sv)
         if (sv->fp32) fill_batch_args(sv, b, sv->op32, *(dgx::StepArgs<float> *)(table.data() + nb * asz));
         else fill_batch_args(sv, b, sv->op64, *(dgx::StepArgs<double> *)(table.data() + nb * asz));
         sched.push_back(tiles);
-        const int per_tile = 256 / sv->group; /* elements per tile: a workgroup covers 256/G elements x G shots */
-        tiles += (int)((sv->meshes[b.mesh].tri + per_tile - 1) / per_tile);
+        if (sv->group == 1) {
+            tiles += (int)sv->meshes[b.mesh].ntile; /* the blocked kernel's tiles of the mesh */
+        } else {
+            const int per_tile = 256 / sv->group; /* a workgroup covers 256/G elements x G shots */
+            tiles += (int)((sv->meshes[b.mesh].tri + per_tile - 1) / per_tile);
+        }
         nb++;
     }
     sched.push_back(tiles);
@@ -1035,7 +1105,12 @@ static int build_graph(dgx_server *sv)
     SV_TRY(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     const void *ops = sv->fp32 ? (const void *)&sv->op32 : (const void *)&sv->op64;
-    for (int k = 0; k < sv->K && e == hipSuccess; k++) {
+    /* single-shot slots: K / kServeBlockSteps launches of the blocked kernel (an even number: every slot's
+     * state is back in copy A at the end of the block) */
+    for (int k = 0; sv->group == 1 && k < sv->K / kServeBlockSteps && e == hipSuccess; k++)
+        e = dgx_launch_serve_block(sv->fp32, sv->p2, sv->d_table, sv->d_sched, sv->d_step_base, k * kServeBlockSteps, k,
+                                   sv->nb, sv->tiles, sv->stream);
+    for (int k = 0; sv->group != 1 && k < sv->K && e == hipSuccess; k++) {
         e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_runmask, sv->d_step_base, k, 1,
                              sv->group, sv->nb, sv->tiles, sv->stream);
         if (e == hipSuccess)

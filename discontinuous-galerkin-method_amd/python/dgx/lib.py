@@ -19,7 +19,7 @@ DGX_FP64, DGX_FP32 = 0, 1
 DGX_SRC_FORCE_Z, DGX_SRC_EXPLOSIVE = 0, 1
 DGX_LAYOUT_AUTO, DGX_LAYOUT_ELEMENT, DGX_LAYOUT_SHOT_INNER = 0, 1, 2
 DGX_OK, DGX_EARG, DGX_EMESH, DGX_EHIP, DGX_ENOMEM, DGX_EIO, DGX_ESTATE = 0, -1, -2, -3, -4, -5, -6
-DGX_STEP_AUTO, DGX_STEP_HALF, DGX_STEP_FULL = 0, 1, 2
+DGX_STEP_AUTO, DGX_STEP_HALF, DGX_STEP_FULL, DGX_STEP_BLOCKED = 0, 1, 2, 3
 
 dptr = C.POINTER(C.c_double)
 lptr = C.POINTER(C.c_long)
@@ -51,7 +51,7 @@ class ProblemC(C.Structure):
                 ("dt", C.c_double), ("precision", C.c_int), ("nshots", C.c_long), ("nsteps", C.c_long),
                 ("nwavelets", C.c_long), ("wavelet", dptr), ("nsrc", C.c_long), ("src", C.POINTER(Source)),
                 ("nrec", C.c_long), ("rec", C.POINTER(Receiver)), ("layout", C.c_int), ("shot_tile", C.c_int),
-                ("step_kernel", C.c_int), ("wg_waves", C.c_int)]
+                ("step_kernel", C.c_int), ("wg_waves", C.c_int), ("block_steps", C.c_int)]
 
 
 class ServerConfig(C.Structure):
@@ -115,6 +115,8 @@ def load() -> C.CDLL:
         "dgx_layout_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), dptr, C.c_int]),
         "dgx_tile_info": (C.c_int, [vp] + [C.POINTER(C.c_int)] * 4 + [dptr, C.c_int]),
         "dgx_compiled_operators": (C.c_int, [vp]),
+        "dgx_block_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_long), C.POINTER(C.c_long),
+                                     C.POINTER(C.c_long)]),
         "dgx_read_ascii": (C.c_long, [C.c_char_p, dptr, C.c_long, C.c_long]),
         "dgx_read_ascii_long": (C.c_long, [C.c_char_p, lptr, C.c_long, C.c_long]),
         "dgx_read_operators": (C.c_int, [C.c_char_p, C.POINTER(Operators)]),

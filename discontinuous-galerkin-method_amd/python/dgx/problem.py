@@ -149,7 +149,7 @@ class Solver:
     """Owns one dgx_solver handle (device-resident fields for ``nshots`` shots on one mesh)."""
 
     def __init__(self, setup: Setup, precision: int = L.DGX_FP64, layout: int = L.DGX_LAYOUT_AUTO,
-                 shot_tile: int = 0, step_kernel: int = L.DGX_STEP_AUTO, wg_waves: int = 0):
+                 shot_tile: int = 0, step_kernel: int = L.DGX_STEP_AUTO, wg_waves: int = 0, block_steps: int = 0):
         self.lib = L.load()
         self.setup = setup
         m = setup.mesh
@@ -169,7 +169,7 @@ class Solver:
         p = L.ProblemC(m.tri, m.point_num, L.dp(pt), L.lp(tr), L.lp(nb), L.dp(la), L.dp(mi), L.dp(de),
                        L.dp(ab) if ab is not None else L.dptr(), setup.dt, precision, setup.nshots, setup.nsteps,
                        wv.shape[0], L.dp(wv), len(setup.sources), srcs, len(setup.receivers), recs, layout,
-                       shot_tile, step_kernel, wg_waves)
+                       shot_tile, step_kernel, wg_waves, block_steps)
         self.handle = C.c_void_p()
         L.check(self.lib.dgx_create(C.byref(p), C.byref(setup.ops), C.byref(self.handle)), "dgx_create")
         self.precision = precision
@@ -246,8 +246,8 @@ class Solver:
         return x[:, :r], z[:, :r]
 
     def kernel_bench(self, which: int, n: int) -> float:
-        """Average ms of one fused kernel (0 velocity, 1 stress, 2 full step) over n back-to-back
-        launches.  Leaves the fields invalid."""
+        """Average ms of one fused kernel (0 velocity, 1 stress, 2 full step, 3 blocked: one launch of
+        block_steps steps) over n back-to-back launches.  Leaves the fields invalid."""
         ms = C.c_double()
         L.check(self.lib.dgx_kernel_bench(self.handle, which, n, C.byref(ms)), "dgx_kernel_bench")
         return ms.value
@@ -275,6 +275,17 @@ class Solver:
         """True when the element kernels run with the P2 operators compiled in (the problem's operators
         equal the shipped ones bit for bit and DGX_GENERIC_OPERATORS=1 was not set at creation)."""
         return bool(L.check(self.lib.dgx_compiled_operators(self.handle), "dgx_compiled_operators"))
+
+    def block_info(self) -> dict:
+        """Tiling of the blocked kernel: steps per launch, tiles, own elements and lanes (threads holding an
+        element) in total; empty for the other step kernels."""
+        k, nt, own, lanes = C.c_int(), C.c_long(), C.c_long(), C.c_long()
+        L.check(self.lib.dgx_block_info(self.handle, C.byref(k), C.byref(nt), C.byref(own), C.byref(lanes)),
+                "dgx_block_info")
+        if not k.value:
+            return {}
+        return {"steps_per_launch": k.value, "tiles": nt.value, "own": own.value, "lanes": lanes.value,
+                "recomputed_fraction": (lanes.value - own.value) / max(own.value, 1)}
 
     def kernel_times(self, n: int) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()

@@ -38,8 +38,11 @@ enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_I
  * next state to a second copy; workgroups recompute the velocities of the elements bordering their tile
  * instead of waiting for them, so each field is read and written once per step.  It gives the same
  * results; on MI355X it is slower than HALF (the kernels wait on memory latency, not bandwidth, and the
- * tile's two phases lengthen that wait).  AUTO: HALF. */
-enum dgx_step_kernel { DGX_STEP_AUTO = 0, DGX_STEP_HALF = 1, DGX_STEP_FULL = 2 };
+ * tile's two phases lengthen that wait).  BLOCKED (element layout): one kernel runs block_steps whole steps
+ * per launch; each workgroup holds a tile of elements and the rings of elements within 2*block_steps - 1
+ * face hops around it in registers and LDS, recomputes the rings instead of exchanging them, and reads and
+ * writes each field once per launch.  AUTO: HALF. */
+enum dgx_step_kernel { DGX_STEP_AUTO = 0, DGX_STEP_HALF = 1, DGX_STEP_FULL = 2, DGX_STEP_BLOCKED = 3 };
 
 /* Reference-element operators as read from input_files/precomputed_matrix/ (row-major), the arrays
  * DG_leap_Frog_OpenMP.c:52-81 reads.  Vb/inverVb are read by the reference but never used. */
@@ -94,6 +97,8 @@ typedef struct {
     int wg_waves;          /* wavefronts per workgroup of the shot-innermost kernels: 2, 4 or 8 with shot
                             * tiles 16-64, 4 with tiles 4 and 8.  0: autotuned with the shot tile when
                             * shot_tile is 0, else 4.  Given with shot_tile 0, only the tile is tuned. */
+    int block_steps;       /* steps per launch of the blocked kernel (step_kernel = DGX_STEP_BLOCKED): 1, 2 or 3;
+                            * 0 = 2 */
 } dgx_problem;
 
 typedef struct dgx_solver dgx_solver;
@@ -138,9 +143,10 @@ long dgx_current_step(const dgx_solver *s);
 /* Average device time of the two fused kernels over n steps, measured with HIP events on the solver's
  * stream (the steps are real steps: the state advances). */
 int dgx_kernel_times(dgx_solver *s, long n, double *ms_velocity, double *ms_stress);
-/* Average duration of one fused kernel (which: 0 velocity, 1 stress, 2 full step) over n back-to-back
- * launches between two HIP events on the solver's stream.  Invalidates the fields (the launches repeat
- * a step).  which = 2 needs the full-step kernel (element layout). */
+/* Average duration of one fused kernel (which: 0 velocity, 1 stress, 2 full step, 3 blocked: one launch of
+ * block_steps steps) over n back-to-back launches between two HIP events on the solver's stream.
+ * Invalidates the fields (the launches repeat a step).  which = 2 and 3 need the full-step and blocked
+ * kernels (element layout). */
 int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms);
 /* Streaming copy of `bytes` bytes (read + write) repeated `reps` times on the default stream: the
  * achievable-bandwidth reference and the calibration point of the HBM byte counters.  variant 0:
@@ -156,6 +162,11 @@ int dgx_layout_info(const dgx_solver *s, int *layout, int *shot_tile, int *tiles
  * one shot per run. */
 int dgx_tile_info(const dgx_solver *s, int *shot_tile, int *wg_waves, int *tiles, int *waves, double *ms_per_step,
                   int cap);
+/* Tiling of the blocked kernel (dgx_problem.step_kernel = DGX_STEP_BLOCKED): steps per launch, tiles,
+ * elements owned by the tiles (the mesh) and lanes (threads holding an element, own or recomputed); lanes/own -
+ * 1 is the fraction of the work that is recomputation.  All zero for the other step kernels.  Diagnostic with
+ * no counterpart in the reference. */
+int dgx_block_info(const dgx_solver *s, int *steps_per_launch, long *ntile, long *own, long *lanes);
 /* 1 when the element-layout velocity kernels of this solver run with the P2 Fekete operators of the
  * shipped operator files compiled in (its operators equal them bit for bit and DGX_GENERIC_OPERATORS=1 is not
  * set), 0 when they read the operators from their kernel argument or the layout is shot-innermost.

@@ -29,10 +29,14 @@ typedef struct {
     int precision;         /* DGX_FP32 (default for serving) or DGX_FP64 */
     long long hbm_budget;  /* bytes for wavefields, wavelets and traces; 0 = 80 % of free HBM */
     int steps_per_block;   /* steps between scheduling points (one graph replay); 0 = 100 */
-    int group_shots;       /* shots per slot group, 4, 8, 16, 32 or 64 (0 = 16): the serving kernel's
-                            * shot tile.  Each node row of a group holds 64/G elements x G shots, so every
-                            * G moves whole lines; 64 is fastest per lane when groups fill, 4-16 waste
-                            * fewer lanes when requests are small and meshes many */
+    int group_shots;       /* shots per slot group, 1, 4, 8, 16, 32 or 64 (0 = 16).  4-64: the shot-innermost
+                            * serving kernel's shot tile, one launch per half-step; each node row of a group
+                            * holds 64/G elements x G shots, so every G moves whole lines; 64 is fastest per
+                            * lane when groups fill, 4-16 waste fewer lanes when requests are small and meshes
+                            * many.  1: every shot is a slot of its own in the element layout, stepped two
+                            * steps per launch by the blocked kernel (tiles of each mesh with recomputed
+                            * rings, two copies of the fields per slot); steps_per_block must then be a
+                            * multiple of 4 */
 } dgx_server_config;
 
 enum dgx_request_state { DGX_REQ_QUEUED = 0, DGX_REQ_RUNNING = 1, DGX_REQ_DONE = 2 };

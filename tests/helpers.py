@@ -67,8 +67,8 @@ def product_setup(w: dgx.configs.Workload, nshots=None) -> dgx.Setup:
 
 
 def product_run(setup: dgx.Setup, precision=L.DGX_FP64, nsteps=None, snap_every=0, shot=0, chunk=None,
-                step_kernel=L.DGX_STEP_AUTO):
-    s = dgx.Solver(setup, precision, step_kernel=step_kernel)
+                step_kernel=L.DGX_STEP_AUTO, **solver_kw):
+    s = dgx.Solver(setup, precision, step_kernel=step_kernel, **solver_kw)
     n = nsteps or setup.nsteps
     snaps = {}
     step = snap_every or chunk or n

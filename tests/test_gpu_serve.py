@@ -10,7 +10,8 @@
   group holds), and may move between them when groups are compacted, without changing its results.
 
 The first two hold for every slot-group size (4, 8, 16, 32 and 64 shots per group, which is also the serving
-kernel's shot tile).
+kernel's shot tile) and for single-shot slots (group 1), which the blocked kernel steps two steps per launch
+(compared with the standalone solver's blocked kernel).
 """
 import numpy as np
 import pytest
@@ -45,6 +46,8 @@ def build(server, rms, spec, k):
 
 
 def new_server(precision=L.DGX_FP32, budget=0, K=50, group=16):
+    if group == 1:
+        K = K // 4 * 4  # single-shot slots: an even number of 2-step launches per block
     sp, ms = meshes()
     sv = S.Server(precision, budget, K, group_shots=group)
     rms = [sv.add_mesh(m, md, sp) for m, md in ms]
@@ -61,7 +64,7 @@ def run_alone(k, group=16, precision=L.DGX_FP32):
     return out, xz, rec
 
 
-@pytest.mark.parametrize("group", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("group", [1, 4, 8, 16, 32, 64])
 def test_results_identical_alone_and_in_the_mix(group):
     print(f"case serve mix, group {group}", flush=True)
     sv, rms, _, _ = new_server(group=group)
@@ -161,7 +164,7 @@ def test_compaction_moves_shots_without_changing_results():
         assert np.array_equal(out[k][0], ax) and np.array_equal(out[k][1], az), k
 
 
-@pytest.mark.parametrize("group", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("group", [1, 4, 8, 16, 32, 64])
 def test_request_equals_standalone_solver_and_oracle(group):
     print(f"case serve vs standalone and oracle, group {group}", flush=True)
     k = 0
@@ -171,12 +174,17 @@ def test_request_equals_standalone_solver_and_oracle(group):
     nsteps, f0 = REQS[k][2], REQS[k][3]
     srcs = [dgx.SourceSpec(float(a), float(b), L.DGX_SRC_EXPLOSIVE, s, f0) for s, (a, b) in enumerate(xz)]
     st = dgx.build_setup(mesh, model, srcs, rec, nsteps, sponge=sp)
-    solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, group)
+    if group == 1:  # the same tile code as the solver's blocked kernel (2 steps per launch, 512 threads)
+        solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_ELEMENT, step_kernel=L.DGX_STEP_BLOCKED, block_steps=2)
+    else:
+        solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, group)
     solver.step(nsteps)
     for s in range(len(xz)):
         gx, gz = solver.traces(s)
         # same arithmetic in two kernels (the compiler may contract FMAs differently): FP32 rounding level
-        assert rel_l2(x[s], gx) <= 1e-5 and rel_l2(z[s], gz) <= 1e-5, s
+        e = max(rel_l2(x[s], gx), rel_l2(z[s], gz))
+        print(f"shot {s}: served vs standalone solver {e:.3e}")
+        assert e <= 1e-5, s
     solver.close()
     w = dgx.configs.Workload("serve", mesh, model, sp, srcs, rec, nsteps, 0)
     oc = OracleCase(w)
@@ -188,7 +196,7 @@ def test_request_equals_standalone_solver_and_oracle(group):
     assert worst <= 1e-4
 
 
-@pytest.mark.parametrize("group", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("group", [1, 4, 8, 16, 32, 64])
 def test_fp64_serving_matches_oracle(group):
     """FP64 serving reproduces the oracle to the plain path's parity bar, a request mixed with others."""
     print(f"case fp64 serve vs oracle, group {group}", flush=True)

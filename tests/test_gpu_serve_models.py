@@ -46,15 +46,16 @@ def make(sv, rm, grad, spec, k):
 
 def server(precision, group=8):
     sp, m, own, grad = mesh_and_models()
-    sv = S.Server(precision, 0, 50, group_shots=group)
+    sv = S.Server(precision, 0, 48 if group == 1 else 50, group_shots=group)
     rm = sv.add_mesh(m, own, sp)
     return sv, rm, sp, m, own, grad
 
 
+@pytest.mark.parametrize("group", [8, 1])
 @pytest.mark.parametrize("precision", [L.DGX_FP32, L.DGX_FP64])
-def test_two_models_on_one_mesh_alone_and_mixed(precision):
-    print(f"case serve two models, precision {precision}", flush=True)
-    sv, rm, sp, m, own, grad = server(precision)
+def test_two_models_on_one_mesh_alone_and_mixed(precision, group):
+    print(f"case serve two models, precision {precision}, group {group}", flush=True)
+    sv, rm, sp, m, own, grad = server(precision, group)
     built = [make(sv, rm, grad, spec, k) for k, spec in enumerate(SPECS)]
     assert built[1][0]["dt"] != built[0][0]["dt"]  # the gradient model has its own CFL step
     ids = [sv.submit(built[0][0]), sv.submit(built[1][0])]
@@ -64,7 +65,7 @@ def test_two_models_on_one_mesh_alone_and_mixed(precision):
     mixed = [sv.result(rid, built[k][0]) for k, rid in enumerate(ids)]
     sv.close()
     for k, spec in enumerate(SPECS):
-        sv, rm, sp, m, own, grad = server(precision)
+        sv, rm, sp, m, own, grad = server(precision, group)
         req, _, _ = make(sv, rm, grad, spec, k)
         rid = sv.submit(req)
         sv.drain()

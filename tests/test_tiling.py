@@ -97,3 +97,99 @@ def test_tiling_tiny_meshes(n):
     mesh = dgx.mesh.structured(n, 1, 10.0)
     tiles, ring, nb2, nbr, vrow0, vrows = tiling(mesh)
     assert len(tiles) == 1 and tiles[0, 1] == mesh.tri and len(ring) == 0
+
+
+# ------------------------------------------------------------------------------------------- blocked kernel
+def block_tiling(mesh, K, cap):
+    lib = L.load()
+    f = lib.dgx_block_tiling
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(L.ProblemC), C.c_int, C.c_int] + [C.POINTER(C.c_long)] * 2 + [C.c_void_p] * 5
+    pt = np.ascontiguousarray(mesh.point, np.float64)
+    tr = np.ascontiguousarray(mesh.triangle, np.int64)
+    nb = np.ascontiguousarray(mesh.neighbor, np.int64)
+    T = mesh.tri
+    ones = np.ones(T)
+    p = L.ProblemC(T, mesh.point_num, L.dp(pt), L.lp(tr), L.lp(nb), L.dp(ones), L.dp(ones), L.dp(ones),
+                   L.dptr(), 1.0, L.DGX_FP32, 1, 1, 0, L.dptr(), 0, None, 0, None, L.DGX_LAYOUT_ELEMENT, 0,
+                   L.DGX_STEP_BLOCKED, 0, K)
+    nt, nl = C.c_long(), C.c_long()
+    assert f(C.byref(p), K, cap, C.byref(nt), C.byref(nl), None, None, None, None, None) == 0
+    tiles = np.zeros((nt.value, 4), np.int32)
+    levels = np.zeros((nt.value, 8), np.int32)
+    gid = np.zeros(nl.value, np.int32)
+    lnb = np.zeros((nl.value, 4), np.int32)
+    nbr = np.zeros((T, 4), np.int32)
+    assert f(C.byref(p), K, cap, C.byref(nt), C.byref(nl), tiles.ctypes.data, levels.ctypes.data, gid.ctypes.data,
+             lnb.ctypes.data, nbr.ctypes.data) == 0
+    return tiles, levels, gid, lnb, nbr
+
+
+def check_block_tiling(mesh, K, cap):
+    """The blocked kernel's tiles: own runs partition the elements in order; each tile's lanes fit `cap`
+    threads; level d holds exactly the elements d face hops from the run (breadth-first distance, d <= 2K,
+    level 2K the ghosts); every face neighbour of a lane below level 2K is a lane of the tile at level d-1, d or
+    d+1, named by its lane index; the code word is the element's.  Returns lanes/own - 1 (recomputed work)."""
+    tiles, levels, gid, lnb, nbr = block_tiling(mesh, K, cap)
+    T = mesh.tri
+    assert tiles[0, 0] == 0 and np.all(tiles[1:, 0] == tiles[:-1, 0] + tiles[:-1, 1])
+    assert tiles[-1, 0] + tiles[-1, 1] == T
+    assert np.all(tiles[:, 3] <= cap) and np.all(tiles[:, 1] >= 1)
+    assert np.all(tiles[1:, 2] == tiles[:-1, 2] + tiles[:-1, 3]) and tiles[-1, 2] + tiles[-1, 3] == len(gid)
+    for t in range(len(tiles)):
+        first, own, base, n = tiles[t]
+        le = levels[t]
+        assert le[0] == own and np.all(np.diff(le) >= 0) and le[2 * K] == n and np.all(le[2 * K:] == n)
+        g = gid[base:base + n]
+        assert np.array_equal(g[:own], np.arange(first, first + own))
+        assert len(set(g.tolist())) == n
+        # breadth-first distance from the run
+        dist = {int(e): 0 for e in g[:own]}
+        frontier = list(g[:own])
+        for d in range(1, 2 * K + 1):
+            nxt = []
+            for e in frontier:
+                for x in nbr[e, :3]:
+                    if x >= 0 and int(x) not in dist:
+                        dist[int(x)] = d
+                        nxt.append(int(x))
+            frontier = nxt
+        lev = np.searchsorted(le[:2 * K + 1], np.arange(n), side="right")
+        assert sorted(dist) == sorted(g.tolist())
+        for l in range(n):
+            e = int(g[l])
+            assert dist[e] == lev[l], (t, l)
+            assert lnb[base + l, 3] == nbr[e, 3]
+            if lev[l] == 2 * K:
+                continue
+            for j in range(3):
+                x, s = nbr[e, j], lnb[base + l, j]
+                if x < 0:
+                    assert s == -1
+                else:
+                    assert 0 <= s < n and g[s] == x and abs(lev[s] - lev[l]) <= 1
+    return (len(gid) - T) / T
+
+
+@pytest.mark.parametrize("K,cap", [(1, 1024), (2, 1024), (3, 1024), (2, 512)])
+def test_block_tiling_jittered(K, cap):
+    w = dgx.configs.gradient_8x4km(10, nx=90, nz=50)
+    check_block_tiling(w.mesh, K, cap)
+
+
+def test_block_tiling_unstructured_and_tiny():
+    pts, _ = U.layered_point_set(900.0, 500.0, 12.0, [lambda x: 250.0 + 0.2 * x], seed=4)
+    check_block_tiling(U.triangulate(pts), 2, 1024)
+    for n in (1, 2, 5):
+        tiles, levels, gid, lnb, nbr = block_tiling(dgx.mesh.structured(n, 1, 10.0), 3, 1024)
+        assert len(tiles) == 1 and tiles[0, 1] == 2 * n
+
+
+def test_block_tiling_recompute_fraction_640k_order():
+    """Work recomputed on the rings, for a mesh of the 640k-element kind at a quarter of its size: 2 steps per
+    launch at 1024 threads recompute well under the own work."""
+    w = dgx.configs.gradient_8x4km(10, nx=400, nz=200)
+    tiles, levels, gid, lnb, nbr = block_tiling(w.mesh, 2, 1024)
+    frac = (len(gid) - w.mesh.tri) / w.mesh.tri
+    print(f"2 steps, 1024 threads: lanes/own - 1 = {frac:.3f}")
+    assert frac < 0.6
