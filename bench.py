@@ -49,14 +49,18 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU time of the reference step-loop sample (steps sized from a short timed run)")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batched FP32 leg (0 = skip)")
-    ap.add_argument("--batched-steps", type=int, default=300)
+    ap.add_argument("--batched-steps", type=int, default=320)
+    ap.add_argument("--block-steps", type=int, default=2, help="steps per launch of the blocked kernel (1-3)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the seeded serving stream to run (0 = skip; 2000 = the whole stream)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-budget-gb", type=float, default=0.0,
                     help="HBM budget of the serving leg's wavefields and request buffers (0 = 80%% of free HBM)")
-    ap.add_argument("--serve-group", type=int, default=4,
-                    help="shots per slot group of the server (4, 8, 16, 32 or 64; 4 suits this stream of 1-16-shot requests)")
+    ap.add_argument("--serve-group", type=int, default=1,
+                    help="shots per slot group of the server (1: single-shot slots on the blocked kernel; 4, 8, 16, "
+                         "32 or 64: shot-innermost groups)")
+    ap.add_argument("--serve-compare-group", type=int, default=4,
+                    help="also run the stream with this group size for comparison (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=1,
                     help="also time a plain device copy (achievable bandwidth, counter calibration)")
     return ap.parse_args()
@@ -78,6 +82,8 @@ def run_path(setup, precision, warmup, steps, layout=L.DGX_LAYOUT_AUTO, bench_la
     x, z = s.traces(0)
     if step_kernel == L.DGX_STEP_FULL:
         kt = {"full": s.kernel_bench(2, bench_launches)}
+    elif step_kernel == L.DGX_STEP_BLOCKED:
+        kt = {"block": s.kernel_bench(3, bench_launches), "info": s.block_info()}
     else:
         kt = {"velocity": s.kernel_bench(0, bench_launches), "stress": s.kernel_bench(1, bench_launches)}
     s.close()
@@ -161,57 +167,101 @@ def accuracy_leg(w, paths):
     return out
 
 
-def batched_leg(a, rate64):
-    """Batched FP32 on the slope model (water over three dipping layers, ~1.2 M elements): S shots in
-    the shot-innermost layout with the autotuned shot tile; FP32 traces compared with an FP64 run of the
-    same shots."""
-    w = dgx.configs.slope_6x3km(a.batched_steps + 25, nshots=a.shots)
-    S, nb = a.shots, a.batched_steps
-    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
-    T = w.mesh.tri
-    secb, ktb, xb, zb, info = run_path(st, L.DGX_FP32, 20, nb, L.DGX_LAYOUT_SHOT_INNER, 20)
-    tinfo = info[3]
-    kvb, ksb = ktb["velocity"], ktb["stress"]
-    rateb = T * 30 * S * nb / secb
-    # FP64 reference on 4 of the shots (same mesh, sources, steps), element layout
-    picks = [0, S // 3, (2 * S) // 3, S - 1]
+def large_leg(a):
+    """Plain FP64 on a ~2M-element jittered mesh (the serving set's largest, 480 MB of fields: twice the 256 MB
+    Infinity Cache, so the fields stream from HBM every step): ms per step, DOF-updates/s and the fraction of
+    HBM peak at 24 algorithmic bytes per DOF-update, for the whole step and per kernel."""
+    mesh, model, sp = dgx.configs.serving_meshes(sizes=(2_000_000,))[0]
+    W, H = mesh.point[:, 0].max(), mesh.point[:, 1].max()
+    n = 320
+    st = dgx.build_setup(mesh, model, [dgx.SourceSpec(W / 2, H - 300.0, L.DGX_SRC_EXPLOSIVE, 0, 20.0)],
+                         np.stack([np.linspace(100.0, W - 100.0, 101), np.full(101, H - 5.0)], 1), n + 200, sponge=sp)
+    sec, kt, _, _, _ = run_path(st, L.DGX_FP64, 64, n)
+    T = mesh.tri
+    rate = T * 30 * n / sec
+    kv, ks = kt["velocity"], kt["stress"]
+    return {"elements": T, "fields_MB": T * 30 * 8 / 1e6, "steps": n, "ms_per_step": sec * 1e3 / n, "value": rate,
+            "step_frac": rate * 24 / HBM_PEAK, "velocity_ms": kv, "stress_ms": ks,
+            "velocity_frac": T * 6 * 8 * 7 / (kv * 1e-3) / HBM_PEAK, "stress_frac": T * 6 * 8 * 8 / (ks * 1e-3) / HBM_PEAK}
+
+
+def fp64_trace_check(w, st, traces_of, picks, n):
+    """Worst FP32 trace error of shots `picks` (traces_of(k) -> (x, z)) against FP64 single-shot runs of the
+    plain path on the same mesh, sources and dt: max over the shots of the set error, of the worst trace at
+    >= 10 % of the set's peak trace norm, and of the worst trace error relative to that peak."""
     worst = 0.0
-    s32 = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, tinfo[0], wg_waves=tinfo[1])
-    s32.step(20 + nb)
     for k in picks:
         one = dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge, [dgx.SourceSpec(
             w.sources[k].x, w.sources[k].z, w.sources[k].kind, 0, w.sources[k].f0)], w.receivers, w.nsteps, 0)
         s1 = dgx.build_setup(one.mesh, one.model, one.sources, one.receivers, one.nsteps, sponge=one.sponge,
                              dt=st.dt)
         r64 = dgx.Solver(s1, L.DGX_FP64, L.DGX_LAYOUT_ELEMENT)
-        r64.step(20 + nb)
+        r64.step(n)
         x64, z64 = r64.traces(0)
         r64.close()
-        x32, z32 = s32.traces(k)
-        n = 20 + nb
+        x32, z32 = traces_of(k)
         e = trace_errors(x32[:n], z32[:n], x64[:n], z64[:n])
-        worst = max(worst, e["all"] or 0.0, e["worst"] or 0.0)
-    s32.close()
+        worst = max(worst, e["all"] or 0.0, e["worst"] or 0.0, e["worst_vs_peak"] or 0.0)
+    return worst
+
+
+def batched_leg(a, rate64):
+    """Batched FP32 on the slope model (water over three dipping layers, ~1.2 M elements), S shots.  Primary:
+    the blocked kernel (element layout, block_steps steps per launch, tiles with recomputed rings); beside it
+    the shot-innermost half-step kernels with the autotuned shot tile.  FP32 traces of 4 shots compared with
+    FP64 runs of the plain path."""
+    w = dgx.configs.slope_6x3km(a.batched_steps + 64, nshots=a.shots)  # room for the launch timing after the run
+    S, nb = a.shots, a.batched_steps
+    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
+    T = w.mesh.tri
+    picks = [0, S // 3, (2 * S) // 3, S - 1]
+    # blocked kernel
+    s = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_ELEMENT, step_kernel=L.DGX_STEP_BLOCKED, block_steps=a.block_steps)
+    binfo = s.block_info()
+    s.step(32)
+    s.synchronize()
+    t = time.perf_counter()
+    s.step(nb)
+    s.synchronize()
+    secb = time.perf_counter() - t
+    worst = fp64_trace_check(w, st, s.traces, picks, 32 + nb)
+    kms = s.kernel_bench(3, 10)
+    s.close()
+    rateb = T * 30 * S * nb / secb
+    K = binfo["steps_per_launch"]
+    # algorithmic bytes of one launch: each field read and written once per element and shot for K steps
+    alg = T * S * 6 * 4 * 10
+    out = {"workload": w.name, "elements": T, "value": rateb, "unit": "DOF-updates/s", "dtype": "f32",
+           "shots": S, "steps": nb, "ms_per_step": secb * 1e3 / nb, "x_vs_plain_fp64": rateb / rate64,
+           "trace_rel_l2_vs_fp64_worst_of_4_shots": worst,
+           "trace_error_note": "max over 4 shots of the set error and of the worst trace at >= 1e-2 of the peak norm",
+           "path": f"blocked kernel, element layout, {K} steps per launch, graphs of 16 launches",
+           "tiling": binfo, "launch_ms": kms, "launch_algorithmic_GBps": alg / (kms * 1e-3) / 1e9,
+           "bytes_per_dof_update_algorithmic": 10 * 4 / (30 / 6) / K}
+    progress(f"batched FP32 (blocked): {rateb:.3e} DOF-updates/s")
+    # shot-innermost half-step kernels on the same shots, for comparison
+    secs, kts, _, _, info = run_path(st, L.DGX_FP32, 20, nb, L.DGX_LAYOUT_SHOT_INNER, 20)
+    tinfo = info[3]
+    kvb, ksb = kts["velocity"], kts["stress"]
     bytes_b = T * 6 * 4 * S
-    return {"workload": w.name, "elements": T, "value": rateb, "unit": "DOF-updates/s", "dtype": "f32",
-            "shots": S, "steps": nb, "ms_per_step": secb * 1e3 / nb, "x_vs_plain_fp64": rateb / rate64,
-            "trace_rel_l2_vs_fp64_worst_of_4_shots": worst,
-            "trace_error_note": "max over 4 shots of the set error and of the worst trace at >= 1e-2 of the peak norm",
-            "layout": "shot-innermost", "shot_tile": tinfo[0],
-            "wavefronts_per_workgroup": tinfo[1], "elements_per_workgroup": tinfo[1] * 64 // tinfo[0],
-            "autotune_ms_per_step": {f"tile {t} x {w} wavefronts": v for (t, w), v in tinfo[2].items()},
-            "velocity_ms": kvb, "stress_ms": ksb,
-            "velocity_GBps": bytes_b * 7 / (kvb * 1e-3) / 1e9, "stress_GBps": bytes_b * 8 / (ksb * 1e-3) / 1e9}
+    out["shot_innermost"] = {
+        "value": T * 30 * S * nb / secs, "ms_per_step": secs * 1e3 / nb, "shot_tile": tinfo[0],
+        "wavefronts_per_workgroup": tinfo[1], "elements_per_workgroup": tinfo[1] * 64 // tinfo[0],
+        "autotune_ms_per_step": {f"tile {t} x {w} wavefronts": v for (t, w), v in tinfo[2].items()},
+        "velocity_ms": kvb, "stress_ms": ksb,
+        "velocity_GBps": bytes_b * 7 / (kvb * 1e-3) / 1e9, "stress_GBps": bytes_b * 8 / (ksb * 1e-3) / 1e9}
+    return out
 
 
-def serving_leg(a, rate64):
+def serving_leg(a, rate64, group):
     """Open-loop serving on the eight resident meshes (20 k .. 2 M elements, random layered models):
-    the first `serve_requests` requests of the seeded Poisson stream (mean rate per second), FP32."""
+    the first `serve_requests` requests of the seeded Poisson stream (mean rate per second), FP32, with
+    `group` shots per slot group (1: single-shot slots stepped by the blocked kernel)."""
     import dgx.serve as SV
     ms = dgx.configs.serving_meshes()
     arrivals, specs = dgx.configs.serving_requests(ms, n=max(a.serve_requests, 1), rate=a.serve_rate)
     arrivals, specs = arrivals[:a.serve_requests], specs[:a.serve_requests]
-    sv = SV.Server(L.DGX_FP32, int(a.serve_budget_gb * 2**30), 100, group_shots=a.serve_group)
+    sv = SV.Server(L.DGX_FP32, int(a.serve_budget_gb * 2**30), 100, group_shots=group)
     rms = [sv.add_mesh(m, md, sp) for m, md, sp in ms]
     reqs = [sv.make_request(rms[r["mesh"]], r["xz"], [r["f0"]] * len(r["xz"]), r["nsteps"], r["rec"])
             for r in specs]
@@ -219,7 +269,9 @@ def serving_leg(a, rate64):
     sv.close()
     dof = st["dof_updates"]
     return {"requests": len(reqs), "arrival_rate_per_s": a.serve_rate, "meshes": [m.tri for m, _, _ in ms],
-            "group_shots": a.serve_group, "hbm_budget_gb": a.serve_budget_gb or "80% of free",
+            "group_shots": group, "hbm_budget_gb": a.serve_budget_gb or "80% of free",
+            "path": ("single-shot slots, blocked kernel (2 steps per launch)" if group == 1 else
+                     f"{group}-shot slot groups, shot-innermost half-step kernels"),
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "wall_s": wall, "dof_updates": dof,
             "value": dof / wall, "unit": "DOF-updates/s", "dtype": "f32",
@@ -311,6 +363,8 @@ def main():
     progress(f"plain FP64: {sec64 * 1e3 / a.steps:.4f} ms/step")
     sec32, kt32, x32, z32, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps)
     progress(f"plain FP32: {sec32 * 1e3 / a.steps:.4f} ms/step")
+    secb1, ktb1, _, _, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps, step_kernel=L.DGX_STEP_BLOCKED)
+    progress(f"blocked FP32: {secb1 * 1e3 / a.steps:.4f} ms/step")
     secf, ktf, xf, zf, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps, step_kernel=L.DGX_STEP_FULL)
     progress(f"full-step FP64: {secf * 1e3 / a.steps:.4f} ms/step")
     rate64 = dof * a.steps / sec64
@@ -363,10 +417,17 @@ def main():
                      "step_frac": T * 30 * 24 / (sec64 / a.steps) / HBM_PEAK},
         "fp32": {"value": rate32, "ms_per_step": sec32 * 1e3 / a.steps,
                  "velocity_ms": kt32["velocity"], "stress_ms": kt32["stress"], "speedup_vs_fp64": rate32 / rate64},
+        "blocked_fp32": {"value": dof * a.steps / secb1, "ms_per_step": secb1 * 1e3 / a.steps,
+                         "launch_ms": ktb1["block"], "tiling": ktb1["info"],
+                         "x_vs_plain_fp64": dof * a.steps / secb1 / rate64},
         "full_step_fp64": {"value": rate64f, "ms_per_step": secf * 1e3 / a.steps, "kernel_ms": ktf["full"],
                            "note": "one kernel per step over tiles + recomputed ring (opt-in DGX_STEP_FULL)"},
     }
+    out["plain_fp64_large"] = large_leg(a)
+    progress(f"plain FP64, 2M elements: {out['plain_fp64_large']['value']:.3e} DOF-updates/s, "
+             f"{out['plain_fp64_large']['step_frac']:.3f} of HBM peak")
     out["accuracy"] = accuracy_leg(dataclasses.replace(w, nsteps=4000), {"fp32": (L.DGX_FP32, L.DGX_STEP_HALF),
+                                       "blocked_fp32": (L.DGX_FP32, L.DGX_STEP_BLOCKED),
                                        "full_step_fp64": (L.DGX_FP64, L.DGX_STEP_FULL)})
     ac = out["accuracy"]
     progress(f"accuracy over {ac['steps']} steps: fp32 cross traces {ac['fp32']['cross_traces']['all']:.2e}, "
@@ -375,10 +436,14 @@ def main():
     progress(f"end to end: {out['end_to_end_fp64']['value']:.3e} DOF-updates/s")
     if a.shots > 0:
         out["batched_fp32"] = batched_leg(a, rate64)
-        progress(f"batched FP32: {out['batched_fp32']['value']:.3e} DOF-updates/s")
+        progress(f"batched FP32 (shot-innermost): {out['batched_fp32']['shot_innermost']['value']:.3e} DOF-updates/s")
     if a.serve_requests > 0:
-        out["serving"] = serving_leg(a, rate64)
-        progress(f"serving: {out['serving']['value']:.3e} DOF-updates/s")
+        out["serving"] = serving_leg(a, rate64, a.serve_group)
+        progress(f"serving ({a.serve_group}-shot groups): {out['serving']['value']:.3e} DOF-updates/s")
+        if a.serve_compare_group and a.serve_compare_group != a.serve_group:
+            out["serving"]["compare"] = serving_leg(a, rate64, a.serve_compare_group)
+            progress(f"serving ({a.serve_compare_group}-shot groups): "
+                     f"{out['serving']['compare']['value']:.3e} DOF-updates/s")
     if a.copy_probe:
         import ctypes as C
         nbytes, reps = 1 << 30, 20

@@ -24,8 +24,8 @@ hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int whic
 hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st);
 hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
-hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double *out, long tri, long spad, int il, long shot,
-                                hipStream_t st);
+hipError_t dgx_launch_unpermute(const void *f, const void *lo, int fp32, const int *perm, double *out, long tri, long spad,
+                                int il, long shot, hipStream_t st);
 hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int variant, hipStream_t st);
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
 int dgx_batched_shape_ok(int shot_tile, int wg_waves);
@@ -110,6 +110,7 @@ inline long long fidx(long long q, int i) { return ((q >> 6) * 6 + i) * 64 + (q 
 
 struct dgx_solver {
     int fp32 = 0;
+    int comp = 0;                    /* compensated FP32: a low word per value (blocked kernel) */
     int layout = DGX_LAYOUT_ELEMENT;
     int shot_tile = 0;               /* shot-innermost: shots per lane group */
     int wg_waves = 4;                /* shot-innermost: wavefronts per workgroup */
@@ -175,6 +176,7 @@ static void swap_copies(dgx_solver *s)
     for (int c = 0; c < 5; c++) {
         std::swap(s->a64.f[c], s->a64.g[c]);
         std::swap(s->a32.f[c], s->a32.g[c]);
+        std::swap(s->a32.fl[c], s->a32.gl[c]);
     }
 }
 
@@ -398,6 +400,9 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     for (int c = 0; c < 5; c++) {
         a.f[c] = f + c * per_field;
         a.g[c] = f2 ? f2 + c * per_field : nullptr;
+        /* compensated FP32: the low words follow the five fields in each copy */
+        a.fl[c] = s->comp ? f + (5 + c) * per_field : nullptr;
+        a.gl[c] = s->comp ? f2 + (5 + c) * per_field : nullptr;
     }
     a.fstride = s->vrows;
     a.nb2 = s->nb2;
@@ -684,12 +689,14 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     if (T > (1L << 28) || p->point_num > (1L << 30)) { dgx_set_error("dgx_create: mesh too large"); return DGX_EARG; }
     if (p->nshots < 1 || p->nsteps < 0 || p->nrec < 0 || p->nsrc < 0 || (p->nsrc > 0 && !p->src) ||
         (p->nrec > 0 && !p->rec) || (p->nsrc > 0 && (!p->wavelet || p->nwavelets < 1)) ||
-        (p->precision != DGX_FP64 && p->precision != DGX_FP32) || !(p->dt > 0)) {
+        (p->precision != DGX_FP64 && p->precision != DGX_FP32 && p->precision != DGX_FP32_COMPENSATED) ||
+        !(p->dt > 0)) {
         dgx_set_error("dgx_create: invalid shots/steps/sources/receivers/precision/dt");
         return DGX_EARG;
     }
     dgx_solver *s = new dgx_solver();
-    s->fp32 = (p->precision == DGX_FP32);
+    s->fp32 = (p->precision != DGX_FP64);
+    s->comp = (p->precision == DGX_FP32_COMPENSATED);
     s->tri = T;
     s->nshots = p->nshots;
     s->nsteps = p->nsteps;
@@ -718,8 +725,14 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         free_solver(s);
         return DGX_EARG;
     }
+    if (s->comp && ((p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_BLOCKED) ||
+                    s->layout != DGX_LAYOUT_ELEMENT)) {
+        dgx_set_error("dgx_create: compensated FP32 runs on the blocked kernel (element layout)");
+        free_solver(s);
+        return DGX_EARG;
+    }
     s->full = p->step_kernel == DGX_STEP_FULL;
-    s->blocked = p->step_kernel == DGX_STEP_BLOCKED;
+    s->blocked = p->step_kernel == DGX_STEP_BLOCKED || s->comp;
     s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : 2) : 0;
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         const int t = p->shot_tile;
@@ -882,6 +895,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     for (auto &ev : s->ev) CREATE_TRY(hipEventCreate(&ev));
     size_t nreal_fields = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)5 * s->tri_pad * 6 * s->spad
                                                                  : (size_t)5 * s->nshots * s->vrows * 6;
+    if (s->comp) nreal_fields *= 2; /* high and low words */
     CREATE_TRY(hipMalloc(&s->fields, nreal_fields * s->real_size()));
     CREATE_TRY(hipMemset(s->fields, 0, nreal_fields * s->real_size()));
     if (s->full || s->blocked) {
@@ -907,7 +921,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         hipError_t e = upload(&m, convert<float>(mat_host));
         s->mat = m;
         CREATE_TRY(e);
-        e = upload(&a, convert<float>(absorb_host));
+        e = upload(&a, dgx_sponge_deficit_f32(absorb_host));
         s->absorb = a;
         CREATE_TRY(e);
     } else {
@@ -1114,12 +1128,36 @@ extern "C" int dgx_step(dgx_solver *s, long n)
     return DGX_OK;
 }
 
-extern "C" int dgx_absorb_boundary(dgx_solver *s) { return s ? launch(s, K_ABSORB, 0) : DGX_EARG; }
-extern "C" int dgx_lu_velocity(dgx_solver *s) { return s ? launch(s, K_VEL, 0) : DGX_EARG; }
-extern "C" int dgx_lu_stress_elastic(dgx_solver *s) { return s ? launch(s, K_STRESS, 0) : DGX_EARG; }
+/* The split entry points act on single FP64 or FP32 values: not on the compensated precision's word pairs. */
+static int split_ok(const dgx_solver *s)
+{
+    if (!s->comp) return DGX_OK;
+    dgx_set_error("the split entry points need DGX_FP64 or DGX_FP32 (compensated FP32 runs through dgx_step)");
+    return DGX_ESTATE;
+}
+
+extern "C" int dgx_absorb_boundary(dgx_solver *s)
+{
+    if (!s) return DGX_EARG;
+    if (int rc = split_ok(s)) return rc;
+    return launch(s, K_ABSORB, 0);
+}
+extern "C" int dgx_lu_velocity(dgx_solver *s)
+{
+    if (!s) return DGX_EARG;
+    if (int rc = split_ok(s)) return rc;
+    return launch(s, K_VEL, 0);
+}
+extern "C" int dgx_lu_stress_elastic(dgx_solver *s)
+{
+    if (!s) return DGX_EARG;
+    if (int rc = split_ok(s)) return rc;
+    return launch(s, K_STRESS, 0);
+}
 
 static int needs_element_layout(const dgx_solver *s)
 {
+    if (int rc = split_ok(s)) return rc;
     if (s->layout == DGX_LAYOUT_ELEMENT) return DGX_OK;
     dgx_set_error("the split entry points need the element layout (DGX_LAYOUT_ELEMENT)");
     return DGX_ESTATE;
@@ -1235,7 +1273,13 @@ extern "C" int dgx_snapshot_begin(dgx_solver *s, long shot, int mask)
         if (!((mask >> c) & 1)) continue;
         size_t off, len;
         raw_span(s, c, shot, &off, &len);
-        HIP_TRY(dgx_launch_unpermute((const char *)s->state() + off, s->fp32, s->d_perm, s->d_snap + c * nf, s->tri,
+        const void *lo = nullptr;
+        if (s->comp) {
+            size_t loff, llen;
+            raw_span(s, c + 5, shot, &loff, &llen);
+            lo = (const char *)s->state() + loff;
+        }
+        HIP_TRY(dgx_launch_unpermute((const char *)s->state() + off, lo, s->fp32, s->d_perm, s->d_snap + c * nf, s->tri,
                                      spad, s->il, shot, s->stream));
     }
     HIP_TRY(hipEventRecord(s->snap_ready, s->stream));
@@ -1292,6 +1336,13 @@ extern "C" int dgx_get_fields(dgx_solver *s, long shot, double *vx, double *vz, 
         HIP_TRY(hipMemcpy(raw.data(), (const char *)s->state() + off, len, hipMemcpyDeviceToHost));
         if (s->fp32) unpack<float>(s, raw, shot, outs[c]);
         else unpack<double>(s, raw, shot, outs[c]);
+        if (s->comp) { /* add the low words */
+            std::vector<double> lo((size_t)s->tri * 6);
+            raw_span(s, c + 5, shot, &off, &len);
+            HIP_TRY(hipMemcpy(raw.data(), (const char *)s->state() + off, len, hipMemcpyDeviceToHost));
+            unpack<float>(s, raw, shot, lo.data());
+            for (size_t k = 0; k < lo.size(); k++) outs[c][k] += lo[k];
+        }
     }
     return DGX_OK;
 }
@@ -1313,6 +1364,14 @@ extern "C" int dgx_set_fields(dgx_solver *s, long shot, const double *vx, const 
         if (s->fp32) pack<float>(s, ins[c], shot, raw);
         else pack<double>(s, ins[c], shot, raw);
         HIP_TRY(hipMemcpy((char *)s->state() + off, raw.data(), len, hipMemcpyHostToDevice));
+        if (s->comp) { /* low word: what the FP32 high word leaves of the value */
+            std::vector<double> rest((size_t)s->tri * 6);
+            for (size_t k = 0; k < rest.size(); k++) rest[k] = ins[c][k] - (double)(float)ins[c][k];
+            raw_span(s, c + 5, shot, &off, &len);
+            HIP_TRY(hipMemcpy(raw.data(), (const char *)s->state() + off, len, hipMemcpyDeviceToHost));
+            pack<float>(s, rest.data(), shot, raw);
+            HIP_TRY(hipMemcpy((char *)s->state() + off, raw.data(), len, hipMemcpyHostToDevice));
+        }
     }
     return DGX_OK;
 }
@@ -1347,8 +1406,15 @@ extern "C" int dgx_kernel_bench(dgx_solver *s, int which, long n, double *ms)
     /* n back-to-back launches of one fused kernel between two events: the kernel's own duration
      * without per-launch event overhead.  The launches repeat the current step, so the fields are
      * no longer a valid state afterwards. */
-    if (!s || n <= 0 || which < 0 || which > 3 || s->step >= s->nsteps) return DGX_EARG;
-    if (which == 3 && s->step + s->bkmax > s->nsteps) return DGX_EARG;
+    if (!s || n <= 0 || which < 0 || which > 3) {
+        dgx_set_error("dgx_kernel_bench: bad arguments");
+        return DGX_EARG;
+    }
+    if (s->step + (which == 3 ? s->bkmax : 1) > s->nsteps) {
+        dgx_set_error("dgx_kernel_bench: the launches repeat the next step, and step %ld + %d exceeds the %ld-step "
+                      "capacity", s->step, which == 3 ? s->bkmax : 1, s->nsteps);
+        return DGX_EARG;
+    }
     int off = (int)(s->step - s->base_host);
     int k = which == 0 ? K_VEL_FUSED : which == 1 ? K_STRESS_FUSED : which == 2 ? K_FULL : K_BLOCK;
     int rc = launch(s, k, off); /* warm */

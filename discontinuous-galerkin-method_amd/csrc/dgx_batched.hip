@@ -25,6 +25,7 @@ using dgx::face_node;
 using dgx::Geom;
 using dgx::geometry;
 using dgx::nb_node;
+using dgx::sponge;
 using dgx::Ops;
 using dgx::StepArgs;
 
@@ -81,7 +82,7 @@ __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned 
 {
     if (code & DGX_CODE_SPONGE) {
         const R f = a.absorb[q * 6 + i];
-        txx *= f; tzz *= f; txz *= f;
+        txx = sponge(txx, f); tzz = sponge(tzz, f); txz = sponge(txz, f);
     }
     const unsigned ax = aux_of(a, q, code);
     if (ax && lstep >= 0) {
@@ -105,7 +106,7 @@ __device__ __forceinline__ void condition_stress6(const StepArgs<R> &a, unsigned
 #pragma unroll
         for (int i = 0; i < 6; i++) {
             const R f = a.absorb[q * 6 + i];
-            txx[i] *= f; tzz[i] *= f; txz[i] *= f;
+            txx[i] = sponge(txx[i], f); tzz[i] = sponge(tzz[i], f); txz[i] = sponge(txz[i], f);
         }
     }
     const unsigned ax = aux_of(a, q, code);
@@ -235,8 +236,8 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 const R f = a.absorb[q * 6 + i];
-                vxe[i] *= f;
-                vze[i] *= f;
+                vxe[i] = sponge(vxe[i], f);
+                vze[i] = sponge(vze[i], f);
             }
         }
         if (ax && lstep >= 0) {
@@ -265,8 +266,8 @@ __device__ __forceinline__ void velocity_body(const Ops<R> &op, const StepArgs<R
         else { vx = Vx[o]; vz = Vz[o]; }
         if (code & DGX_CODE_SPONGE) {
             const R f = a.absorb[q * 6 + i];
-            vx *= f;
-            vz *= f;
+            vx = sponge(vx, f);
+            vz = sponge(vz, f);
         }
         if (ax && lstep >= 0) {
             const dgx_aux x = a.aux[ax - 1];

@@ -26,6 +26,18 @@ __device__ __forceinline__ T *as_global(T *p)
     return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
 }
 
+/* Sponge factor applied to a value (Absorb_Boundary, OpenMP_Function.c:25-46).  FP64 arrays hold the factor
+ * f itself, so the product is the reference's bit for bit.  FP32 arrays hold its deficit d = 1 - f (f lies in
+ * [0.9755, 1] for the reference's beta = 0.1, so d in [0, 0.0245] is represented about 40x more accurately
+ * than f), applied as x - x d in one fused operation: the factor's representation error, which compounds
+ * once per step, stays below 2e-9 instead of 3e-8. */
+template <typename R>
+__device__ __forceinline__ R sponge(R x, R a)
+{
+    if constexpr (sizeof(R) == 8) return x * a;
+    else return __builtin_fmaf(-x, a, x);
+}
+
 /* Local node of face j, position k, in the reference's face order (ibe, fun_DG.c:1048-1056). */
 __device__ __forceinline__ constexpr int face_node(int j, int k)
 {

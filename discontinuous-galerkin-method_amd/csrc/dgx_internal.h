@@ -23,6 +23,15 @@ struct dgx_mesh_host {
     std::vector<double> absorb; /* 6 * tri_pad */
 };
 
+/* Sponge factors as the FP32 kernels read them: the deficit 1 - f, rounded once (the device helper sponge()
+ * applies x - x (1 - f)).  FP64 kernels read f itself. */
+inline std::vector<float> dgx_sponge_deficit_f32(const std::vector<double> &f)
+{
+    std::vector<float> d(f.size());
+    for (size_t k = 0; k < f.size(); k++) d[k] = (float)(1.0 - f[k]);
+    return d;
+}
+
 int dgx_build_mesh_host(const dgx_problem *p, int shot_inner, long tri_pad, const std::vector<char> *explosive,
                         dgx_mesh_host *out);
 

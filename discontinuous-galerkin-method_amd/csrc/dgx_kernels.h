@@ -110,6 +110,7 @@ struct StepArgs {
     const int *blevels;    /* per tile DGX_BLOCK_LEVELS ints: end lane of each level (hops from the own elements) */
     const int *bgid;       /* per lane record: the element */
     const int4 *blnb;      /* per lane record: {lane of face j's neighbour (j = 0..2), code}; -1 boundary */
+    R *fl[5], *gl[5];      /* compensated FP32 (blocked kernel): the low words of f and g; NULL otherwise */
     int bkmax;             /* steps the tiles' levels allow per launch */
     int bthreads;          /* threads per workgroup (lanes per tile at most) */
     int bsteps;            /* steps of this launch (1..bkmax) */

@@ -78,7 +78,7 @@ __device__ __forceinline__ void condition_stress(const StepArgs<R> &a, unsigned 
 {
     if (code & DGX_CODE_SPONGE) {
         const R f = a.absorb[fidx(q, i)];
-        txx *= f; tzz *= f; txz *= f;
+        txx = sponge(txx, f); tzz = sponge(tzz, f); txz = sponge(txz, f);
     }
     const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
     if (ax) {
@@ -103,7 +103,7 @@ __device__ __forceinline__ void condition_stress6(const StepArgs<R> &a, unsigned
 #pragma unroll
         for (int i = 0; i < 6; i++) {
             const R f = a.absorb[fidx(q, i)];
-            txx[i] *= f; tzz[i] *= f; txz[i] *= f;
+            txx[i] = sponge(txx[i], f); tzz[i] = sponge(tzz[i], f); txz[i] = sponge(txz[i], f);
         }
     }
     const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
@@ -261,8 +261,8 @@ __device__ __forceinline__ void condition_velocity6(const StepArgs<R> &a, unsign
 #pragma unroll
         for (int i = 0; i < 6; i++) {
             const R f = a.absorb[fidx(q, i)];
-            vx[i] *= f;
-            vz[i] *= f;
+            vx[i] = sponge(vx[i], f);
+            vz[i] = sponge(vz[i], f);
         }
     }
     const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
@@ -286,8 +286,8 @@ __device__ __forceinline__ void condition_velocity(const StepArgs<R> &a, unsigne
 {
     if (code & DGX_CODE_SPONGE) {
         const R f = a.absorb[o];
-        vx *= f;
-        vz *= f;
+        vx = sponge(vx, f);
+        vz = sponge(vz, f);
     }
     const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
     if (ax) {
@@ -761,19 +761,46 @@ __device__ __forceinline__ double block_wavelet(const StepArgs<R> &a, const dgx_
     else return a.wav[(long long)src.wavelet * a.nsteps + lstep];
 }
 
+/* Compensated accumulation: the value h + l (l the low word) plus d, with the sum rounded into h and its
+ * exact rounding error (TwoSum: Knuth, TAOCP vol. 2, 4.2.2) kept in l.  The increment absorbs the old low
+ * word first; that rounding is second order (|l| <= ulp(h)/2). */
+__device__ __forceinline__ void comp_add(float &h, float &l, float d)
+{
+    const float y = d + l;
+    const float s = h + y;
+    const float bb = s - h;
+    l = (h - (s - bb)) + (y - bb);
+    h = s;
+}
+
+/* Compensated sponge (h + l)(1 - d), d the FP32 deficit of the factor (sponge()): h d = p + e exactly (FMA), and
+ * h - p is accumulated with TwoSum. */
+__device__ __forceinline__ void comp_sponge(float &h, float &l, float d)
+{
+    const float p = h * d;
+    const float e = __builtin_fmaf(h, d, -p);
+    l = l - l * d - e;
+    comp_add(h, l, -p);
+}
+
 /* Sponge factors (ab, loaded once per launch) and the element's sources (aux record ax, 0 = none) at local
  * step `lstep` applied to its stresses and, with VEL, its velocities: Absorb_Boundary then source_kernel
  * (OpenMP_Function.c:25-58); explosive sources add to Txx and Tzz, force sources to Vz. */
-template <typename R, bool VEL, bool SERVE>
+template <typename R, bool VEL, bool SERVE, bool COMP>
 __device__ __forceinline__ void condition_block(const StepArgs<R> &a, unsigned code, unsigned ax, int shot, int lstep,
                                                 const R (&ab)[6], R (&vx)[6], R (&vz)[6], R (&txx)[6],
-                                                R (&tzz)[6], R (&txz)[6])
+                                                R (&tzz)[6], R (&txz)[6], R (&lo)[5][6])
 {
     if (code & DGX_CODE_SPONGE) {
 #pragma unroll
         for (int i = 0; i < 6; i++) {
-            txx[i] *= ab[i]; tzz[i] *= ab[i]; txz[i] *= ab[i];
-            if (VEL) { vx[i] *= ab[i]; vz[i] *= ab[i]; }
+            if constexpr (COMP) {
+                comp_sponge(txx[i], lo[2][i], ab[i]); comp_sponge(tzz[i], lo[3][i], ab[i]); comp_sponge(txz[i], lo[4][i], ab[i]);
+                if (VEL) { comp_sponge(vx[i], lo[0][i], ab[i]); comp_sponge(vz[i], lo[1][i], ab[i]); }
+            } else {
+                txx[i] = sponge(txx[i], ab[i]); tzz[i] = sponge(tzz[i], ab[i]); txz[i] = sponge(txz[i], ab[i]);
+                if (VEL) { vx[i] = sponge(vx[i], ab[i]); vz[i] = sponge(vz[i], ab[i]); }
+            }
         }
     }
     if (ax) {
@@ -785,8 +812,13 @@ __device__ __forceinline__ void condition_block(const StepArgs<R> &a, unsigned c
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 const R inc = (R)(src.fun[i] * w);
-                if (src.kind == 1) { txx[i] += inc; tzz[i] += inc; }
-                else if (VEL) vz[i] += inc;
+                if constexpr (COMP) {
+                    if (src.kind == 1) { comp_add(txx[i], lo[2][i], inc); comp_add(tzz[i], lo[3][i], inc); }
+                    else if (VEL) comp_add(vz[i], lo[1][i], inc);
+                } else {
+                    if (src.kind == 1) { txx[i] += inc; tzz[i] += inc; }
+                    else if (VEL) vz[i] += inc;
+                }
             }
         }
     }
@@ -794,9 +826,9 @@ __device__ __forceinline__ void condition_block(const StepArgs<R> &a, unsigned c
 
 /* Receivers of the element (aux record ax) from the velocities it holds (Record, OpenMP_Function.c:61-72, with
  * interpolation weights): trace slot lstep of the solver's shot, or of the serving slot's trace buffer. */
-template <typename R, bool SERVE>
+template <typename R, bool SERVE, bool COMP>
 __device__ __forceinline__ void record_block(const StepArgs<R> &a, unsigned ax, int shot, int lstep, const R (&vx)[6],
-                                             const R (&vz)[6])
+                                             const R (&vz)[6], const R (&lo)[5][6])
 {
     if (!ax || (!SERVE && lstep >= a.nsteps)) return;
     const dgx_aux x = a.aux[ax - 1];
@@ -806,8 +838,13 @@ __device__ __forceinline__ void record_block(const StepArgs<R> &a, unsigned ax, 
         double sx_ = 0, sz_ = 0;
 #pragma unroll
         for (int i = 0; i < 6; i++) {
-            sx_ += rc.w[i] * (double)vx[i];
-            sz_ += rc.w[i] * (double)vz[i];
+            if constexpr (COMP) { /* the compensated value h + l */
+                sx_ += rc.w[i] * ((double)vx[i] + (double)lo[0][i]);
+                sz_ += rc.w[i] * ((double)vz[i] + (double)lo[1][i]);
+            } else {
+                sx_ += rc.w[i] * (double)vx[i];
+                sz_ += rc.w[i] * (double)vz[i];
+            }
         }
         const int nrec = SERVE ? a.slot_nrec[shot] : a.nrec;
         double *t = SERVE ? as_global(a.slot_trace[shot]) + (long long)lstep * 2 * nrec
@@ -835,7 +872,7 @@ __device__ __forceinline__ void record_block(const StepArgs<R> &a, unsigned ax, 
  * element and written once for every own element, against 7 reads and writes per step and element for the
  * half-step kernel pair.  Every thread of the workgroup reaches every barrier.
  */
-template <typename R, int NT, bool P2, bool SERVE>
+template <typename R, int NT, bool P2, bool SERVE, bool COMP = false>
 __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], R *const gout[5], int t, int shot,
                                            int nst, int lstep0, R *sts, R *svl)
 {
@@ -861,6 +898,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
     /* the stresses live in registers from their update to the next step's conditioning and in LDS (sts) in
      * between; the velocities stay in registers */
     R vx[6], vz[6], txx[6], tzz[6], txz[6], ab[6];
+    R lo[5][6]; /* compensated FP32: the low words of Vx, Vz, Txx, Tzz, Txz (unused otherwise) */
     GeomR<R> g;
     R rho_inv = 0, lam = 0, mu = 0;
     if (lev <= ghost) {
@@ -868,7 +906,8 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
         for (int i = 0; i < 6; i++) {
             const int o = fidx(q, i);
             txx[i] = f[2][so + o]; tzz[i] = f[3][so + o]; txz[i] = f[4][so + o];
-            ab[i] = (code & DGX_CODE_SPONGE) ? a.absorb[o] : (R)1;
+            ab[i] = (code & DGX_CODE_SPONGE) ? a.absorb[o] : (R)0; /* used only with the sponge bit */
+            if constexpr (COMP) { lo[2][i] = a.fl[2][so + o]; lo[3][i] = a.fl[3][so + o]; lo[4][i] = a.fl[4][so + o]; }
         }
     }
     if (upd) {
@@ -876,6 +915,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
         for (int i = 0; i < 6; i++) {
             const int o = fidx(q, i);
             vx[i] = f[0][so + o]; vz[i] = f[1][so + o];
+            if constexpr (COMP) { lo[0][i] = a.fl[0][so + o]; lo[1][i] = a.fl[1][so + o]; }
         }
         g = geometry_r<R>(geometry(a.pt, a.vtx[q]));
         rho_inv = a.rho_inv[q];
@@ -885,8 +925,8 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
     for (int j = 0; j < nst; j++) {
         const int lstep = lstep0 + j;
         if (upd || (lev == ghost && j == 0)) {
-            if (upd) condition_block<R, true, SERVE>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz);
-            else condition_block<R, false, SERVE>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz);
+            if (upd) condition_block<R, true, SERVE, COMP>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz, lo);
+            else condition_block<R, false, SERVE, COMP>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz, lo);
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 sts[(0 * 6 + i) * NT + l] = txx[i];
@@ -921,8 +961,13 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             const R c = rho_inv * (R)a.dt;
 #pragma unroll
             for (int i = 0; i < 6; i++) {
-                vx[i] = (rhx[i] + flx[i] * g.invD) * c + vx[i];
-                vz[i] = (rhz[i] + flz[i] * g.invD) * c + vz[i];
+                if constexpr (COMP) {
+                    comp_add(vx[i], lo[0][i], (rhx[i] + flx[i] * g.invD) * c);
+                    comp_add(vz[i], lo[1][i], (rhz[i] + flz[i] * g.invD) * c);
+                } else {
+                    vx[i] = (rhx[i] + flx[i] * g.invD) * c + vx[i];
+                    vz[i] = (rhz[i] + flz[i] * g.invD) * c + vz[i];
+                }
             }
         }
         if (upd) {
@@ -958,11 +1003,17 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 const R axx = exx[i] + fx[i] * g.invD, azz = ezz[i] + fz[i] * g.invD, axz = exz[i] + fxz[i] * g.invD;
-                txx[i] = (l2m * axx + lam * azz) * dt + txx[i];
-                tzz[i] = (lam * axx + l2m * azz) * dt + tzz[i];
-                txz[i] = (mu * axz) * dt + txz[i];
+                if constexpr (COMP) {
+                    comp_add(txx[i], lo[2][i], (l2m * axx + lam * azz) * dt);
+                    comp_add(tzz[i], lo[3][i], (lam * axx + l2m * azz) * dt);
+                    comp_add(txz[i], lo[4][i], (mu * axz) * dt);
+                } else {
+                    txx[i] = (l2m * axx + lam * azz) * dt + txx[i];
+                    tzz[i] = (lam * axx + l2m * azz) * dt + tzz[i];
+                    txz[i] = (mu * axz) * dt + txz[i];
+                }
             }
-            if (lev == 0) record_block<R, SERVE>(a, ax, shot, lstep, vx, vz);
+            if (lev == 0) record_block<R, SERVE, COMP>(a, ax, shot, lstep, vx, vz, lo);
         }
     }
     if (lev == 0) {
@@ -971,6 +1022,10 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             const int o = fidx(q, i);
             gout[0][so + o] = vx[i]; gout[1][so + o] = vz[i];
             gout[2][so + o] = txx[i]; gout[3][so + o] = tzz[i]; gout[4][so + o] = txz[i];
+            if constexpr (COMP) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) a.gl[c][so + o] = lo[c][i];
+            }
         }
     }
 }
@@ -978,7 +1033,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
 /* The solver's blocked kernel: workgroup (b, shot) runs tile (b mod 8) * tper + b / 8 (XCD-contiguous runs of
  * tiles) of that shot for bsteps steps.  MINW: at least that many wavefronts per SIMD (512-thread tiles in
  * FP32: 4, two workgroups per CU). */
-template <typename R, int NT, int MINW, bool P2>
+template <typename R, int NT, int MINW, bool P2, bool COMP = false>
 __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
 {
     __shared__ R sts[18 * NT]; /* conditioned stresses, (c*6 + i)*NT + lane */
@@ -987,7 +1042,7 @@ __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
     const int t = (b & (DGX_NXCD - 1)) * a.tper + (b >> 3);
     if (t >= a.ntile) return; /* uniform over the workgroup */
     const int step0 = *a.step_base + a.step_off;
-    block_tile<R, NT, P2, false>(a, a.f, a.g, t, blockIdx.y, a.bsteps, step0, sts, svl);
+    block_tile<R, NT, P2, false, COMP>(a, a.f, a.g, t, blockIdx.y, a.bsteps, step0, sts, svl);
 }
 
 /*
@@ -1058,7 +1113,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_absorb(StepArgs<R> a)
         const int o = fidx(q, i);
         const R f = a.absorb[o];
 #pragma unroll
-        for (int c = 0; c < 5; c++) a.f[c][so + o] *= f;
+        for (int c = 0; c < 5; c++) a.f[c][so + o] = sponge(a.f[c][so + o], f);
     }
 }
 
@@ -1151,17 +1206,19 @@ __global__ void __launch_bounds__(256) k_read4(const double4 *__restrict__ src, 
  * shot-innermost layout (node rows of spad shots interleaving 2^il elements:
  * ((((q >> il)*6 + i) << il) + q mod 2^il)*spad + shot). */
 template <typename R>
-__global__ void __launch_bounds__(256) k_unpermute(const R *__restrict__ f, const int *__restrict__ perm,
-                                                   double *__restrict__ out, int tri, long long spad, int il,
-                                                   long long shot)
+__global__ void __launch_bounds__(256) k_unpermute(const R *__restrict__ f, const R *__restrict__ lo,
+                                                   const int *__restrict__ perm, double *__restrict__ out, int tri,
+                                                   long long spad, int il, long long shot)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= tri) return;
     double *o = out + 6 * (long long)perm[q];
     const long long row0 = (long long)(q >> il) * 6, sub = q & ((1 << il) - 1);
 #pragma unroll
-    for (int i = 0; i < 6; i++)
-        o[i] = (double)(spad > 0 ? f[(((row0 + i) << il) + sub) * spad + shot] : f[fidx(q, i)]);
+    for (int i = 0; i < 6; i++) {
+        const long long k = spad > 0 ? (((row0 + i) << il) + sub) * spad + shot : fidx(q, i);
+        o[i] = lo ? (double)f[k] + (double)lo[k] : (double)f[k]; /* compensated FP32: high + low word */
+    }
 }
 
 __global__ void k_advance(int *step_base, int n)
@@ -1189,7 +1246,9 @@ static void launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st
         /* one element per thread, a.bthreads threads (the tiles are built for that count): FP32 1024 (one
          * workgroup per CU) or 512 (two), FP64 512 */
         if constexpr (sizeof(R) == 4) {
-            if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 1024, 0, st>>>(a);
+            /* compensated FP32: twice the registers for the low words, one 512-thread workgroup per CU */
+            if (a.fl[0]) dgx::k_block<R, 512, 2, P2, true><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
+            else if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 1024, 0, st>>>(a);
             else dgx::k_block<R, 512, 4, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
         } else {
             dgx::k_block<R, 512, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
@@ -1237,12 +1296,12 @@ hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int varian
     else dgx::k_read4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double *)dst, n32);
     return hipGetLastError();
 }
-hipError_t dgx_launch_unpermute(const void *f, int fp32, const int *perm, double *out, long tri, long spad, int il,
-                                long shot, hipStream_t st)
+hipError_t dgx_launch_unpermute(const void *f, const void *lo, int fp32, const int *perm, double *out, long tri,
+                                long spad, int il, long shot, hipStream_t st)
 {
     const unsigned blocks = (unsigned)((tri + 255) / 256);
-    if (fp32) dgx::k_unpermute<float><<<blocks, 256, 0, st>>>((const float *)f, perm, out, (int)tri, spad, il, shot);
-    else dgx::k_unpermute<double><<<blocks, 256, 0, st>>>((const double *)f, perm, out, (int)tri, spad, il, shot);
+    if (fp32) dgx::k_unpermute<float><<<blocks, 256, 0, st>>>((const float *)f, (const float *)lo, perm, out, (int)tri, spad, il, shot);
+    else dgx::k_unpermute<double><<<blocks, 256, 0, st>>>((const double *)f, (const double *)lo, perm, out, (int)tri, spad, il, shot);
     return hipGetLastError();
 }
 /* Serving launch of the blocked kernel over the tiles of nb single-shot batches (ks_block). */

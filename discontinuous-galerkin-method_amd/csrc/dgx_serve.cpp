@@ -450,7 +450,15 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     for (int c = 0; c < 3; c++)
         std::copy(mh.mat.begin() + c * tri_pad, mh.mat.begin() + c * tri_pad + p->tri, mat0.begin() + c * p->tri);
     if (e == hipSuccess) e = upload_reals(sv, &md.d_mat, mat0);
-    if (e == hipSuccess) e = upload_reals(sv, &m.absorb, mh.absorb);
+    if (e == hipSuccess) {
+        if (sv->fp32) {
+            float *d = nullptr;
+            e = upload_vec(&d, dgx_sponge_deficit_f32(mh.absorb));
+            m.absorb = d;
+        } else {
+            e = upload_reals(sv, &m.absorb, mh.absorb);
+        }
+    }
     if (e == hipSuccess && sv->group == 1) {
         e = upload_vec(&m.btiles, bt);
         if (e == hipSuccess) e = upload_vec(&m.blnb, bl);

@@ -86,7 +86,6 @@ def run_slope(a):
         x32, z32 = s32.traces(k)
         f32 = s32.fields(k)
         checks.append({"shot": k, "traces": trace_errors(x32, z32, x64, z64),
-                       "traces_floor_1e-4": trace_errors(x32, z32, x64, z64, floor=1e-4),
                        "snapshot": snapshot_errors(f32, f64),
                        "fp64_finite": bool(np.all(np.isfinite(f64))), "fp32_finite": bool(np.all(np.isfinite(f32)))})
         log(f"  shot {k}: {checks[-1]}")
@@ -137,8 +136,7 @@ def run_carbonate(a):
     for k in picks:
         ((x64, z64), _), _ = single_shot_reference(w, k, st.dt, a.steps)
         x32, z32 = res[k // per_req][0][k % per_req], res[k // per_req][1][k % per_req]
-        errs.append({"shot": k, "floor_1e-2": trace_errors(x32, z32, x64, z64),
-                     "floor_1e-4": trace_errors(x32, z32, x64, z64, floor=1e-4)})
+        errs.append({"shot": k, "traces": trace_errors(x32, z32, x64, z64)})
     return {"workload": w.name, "elements": T, "shots": a.shots, "steps": a.steps, "dt": st.dt, "group_shots": a.group,
             "served_fp32_dof_updates_per_s": stats["dof_updates"] / wall, "wall_s": wall,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),

@@ -27,7 +27,11 @@ extern "C" {
 #define DGX_EIO (-5)    /* file could not be read or written */
 #define DGX_ESTATE (-6) /* call not valid in the solver's current state */
 
-enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1 };
+/* DGX_FP32_COMPENSATED: FP32 arithmetic and FP32 storage with a second FP32 word per value that carries the
+ * rounding error of every update (TwoSum accumulation), so that accumulated rounding does not grow with the
+ * number of steps; snapshots and traces return the sum of the two words.  Runs on the blocked kernel only
+ * (element layout) and moves twice the field bytes. */
+enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1, DGX_FP32_COMPENSATED = 2 };
 enum dgx_source_kind { DGX_SRC_FORCE_Z = 0, DGX_SRC_EXPLOSIVE = 1 };
 /* Field layout on the device.  ELEMENT: 64-element blocks, one thread per element and shot (the
  * single-shot path).  SHOT_INNER: shots innermost, lanes over shots, for many shots on one mesh.
