@@ -24,6 +24,7 @@ hipError_t dgx_launch_f32(const void *args, int nshots, hipStream_t st, int whic
 hipError_t dgx_launch_source(const void *args, int fp32, int nsrc, double w, hipStream_t st);
 hipError_t dgx_launch_record(const void *args, int fp32, int t, int nshots, int nrec, hipStream_t st);
 hipError_t dgx_launch_advance(int *step_base, int n, hipStream_t st);
+hipError_t dgx_noalloc_scratch(double **ptr);
 hipError_t dgx_launch_unpermute(const void *f, const void *lo, int fp32, const int *perm, double *out, long tri, long spad,
                                 int il, long shot, hipStream_t st);
 hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int variant, hipStream_t st);
@@ -124,6 +125,8 @@ struct dgx_solver {
     int nper = 0;
     std::vector<long> perm; /* device element -> caller element (0-based) */
     std::vector<long> inv;  /* caller element -> device element */
+    int device = 0;         /* the HIP device current at dgx_create */
+    int *d_perm = nullptr;  /* device element -> caller element, on the device */
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; /* step graphs, by which field copy holds the state */
     int full = 0;           /* one full-step kernel per step (element layout) */
@@ -157,7 +160,6 @@ struct dgx_solver {
     std::vector<dgx_src_dev> src_host;
     /* snapshots (allocated at the first one): the fields of one shot in the caller's element order as
      * doubles on the device, their pinned host copy, and the stream that moves them while stepping goes on */
-    int *d_perm = nullptr;
     double *d_snap = nullptr, *h_snap = nullptr;
     hipStream_t copy_stream = nullptr;
     hipEvent_t snap_ready = nullptr, snap_done = nullptr;
@@ -891,6 +893,11 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         free_solver(s);
         return DGX_EHIP;
     }
+    CREATE_TRY(hipGetDevice(&s->device));
+    {
+        double *scr = nullptr; /* dgx_get_fields_noalloc's staging on this device: look it up now */
+        CREATE_TRY(dgx_noalloc_scratch(&scr));
+    }
     CREATE_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (auto &ev : s->ev) CREATE_TRY(hipEventCreate(&ev));
     size_t nreal_fields = (s->layout == DGX_LAYOUT_SHOT_INNER) ? (size_t)5 * s->tri_pad * 6 * s->spad
@@ -950,6 +957,11 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     if (ntrace) {
         CREATE_TRY(hipMalloc((void **)&s->trace, ntrace * sizeof(double)));
         CREATE_TRY(hipMemset(s->trace, 0, ntrace * sizeof(double)));
+    }
+    {
+        /* device element -> caller element, for the reorders of dgx_get_fields and dgx_get_fields_noalloc */
+        std::vector<int> perm(s->perm.begin(), s->perm.end());
+        CREATE_TRY(upload(&s->d_perm, perm));
     }
     CREATE_TRY(hipMalloc((void **)&s->step_base, sizeof(int)));
     CREATE_TRY(hipMemset(s->step_base, 0, sizeof(int)));
@@ -1238,14 +1250,6 @@ static int snapshot_buffers(dgx_solver *s)
 {
     /* each piece once, so that a call after a failed allocation neither leaks nor repeats the others */
     const size_t n = (size_t)s->tri * 6 * 5;
-    if (!s->d_perm) {
-        std::vector<int> perm(s->perm.begin(), s->perm.end());
-        int *d = nullptr;
-        const hipError_t e = upload(&d, perm);
-        if (e != hipSuccess && d) (void)hipFree(d);
-        HIP_TRY(e);
-        s->d_perm = d;
-    }
     if (!s->d_snap) HIP_TRY(hipMalloc((void **)&s->d_snap, n * sizeof(double)));
     if (!s->copy_stream) HIP_TRY(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
     if (!s->snap_ready) HIP_TRY(hipEventCreateWithFlags(&s->snap_ready, hipEventDisableTiming));
@@ -1308,6 +1312,70 @@ extern "C" int dgx_snapshot_wait(dgx_solver *s, double *vx, double *vz, double *
     HIP_TRY(hipEventSynchronize(s->snap_done));
     for (int c = 0; c < 5; c++)
         if (((mask >> c) & 1) && outs[c]) std::memcpy(outs[c], s->h_snap + c * nf, nf * sizeof(double));
+    return DGX_OK;
+}
+
+/* The caller's current device, restored when a call that switched to the solver's device returns. */
+struct DeviceGuard {
+    int prev = -1;
+    hipError_t use(int dev)
+    {
+        hipError_t e = hipGetDevice(&prev);
+        if (e != hipSuccess) { prev = -1; return e; }
+        if (prev == dev) { prev = -1; return hipSuccess; }
+        return hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+extern "C" int dgx_get_fields_noalloc(dgx_solver *s, long shot, double *vx, double *vz, double *txx, double *tzz,
+                                      double *txz)
+{
+    if (!s || shot < 0 || shot >= s->nshots) {
+        dgx_set_error("dgx_get_fields_noalloc: bad solver or shot");
+        return DGX_EARG;
+    }
+    double *outs[5] = {vx, vz, txx, tzz, txz};
+    int nreq = 0;
+    for (int c = 0; c < 5; c++) nreq += outs[c] != nullptr;
+    if (!nreq) return DGX_OK;
+    const size_t nf = (size_t)s->tri * 6;
+    if ((size_t)nreq * nf * sizeof(double) > (size_t)DGX_NOALLOC_SCRATCH_BYTES) {
+        dgx_set_error("dgx_get_fields_noalloc: %d field(s) of %ld elements need %zu bytes of staging, more than the "
+                      "%ld bytes of the static array; request fewer fields per call or use dgx_get_fields",
+                      nreq, s->tri, (size_t)nreq * nf * sizeof(double), (long)DGX_NOALLOC_SCRATCH_BYTES);
+        return DGX_EARG;
+    }
+    DeviceGuard dg;
+    HIP_TRY(dg.use(s->device));
+    double *scr = nullptr;
+    HIP_TRY(dgx_noalloc_scratch(&scr));
+    const long spad = s->layout == DGX_LAYOUT_SHOT_INNER ? s->spad : 0;
+    /* on the solver's stream, after the steps enqueued so far: reorder each field into its slice of the
+     * array, then copy the slices out */
+    for (int c = 0, k = 0; c < 5; c++) {
+        if (!outs[c]) continue;
+        size_t off, len;
+        raw_span(s, c, shot, &off, &len);
+        const void *lo = nullptr;
+        if (s->comp) {
+            size_t loff, llen;
+            raw_span(s, c + 5, shot, &loff, &llen);
+            lo = (const char *)s->state() + loff;
+        }
+        HIP_TRY(dgx_launch_unpermute((const char *)s->state() + off, lo, s->fp32, s->d_perm, scr + k * nf, s->tri,
+                                     spad, s->il, shot, s->stream));
+        k++;
+    }
+    for (int c = 0, k = 0; c < 5; c++) {
+        if (!outs[c]) continue;
+        HIP_TRY(hipMemcpyAsync(outs[c], scr + k * nf, nf * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        k++;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return DGX_OK;
 }
 

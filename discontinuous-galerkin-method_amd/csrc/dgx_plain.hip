@@ -28,6 +28,9 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
+#include "dgx.h"
 #include "dgx_device.h"
 #include "dgx_kernels.h"
 #include "dgx_p2ops.h"
@@ -1296,6 +1299,34 @@ hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int varian
     else dgx::k_read4<<<(unsigned)((n32 + 1023) / 1024), 256, 0, st>>>((const double4 *)src, (double *)dst, n32);
     return hipGetLastError();
 }
+/* Staging of dgx_get_fields_noalloc: the requested fields of one shot, reordered into the caller's element
+ * order, before they are copied to the host.  One copy per GPU the library runs on, held until the process
+ * ends; host code reaches it only through the address hipGetSymbolAddress returns for the current device.
+ * Every entry a call reads is written by that call's reorder first, so nothing needs clearing. */
+static __device__ __align__(256) double scratch[DGX_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+#define DGX_MAX_DEVICES 64
+static std::mutex scratch_mu;
+static double *scratch_dev[DGX_MAX_DEVICES]; /* per device: the address of scratch on it, once looked up */
+
+/* Address of scratch on the current device (looked up on the first call on that device, with it current). */
+hipError_t dgx_noalloc_scratch(double **ptr)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= DGX_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_dev[dev] = (double *)p;
+    }
+    *ptr = scratch_dev[dev];
+    return hipSuccess;
+}
+
 hipError_t dgx_launch_unpermute(const void *f, const void *lo, int fp32, const int *perm, double *out, long tri,
                                 long spad, int il, long shot, hipStream_t st)
 {

@@ -20,6 +20,7 @@ DGX_SRC_FORCE_Z, DGX_SRC_EXPLOSIVE = 0, 1
 DGX_LAYOUT_AUTO, DGX_LAYOUT_ELEMENT, DGX_LAYOUT_SHOT_INNER = 0, 1, 2
 DGX_OK, DGX_EARG, DGX_EMESH, DGX_EHIP, DGX_ENOMEM, DGX_EIO, DGX_ESTATE = 0, -1, -2, -3, -4, -5, -6
 DGX_STEP_AUTO, DGX_STEP_HALF, DGX_STEP_FULL, DGX_STEP_BLOCKED = 0, 1, 2, 3
+DGX_NOALLOC_SCRATCH_BYTES = 32 << 20  # staging array of dgx_get_fields_noalloc, per GPU
 
 dptr = C.POINTER(C.c_double)
 lptr = C.POINTER(C.c_long)
@@ -102,6 +103,7 @@ def load() -> C.CDLL:
         "dgx_lu_stress_elastic": (C.c_int, [vp]),
         "dgx_record": (C.c_int, [vp, C.c_long]),
         "dgx_get_fields": (C.c_int, [vp, C.c_long] + [dptr] * 5),
+        "dgx_get_fields_noalloc": (C.c_int, [vp, C.c_long] + [dptr] * 5),
         "dgx_snapshot_begin": (C.c_int, [vp, C.c_long, C.c_int]),
         "dgx_snapshot_wait": (C.c_int, [vp] + [dptr] * 5),
         "dgx_set_fields": (C.c_int, [vp, C.c_long] + [dptr] * 5),

@@ -177,6 +177,19 @@ int dgx_block_info(const dgx_solver *s, int *steps_per_launch, long *ntile, long
  * Diagnostic with no counterpart in the reference, which reads its operators from files at start-up
  * (DG_leap_Frog_OpenMP.c:52-81); the two variants agree to rounding. */
 int dgx_compiled_operators(const dgx_solver *s);
+/* dgx_get_fields without allocating: the same fields, bit for bit, reordered on the device into a static
+ * staging array of DGX_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on (looked up at
+ * dgx_create), then copied into the caller's arrays; no device or host memory is allocated, on any call.  For
+ * callers that cannot allocate (tight serving loops, code that forbids device allocation); dgx_get_fields
+ * allocates its snapshot buffers on first use.  The requested fields must fit the array: (fields requested) *
+ * tri * 6 * 8 bytes <= DGX_NOALLOC_SCRATCH_BYTES, else DGX_EARG with nothing done (one field of a mesh of up to
+ * 699,050 elements, all five up to 139,810).  The array is shared by every call on one GPU and keeps the last
+ * call's contents: calls that overlap on one GPU (from several threads, or solvers on several streams) race;
+ * calls made one after another are safe, as each waits for its copies before it returns.  Blocks until the
+ * steps enqueued so far are done.  No counterpart in the reference. */
+#define DGX_NOALLOC_SCRATCH_BYTES (32L << 20)
+int dgx_get_fields_noalloc(dgx_solver *s, long shot, double *vx, double *vz, double *txx, double *tzz,
+                           double *txz);
 const char *dgx_last_error(void);
 
 #ifdef __cplusplus
