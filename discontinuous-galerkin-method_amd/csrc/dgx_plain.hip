@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <utility>
 
 #include "dgx.h"
 #include "dgx_device.h"
@@ -755,6 +756,84 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
                                a.g[3] + so, a.g[4] + so);
 }
 
+/* ---- Matrix-core contractions of the blocked FP32 kernel ----
+ * v_mfma_f32_4x4x1f32 in its 16-block form: block b (lanes 4b..4b+3) computes D_b = A_b B_b + C_b with A_b a
+ * 4x1 column (lane 4b+i holds row i), B_b a 1x4 row (lane 4b+j holds column j) and D_b[i][j] in register i of
+ * lane 4b+j.  With A broadcast from block c (cbsz 4, abid c) every lane receives column c of a table times its
+ * own B.  A chain over k of (operator rows 4g..4g+3 at input k) x (the lane's input k) therefore leaves rows
+ * 4g..4g+3 of the operator applied to the lane's own element in its four registers: 64 elements per
+ * instruction, nothing exchanged between lanes, each lane's result depending on its own inputs only.  The
+ * tables hold 16 (row group, input) columns per register, column c in lanes 4(c mod 16)..+3 of register
+ * c / 16.  An MFMA ignores the execution mask, so the chains are issued only where every lane of the wave is
+ * active (the tables are written with all lanes active at the kernel's start). */
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+struct MfOps {
+    float vol[2];  /* [Dr | Ds], 6 x 12, column c = 12 g + k: the velocity volume term on (r, s)-combined stresses */
+    float grad[2]; /* [Dr ; Ds], 12 x 6, c = 6 g + k: both derivatives of a velocity component */
+    float lift[2]; /* [S1 | S2 | S3], 6 x 9, c = 9 g + 3 j + k: the three face lifts */
+};
+
+template <bool P2>
+__device__ __forceinline__ MfOps mf_ops(const StepArgs<float> &a)
+{
+    MfOps m;
+    const int l = threadIdx.x & 63, i = l & 3;
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+        const int c = 16 * v + (l >> 2);
+        float x = 0.0f;
+        if (c < 24) {
+            const int row = 4 * (c / 12) + i, k = c % 12;
+            if (row < 6) x = k < 6 ? op_dr<P2>(a, row * 6 + k) : op_ds<P2>(a, row * 6 + k - 6);
+        }
+        m.vol[v] = x;
+        x = 0.0f;
+        if (c < 18) {
+            const int row = 4 * (c / 6) + i, k = c % 6;
+            x = row < 6 ? op_dr<P2>(a, row * 6 + k) : op_ds<P2>(a, (row - 6) * 6 + k);
+        }
+        m.grad[v] = x;
+        x = 0.0f;
+        if (c < 18) {
+            const int row = 4 * (c / 9) + i, kk = c % 9;
+            if (row < 6) x = op_l<P2>(a, (kk / 3) * 18 + row * 3 + kk % 3);
+        }
+        m.lift[v] = x;
+    }
+    return m;
+}
+
+template <int C>
+__device__ __forceinline__ mf4 mf_step(const float (&tab)[2], float x, mf4 acc)
+{
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(tab[C >> 4], x, acc, 4, C & 15, 0);
+}
+
+/* Rows 0..3 (r0) and 4..5 (r1[0..1]) of a 6-row table applied to the lane's inputs x[0..K-1]. */
+template <int K, int... I>
+__device__ __forceinline__ void mf_6rows(const float (&tab)[2], const float (&x)[K], mf4 &r0, mf4 &r1,
+                                         std::integer_sequence<int, I...>)
+{
+    r0 = mf4{0.0f, 0.0f, 0.0f, 0.0f};
+    r1 = r0;
+    ((r0 = mf_step<I>(tab, x[I], r0), r1 = mf_step<K + I>(tab, x[I], r1)), ...);
+}
+
+/* Rows 0..11 of the 12-row table [Dr ; Ds] applied to the lane's x[0..5]: Dr x in r0, r1[0..1]; Ds x in
+ * r1[2..3], r2. */
+template <int... I>
+__device__ __forceinline__ void mf_12rows(const float (&tab)[2], const float (&x)[6], mf4 &r0, mf4 &r1, mf4 &r2,
+                                          std::integer_sequence<int, I...>)
+{
+    r0 = mf4{0.0f, 0.0f, 0.0f, 0.0f};
+    r1 = r0;
+    r2 = r0;
+    ((r0 = mf_step<I>(tab, x[I], r0), r1 = mf_step<6 + I>(tab, x[I], r1), r2 = mf_step<12 + I>(tab, x[I], r2)), ...);
+}
+
+__device__ __forceinline__ float mf_row6(const mf4 &r0, const mf4 &r1, int i) { return i < 4 ? r0[i] : r1[i - 4]; }
+
 /* Wavelet sample of a source at local step `lstep`: the solver's wavelet table row src.wavelet, or in a
  * serving batch (SERVE) the source's slot row. */
 template <typename R, bool SERVE>
@@ -875,10 +954,13 @@ __device__ __forceinline__ void record_block(const StepArgs<R> &a, unsigned ax, 
  * element and written once for every own element, against 7 reads and writes per step and element for the
  * half-step kernel pair.  Every thread of the workgroup reaches every barrier.
  */
-template <typename R, int NT, bool P2, bool SERVE, bool COMP = false>
+template <typename R, int NT, bool P2, bool SERVE, bool COMP = false, bool MF = false>
 __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], R *const gout[5], int t, int shot,
                                            int nst, int lstep0, R *sts, R *svl)
 {
+    static_assert(!MF || sizeof(R) == 4, "matrix-core contractions are FP32 only");
+    MfOps mo;
+    if constexpr (MF) mo = mf_ops<P2>(a); /* every lane active here */
     const int4 td = a.tiles[t];
     const int l = threadIdx.x;
     const int ghost = 2 * a.bkmax;
@@ -900,9 +982,9 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
 
     /* the stresses live in registers from their update to the next step's conditioning and in LDS (sts) in
      * between; the velocities stay in registers */
-    R vx[6], vz[6], txx[6], tzz[6], txz[6], ab[6];
+    R vx[6] = {}, vz[6] = {}, txx[6] = {}, tzz[6] = {}, txz[6] = {}, ab[6] = {};
     R lo[5][6]; /* compensated FP32: the low words of Vx, Vz, Txx, Tzz, Txz (unused otherwise) */
-    GeomR<R> g;
+    GeomR<R> g = {};
     R rho_inv = 0, lam = 0, mu = 0;
     if (lev <= ghost) {
 #pragma unroll
@@ -938,7 +1020,57 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             }
         }
         __syncthreads();
-        if (lev <= 2 * nst - 1 - 2 * j) {
+        if constexpr (MF) {
+            const bool act = lev <= 2 * nst - 1 - 2 * j;
+            if (__any(act)) { /* wave-uniform: the MFMA chains run with every lane of the wave active */
+                float xin[12], zin[12], gx[9], gz[9];
+#pragma unroll
+                for (int n = 0; n < 6; n++) {
+                    xin[n] = g.rx * txx[n] + g.rz * txz[n];
+                    xin[6 + n] = g.sx * txx[n] + g.sz * txz[n];
+                    zin[n] = g.rx * txz[n] + g.rz * tzz[n];
+                    zin[6 + n] = g.sx * txz[n] + g.sz * tzz[n];
+                }
+#pragma unroll
+                for (int fc = 0; fc < 3; fc++) {
+                    const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
+                    const int s = nbs[fc];
+                    const float ex = g.ex[fc], ez = g.ez[fc];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const int on = face_node(fc, k), nn = nb_node(m, rev, k);
+                        float dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
+                        if (s >= 0) {
+                            dxx += sts[(0 * 6 + nn) * NT + s];
+                            dzz += sts[(1 * 6 + nn) * NT + s];
+                            dxz += sts[(2 * 6 + nn) * NT + s];
+                        }
+                        gx[3 * fc + k] = ez * dxx - ex * dxz;
+                        gz[3 * fc + k] = ez * dxz - ex * dzz;
+                    }
+                }
+                mf4 hx0, hx1, hz0, hz1, lx0, lx1, lz0, lz1;
+                mf_6rows<12>(mo.vol, xin, hx0, hx1, std::make_integer_sequence<int, 12>{});
+                mf_6rows<12>(mo.vol, zin, hz0, hz1, std::make_integer_sequence<int, 12>{});
+                mf_6rows<9>(mo.lift, gx, lx0, lx1, std::make_integer_sequence<int, 9>{});
+                mf_6rows<9>(mo.lift, gz, lz0, lz1, std::make_integer_sequence<int, 9>{});
+                if (act) {
+                    const R c = rho_inv * (R)a.dt;
+#pragma unroll
+                    for (int i = 0; i < 6; i++) {
+                        const R ix = (mf_row6(hx0, hx1, i) + mf_row6(lx0, lx1, i) * g.invD) * c;
+                        const R iz = (mf_row6(hz0, hz1, i) + mf_row6(lz0, lz1, i) * g.invD) * c;
+                        if constexpr (COMP) {
+                            comp_add(vx[i], lo[0][i], ix);
+                            comp_add(vz[i], lo[1][i], iz);
+                        } else {
+                            vx[i] = ix + vx[i];
+                            vz[i] = iz + vz[i];
+                        }
+                    }
+                }
+            }
+        } else if (lev <= 2 * nst - 1 - 2 * j) {
             R rhx[6], rhz[6], flx[6], flz[6];
             velocity_volume<R, P2>(a, g, txx, tzz, txz, rhx, rhz);
 #pragma unroll
@@ -981,7 +1113,57 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             }
         }
         __syncthreads();
-        if (lev <= 2 * nst - 2 - 2 * j) {
+        if constexpr (MF) {
+            const bool act = lev <= 2 * nst - 2 - 2 * j;
+            if (__any(act)) { /* wave-uniform, as above */
+                float px[9], pz[9], pxz[9];
+#pragma unroll
+                for (int fc = 0; fc < 3; fc++) {
+                    const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
+                    const int s = nbs[fc];
+                    const float ex = g.ex[fc], ez = g.ez[fc];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const int on = face_node(fc, k), nn = nb_node(m, rev, k);
+                        float dvx = -vx[on], dvz = -vz[on];
+                        if (s >= 0) {
+                            dvx += svl[(0 * 6 + nn) * NT + s];
+                            dvz += svl[(1 * 6 + nn) * NT + s];
+                        }
+                        px[3 * fc + k] = ez * dvx;
+                        pz[3 * fc + k] = -ex * dvz;
+                        pxz[3 * fc + k] = ez * dvz - ex * dvx;
+                    }
+                }
+                mf4 dx0, dx1, dx2, dz0, dz1, dz2, fx0, fx1, fz0, fz1, fq0, fq1;
+                mf_12rows(mo.grad, vx, dx0, dx1, dx2, std::make_integer_sequence<int, 6>{});
+                mf_12rows(mo.grad, vz, dz0, dz1, dz2, std::make_integer_sequence<int, 6>{});
+                mf_6rows<9>(mo.lift, px, fx0, fx1, std::make_integer_sequence<int, 9>{});
+                mf_6rows<9>(mo.lift, pz, fz0, fz1, std::make_integer_sequence<int, 9>{});
+                mf_6rows<9>(mo.lift, pxz, fq0, fq1, std::make_integer_sequence<int, 9>{});
+                if (act) {
+                    const R l2m = lam + 2 * mu, dt = (R)a.dt;
+#pragma unroll
+                    for (int i = 0; i < 6; i++) {
+                        const float rvx = mf_row6(dx0, dx1, i), rvz = mf_row6(dz0, dz1, i);
+                        const float svx = i < 2 ? dx1[2 + i] : dx2[i - 2], svz = i < 2 ? dz1[2 + i] : dz2[i - 2];
+                        const R axx = g.rx * rvx + g.sx * svx + mf_row6(fx0, fx1, i) * g.invD;
+                        const R azz = g.rz * rvz + g.sz * svz + mf_row6(fz0, fz1, i) * g.invD;
+                        const R axz = g.rx * rvz + g.sx * svz + g.rz * rvx + g.sz * svx + mf_row6(fq0, fq1, i) * g.invD;
+                        if constexpr (COMP) {
+                            comp_add(txx[i], lo[2][i], (l2m * axx + lam * azz) * dt);
+                            comp_add(tzz[i], lo[3][i], (lam * axx + l2m * azz) * dt);
+                            comp_add(txz[i], lo[4][i], (mu * axz) * dt);
+                        } else {
+                            txx[i] = (l2m * axx + lam * azz) * dt + txx[i];
+                            tzz[i] = (lam * axx + l2m * azz) * dt + tzz[i];
+                            txz[i] = (mu * axz) * dt + txz[i];
+                        }
+                    }
+                    if (lev == 0) record_block<R, SERVE, COMP>(a, ax, shot, lstep, vx, vz, lo);
+                }
+            }
+        } else if (lev <= 2 * nst - 2 - 2 * j) {
             R exx[6], ezz[6], exz[6], fx[6], fz[6], fxz[6];
             strain_volume<R, GeomR<R>, P2>(a, g, vx, vz, exx, ezz, exz);
 #pragma unroll
@@ -1036,7 +1218,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
 /* The solver's blocked kernel: workgroup (b, shot) runs tile (b mod 8) * tper + b / 8 (XCD-contiguous runs of
  * tiles) of that shot for bsteps steps.  MINW: at least that many wavefronts per SIMD (512-thread tiles in
  * FP32: 4, two workgroups per CU). */
-template <typename R, int NT, int MINW, bool P2, bool COMP = false>
+template <typename R, int NT, int MINW, bool P2, bool COMP = false, bool MF = false>
 __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
 {
     __shared__ R sts[18 * NT]; /* conditioned stresses, (c*6 + i)*NT + lane */
@@ -1045,7 +1227,7 @@ __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
     const int t = (b & (DGX_NXCD - 1)) * a.tper + (b >> 3);
     if (t >= a.ntile) return; /* uniform over the workgroup */
     const int step0 = *a.step_base + a.step_off;
-    block_tile<R, NT, P2, false, COMP>(a, a.f, a.g, t, blockIdx.y, a.bsteps, step0, sts, svl);
+    block_tile<R, NT, P2, false, COMP, MF>(a, a.f, a.g, t, blockIdx.y, a.bsteps, step0, sts, svl);
 }
 
 /*
@@ -1055,7 +1237,7 @@ __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
  * batch by a scalar binary search; the batch's slot runs min(bkmax, steps left) steps from its local step.
  * Launch `launch` of a block reads copy (launch & 1) of the batch's fields and writes the other.
  */
-template <typename R, int NT, int MINW, bool P2>
+template <typename R, int NT, int MINW, bool P2, bool MF = false>
 __global__ void __launch_bounds__(NT, MINW) ks_block(const StepArgs<R> *__restrict__ table,
                                                     const int *__restrict__ sched, const int *step_base,
                                                     int step_off, int launch, int nb, int total, int nper)
@@ -1099,7 +1281,7 @@ __global__ void __launch_bounds__(NT, MINW) ks_block(const StepArgs<R> *__restri
         f[c] = as_global((launch & 1) ? a.g[c] : a.f[c]);
         gout[c] = as_global((launch & 1) ? a.f[c] : a.g[c]);
     }
-    block_tile<R, NT, P2, true>(a, f, gout, tile - sched[2 + lo], 0, nst, lstep0, sts, svl);
+    block_tile<R, NT, P2, true, false, MF>(a, f, gout, tile - sched[2 + lo], 0, nst, lstep0, sts, svl);
 }
 
 /* ---- split entry points (unfused reference step functions) ---- */
@@ -1250,9 +1432,12 @@ static void launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st
          * workgroup per CU) or 512 (two), FP64 512 */
         if constexpr (sizeof(R) == 4) {
             /* compensated FP32: twice the registers for the low words, one 512-thread workgroup per CU */
-            if (a.fl[0]) dgx::k_block<R, 512, 2, P2, true><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
-            else if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 1024, 0, st>>>(a);
-            else dgx::k_block<R, 512, 4, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
+            const dim3 grid(DGX_NXCD * a.tper, nshots);
+            if (a.fl[0] && a.bmfma) dgx::k_block<R, 512, 2, P2, true, true><<<grid, 512, 0, st>>>(a);
+            else if (a.fl[0]) dgx::k_block<R, 512, 2, P2, true><<<grid, 512, 0, st>>>(a);
+            else if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<grid, 1024, 0, st>>>(a);
+            else if (a.bmfma) dgx::k_block<R, 512, 4, P2, false, true><<<grid, 512, 0, st>>>(a);
+            else dgx::k_block<R, 512, 4, P2><<<grid, 512, 0, st>>>(a);
         } else {
             dgx::k_block<R, 512, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
         }
