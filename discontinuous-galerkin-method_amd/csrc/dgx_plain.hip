@@ -769,8 +769,7 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
 typedef float mf4 __attribute__((ext_vector_type(4)));
 
 struct MfOps {
-    float vol[2];  /* [Dr | Ds], 6 x 12, column c = 12 g + k: the velocity volume term on (r, s)-combined stresses */
-    float grad[2]; /* [Dr ; Ds], 12 x 6, c = 6 g + k: both derivatives of a velocity component */
+    float vol[2];  /* [Dr | Ds], 6 x 12, column c = 12 g + k: a volume term on (r, s)-weighted inputs */
     float lift[2]; /* [S1 | S2 | S3], 6 x 9, c = 9 g + 3 j + k: the three face lifts */
 };
 
@@ -790,12 +789,6 @@ __device__ __forceinline__ MfOps mf_ops(const StepArgs<float> &a)
         m.vol[v] = x;
         x = 0.0f;
         if (c < 18) {
-            const int row = 4 * (c / 6) + i, k = c % 6;
-            x = row < 6 ? op_dr<P2>(a, row * 6 + k) : op_ds<P2>(a, (row - 6) * 6 + k);
-        }
-        m.grad[v] = x;
-        x = 0.0f;
-        if (c < 18) {
             const int row = 4 * (c / 9) + i, kk = c % 9;
             if (row < 6) x = op_l<P2>(a, (kk / 3) * 18 + row * 3 + kk % 3);
         }
@@ -810,26 +803,16 @@ __device__ __forceinline__ mf4 mf_step(const float (&tab)[2], float x, mf4 acc)
     return __builtin_amdgcn_mfma_f32_4x4x1f32(tab[C >> 4], x, acc, 4, C & 15, 0);
 }
 
-/* Rows 0..3 (r0) and 4..5 (r1[0..1]) of a 6-row table applied to the lane's inputs x[0..K-1]. */
-template <int K, int... I>
-__device__ __forceinline__ void mf_6rows(const float (&tab)[2], const float (&x)[K], mf4 &r0, mf4 &r1,
-                                         std::integer_sequence<int, I...>)
+/* f(std::integral_constant<int, I>) for I = 0..N-1, unrolled (MFMA broadcast indices must be constants). */
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &f, std::integer_sequence<int, I...>)
 {
-    r0 = mf4{0.0f, 0.0f, 0.0f, 0.0f};
-    r1 = r0;
-    ((r0 = mf_step<I>(tab, x[I], r0), r1 = mf_step<K + I>(tab, x[I], r1)), ...);
+    (f(std::integral_constant<int, I>{}), ...);
 }
-
-/* Rows 0..11 of the 12-row table [Dr ; Ds] applied to the lane's x[0..5]: Dr x in r0, r1[0..1]; Ds x in
- * r1[2..3], r2. */
-template <int... I>
-__device__ __forceinline__ void mf_12rows(const float (&tab)[2], const float (&x)[6], mf4 &r0, mf4 &r1, mf4 &r2,
-                                          std::integer_sequence<int, I...>)
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
 {
-    r0 = mf4{0.0f, 0.0f, 0.0f, 0.0f};
-    r1 = r0;
-    r2 = r0;
-    ((r0 = mf_step<I>(tab, x[I], r0), r1 = mf_step<6 + I>(tab, x[I], r1), r2 = mf_step<12 + I>(tab, x[I], r2)), ...);
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 __device__ __forceinline__ float mf_row6(const mf4 &r0, const mf4 &r1, int i) { return i < 4 ? r0[i] : r1[i - 4]; }
@@ -1023,43 +1006,47 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
         if constexpr (MF) {
             const bool act = lev <= 2 * nst - 1 - 2 * j;
             if (__any(act)) { /* wave-uniform: the MFMA chains run with every lane of the wave active */
-                float xin[12], zin[12], gx[9], gz[9];
-#pragma unroll
-                for (int n = 0; n < 6; n++) {
-                    xin[n] = g.rx * txx[n] + g.rz * txz[n];
-                    xin[6 + n] = g.sx * txx[n] + g.sz * txz[n];
-                    zin[n] = g.rx * txz[n] + g.rz * tzz[n];
-                    zin[6 + n] = g.sx * txz[n] + g.sz * tzz[n];
-                }
-#pragma unroll
-                for (int fc = 0; fc < 3; fc++) {
+                /* per output row group, one chain: Dr (rx Txx + rz Txz) + Ds (sx Txx + sz Txz) for Vx (Txz, Tzz for
+                 * Vz), then the three face lifts of the flux jumps scaled by 1/D; four chains interleaved */
+                mf4 hx0 = {0.0f, 0.0f, 0.0f, 0.0f}, hx1 = hx0, hz0 = hx0, hz1 = hx0;
+                static_for<6>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    const float ux = g.rx * txx[n] + g.rz * txz[n], uz = g.rx * txz[n] + g.rz * tzz[n];
+                    hx0 = mf_step<n>(mo.vol, ux, hx0);
+                    hx1 = mf_step<12 + n>(mo.vol, ux, hx1);
+                    hz0 = mf_step<n>(mo.vol, uz, hz0);
+                    hz1 = mf_step<12 + n>(mo.vol, uz, hz1);
+                });
+                static_for<6>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    const float wx = g.sx * txx[n] + g.sz * txz[n], wz = g.sx * txz[n] + g.sz * tzz[n];
+                    hx0 = mf_step<6 + n>(mo.vol, wx, hx0);
+                    hx1 = mf_step<18 + n>(mo.vol, wx, hx1);
+                    hz0 = mf_step<6 + n>(mo.vol, wz, hz0);
+                    hz1 = mf_step<18 + n>(mo.vol, wz, hz1);
+                });
+                static_for<9>([&](auto kc) {
+                    constexpr int kk = decltype(kc)::value, fc = kk / 3, k = kk % 3;
                     const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
-                    const int s = nbs[fc];
-                    const float ex = g.ex[fc], ez = g.ez[fc];
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        const int on = face_node(fc, k), nn = nb_node(m, rev, k);
-                        float dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
-                        if (s >= 0) {
-                            dxx += sts[(0 * 6 + nn) * NT + s];
-                            dzz += sts[(1 * 6 + nn) * NT + s];
-                            dxz += sts[(2 * 6 + nn) * NT + s];
-                        }
-                        gx[3 * fc + k] = ez * dxx - ex * dxz;
-                        gz[3 * fc + k] = ez * dxz - ex * dzz;
+                    const int s = nbs[fc], on = face_node(fc, k), nn = nb_node(m, rev, k);
+                    float dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
+                    if (s >= 0) {
+                        dxx += sts[(0 * 6 + nn) * NT + s];
+                        dzz += sts[(1 * 6 + nn) * NT + s];
+                        dxz += sts[(2 * 6 + nn) * NT + s];
                     }
-                }
-                mf4 hx0, hx1, hz0, hz1, lx0, lx1, lz0, lz1;
-                mf_6rows<12>(mo.vol, xin, hx0, hx1, std::make_integer_sequence<int, 12>{});
-                mf_6rows<12>(mo.vol, zin, hz0, hz1, std::make_integer_sequence<int, 12>{});
-                mf_6rows<9>(mo.lift, gx, lx0, lx1, std::make_integer_sequence<int, 9>{});
-                mf_6rows<9>(mo.lift, gz, lz0, lz1, std::make_integer_sequence<int, 9>{});
+                    const float ex = g.ex[fc] * g.invD, ez = g.ez[fc] * g.invD;
+                    const float gx = ez * dxx - ex * dxz, gz = ez * dxz - ex * dzz;
+                    hx0 = mf_step<kk>(mo.lift, gx, hx0);
+                    hx1 = mf_step<9 + kk>(mo.lift, gx, hx1);
+                    hz0 = mf_step<kk>(mo.lift, gz, hz0);
+                    hz1 = mf_step<9 + kk>(mo.lift, gz, hz1);
+                });
                 if (act) {
                     const R c = rho_inv * (R)a.dt;
 #pragma unroll
                     for (int i = 0; i < 6; i++) {
-                        const R ix = (mf_row6(hx0, hx1, i) + mf_row6(lx0, lx1, i) * g.invD) * c;
-                        const R iz = (mf_row6(hz0, hz1, i) + mf_row6(lz0, lz1, i) * g.invD) * c;
+                        const R ix = mf_row6(hx0, hx1, i) * c, iz = mf_row6(hz0, hz1, i) * c;
                         if constexpr (COMP) {
                             comp_add(vx[i], lo[0][i], ix);
                             comp_add(vz[i], lo[1][i], iz);
@@ -1116,40 +1103,53 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
         if constexpr (MF) {
             const bool act = lev <= 2 * nst - 2 - 2 * j;
             if (__any(act)) { /* wave-uniform, as above */
-                float px[9], pz[9], pxz[9];
-#pragma unroll
-                for (int fc = 0; fc < 3; fc++) {
+                /* strain rates as volume + surface / D, one chain per output row group: exx = Dr (rx Vx) + Ds (sx Vx)
+                 * + lifts of ez dVx / D; ezz from rz Vz, sz Vz and -ex dVz / D; exz from rx Vz + rz Vx, sx Vz + sz Vx
+                 * and (ez dVz - ex dVx) / D; six chains interleaved */
+                mf4 ax0 = {0.0f, 0.0f, 0.0f, 0.0f}, ax1 = ax0, az0 = ax0, az1 = ax0, aq0 = ax0, aq1 = ax0;
+                static_for<6>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    const float px = g.rx * vx[n], pz = g.rz * vz[n], pq = g.rx * vz[n] + g.rz * vx[n];
+                    ax0 = mf_step<n>(mo.vol, px, ax0);
+                    ax1 = mf_step<12 + n>(mo.vol, px, ax1);
+                    az0 = mf_step<n>(mo.vol, pz, az0);
+                    az1 = mf_step<12 + n>(mo.vol, pz, az1);
+                    aq0 = mf_step<n>(mo.vol, pq, aq0);
+                    aq1 = mf_step<12 + n>(mo.vol, pq, aq1);
+                });
+                static_for<6>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    const float px = g.sx * vx[n], pz = g.sz * vz[n], pq = g.sx * vz[n] + g.sz * vx[n];
+                    ax0 = mf_step<6 + n>(mo.vol, px, ax0);
+                    ax1 = mf_step<18 + n>(mo.vol, px, ax1);
+                    az0 = mf_step<6 + n>(mo.vol, pz, az0);
+                    az1 = mf_step<18 + n>(mo.vol, pz, az1);
+                    aq0 = mf_step<6 + n>(mo.vol, pq, aq0);
+                    aq1 = mf_step<18 + n>(mo.vol, pq, aq1);
+                });
+                static_for<9>([&](auto kc) {
+                    constexpr int kk = decltype(kc)::value, fc = kk / 3, k = kk % 3;
                     const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
-                    const int s = nbs[fc];
-                    const float ex = g.ex[fc], ez = g.ez[fc];
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        const int on = face_node(fc, k), nn = nb_node(m, rev, k);
-                        float dvx = -vx[on], dvz = -vz[on];
-                        if (s >= 0) {
-                            dvx += svl[(0 * 6 + nn) * NT + s];
-                            dvz += svl[(1 * 6 + nn) * NT + s];
-                        }
-                        px[3 * fc + k] = ez * dvx;
-                        pz[3 * fc + k] = -ex * dvz;
-                        pxz[3 * fc + k] = ez * dvz - ex * dvx;
+                    const int s = nbs[fc], on = face_node(fc, k), nn = nb_node(m, rev, k);
+                    float dvx = -vx[on], dvz = -vz[on];
+                    if (s >= 0) {
+                        dvx += svl[(0 * 6 + nn) * NT + s];
+                        dvz += svl[(1 * 6 + nn) * NT + s];
                     }
-                }
-                mf4 dx0, dx1, dx2, dz0, dz1, dz2, fx0, fx1, fz0, fz1, fq0, fq1;
-                mf_12rows(mo.grad, vx, dx0, dx1, dx2, std::make_integer_sequence<int, 6>{});
-                mf_12rows(mo.grad, vz, dz0, dz1, dz2, std::make_integer_sequence<int, 6>{});
-                mf_6rows<9>(mo.lift, px, fx0, fx1, std::make_integer_sequence<int, 9>{});
-                mf_6rows<9>(mo.lift, pz, fz0, fz1, std::make_integer_sequence<int, 9>{});
-                mf_6rows<9>(mo.lift, pxz, fq0, fq1, std::make_integer_sequence<int, 9>{});
+                    const float ex = g.ex[fc] * g.invD, ez = g.ez[fc] * g.invD;
+                    const float px = ez * dvx, pz = -ex * dvz, pq = ez * dvz - ex * dvx;
+                    ax0 = mf_step<kk>(mo.lift, px, ax0);
+                    ax1 = mf_step<9 + kk>(mo.lift, px, ax1);
+                    az0 = mf_step<kk>(mo.lift, pz, az0);
+                    az1 = mf_step<9 + kk>(mo.lift, pz, az1);
+                    aq0 = mf_step<kk>(mo.lift, pq, aq0);
+                    aq1 = mf_step<9 + kk>(mo.lift, pq, aq1);
+                });
                 if (act) {
                     const R l2m = lam + 2 * mu, dt = (R)a.dt;
 #pragma unroll
                     for (int i = 0; i < 6; i++) {
-                        const float rvx = mf_row6(dx0, dx1, i), rvz = mf_row6(dz0, dz1, i);
-                        const float svx = i < 2 ? dx1[2 + i] : dx2[i - 2], svz = i < 2 ? dz1[2 + i] : dz2[i - 2];
-                        const R axx = g.rx * rvx + g.sx * svx + mf_row6(fx0, fx1, i) * g.invD;
-                        const R azz = g.rz * rvz + g.sz * svz + mf_row6(fz0, fz1, i) * g.invD;
-                        const R axz = g.rx * rvz + g.sx * svz + g.rz * rvx + g.sz * svx + mf_row6(fq0, fq1, i) * g.invD;
+                        const R axx = mf_row6(ax0, ax1, i), azz = mf_row6(az0, az1, i), axz = mf_row6(aq0, aq1, i);
                         if constexpr (COMP) {
                             comp_add(txx[i], lo[2][i], (l2m * axx + lam * azz) * dt);
                             comp_add(tzz[i], lo[3][i], (lam * axx + l2m * azz) * dt);
