@@ -132,7 +132,6 @@ struct dgx_solver {
     int full = 0;           /* one full-step kernel per step (element layout) */
     int blocked = 0;        /* blocked kernel: bkmax steps per launch (element layout) */
     int bkmax = 0, bthreads = 0;
-    int bmfma = 0;          /* blocked FP32: operator contractions on the matrix cores */
     long bown = 0, blanes = 0; /* own elements and lanes over all tiles of the blocked kernel */
     int *blevels = nullptr, *bgid = nullptr;
     int4 *blnb = nullptr;
@@ -415,7 +414,6 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     a.bkmax = s->bkmax;
     a.bthreads = s->bthreads;
     a.bsteps = s->bkmax;
-    a.bmfma = s->bmfma;
     a.tiles = s->tiles;
     a.ring = s->ring;
     a.ntile = (int)s->ntile;
@@ -877,7 +875,6 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         if (const char *bt = getenv("DGX_BLOCK_THREADS"))
             if (s->fp32 && atoi(bt) == 1024) cap = 1024;
         s->bthreads = cap;
-        if (const char *mf = getenv("DGX_BLOCK_MFMA")) s->bmfma = s->fp32 && cap == 512 && atoi(mf) == 1;
         if (!dgx_build_block_tiles(T, nbr_host, s->bkmax, cap, btiles_host, blevels_host, bgid_host, blnb_host)) {
             dgx_set_error("dgx_create: an element and its %d levels of neighbours exceed %d threads", 2 * s->bkmax,
                           cap);

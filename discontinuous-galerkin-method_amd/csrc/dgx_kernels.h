@@ -114,7 +114,6 @@ struct StepArgs {
     int bkmax;             /* steps the tiles' levels allow per launch */
     int bthreads;          /* threads per workgroup (lanes per tile at most) */
     int bsteps;            /* steps of this launch (1..bkmax) */
-    int bmfma;             /* FP32: operator contractions on the matrix cores (v_mfma_f32_4x4x1f32) */
 };
 
 } // namespace dgx

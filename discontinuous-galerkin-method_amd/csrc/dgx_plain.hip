@@ -29,7 +29,6 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
-#include <utility>
 
 #include "dgx.h"
 #include "dgx_device.h"
@@ -756,67 +755,6 @@ __global__ void __launch_bounds__(DGX_BLOCK, 3) k_fullstep(StepArgs<R> a)
                                a.g[3] + so, a.g[4] + so);
 }
 
-/* ---- Matrix-core contractions of the blocked FP32 kernel ----
- * v_mfma_f32_4x4x1f32 in its 16-block form: block b (lanes 4b..4b+3) computes D_b = A_b B_b + C_b with A_b a
- * 4x1 column (lane 4b+i holds row i), B_b a 1x4 row (lane 4b+j holds column j) and D_b[i][j] in register i of
- * lane 4b+j.  With A broadcast from block c (cbsz 4, abid c) every lane receives column c of a table times its
- * own B.  A chain over k of (operator rows 4g..4g+3 at input k) x (the lane's input k) therefore leaves rows
- * 4g..4g+3 of the operator applied to the lane's own element in its four registers: 64 elements per
- * instruction, nothing exchanged between lanes, each lane's result depending on its own inputs only.  The
- * tables hold 16 (row group, input) columns per register, column c in lanes 4(c mod 16)..+3 of register
- * c / 16.  An MFMA ignores the execution mask, so the chains are issued only where every lane of the wave is
- * active (the tables are written with all lanes active at the kernel's start). */
-typedef float mf4 __attribute__((ext_vector_type(4)));
-
-struct MfOps {
-    float vol[2];  /* [Dr | Ds], 6 x 12, column c = 12 g + k: a volume term on (r, s)-weighted inputs */
-    float lift[2]; /* [S1 | S2 | S3], 6 x 9, c = 9 g + 3 j + k: the three face lifts */
-};
-
-template <bool P2>
-__device__ __forceinline__ MfOps mf_ops(const StepArgs<float> &a)
-{
-    MfOps m;
-    const int l = threadIdx.x & 63, i = l & 3;
-#pragma unroll
-    for (int v = 0; v < 2; v++) {
-        const int c = 16 * v + (l >> 2);
-        float x = 0.0f;
-        if (c < 24) {
-            const int row = 4 * (c / 12) + i, k = c % 12;
-            if (row < 6) x = k < 6 ? op_dr<P2>(a, row * 6 + k) : op_ds<P2>(a, row * 6 + k - 6);
-        }
-        m.vol[v] = x;
-        x = 0.0f;
-        if (c < 18) {
-            const int row = 4 * (c / 9) + i, kk = c % 9;
-            if (row < 6) x = op_l<P2>(a, (kk / 3) * 18 + row * 3 + kk % 3);
-        }
-        m.lift[v] = x;
-    }
-    return m;
-}
-
-template <int C>
-__device__ __forceinline__ mf4 mf_step(const float (&tab)[2], float x, mf4 acc)
-{
-    return __builtin_amdgcn_mfma_f32_4x4x1f32(tab[C >> 4], x, acc, 4, C & 15, 0);
-}
-
-/* f(std::integral_constant<int, I>) for I = 0..N-1, unrolled (MFMA broadcast indices must be constants). */
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F &f, std::integer_sequence<int, I...>)
-{
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F &&f)
-{
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-__device__ __forceinline__ float mf_row6(const mf4 &r0, const mf4 &r1, int i) { return i < 4 ? r0[i] : r1[i - 4]; }
-
 /* Wavelet sample of a source at local step `lstep`: the solver's wavelet table row src.wavelet, or in a
  * serving batch (SERVE) the source's slot row. */
 template <typename R, bool SERVE>
@@ -937,13 +875,10 @@ __device__ __forceinline__ void record_block(const StepArgs<R> &a, unsigned ax, 
  * element and written once for every own element, against 7 reads and writes per step and element for the
  * half-step kernel pair.  Every thread of the workgroup reaches every barrier.
  */
-template <typename R, int NT, bool P2, bool SERVE, bool COMP = false, bool MF = false>
+template <typename R, int NT, bool P2, bool SERVE, bool COMP = false>
 __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], R *const gout[5], int t, int shot,
                                            int nst, int lstep0, R *sts, R *svl)
 {
-    static_assert(!MF || sizeof(R) == 4, "matrix-core contractions are FP32 only");
-    MfOps mo;
-    if constexpr (MF) mo = mf_ops<P2>(a); /* every lane active here */
     const int4 td = a.tiles[t];
     const int l = threadIdx.x;
     const int ghost = 2 * a.bkmax;
@@ -965,9 +900,9 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
 
     /* the stresses live in registers from their update to the next step's conditioning and in LDS (sts) in
      * between; the velocities stay in registers */
-    R vx[6] = {}, vz[6] = {}, txx[6] = {}, tzz[6] = {}, txz[6] = {}, ab[6] = {};
+    R vx[6], vz[6], txx[6], tzz[6], txz[6], ab[6];
     R lo[5][6]; /* compensated FP32: the low words of Vx, Vz, Txx, Tzz, Txz (unused otherwise) */
-    GeomR<R> g = {};
+    GeomR<R> g;
     R rho_inv = 0, lam = 0, mu = 0;
     if (lev <= ghost) {
 #pragma unroll
@@ -1003,61 +938,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             }
         }
         __syncthreads();
-        if constexpr (MF) {
-            const bool act = lev <= 2 * nst - 1 - 2 * j;
-            if (__any(act)) { /* wave-uniform: the MFMA chains run with every lane of the wave active */
-                /* per output row group, one chain: Dr (rx Txx + rz Txz) + Ds (sx Txx + sz Txz) for Vx (Txz, Tzz for
-                 * Vz), then the three face lifts of the flux jumps scaled by 1/D; four chains interleaved */
-                mf4 hx0 = {0.0f, 0.0f, 0.0f, 0.0f}, hx1 = hx0, hz0 = hx0, hz1 = hx0;
-                static_for<6>([&](auto nc) {
-                    constexpr int n = decltype(nc)::value;
-                    const float ux = g.rx * txx[n] + g.rz * txz[n], uz = g.rx * txz[n] + g.rz * tzz[n];
-                    hx0 = mf_step<n>(mo.vol, ux, hx0);
-                    hx1 = mf_step<12 + n>(mo.vol, ux, hx1);
-                    hz0 = mf_step<n>(mo.vol, uz, hz0);
-                    hz1 = mf_step<12 + n>(mo.vol, uz, hz1);
-                });
-                static_for<6>([&](auto nc) {
-                    constexpr int n = decltype(nc)::value;
-                    const float wx = g.sx * txx[n] + g.sz * txz[n], wz = g.sx * txz[n] + g.sz * tzz[n];
-                    hx0 = mf_step<6 + n>(mo.vol, wx, hx0);
-                    hx1 = mf_step<18 + n>(mo.vol, wx, hx1);
-                    hz0 = mf_step<6 + n>(mo.vol, wz, hz0);
-                    hz1 = mf_step<18 + n>(mo.vol, wz, hz1);
-                });
-                static_for<9>([&](auto kc) {
-                    constexpr int kk = decltype(kc)::value, fc = kk / 3, k = kk % 3;
-                    const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
-                    const int s = nbs[fc], on = face_node(fc, k), nn = nb_node(m, rev, k);
-                    float dxx = -txx[on], dzz = -tzz[on], dxz = -txz[on];
-                    if (s >= 0) {
-                        dxx += sts[(0 * 6 + nn) * NT + s];
-                        dzz += sts[(1 * 6 + nn) * NT + s];
-                        dxz += sts[(2 * 6 + nn) * NT + s];
-                    }
-                    const float ex = g.ex[fc] * g.invD, ez = g.ez[fc] * g.invD;
-                    const float gx = ez * dxx - ex * dxz, gz = ez * dxz - ex * dzz;
-                    hx0 = mf_step<kk>(mo.lift, gx, hx0);
-                    hx1 = mf_step<9 + kk>(mo.lift, gx, hx1);
-                    hz0 = mf_step<kk>(mo.lift, gz, hz0);
-                    hz1 = mf_step<9 + kk>(mo.lift, gz, hz1);
-                });
-                if (act) {
-                    const R c = rho_inv * (R)a.dt;
-#pragma unroll
-                    for (int i = 0; i < 6; i++) {
-                        const R ix = mf_row6(hx0, hx1, i) * c, iz = mf_row6(hz0, hz1, i) * c;
-                        if constexpr (COMP) {
-                            comp_add(vx[i], lo[0][i], ix);
-                            comp_add(vz[i], lo[1][i], iz);
-                        } else {
-                            vx[i] = ix + vx[i];
-                            vz[i] = iz + vz[i];
-                        }
-                    }
-                }
-            }
-        } else if (lev <= 2 * nst - 1 - 2 * j) {
+        if (lev <= 2 * nst - 1 - 2 * j) {
             R rhx[6], rhz[6], flx[6], flz[6];
             velocity_volume<R, P2>(a, g, txx, tzz, txz, rhx, rhz);
 #pragma unroll
@@ -1100,70 +981,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             }
         }
         __syncthreads();
-        if constexpr (MF) {
-            const bool act = lev <= 2 * nst - 2 - 2 * j;
-            if (__any(act)) { /* wave-uniform, as above */
-                /* strain rates as volume + surface / D, one chain per output row group: exx = Dr (rx Vx) + Ds (sx Vx)
-                 * + lifts of ez dVx / D; ezz from rz Vz, sz Vz and -ex dVz / D; exz from rx Vz + rz Vx, sx Vz + sz Vx
-                 * and (ez dVz - ex dVx) / D; six chains interleaved */
-                mf4 ax0 = {0.0f, 0.0f, 0.0f, 0.0f}, ax1 = ax0, az0 = ax0, az1 = ax0, aq0 = ax0, aq1 = ax0;
-                static_for<6>([&](auto nc) {
-                    constexpr int n = decltype(nc)::value;
-                    const float px = g.rx * vx[n], pz = g.rz * vz[n], pq = g.rx * vz[n] + g.rz * vx[n];
-                    ax0 = mf_step<n>(mo.vol, px, ax0);
-                    ax1 = mf_step<12 + n>(mo.vol, px, ax1);
-                    az0 = mf_step<n>(mo.vol, pz, az0);
-                    az1 = mf_step<12 + n>(mo.vol, pz, az1);
-                    aq0 = mf_step<n>(mo.vol, pq, aq0);
-                    aq1 = mf_step<12 + n>(mo.vol, pq, aq1);
-                });
-                static_for<6>([&](auto nc) {
-                    constexpr int n = decltype(nc)::value;
-                    const float px = g.sx * vx[n], pz = g.sz * vz[n], pq = g.sx * vz[n] + g.sz * vx[n];
-                    ax0 = mf_step<6 + n>(mo.vol, px, ax0);
-                    ax1 = mf_step<18 + n>(mo.vol, px, ax1);
-                    az0 = mf_step<6 + n>(mo.vol, pz, az0);
-                    az1 = mf_step<18 + n>(mo.vol, pz, az1);
-                    aq0 = mf_step<6 + n>(mo.vol, pq, aq0);
-                    aq1 = mf_step<18 + n>(mo.vol, pq, aq1);
-                });
-                static_for<9>([&](auto kc) {
-                    constexpr int kk = decltype(kc)::value, fc = kk / 3, k = kk % 3;
-                    const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
-                    const int s = nbs[fc], on = face_node(fc, k), nn = nb_node(m, rev, k);
-                    float dvx = -vx[on], dvz = -vz[on];
-                    if (s >= 0) {
-                        dvx += svl[(0 * 6 + nn) * NT + s];
-                        dvz += svl[(1 * 6 + nn) * NT + s];
-                    }
-                    const float ex = g.ex[fc] * g.invD, ez = g.ez[fc] * g.invD;
-                    const float px = ez * dvx, pz = -ex * dvz, pq = ez * dvz - ex * dvx;
-                    ax0 = mf_step<kk>(mo.lift, px, ax0);
-                    ax1 = mf_step<9 + kk>(mo.lift, px, ax1);
-                    az0 = mf_step<kk>(mo.lift, pz, az0);
-                    az1 = mf_step<9 + kk>(mo.lift, pz, az1);
-                    aq0 = mf_step<kk>(mo.lift, pq, aq0);
-                    aq1 = mf_step<9 + kk>(mo.lift, pq, aq1);
-                });
-                if (act) {
-                    const R l2m = lam + 2 * mu, dt = (R)a.dt;
-#pragma unroll
-                    for (int i = 0; i < 6; i++) {
-                        const R axx = mf_row6(ax0, ax1, i), azz = mf_row6(az0, az1, i), axz = mf_row6(aq0, aq1, i);
-                        if constexpr (COMP) {
-                            comp_add(txx[i], lo[2][i], (l2m * axx + lam * azz) * dt);
-                            comp_add(tzz[i], lo[3][i], (lam * axx + l2m * azz) * dt);
-                            comp_add(txz[i], lo[4][i], (mu * axz) * dt);
-                        } else {
-                            txx[i] = (l2m * axx + lam * azz) * dt + txx[i];
-                            tzz[i] = (lam * axx + l2m * azz) * dt + tzz[i];
-                            txz[i] = (mu * axz) * dt + txz[i];
-                        }
-                    }
-                    if (lev == 0) record_block<R, SERVE, COMP>(a, ax, shot, lstep, vx, vz, lo);
-                }
-            }
-        } else if (lev <= 2 * nst - 2 - 2 * j) {
+        if (lev <= 2 * nst - 2 - 2 * j) {
             R exx[6], ezz[6], exz[6], fx[6], fz[6], fxz[6];
             strain_volume<R, GeomR<R>, P2>(a, g, vx, vz, exx, ezz, exz);
 #pragma unroll
@@ -1218,7 +1036,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
 /* The solver's blocked kernel: workgroup (b, shot) runs tile (b mod 8) * tper + b / 8 (XCD-contiguous runs of
  * tiles) of that shot for bsteps steps.  MINW: at least that many wavefronts per SIMD (512-thread tiles in
  * FP32: 4, two workgroups per CU). */
-template <typename R, int NT, int MINW, bool P2, bool COMP = false, bool MF = false>
+template <typename R, int NT, int MINW, bool P2, bool COMP = false>
 __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
 {
     __shared__ R sts[18 * NT]; /* conditioned stresses, (c*6 + i)*NT + lane */
@@ -1227,7 +1045,7 @@ __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
     const int t = (b & (DGX_NXCD - 1)) * a.tper + (b >> 3);
     if (t >= a.ntile) return; /* uniform over the workgroup */
     const int step0 = *a.step_base + a.step_off;
-    block_tile<R, NT, P2, false, COMP, MF>(a, a.f, a.g, t, blockIdx.y, a.bsteps, step0, sts, svl);
+    block_tile<R, NT, P2, false, COMP>(a, a.f, a.g, t, blockIdx.y, a.bsteps, step0, sts, svl);
 }
 
 /*
@@ -1237,7 +1055,7 @@ __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
  * batch by a scalar binary search; the batch's slot runs min(bkmax, steps left) steps from its local step.
  * Launch `launch` of a block reads copy (launch & 1) of the batch's fields and writes the other.
  */
-template <typename R, int NT, int MINW, bool P2, bool MF = false>
+template <typename R, int NT, int MINW, bool P2>
 __global__ void __launch_bounds__(NT, MINW) ks_block(const StepArgs<R> *__restrict__ table,
                                                     const int *__restrict__ sched, const int *step_base,
                                                     int step_off, int launch, int nb, int total, int nper)
@@ -1281,7 +1099,7 @@ __global__ void __launch_bounds__(NT, MINW) ks_block(const StepArgs<R> *__restri
         f[c] = as_global((launch & 1) ? a.g[c] : a.f[c]);
         gout[c] = as_global((launch & 1) ? a.f[c] : a.g[c]);
     }
-    block_tile<R, NT, P2, true, false, MF>(a, f, gout, tile - sched[2 + lo], 0, nst, lstep0, sts, svl);
+    block_tile<R, NT, P2, true>(a, f, gout, tile - sched[2 + lo], 0, nst, lstep0, sts, svl);
 }
 
 /* ---- split entry points (unfused reference step functions) ---- */
@@ -1432,12 +1250,9 @@ static void launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st
          * workgroup per CU) or 512 (two), FP64 512 */
         if constexpr (sizeof(R) == 4) {
             /* compensated FP32: twice the registers for the low words, one 512-thread workgroup per CU */
-            const dim3 grid(DGX_NXCD * a.tper, nshots);
-            if (a.fl[0] && a.bmfma) dgx::k_block<R, 512, 2, P2, true, true><<<grid, 512, 0, st>>>(a);
-            else if (a.fl[0]) dgx::k_block<R, 512, 2, P2, true><<<grid, 512, 0, st>>>(a);
-            else if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<grid, 1024, 0, st>>>(a);
-            else if (a.bmfma) dgx::k_block<R, 512, 4, P2, false, true><<<grid, 512, 0, st>>>(a);
-            else dgx::k_block<R, 512, 4, P2><<<grid, 512, 0, st>>>(a);
+            if (a.fl[0]) dgx::k_block<R, 512, 2, P2, true><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
+            else if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 1024, 0, st>>>(a);
+            else dgx::k_block<R, 512, 4, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
         } else {
             dgx::k_block<R, 512, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
         }
