@@ -5,12 +5,14 @@
  * a single shot).  The server keeps meshes, materials and sponge factors resident in HBM and runs many
  * shots of many requests at once: every request is 1..DGX_SERVE_MAX_SHOTS shots on one resident mesh, each
  * shot with its own source, FP64 wavelet samples and start step, all shots of a request sharing one
- * receiver spread.  Shots run in slot groups of group_shots lanes (4, 8, 16, 32 or 64, dgx_server_config); a
- * request's shots may spread over several groups of its mesh.
- * Requests are admitted as slots and memory free up (continuous batching), stepped by one kernel per
- * half-step over the tiles of all active (mesh, slot group) batches, in hipGraph-captured blocks of steps,
- * and retired with their traces copied back.  Every shot computes exactly what a single-shot solve of
- * the same problem on the shot-innermost layout computes; the mix of requests does not change a result.
+ * receiver spread and one material model (the mesh's own, or the request's).  Shots run in slot groups of
+ * group_shots lanes (dgx_server_config): 4, 8, 16, 32 or 64 shots in the shot-innermost layout, stepped by one
+ * kernel per half-step, or single-shot slots in the element layout, stepped two steps per launch by the
+ * blocked kernel; a request's shots may spread over several groups of its mesh.
+ * Requests are admitted as slots and memory free up (continuous batching), stepped by launches over the
+ * tiles of all active (mesh, slot group) batches, in hipGraph-captured blocks of steps, and retired with
+ * their traces copied back.  The mix of requests does not change a result: every shot's traces are bit for
+ * bit those of the same request served alone.
  */
 #ifndef DGX_SERVE_H
 #define DGX_SERVE_H
