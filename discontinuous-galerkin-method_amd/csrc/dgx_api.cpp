@@ -478,12 +478,6 @@ static void free_solver(dgx_solver *s)
     delete s;
 }
 
-template <typename R>
-static std::vector<R> convert(const std::vector<double> &v)
-{
-    return std::vector<R>(v.begin(), v.end());
-}
-
 static void set_tile(dgx_solver *s, int tile, int waves)
 {
     s->shot_tile = tile;
@@ -799,18 +793,39 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     if (aux_elems.size() > DGX_MAX_AUX) { dgx_set_error("too many source/receiver elements"); free_solver(s); return DGX_EARG; }
 
-    /* ---- mesh: order, topology, vertices, materials, sponge ---- */
-    dgx_mesh_host mh;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        dgx_set_error("dgx_create: no HIP device available");
+        free_solver(s);
+        return DGX_EHIP;
+    }
+    CREATE_TRY(hipGetDevice(&s->device));
+
+    /* ---- mesh: order, topology, vertices, materials, sponge, built on the device; the solver owns the
+     * arrays from here (free_solver releases them) ---- */
+    dgx_mesh_dev md;
     {
-        int rc = dgx_build_mesh_host(p, s->layout == DGX_LAYOUT_SHOT_INNER, s->tri_pad, &explosive, &mh);
+        int rc = dgx_build_mesh_device(p, s->layout == DGX_LAYOUT_SHOT_INNER, s->tri_pad, &explosive, &md);
         if (rc) { free_solver(s); return rc; }
     }
-    s->perm = mh.perm;
-    s->inv = mh.inv;
-    std::vector<int4> &nbr_host = mh.nbr;
-    std::vector<int4> &vtx_host = mh.vtx;
-    std::vector<double2> &pt_host = mh.pt;
-    std::vector<double> &mat_host = mh.mat, &absorb_host = mh.absorb;
+    s->nbr = md.d_nbr;
+    s->vtx = md.d_vtx;
+    s->pt = md.d_pt;
+    if (s->fp32) {
+        float *m = nullptr, *a = nullptr;
+        const hipError_t e = dgx_mesh_f32(md.d_mat, 3 * s->tri_pad, md.d_absorb, 6 * s->tri_pad, &m, &a);
+        (void)hipFree(md.d_mat);
+        (void)hipFree(md.d_absorb);
+        s->mat = m;
+        s->absorb = a;
+        CREATE_TRY(e);
+    } else {
+        s->mat = md.d_mat;
+        s->absorb = md.d_absorb;
+    }
+    s->perm = std::move(md.perm);
+    s->inv = std::move(md.inv);
+    std::vector<int4> &nbr_host = md.nbr;
 
     std::vector<dgx_aux> aux_host(aux_elems.size());
     std::vector<dgx_src_dev> src_host;
@@ -887,13 +902,6 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
 
     /* ---- device allocations (all host-side validation is done above) ---- */
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        dgx_set_error("dgx_create: no HIP device available");
-        free_solver(s);
-        return DGX_EHIP;
-    }
-    CREATE_TRY(hipGetDevice(&s->device));
     {
         double *scr = nullptr; /* dgx_get_fields_noalloc's staging on this device: look it up now */
         CREATE_TRY(dgx_noalloc_scratch(&scr));
@@ -920,26 +928,8 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         CREATE_TRY(upload(&s->bgid, bgid_host));
         CREATE_TRY(upload(&s->blnb, blnb_host));
     }
-    CREATE_TRY(upload(&s->nbr, nbr_host));
-    CREATE_TRY(upload(&s->vtx, vtx_host));
-    CREATE_TRY(upload(&s->pt, pt_host));
-    if (s->fp32) {
-        float *m = nullptr, *a = nullptr;
-        hipError_t e = upload(&m, convert<float>(mat_host));
-        s->mat = m;
-        CREATE_TRY(e);
-        e = upload(&a, dgx_sponge_deficit_f32(absorb_host));
-        s->absorb = a;
-        CREATE_TRY(e);
-    } else {
-        double *m = nullptr, *a = nullptr;
-        hipError_t e = upload(&m, mat_host);
-        s->mat = m;
-        CREATE_TRY(e);
-        e = upload(&a, absorb_host);
-        s->absorb = a;
-        CREATE_TRY(e);
-    }
+    if (!aux_elems.empty()) /* the code words now carry their source/receiver records */
+        CREATE_TRY(hipMemcpy(s->nbr, nbr_host.data(), nbr_host.size() * sizeof(int4), hipMemcpyHostToDevice));
     CREATE_TRY(upload(&s->aux, aux_host));
     {
         /* fused kernels read sources through the aux ranges; the split source kernel reads the list in

@@ -23,17 +23,34 @@ struct dgx_mesh_host {
     std::vector<double> absorb; /* 6 * tri_pad */
 };
 
-/* Sponge factors as the FP32 kernels read them: the deficit 1 - f, rounded once (the device helper sponge()
- * applies x - x (1 - f)).  FP64 kernels read f itself. */
-inline std::vector<float> dgx_sponge_deficit_f32(const std::vector<double> &f)
-{
-    std::vector<float> d(f.size());
-    for (size_t k = 0; k < f.size(); k++) d[k] = (float)(1.0 - f[k]);
-    return d;
-}
-
 int dgx_build_mesh_host(const dgx_problem *p, int shot_inner, long tri_pad, const std::vector<char> *explosive,
                         dgx_mesh_host *out);
+
+/* The same mesh built on the device (dgx_meshdev.hip), bit for bit the arrays of dgx_build_mesh_host: nbr, vtx,
+ * pt (npt used vertices), mat and absorb (FP64, in the layout of shot_inner) stay in device memory, owned by the
+ * caller (dgx_mesh_dev_free); perm, inv and a copy of nbr come back to the host. */
+struct dgx_mesh_dev {
+    long tri = 0, tri_pad = 0, npt = 0;
+    std::vector<long> perm, inv;
+    std::vector<int4> nbr;
+    int4 *d_nbr = nullptr, *d_vtx = nullptr;
+    double2 *d_pt = nullptr;
+    double *d_mat = nullptr, *d_absorb = nullptr;
+};
+
+int dgx_build_mesh_device(const dgx_problem *p, int shot_inner, long tri_pad, const std::vector<char> *explosive,
+                          dgx_mesh_dev *out);
+void dgx_mesh_dev_free(dgx_mesh_dev *m);
+/* FP32 copies on the device for the FP32 kernels: materials rounded once, sponge factors as their deficit 1 - f
+ * rounded once (the device helper sponge() applies x - x (1 - f)). */
+hipError_t dgx_mesh_f32(const double *mat, long nmat, const double *absorb, long nab, float **mat32, float **absorb32);
+/* A mesh's materials 1/rho | lambda | mu with stride tri, from the build's stride tri_pad, in FP64 or FP32. */
+hipError_t dgx_mesh_mat_packed(const double *mat, long tri_pad, long tri, int fp32, void **out);
+
+/* Both builds of one mesh compared array by array, bit for bit (tests): *mismatch = entries that differ.  Not
+ * part of the C-ABI. */
+extern "C" int dgx_mesh_build_compare(const dgx_problem *p, int shot_inner, long tri_pad, const char *explosive,
+                                      long *mismatch);
 
 extern "C" void dgx_set_error(const char *fmt, ...);
 

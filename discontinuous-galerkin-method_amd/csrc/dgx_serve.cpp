@@ -412,15 +412,17 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
         dgx_set_error("dgx_server_add_mesh: incomplete mesh");
         return DGX_EARG;
     }
-    dgx_mesh_host mh;
-    /* single-shot slots: the element layout (64-element blocks), whose sponge factors are blocked the same way */
+    /* the mesh built on the device (single-shot slots: the element layout, 64-element blocks, whose sponge
+     * factors are blocked the same way) */
     const long tri_pad = sv->group == 1 ? (p->tri + 63) / 64 * 64 : p->tri;
-    int rc = dgx_build_mesh_host(p, sv->group == 1 ? 0 : 1, tri_pad, nullptr, &mh);
+    dgx_mesh_dev mh;
+    int rc = dgx_build_mesh_device(p, sv->group == 1 ? 0 : 1, tri_pad, nullptr, &mh);
     if (rc) return rc;
     std::vector<int4> bt, bl;
     std::vector<int> blv, bg;
     if (sv->group == 1 &&
         !dgx_build_block_tiles(p->tri, mh.nbr, kServeBlockSteps, kServeBlockThreads, bt, blv, bg, bl)) {
+        dgx_mesh_dev_free(&mh);
         dgx_set_error("dgx_server_add_mesh: an element and its neighbour levels exceed %d threads", kServeBlockThreads);
         return DGX_EMESH;
     }
@@ -430,35 +432,37 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     m.ntile = (long)bt.size();
     m.inv = mh.inv;
     m.nbr_host.assign(mh.nbr.begin(), mh.nbr.begin() + mh.tri);
+    m.nbr = mh.d_nbr;
+    m.vtx = mh.d_vtx;
+    m.pt = mh.d_pt;
     ModelDev md;
     md.used = true;
     md.dt = p->dt;
     md.refs = 1; /* never released */
     /* a failure frees what this call allocated */
     auto fail = [&](hipError_t e) {
-        void *q[] = {m.nbr, m.vtx, m.pt, m.absorb, md.d_mat, m.btiles, m.blnb, m.blevels, m.bgid};
+        void *q[] = {m.nbr, m.vtx, m.pt, m.absorb, md.d_mat, m.btiles, m.blnb, m.blevels, m.bgid, mh.d_mat, mh.d_absorb};
         for (void *x : q)
             if (x) (void)hipFree(x);
         dgx_set_error("dgx_server_add_mesh: %s", hipGetErrorString(e));
         return e == hipErrorOutOfMemory ? DGX_ENOMEM : DGX_EHIP;
     };
-    hipError_t e = upload_vec(&m.nbr, mh.nbr);
-    if (e == hipSuccess) e = upload_vec(&m.vtx, mh.vtx);
-    if (e == hipSuccess) e = upload_vec(&m.pt, mh.pt);
-    /* every model of the mesh is stored 1/rho | lambda | mu with stride tri (dgx_build_mesh_host pads to tri_pad) */
-    std::vector<double> mat0(3 * p->tri);
-    for (int c = 0; c < 3; c++)
-        std::copy(mh.mat.begin() + c * tri_pad, mh.mat.begin() + c * tri_pad + p->tri, mat0.begin() + c * p->tri);
-    if (e == hipSuccess) e = upload_reals(sv, &md.d_mat, mat0);
+    /* every model of the mesh is stored 1/rho | lambda | mu with stride tri (the build pads to tri_pad) */
+    hipError_t e = dgx_mesh_mat_packed(mh.d_mat, tri_pad, p->tri, sv->fp32, &md.d_mat);
     if (e == hipSuccess) {
         if (sv->fp32) {
-            float *d = nullptr;
-            e = upload_vec(&d, dgx_sponge_deficit_f32(mh.absorb));
-            m.absorb = d;
+            float *m32 = nullptr, *a32 = nullptr;
+            e = dgx_mesh_f32(nullptr, 0, mh.d_absorb, 6 * tri_pad, &m32, &a32);
+            if (m32) (void)hipFree(m32);
+            m.absorb = a32;
+            (void)hipFree(mh.d_absorb);
         } else {
-            e = upload_reals(sv, &m.absorb, mh.absorb);
+            m.absorb = mh.d_absorb;
         }
+        mh.d_absorb = nullptr;
     }
+    (void)hipFree(mh.d_mat);
+    mh.d_mat = nullptr;
     if (e == hipSuccess && sv->group == 1) {
         e = upload_vec(&m.btiles, bt);
         if (e == hipSuccess) e = upload_vec(&m.blnb, bl);

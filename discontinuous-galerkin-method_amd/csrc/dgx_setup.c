@@ -575,10 +575,10 @@ static long match_vertex(const dgx_locator *L, double x, double z, double tol, i
 }
 
 /* Barycentric test with the same formulas and tolerance as the oracle's locator. */
-static int inside(const dgx_locator *L, long e, double x, double z, double *r, double *s)
+static int inside_elem(const double *point, const long *triangle, long e, double x, double z, double *r, double *s)
 {
     double v[3][2];
-    element_vertices(L->point, L->triangle, e, v);
+    element_vertices(point, triangle, e, v);
     double xr = v[1][0] - v[0][0], zr = v[1][1] - v[0][1];
     double xs = v[2][0] - v[0][0], zs = v[2][1] - v[0][1];
     double det = xr * zs - xs * zr;
@@ -591,6 +591,11 @@ static int inside(const dgx_locator *L, long e, double x, double z, double *r, d
         return 1;
     }
     return 0;
+}
+
+static int inside(const dgx_locator *L, long e, double x, double z, double *r, double *s)
+{
+    return inside_elem(L->point, L->triangle, e, x, z, r, s);
 }
 
 static long locate_point(const dgx_locator *L, double x, double z, double *r, double *s)
@@ -623,32 +628,34 @@ static int vertex_slot(const long *tv, long p)
     return -1;
 }
 
-static double element_jacobian(const dgx_locator *L, long e)
+static double element_jacobian(const double *point, const long *triangle, long e)
 {
     double v[3][2];
-    element_vertices(L->point, L->triangle, e, v);
+    element_vertices(point, triangle, e, v);
     double xr = (v[1][0] - v[0][0]) * 0.5, yr = (v[1][1] - v[0][1]) * 0.5;
     double xs = (v[2][0] - v[0][0]) * 0.5, ys = (v[2][1] - v[0][1]) * 0.5;
     return xr * ys - xs * yr; /* fun_DG.c:441-447 */
 }
 
-int dgx_locate_source(const dgx_locator *L, const dgx_operators *ops, double x, double z, int kind, int shot,
-                      long wavelet, dgx_source *out)
+/* Source vector of a located source: element e (0-based) and, on a vertex, that vertex (0-based), else -1
+ * and the point (x, z) inside e.  Shared by the host and device locators (dgx_locate_sources_device). */
+int dgx_source_at(const double *point, const long *triangle, const dgx_operators *ops, long e, long vertex,
+                  double x, double z, int kind, int shot, long wavelet, dgx_source *out)
 {
-    if (!L || !ops || !out) return DGX_EARG;
     double phi[6] = {0, 0, 0, 0, 0, 0};
-    long e = -1;
-    long p = match_vertex(L, x, z, 1e-3, 0);
-    if (p >= 0 && L->inc_start[p] < L->inc_start[p + 1]) {
-        e = L->inc_elem[L->inc_start[p]]; /* first element holding the vertex */
-        phi[node_of_vertex[vertex_slot(&L->triangle[e * 3], p)]] = 1.0;
+    if (vertex >= 0) {
+        int k = vertex_slot(&triangle[e * 3], vertex);
+        if (k < 0) { dgx_set_error("locator: element %ld does not hold vertex %ld", e + 1, vertex + 1); return DGX_EMESH; }
+        phi[node_of_vertex[k]] = 1.0;
     } else {
         double r, s;
-        e = locate_point(L, x, z, &r, &s);
-        if (e < 0) { dgx_set_error("source (%g, %g) lies outside the mesh", x, z); return DGX_EARG; }
+        if (!inside_elem(point, triangle, e, x, z, &r, &s)) {
+            dgx_set_error("locator: (%g, %g) is not inside element %ld", x, z, e + 1);
+            return DGX_EMESH;
+        }
         dgx_p2_basis(r, s, phi);
     }
-    double J = element_jacobian(L, e);
+    double J = element_jacobian(point, triangle, e);
     for (int i = 0; i < 6; i++) {
         double acc = 0;
         for (int k = 0; k < 6; k++) {
@@ -665,24 +672,61 @@ int dgx_locate_source(const dgx_locator *L, const dgx_operators *ops, double x, 
     return DGX_OK;
 }
 
+/* Receiver weights of a located receiver (element e, vertex or -1, point (x, z)); shared like dgx_source_at. */
+int dgx_receiver_at(const double *point, const long *triangle, long e, long vertex, double x, double z,
+                    dgx_receiver *out)
+{
+    for (int i = 0; i < 6; i++) out->weight[i] = 0;
+    out->elem = e + 1;
+    if (vertex >= 0) {
+        int k = vertex_slot(&triangle[e * 3], vertex);
+        if (k < 0) { dgx_set_error("locator: element %ld does not hold vertex %ld", e + 1, vertex + 1); return DGX_EMESH; }
+        out->weight[node_of_vertex[k]] = 1.0;
+        return DGX_OK;
+    }
+    double r, s;
+    if (!inside_elem(point, triangle, e, x, z, &r, &s)) {
+        dgx_set_error("locator: (%g, %g) is not inside element %ld", x, z, e + 1);
+        return DGX_EMESH;
+    }
+    dgx_p2_basis(r, s, out->weight);
+    return DGX_OK;
+}
+
+int dgx_locate_source(const dgx_locator *L, const dgx_operators *ops, double x, double z, int kind, int shot,
+                      long wavelet, dgx_source *out)
+{
+    if (!L || !ops || !out) return DGX_EARG;
+    long e = -1, vertex = -1;
+    long p = match_vertex(L, x, z, 1e-3, 0);
+    if (p >= 0 && L->inc_start[p] < L->inc_start[p + 1]) {
+        e = L->inc_elem[L->inc_start[p]]; /* first element holding the vertex */
+        vertex = p;
+    } else {
+        double r, s;
+        e = locate_point(L, x, z, &r, &s);
+        if (e < 0) { dgx_set_error("source (%g, %g) lies outside the mesh", x, z); return DGX_EARG; }
+    }
+    return dgx_source_at(L->point, L->triangle, ops, e, vertex, x, z, kind, shot, wavelet, out);
+}
+
 int dgx_locate_receivers(const dgx_locator *L, long n, const double *xz, dgx_receiver *out)
 {
     if (!L || (n > 0 && (!xz || !out))) return DGX_EARG;
     for (long k = 0; k < n; k++) {
         double x = xz[k * 2], z = xz[k * 2 + 1];
-        for (int i = 0; i < 6; i++) out[k].weight[i] = 0;
+        long e = -1, vertex = -1;
         long p = match_vertex(L, x, z, 1e-2, 1);
         if (p >= 0 && L->inc_start[p] < L->inc_start[p + 1]) {
-            long e = L->inc_elem[L->inc_start[p + 1] - 1]; /* last element holding the vertex */
-            out[k].elem = e + 1;
-            out[k].weight[node_of_vertex[vertex_slot(&L->triangle[e * 3], p)]] = 1.0;
-            continue;
+            e = L->inc_elem[L->inc_start[p + 1] - 1]; /* last element holding the vertex */
+            vertex = p;
+        } else {
+            double r, s;
+            e = locate_point(L, x, z, &r, &s);
+            if (e < 0) { dgx_set_error("receiver %ld at (%g, %g) lies outside the mesh", k, x, z); return DGX_EARG; }
         }
-        double r, s;
-        long e = locate_point(L, x, z, &r, &s);
-        if (e < 0) { dgx_set_error("receiver %ld at (%g, %g) lies outside the mesh", k, x, z); return DGX_EARG; }
-        out[k].elem = e + 1;
-        dgx_p2_basis(r, s, out[k].weight);
+        int rc = dgx_receiver_at(L->point, L->triangle, e, vertex, x, z, &out[k]);
+        if (rc) return rc;
     }
     return DGX_OK;
 }

@@ -228,14 +228,20 @@ int main(int argc, char **argv)
     double *absorb = malloc(sizeof(double) * tri * 6);
     CHECK(dgx_sponge(tri, point, triangle, ops.FeketeNew, &c.sp, absorb));
 
-    /* source and receivers (Get_Source, Get_Receiver, DG_leap_Frog_OpenMP.c:207-233) */
-    dgx_locator *loc;
-    CHECK(dgx_locator_create(tri, np, point, triangle, &loc));
+    /* source and receivers (Get_Source, Get_Receiver, DG_leap_Frog_OpenMP.c:207-233), searched on the GPU */
     dgx_source src[MAXSRC];
-    for (int k = 0; k < c.nsrc; k++) CHECK(dgx_locate_source(loc, &ops, c.src_x[k], c.src_z[k], c.src_kind[k], 0, 0, &src[k]));
+    double src_xz[2 * MAXSRC];
+    int src_shot[MAXSRC];
+    long src_wav[MAXSRC];
+    for (int k = 0; k < c.nsrc; k++) {
+        src_xz[2 * k] = c.src_x[k];
+        src_xz[2 * k + 1] = c.src_z[k];
+        src_shot[k] = 0;
+        src_wav[k] = 0;
+    }
+    CHECK(dgx_locate_sources_device(tri, np, point, triangle, &ops, c.nsrc, src_xz, c.src_kind, src_shot, src_wav, src));
     dgx_receiver *rec = malloc(sizeof(dgx_receiver) * (c.nrec ? c.nrec : 1));
-    CHECK(dgx_locate_receivers(loc, c.nrec, c.rec_xz, rec));
-    dgx_locator_destroy(loc);
+    CHECK(dgx_locate_receivers_device(tri, np, point, triangle, c.nrec, c.rec_xz, rec));
     for (int k = 0; k < c.nsrc; k++) printf("source[%d]  element %ld\n", k, src[k].elem);
 
     /* node coordinates for plotting (set_All_node + Save_Data, DG_leap_Frog_OpenMP.c:363-367) */

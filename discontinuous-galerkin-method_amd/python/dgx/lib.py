@@ -141,6 +141,11 @@ def load() -> C.CDLL:
         "dgx_locate_source": (C.c_int, [vp, C.POINTER(Operators), C.c_double, C.c_double, C.c_int, C.c_int,
                                         C.c_long, C.POINTER(Source)]),
         "dgx_locate_receivers": (C.c_int, [vp, C.c_long, dptr, C.POINTER(Receiver)]),
+        "dgx_locate_receivers_device": (C.c_int, [C.c_long, C.c_long, dptr, lptr, C.c_long, dptr,
+                                                  C.POINTER(Receiver)]),
+        "dgx_locate_sources_device": (C.c_int, [C.c_long, C.c_long, dptr, lptr, C.POINTER(Operators), C.c_long,
+                                                dptr, C.POINTER(C.c_int), C.POINTER(C.c_int), lptr,
+                                                C.POINTER(Source)]),
         "dgx_p2_basis": (None, [C.c_double, C.c_double, dptr]),
         "dgx_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(Operators), C.POINTER(vp)]),
         "dgx_server_destroy": (C.c_int, [vp]),

@@ -101,7 +101,12 @@ class Setup:
 
 def build_setup(mesh: Mesh, model: Model, sources: Sequence[SourceSpec], receiver_xz, nsteps: int, *,
                 sponge: Sponge | None, ops: L.Operators | None = None, dt: float | None = None,
-                pi: float = PI_REFERENCE, nshots: int | None = None) -> Setup:
+                pi: float = PI_REFERENCE, nshots: int | None = None, locate: str = "host") -> Setup:
+    """Host-side set-up of a problem (dgx_setup.h).  locate: "host" searches sources and receivers with the
+    host grid locator, "device" with the same search on the GPU (dgx_locate_*_device); both give the same
+    elements and weights bit for bit."""
+    if locate not in ("host", "device"):
+        raise ValueError(f"locate must be 'host' or 'device', not {locate!r}")
     lib = L.load()
     ops = ops or L.read_operators()
     point = np.ascontiguousarray(mesh.point, np.float64)
@@ -115,25 +120,43 @@ def build_setup(mesh: Mesh, model: Model, sources: Sequence[SourceSpec], receive
         L.check(lib.dgx_sponge(T, L.dp(point), L.lp(tri), L.dp(fek), C.byref(sp), L.dp(absorb)), "dgx_sponge")
     if dt is None:
         dt = lib.dgx_cfl_dt(T, L.dp(point), L.lp(tri), L.dp(np.ascontiguousarray(model.vp, np.float64)))
-    loc = C.c_void_p()
-    L.check(lib.dgx_locator_create(T, mesh.point_num, L.dp(point), L.lp(tri), C.byref(loc)), "locator")
-    try:
-        freqs: list[float] = []
-        srcs = []
-        for s in sources:
-            if s.f0 not in freqs:
-                freqs.append(s.f0)
-            out = L.Source()
-            L.check(lib.dgx_locate_source(loc, C.byref(ops), s.x, s.z, s.kind, s.shot, freqs.index(s.f0),
-                                          C.byref(out)), "dgx_locate_source")
-            srcs.append(out)
-        rxz = np.ascontiguousarray(np.asarray(receiver_xz, np.float64).reshape(-1, 2))
-        recs = (L.Receiver * max(1, len(rxz)))()
+    freqs: list[float] = []
+    for s in sources:
+        if s.f0 not in freqs:
+            freqs.append(s.f0)
+    rxz = np.ascontiguousarray(np.asarray(receiver_xz, np.float64).reshape(-1, 2))
+    recs = (L.Receiver * max(1, len(rxz)))()
+    if locate == "device":
+        n = len(sources)
+        out = (L.Source * max(1, n))()
+        if n:
+            sxz = np.ascontiguousarray([[s.x, s.z] for s in sources], np.float64)
+            kind = np.ascontiguousarray([s.kind for s in sources], np.int32)
+            shot = np.ascontiguousarray([s.shot for s in sources], np.int32)
+            wrow = np.ascontiguousarray([freqs.index(s.f0) for s in sources], np.int64)
+            L.check(lib.dgx_locate_sources_device(T, mesh.point_num, L.dp(point), L.lp(tri), C.byref(ops), n,
+                                                  L.dp(sxz), kind.ctypes.data_as(C.POINTER(C.c_int)),
+                                                  shot.ctypes.data_as(C.POINTER(C.c_int)), L.lp(wrow), out),
+                    "dgx_locate_sources_device")
+        srcs = [out[i] for i in range(n)]
         if len(rxz):
-            L.check(lib.dgx_locate_receivers(loc, len(rxz), L.dp(rxz), recs), "dgx_locate_receivers")
-        recs = [recs[i] for i in range(len(rxz))]
-    finally:
-        lib.dgx_locator_destroy(loc)
+            L.check(lib.dgx_locate_receivers_device(T, mesh.point_num, L.dp(point), L.lp(tri), len(rxz), L.dp(rxz),
+                                                    recs), "dgx_locate_receivers_device")
+    else:
+        loc = C.c_void_p()
+        L.check(lib.dgx_locator_create(T, mesh.point_num, L.dp(point), L.lp(tri), C.byref(loc)), "locator")
+        try:
+            srcs = []
+            for s in sources:
+                out1 = L.Source()
+                L.check(lib.dgx_locate_source(loc, C.byref(ops), s.x, s.z, s.kind, s.shot, freqs.index(s.f0),
+                                              C.byref(out1)), "dgx_locate_source")
+                srcs.append(out1)
+            if len(rxz):
+                L.check(lib.dgx_locate_receivers(loc, len(rxz), L.dp(rxz), recs), "dgx_locate_receivers")
+        finally:
+            lib.dgx_locator_destroy(loc)
+    recs = [recs[i] for i in range(len(rxz))]
     t0s, wav = [], np.zeros((max(1, len(freqs)), nsteps), np.float64)
     for k, f in enumerate(freqs):
         t0 = lib.dgx_ricker_delay(dt, f, pi)

@@ -77,6 +77,15 @@ int dgx_locate_source(const dgx_locator *loc, const dgx_operators *ops, double x
  * last element holding it, unit weight); elsewhere the first containing element and the P2 Lagrange
  * weights of the point. */
 int dgx_locate_receivers(const dgx_locator *loc, long n, const double *xz, dgx_receiver *out);
+/* The same two searches on the current HIP device (a vertex match over all vertices, the element holding it, and
+ * the smallest element containing the point among those the grid files under the point's cell), for meshes
+ * and receiver sets too large for the host search to be cheap.  The results equal dgx_locate_source and
+ * dgx_locate_receivers on a locator of the same mesh bit for bit.  Sources: kind, shot and wavelet per point. */
+int dgx_locate_receivers_device(long tri, long point_num, const double *point, const long *triangle, long n,
+                                const double *xz, dgx_receiver *out);
+int dgx_locate_sources_device(long tri, long point_num, const double *point, const long *triangle,
+                              const dgx_operators *ops, long n, const double *xz, const int *kind, const int *shot,
+                              const long *wavelet, dgx_source *out);
 /* P2 Lagrange basis at reference coordinates (r,s) in the node order of FeketeNew.txt. */
 void dgx_p2_basis(double r, double s, double *phi);
 
