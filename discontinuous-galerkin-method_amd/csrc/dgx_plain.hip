@@ -202,9 +202,11 @@ __device__ __forceinline__ void velocity_volume(const StepArgs<R> &a, const G &g
         const R ux = rx * txx[n] + rz * txz[n], wx = sx * txx[n] + sz * txz[n];
         const R uz = rx * txz[n] + rz * tzz[n], wz = sx * txz[n] + sz * tzz[n];
 #pragma unroll
-        for (int i = 0; i < 6; i++) {
-            rhx[i] += op_dr<P2>(a, i * 6 + n) * ux + op_ds<P2>(a, i * 6 + n) * wx;
-            rhz[i] += op_dr<P2>(a, i * 6 + n) * uz + op_ds<P2>(a, i * 6 + n) * wz;
+        for (int i = 0; i < 6; i++) { /* one accumulation per product: two fused multiply-adds, no separate add */
+            rhx[i] += op_dr<P2>(a, i * 6 + n) * ux;
+            rhx[i] += op_ds<P2>(a, i * 6 + n) * wx;
+            rhz[i] += op_dr<P2>(a, i * 6 + n) * uz;
+            rhz[i] += op_ds<P2>(a, i * 6 + n) * wz;
         }
     }
 }
