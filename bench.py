@@ -370,7 +370,7 @@ def main():
     progress(f"{w.name}: {T} elements set up")
     sec64, kt64, x64, z64, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps)
     progress(f"plain FP64: {sec64 * 1e3 / a.steps:.4f} ms/step")
-    sec32, kt32, x32, z32, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps)
+    sec32, kt32, x32, z32, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps, step_kernel=L.DGX_STEP_HALF)
     progress(f"plain FP32: {sec32 * 1e3 / a.steps:.4f} ms/step")
     secb1, ktb1, _, _, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps, step_kernel=L.DGX_STEP_BLOCKED)
     progress(f"blocked FP32: {secb1 * 1e3 / a.steps:.4f} ms/step")

@@ -703,7 +703,18 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         free_solver(s);
         return DGX_EARG;
     }
-    s->layout = (p->layout == DGX_LAYOUT_AUTO) ? (p->nshots >= 8 ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT) : p->layout;
+    /* automatic choices: FP32 runs the blocked kernel in the element layout, one shot per workgroup row, at
+     * any shot count (measured fastest: 1.5x the half-step pair on one shot, 1.26x the shot-innermost kernels
+     * on 64); FP64 takes the half-step pair, shot-innermost from 8 shots; a shot tile or workgroup shape given
+     * with the automatic layout asks for the shot-innermost kernels */
+    const bool fp32_plain = p->precision == DGX_FP32;
+    const bool wants_inner = p->shot_tile != 0 || p->wg_waves != 0;
+    if (p->layout == DGX_LAYOUT_AUTO)
+        s->layout = (fp32_plain && !wants_inner) ? DGX_LAYOUT_ELEMENT
+                                                 : (p->nshots >= 8 ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT);
+    else
+        s->layout = p->layout;
+    const bool auto_blocked = p->step_kernel == DGX_STEP_AUTO && fp32_plain && s->layout == DGX_LAYOUT_ELEMENT;
     if (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF && p->step_kernel != DGX_STEP_FULL &&
         p->step_kernel != DGX_STEP_BLOCKED) {
         dgx_set_error("dgx_create: unknown step_kernel %d", p->step_kernel);
@@ -716,8 +727,9 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         return DGX_EARG;
     }
     if (p->block_steps < 0 || p->block_steps > DGX_BLOCK_KMAX ||
-        (p->block_steps && p->step_kernel != DGX_STEP_BLOCKED)) {
-        dgx_set_error("dgx_create: block_steps must be 0 or 1..%d, with step_kernel DGX_STEP_BLOCKED", DGX_BLOCK_KMAX);
+        (p->block_steps && p->step_kernel != DGX_STEP_BLOCKED && !auto_blocked &&
+         !(s->comp && p->step_kernel == DGX_STEP_AUTO))) {
+        dgx_set_error("dgx_create: block_steps must be 0 or 1..%d, with the blocked kernel", DGX_BLOCK_KMAX);
         free_solver(s);
         return DGX_EARG;
     }
@@ -728,7 +740,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         return DGX_EARG;
     }
     s->full = p->step_kernel == DGX_STEP_FULL;
-    s->blocked = p->step_kernel == DGX_STEP_BLOCKED || s->comp;
+    s->blocked = p->step_kernel == DGX_STEP_BLOCKED || s->comp || auto_blocked;
     s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : 2) : 0;
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         const int t = p->shot_tile;
@@ -890,15 +902,21 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         if (const char *bt = getenv("DGX_BLOCK_THREADS"))
             if (s->fp32 && atoi(bt) == 1024) cap = 1024;
         s->bthreads = cap;
-        if (!dgx_build_block_tiles(T, nbr_host, s->bkmax, cap, btiles_host, blevels_host, bgid_host, blnb_host)) {
+        if (dgx_build_block_tiles(T, nbr_host, s->bkmax, cap, btiles_host, blevels_host, bgid_host, blnb_host)) {
+            s->ntile = (long)btiles_host.size();
+            s->bown = T;
+            s->blanes = (long)bgid_host.size();
+        } else if (auto_blocked) { /* the automatic choice falls back to the half-step pair */
+            s->blocked = 0;
+            s->bkmax = 0;
+            s->bthreads = 0;
+            btiles_host.clear(); blevels_host.clear(); bgid_host.clear(); blnb_host.clear();
+        } else {
             dgx_set_error("dgx_create: an element and its %d levels of neighbours exceed %d threads", 2 * s->bkmax,
                           cap);
             free_solver(s);
             return DGX_EMESH;
         }
-        s->ntile = (long)btiles_host.size();
-        s->bown = T;
-        s->blanes = (long)bgid_host.size();
     }
 
     /* ---- device allocations (all host-side validation is done above) ---- */

@@ -25,7 +25,8 @@
  *   receiver <x> <z>         one receiver (repeatable)                   [1600 1600]
  *   receiver_line <x0> <z0> <x1> <z1> <n>   n receivers on a segment
  *   precision <fp64|fp32>                                                [fp64]
- *   step_kernel <auto|half|full>   one full-step kernel or two half-step kernels per step   [auto]
+ *   step_kernel <auto|half|full|blocked>   two half-step kernels per step, one full-step kernel, or two whole
+ *                                  steps per launch (auto: blocked in FP32, half in FP64)   [auto]
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -128,7 +129,10 @@ static int read_config(const char *path, config *c)
         } else if (!strcmp(key, "precision") && sscanf(line, "%*s %31s", kind) == 1)
             c->precision = !strcmp(kind, "fp32") ? DGX_FP32 : DGX_FP64;
         else if (!strcmp(key, "step_kernel") && sscanf(line, "%*s %31s", kind) == 1)
-            c->step_kernel = !strcmp(kind, "half") ? DGX_STEP_HALF : !strcmp(kind, "full") ? DGX_STEP_FULL : DGX_STEP_AUTO;
+            c->step_kernel = !strcmp(kind, "half")      ? DGX_STEP_HALF
+                             : !strcmp(kind, "full")    ? DGX_STEP_FULL
+                             : !strcmp(kind, "blocked") ? DGX_STEP_BLOCKED
+                                                        : DGX_STEP_AUTO;
         else {
             fprintf(stderr, "config: cannot parse line: %s", line);
             fclose(f);

@@ -35,7 +35,8 @@ enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1, DGX_FP32_COMPENSATED = 2 };
 enum dgx_source_kind { DGX_SRC_FORCE_Z = 0, DGX_SRC_EXPLOSIVE = 1 };
 /* Field layout on the device.  ELEMENT: 64-element blocks, one thread per element and shot (the
  * single-shot path).  SHOT_INNER: shots innermost, lanes over shots, for many shots on one mesh.
- * AUTO picks SHOT_INNER from 8 shots on. */
+ * AUTO: ELEMENT for DGX_FP32 (with the blocked kernel, faster at every shot count measured) unless a shot tile
+ * or workgroup shape is given; otherwise SHOT_INNER from 8 shots on. */
 enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_INNER = 2 };
 /* How dgx_step runs a step.  HALF: two kernels per step (velocity, then stress), each updating the
  * fields in place.  FULL (element layout only): one kernel per step that reads the fields and writes the
@@ -45,7 +46,9 @@ enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_I
  * tile's two phases lengthen that wait).  BLOCKED (element layout): one kernel runs block_steps whole steps
  * per launch; each workgroup holds a tile of elements and the rings of elements within 2*block_steps - 1
  * face hops around it in registers and LDS, recomputes the rings instead of exchanging them, and reads and
- * writes each field once per launch.  AUTO: HALF. */
+ * writes each field once per launch; it keeps a second copy of the fields.  AUTO: BLOCKED for DGX_FP32 and
+ * DGX_FP32_COMPENSATED in the element layout (HALF if no tile fits an element and its rings, which only
+ * extreme vertex valences cause), HALF otherwise. */
 enum dgx_step_kernel { DGX_STEP_AUTO = 0, DGX_STEP_HALF = 1, DGX_STEP_FULL = 2, DGX_STEP_BLOCKED = 3 };
 
 /* Reference-element operators as read from input_files/precomputed_matrix/ (row-major), the arrays
@@ -101,8 +104,8 @@ typedef struct {
     int wg_waves;          /* wavefronts per workgroup of the shot-innermost kernels: 2, 4 or 8 with shot
                             * tiles 16-64, 4 with tiles 4 and 8.  0: autotuned with the shot tile when
                             * shot_tile is 0, else 4.  Given with shot_tile 0, only the tile is tuned. */
-    int block_steps;       /* steps per launch of the blocked kernel (step_kernel = DGX_STEP_BLOCKED): 1, 2 or 3;
-                            * 0 = 2 */
+    int block_steps;       /* steps per launch of the blocked kernel (DGX_STEP_BLOCKED, or AUTO when it picks the
+                            * blocked kernel): 1, 2 or 3; 0 = 2 */
 } dgx_problem;
 
 typedef struct dgx_solver dgx_solver;

@@ -39,12 +39,14 @@ def test_homogeneous_fp64_traces_and_snapshots(homog_case):
             assert e <= FP64_TOL, (k, c, e)
 
 
-def test_homogeneous_fp32_traces(homog_case):
+@pytest.mark.parametrize("step_kernel", [L.DGX_STEP_AUTO, L.DGX_STEP_HALF])
+def test_homogeneous_fp32_traces(homog_case, step_kernel):
+    """FP32 against the FP64 oracle: the default kernel (the blocked kernel for FP32) and the half-step pair."""
     w, oc, ox, oz, osnap = homog_case
     st = product_setup(w)
-    gx, gz, gsnap = product_run(st, L.DGX_FP32, snap_every=250)
+    gx, gz, gsnap = product_run(st, L.DGX_FP32, snap_every=250, step_kernel=step_kernel)
     ex, ez = rel_l2(gx, ox), rel_l2(gz, oz)
-    print(f"fp32 traces rel L2: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
+    print(f"fp32 (step kernel {step_kernel}) traces rel L2: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
     assert ex <= FP32_TOL and ez <= FP32_TOL
     for k, f in osnap.items():
         print(f"fp32 snapshot {k}: Vx {rel_l2(gsnap[k][0], f[0]):.3e} Vz {rel_l2(gsnap[k][1], f[1]):.3e}")
