@@ -151,12 +151,13 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--group", type=int, default=0,
-                    help="shots per slot group of the server, 4, 8, 16, 32 or 64 (default: 8 for serving, 64 for carbonate)")
+                    help="shots per slot group of the server, 1 (single-shot slots on the blocked kernel), 4, 8, 16, 32 or 64 "
+                         "(default: 1 for serving, 64 for carbonate)")
     a = ap.parse_args(argv)
     defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256)}
     a.steps = a.steps or defaults[a.which][0]
     a.shots = a.shots or defaults[a.which][1]
-    a.group = a.group or (64 if a.which == "carbonate" else 8)
+    a.group = a.group or (64 if a.which == "carbonate" else 1)
     fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate}[a.which]
     print(json.dumps(fn(a)), flush=True)
 
