@@ -7,8 +7,9 @@ checks the solver on it at a reduced step count:
   OpenMP_Function.c:25-306) to 1e-12 on traces and all five fields; FP32 against the oracle to 1e-4 on the
   trace set, on every trace at or above 10 % of the peak trace norm, and on every trace against that peak;
 * slope 6 km x 3 km, ~1.2M Delaunay triangles, water over three dipping layers, 64 FP32 shots in the
-  shot-innermost layout: sampled shots against FP64 single-shot GPU runs to 1e-4; one shot FP64 against the
-  oracle to 1e-12;
+  shot-innermost layout and on the blocked kernel (the FP32 default): sampled shots against FP64 single-shot
+  GPU runs to the per-trace bar; sources and receivers located on the GPU equal the host locator's; one shot
+  FP64 against the oracle to 1e-12;
 * the eight resident serving meshes (20k - 2M triangles, random layered models): short requests on several
   meshes are bit-identical alone and in the mix; one request served in FP64 matches the oracle to 1e-12;
 * carbonate 4 km x 3 km, ~2.4M triangles refined to 1 m around 300 vugs and 50 fractures: one shot FP64
@@ -98,8 +99,17 @@ def test_slope_1200k_64_shots():
     w = dgx.configs.slope_6x3km(1500, nshots=64)
     print(f"case {w.name}: {w.mesh.tri} elements, 64 shots, 1500 steps", flush=True)
     st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
+    # the 64 sources and 2001 receivers located on the GPU equal the host locator's, bit for bit
+    sd = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge, locate="device")
+    assert all(bytes(a) == bytes(b) for a, b in zip(st.sources, sd.sources)) and len(sd.sources) == 64
+    assert all(bytes(a) == bytes(b) for a, b in zip(st.receivers, sd.receivers)) and len(sd.receivers) == 2001
+    # the shot-innermost half-step kernels, and the default FP32 path (the blocked kernel, one row of
+    # workgroups per shot)
     b = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER)
     b.step(w.nsteps)
+    k2 = dgx.Solver(st, L.DGX_FP32)
+    assert k2.block_info()["steps_per_launch"] == 2
+    k2.step(w.nsteps)
     for k in (0, 21, 42, 63):
         s = w.sources[k]
         one = dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge,
@@ -108,7 +118,10 @@ def test_slope_1200k_64_shots():
         x64, z64, _ = product_run(st1, L.DGX_FP64)
         x32, z32 = b.traces(k)
         check_fp32_traces(x32, z32, x64, z64, f"slope shot {k} batched fp32 vs fp64")
+        x32, z32 = k2.traces(k)
+        check_fp32_traces(x32, z32, x64, z64, f"slope shot {k} blocked fp32 vs fp64")
     b.close()
+    k2.close()
     # one shot, FP64, against the oracle
     s = w.sources[21]
     one = dgx.configs.Workload(w.name, w.mesh, w.model, w.sponge, [dgx.SourceSpec(s.x, s.z, s.kind, 0, s.f0)],
