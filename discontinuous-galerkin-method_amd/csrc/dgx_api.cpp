@@ -741,7 +741,10 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     s->full = p->step_kernel == DGX_STEP_FULL;
     s->blocked = p->step_kernel == DGX_STEP_BLOCKED || s->comp || auto_blocked;
-    s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : 2) : 0;
+    /* steps per launch by default: 3 up to 50,000 elements, where the launch itself is a large part of a step,
+     * 2 above, where the rings' recomputation dominates (FP32, ms per step, K = 1 / 2 / 3: 20k elements
+     * 0.0084 / 0.0058 / 0.0051; 120k 0.0147 / 0.0104 / 0.0110; 640k 0.0421 / 0.0334 / 0.0373) */
+    s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : (T <= 50000 ? 3 : 2)) : 0;
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         const int t = p->shot_tile;
         if (t != 0 && t != 4 && t != 8 && t != 16 && t != 32 && t != 64) {
