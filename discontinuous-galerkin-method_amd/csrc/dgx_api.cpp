@@ -705,8 +705,10 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     /* automatic choices: FP32 runs the blocked kernel in the element layout, one shot per workgroup row, at
      * any shot count (measured fastest: 1.5x the half-step pair on one shot, 1.26x the shot-innermost kernels
-     * on 64); FP64 takes the half-step pair, shot-innermost from 8 shots; a shot tile or workgroup shape given
-     * with the automatic layout asks for the shot-innermost kernels */
+     * on 64).  FP64 runs it while its two field copies (2 x 240 B per element and shot) stay within the 256 MB
+     * Infinity Cache, elements x shots <= 400,000 (measured: 2.3x the half-step pair at 20k elements, 1.10x
+     * at 400k, 0.91x at 500k), else the half-step pair, shot-innermost from 8 shots.  A shot tile or
+     * workgroup shape given with the automatic layout asks for the shot-innermost kernels */
     const bool fp32_plain = p->precision == DGX_FP32;
     const bool wants_inner = p->shot_tile != 0 || p->wg_waves != 0;
     if (p->layout == DGX_LAYOUT_AUTO)
@@ -714,7 +716,9 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
                                                  : (p->nshots >= 8 ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT);
     else
         s->layout = p->layout;
-    const bool auto_blocked = p->step_kernel == DGX_STEP_AUTO && fp32_plain && s->layout == DGX_LAYOUT_ELEMENT;
+    const bool fp64_cached = p->precision == DGX_FP64 && T * p->nshots <= 400000;
+    const bool auto_blocked =
+        p->step_kernel == DGX_STEP_AUTO && (fp32_plain || fp64_cached) && s->layout == DGX_LAYOUT_ELEMENT;
     if (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF && p->step_kernel != DGX_STEP_FULL &&
         p->step_kernel != DGX_STEP_BLOCKED) {
         dgx_set_error("dgx_create: unknown step_kernel %d", p->step_kernel);

@@ -46,9 +46,10 @@ enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_I
  * tile's two phases lengthen that wait).  BLOCKED (element layout): one kernel runs block_steps whole steps
  * per launch; each workgroup holds a tile of elements and the rings of elements within 2*block_steps - 1
  * face hops around it in registers and LDS, recomputes the rings instead of exchanging them, and reads and
- * writes each field once per launch; it keeps a second copy of the fields.  AUTO: BLOCKED for DGX_FP32 and
- * DGX_FP32_COMPENSATED in the element layout (HALF if no tile fits an element and its rings, which only
- * extreme vertex valences cause), HALF otherwise. */
+ * writes each field once per launch; it keeps a second copy of the fields.  AUTO: BLOCKED in the element
+ * layout for DGX_FP32 and DGX_FP32_COMPENSATED, and for DGX_FP64 while elements x shots <= 400,000 (both field
+ * copies within the Infinity Cache); HALF otherwise, and HALF if no tile fits an element and its rings (which
+ * only extreme vertex valences cause). */
 enum dgx_step_kernel { DGX_STEP_AUTO = 0, DGX_STEP_HALF = 1, DGX_STEP_FULL = 2, DGX_STEP_BLOCKED = 3 };
 
 /* Reference-element operators as read from input_files/precomputed_matrix/ (row-major), the arrays

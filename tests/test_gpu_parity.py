@@ -25,13 +25,16 @@ def homog_case():
     return w, oc, ox, oz, osnap
 
 
-def test_homogeneous_fp64_traces_and_snapshots(homog_case):
+@pytest.mark.parametrize("step_kernel", [L.DGX_STEP_AUTO, L.DGX_STEP_HALF])
+def test_homogeneous_fp64_traces_and_snapshots(homog_case, step_kernel):
+    """FP64 against the oracle: the default kernel (the blocked kernel on this 20k-element mesh) and the
+    half-step pair."""
     w, oc, ox, oz, osnap = homog_case
     st = product_setup(w)
     assert st.dt == oc.dt
-    gx, gz, gsnap = product_run(st, L.DGX_FP64, snap_every=250)
+    gx, gz, gsnap = product_run(st, L.DGX_FP64, snap_every=250, step_kernel=step_kernel)
     ex, ez = rel_l2(gx, ox), rel_l2(gz, oz)
-    print(f"fp64 traces rel L2: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
+    print(f"fp64 (step kernel {step_kernel}) traces rel L2: Vx {ex:.3e} Vz {ez:.3e}", flush=True)
     assert ex <= FP64_TOL and ez <= FP64_TOL
     for k, f in osnap.items():
         for c in range(5):
