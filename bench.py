@@ -108,7 +108,7 @@ def end_to_end(setup, nsteps, snap_every):
     buf = np.zeros((5, T * 6))
     buf[:] = 1.0  # touch the pages before the timed region
     t = time.perf_counter()
-    s = dgx.Solver(setup, L.DGX_FP64)
+    s = dgx.Solver(setup, L.DGX_FP64, step_kernel=L.DGX_STEP_HALF)
     s.synchronize()
     create = time.perf_counter() - t
     done, pending, nsnap = 0, False, 0
@@ -185,7 +185,7 @@ def large_leg(a):
     n = 320
     st = dgx.build_setup(mesh, model, [dgx.SourceSpec(W / 2, H - 300.0, L.DGX_SRC_EXPLOSIVE, 0, 20.0)],
                          np.stack([np.linspace(100.0, W - 100.0, 101), np.full(101, H - 5.0)], 1), n + 200, sponge=sp)
-    sec, kt, _, _, _ = run_path(st, L.DGX_FP64, 64, n)
+    sec, kt, _, _, _ = run_path(st, L.DGX_FP64, 64, n, step_kernel=L.DGX_STEP_HALF)
     T = mesh.tri
     rate = T * 30 * n / sec
     kv, ks = kt["velocity"], kt["stress"]
@@ -368,7 +368,7 @@ def main():
     dof = T * 30
 
     progress(f"{w.name}: {T} elements set up")
-    sec64, kt64, x64, z64, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps)
+    sec64, kt64, x64, z64, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps, step_kernel=L.DGX_STEP_HALF)
     progress(f"plain FP64: {sec64 * 1e3 / a.steps:.4f} ms/step")
     sec32, kt32, x32, z32, _ = run_path(setup, L.DGX_FP32, a.warmup, a.steps, step_kernel=L.DGX_STEP_HALF)
     progress(f"plain FP32: {sec32 * 1e3 / a.steps:.4f} ms/step")
