@@ -194,6 +194,29 @@ def large_leg(a):
             "velocity_frac": T * 6 * 8 * 7 / (kv * 1e-3) / HBM_PEAK, "stress_frac": T * 6 * 8 * 8 / (ks * 1e-3) / HBM_PEAK}
 
 
+def small_leg(a):
+    """FP64 on the 2 km homogeneous case (20,000 elements, the reference's own problem size): the default
+    step kernel (the blocked kernel, 3 steps per launch at this size: both field copies stay in the Infinity
+    Cache) against the half-step pair, ms per step over 1440 steps after 96 of warm-up."""
+    w = dgx.configs.homogeneous_2km(1600)
+    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, 1600, sponge=w.sponge)
+    out = {"workload": w.name, "elements": w.mesh.tri}
+    for name, sk in (("default", L.DGX_STEP_AUTO), ("half_step", L.DGX_STEP_HALF)):
+        s = dgx.Solver(st, L.DGX_FP64, step_kernel=sk)
+        s.step(96)
+        s.synchronize()
+        t = time.perf_counter()
+        s.step(1440)
+        s.synchronize()
+        sec = time.perf_counter() - t
+        out[name] = {"ms_per_step": sec * 1e3 / 1440, "value": w.mesh.tri * 30 * 1440 / sec}
+        if sk == L.DGX_STEP_AUTO:
+            out[name]["tiling"] = s.block_info()
+        s.close()
+    out["default_vs_half_step"] = out["default"]["value"] / out["half_step"]["value"]
+    return out
+
+
 def fp64_trace_check(w, st, traces_of, picks, n):
     """Worst FP32 trace error of shots `picks` (traces_of(k) -> (x, z)) against FP64 single-shot runs of the
     plain path on the same mesh, sources and dt: max over the shots of the set error, of the worst trace at
@@ -436,6 +459,9 @@ def main():
         out["plain_fp64_large"] = large_leg(a)
         progress(f"plain FP64, 2M elements: {out['plain_fp64_large']['value']:.3e} DOF-updates/s, "
                  f"{out['plain_fp64_large']['step_frac']:.3f} of HBM peak")
+    out["plain_fp64_small"] = small_leg(a)
+    progress(f"plain FP64, 20k elements: default {out['plain_fp64_small']['default']['ms_per_step']:.4f} ms/step, "
+             f"half-step {out['plain_fp64_small']['half_step']['ms_per_step']:.4f}")
     if "accuracy" not in a.skip:
         out["accuracy"] = accuracy_leg(dataclasses.replace(w, nsteps=4000), {
             "fp32": (L.DGX_FP32, L.DGX_STEP_HALF), "blocked_fp32": (L.DGX_FP32, L.DGX_STEP_BLOCKED),
