@@ -705,10 +705,11 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     /* automatic choices: FP32 runs the blocked kernel in the element layout, one shot per workgroup row, at
      * any shot count (measured fastest: 1.5x the half-step pair on one shot, 1.26x the shot-innermost kernels
-     * on 64).  FP64 runs it while its two field copies (2 x 240 B per element and shot) stay within the 256 MB
-     * Infinity Cache, elements x shots <= 400,000 (measured: 2.3x the half-step pair at 20k elements, 1.10x
-     * at 400k, 0.91x at 500k), else the half-step pair, shot-innermost from 8 shots.  A shot tile or
-     * workgroup shape given with the automatic layout asks for the shot-innermost kernels */
+     * on 64).  FP64 runs it (256-thread tiles) while elements x shots <= 550,000, about where its two field
+     * copies (2 x 240 B per element and shot) outgrow the 256 MB Infinity Cache (measured: 2.7x the half-step
+     * pair at 20k elements, 1.05x at 500k, 0.95x at 640k), else the half-step pair, shot-innermost from 8
+     * shots.  A shot tile or workgroup shape given with the automatic layout asks for the shot-innermost
+     * kernels */
     const bool fp32_plain = p->precision == DGX_FP32;
     const bool wants_inner = p->shot_tile != 0 || p->wg_waves != 0;
     if (p->layout == DGX_LAYOUT_AUTO)
@@ -716,7 +717,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
                                                  : (p->nshots >= 8 ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT);
     else
         s->layout = p->layout;
-    const bool fp64_cached = p->precision == DGX_FP64 && T * p->nshots <= 400000;
+    const bool fp64_cached = p->precision == DGX_FP64 && T * p->nshots <= 550000;
     const bool auto_blocked =
         p->step_kernel == DGX_STEP_AUTO && (fp32_plain || fp64_cached) && s->layout == DGX_LAYOUT_ELEMENT;
     if (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF && p->step_kernel != DGX_STEP_FULL &&
@@ -745,10 +746,11 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     s->full = p->step_kernel == DGX_STEP_FULL;
     s->blocked = p->step_kernel == DGX_STEP_BLOCKED || s->comp || auto_blocked;
-    /* steps per launch by default: 3 up to 50,000 elements, where the launch itself is a large part of a step,
-     * 2 above, where the rings' recomputation dominates (FP32, ms per step, K = 1 / 2 / 3: 20k elements
-     * 0.0084 / 0.0058 / 0.0051; 120k 0.0147 / 0.0104 / 0.0110; 640k 0.0421 / 0.0334 / 0.0373) */
-    s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : (T <= 50000 ? 3 : 2)) : 0;
+    /* steps per launch by default: FP32 3 up to 50,000 elements, where the launch itself is a large part of a
+     * step, 2 above, where the rings' recomputation dominates (ms per step, K = 1 / 2 / 3: 20k elements
+     * 0.0084 / 0.0058 / 0.0051; 120k 0.0147 / 0.0104 / 0.0110; 640k 0.0421 / 0.0334 / 0.0373); FP64 2 (its
+     * 256-thread tiles: 20k 0.0063 / 0.0073 for K = 2 / 3, 120k 0.0191 / 0.0239) */
+    s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : (s->fp32 && T <= 50000 ? 3 : 2)) : 0;
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         const int t = p->shot_tile;
         if (t != 0 && t != 4 && t != 8 && t != 16 && t != 32 && t != 64) {
@@ -902,12 +904,17 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     std::vector<int4> btiles_host, blnb_host;
     std::vector<int> blevels_host, bgid_host;
     if (s->blocked) {
-        /* threads per workgroup of k_block: 512 (FP32: two workgroups per CU, so that one's loads overlap the
-         * other's arithmetic; measured 1.2-1.3x faster than one 1024-thread workgroup per CU despite the larger
-         * rings); DGX_BLOCK_THREADS=1024 selects 1024-thread FP32 tiles */
-        int cap = 512;
-        if (const char *bt = getenv("DGX_BLOCK_THREADS"))
-            if (s->fp32 && atoi(bt) == 1024) cap = 1024;
+        /* threads per workgroup of k_block: two workgroups per CU, so that one's loads and barriers overlap the
+         * other's arithmetic.  FP32: 512 (128 VGPRs; measured 1.2-1.3x faster than one 1024-thread workgroup per
+         * CU despite the larger rings).  FP64: 256 (about 235 VGPRs allow only two waves per SIMD; measured 1.07-
+         * 1.16x faster than 512-thread tiles although their rings recompute 1.4-1.6 instead of 0.9 of the own
+         * work).  DGX_BLOCK_THREADS selects 1024 (FP32) or 512 (FP64) instead */
+        int cap = s->fp32 ? 512 : 256;
+        if (const char *bt = getenv("DGX_BLOCK_THREADS")) {
+            const int v = atoi(bt);
+            if (s->fp32 && v == 1024) cap = 1024;
+            if (!s->fp32 && v == 512) cap = 512;
+        }
         s->bthreads = cap;
         if (dgx_build_block_tiles(T, nbr_host, s->bkmax, cap, btiles_host, blevels_host, bgid_host, blnb_host)) {
             s->ntile = (long)btiles_host.size();

@@ -1249,12 +1249,14 @@ static void launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st
     case 5: dgx::k_fullstep<R, P2><<<dim3(DGX_NXCD * a.tper, nshots), block, 0, st>>>(a); break;
     case 6:
         /* one element per thread, a.bthreads threads (the tiles are built for that count): FP32 1024 (one
-         * workgroup per CU) or 512 (two), FP64 512 */
+         * workgroup per CU) or 512 (two), FP64 256 (two) or 512 (one) */
         if constexpr (sizeof(R) == 4) {
             /* compensated FP32: twice the registers for the low words, one 512-thread workgroup per CU */
             if (a.fl[0]) dgx::k_block<R, 512, 2, P2, true><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
             else if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 1024, 0, st>>>(a);
             else dgx::k_block<R, 512, 4, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
+        } else if (a.bthreads == 256) {
+            dgx::k_block<R, 256, 2, P2><<<dim3(DGX_NXCD * a.tper, nshots), 256, 0, st>>>(a);
         } else {
             dgx::k_block<R, 512, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
         }

@@ -47,9 +47,9 @@ enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_I
  * per launch; each workgroup holds a tile of elements and the rings of elements within 2*block_steps - 1
  * face hops around it in registers and LDS, recomputes the rings instead of exchanging them, and reads and
  * writes each field once per launch; it keeps a second copy of the fields.  AUTO: BLOCKED in the element
- * layout for DGX_FP32 and DGX_FP32_COMPENSATED, and for DGX_FP64 while elements x shots <= 400,000 (both field
- * copies within the Infinity Cache); HALF otherwise, and HALF if no tile fits an element and its rings (which
- * only extreme vertex valences cause). */
+ * layout for DGX_FP32 and DGX_FP32_COMPENSATED, and for DGX_FP64 while elements x shots <= 550,000 (about
+ * where its two field copies outgrow the Infinity Cache); HALF otherwise, and HALF if no tile fits an element
+ * and its rings (which only extreme vertex valences cause). */
 enum dgx_step_kernel { DGX_STEP_AUTO = 0, DGX_STEP_HALF = 1, DGX_STEP_FULL = 2, DGX_STEP_BLOCKED = 3 };
 
 /* Reference-element operators as read from input_files/precomputed_matrix/ (row-major), the arrays
@@ -106,7 +106,7 @@ typedef struct {
                             * tiles 16-64, 4 with tiles 4 and 8.  0: autotuned with the shot tile when
                             * shot_tile is 0, else 4.  Given with shot_tile 0, only the tile is tuned. */
     int block_steps;       /* steps per launch of the blocked kernel (DGX_STEP_BLOCKED, or AUTO when it picks the
-                            * blocked kernel): 1, 2 or 3; 0 = 3 up to 50,000 elements, else 2 */
+                            * blocked kernel): 1, 2 or 3; 0 = 2, and 3 in FP32 up to 50,000 elements */
 } dgx_problem;
 
 typedef struct dgx_solver dgx_solver;
