@@ -196,8 +196,8 @@ def large_leg(a):
 
 def small_leg(a):
     """FP64 on the 2 km homogeneous case (20,000 elements, the reference's own problem size): the default
-    step kernel (the blocked kernel, 3 steps per launch at this size: both field copies stay in the Infinity
-    Cache) against the half-step pair, ms per step over 1440 steps after 96 of warm-up."""
+    step kernel (the blocked kernel on 256-thread tiles, 2 steps per launch: both field copies stay in the
+    Infinity Cache) against the half-step pair, ms per step over 1440 steps after 96 of warm-up."""
     w = dgx.configs.homogeneous_2km(1600)
     st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, 1600, sponge=w.sponge)
     out = {"workload": w.name, "elements": w.mesh.tri}
