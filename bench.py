@@ -64,11 +64,11 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=1,
                     help="also time a plain device copy (achievable bandwidth, counter calibration)")
     ap.add_argument("--skip", default="",
-                    help="comma-separated legs to leave out (large, accuracy, end_to_end), for profiling runs; the "
-                         "headline fields are always measured")
+                    help="comma-separated legs to leave out (large, small, accuracy, end_to_end), for profiling runs; "
+                         "the headline fields are always measured")
     a = ap.parse_args()
     a.skip = {x for x in a.skip.split(",") if x}
-    bad = a.skip - {"large", "accuracy", "end_to_end"}
+    bad = a.skip - {"large", "small", "accuracy", "end_to_end"}
     if bad:
         ap.error(f"--skip: unknown legs {sorted(bad)}")
     return a
@@ -459,9 +459,10 @@ def main():
         out["plain_fp64_large"] = large_leg(a)
         progress(f"plain FP64, 2M elements: {out['plain_fp64_large']['value']:.3e} DOF-updates/s, "
                  f"{out['plain_fp64_large']['step_frac']:.3f} of HBM peak")
-    out["plain_fp64_small"] = small_leg(a)
-    progress(f"plain FP64, 20k elements: default {out['plain_fp64_small']['default']['ms_per_step']:.4f} ms/step, "
-             f"half-step {out['plain_fp64_small']['half_step']['ms_per_step']:.4f}")
+    if "small" not in a.skip:
+        out["plain_fp64_small"] = small_leg(a)
+        progress(f"plain FP64, 20k elements: default {out['plain_fp64_small']['default']['ms_per_step']:.4f} "
+                 f"ms/step, half-step {out['plain_fp64_small']['half_step']['ms_per_step']:.4f}")
     if "accuracy" not in a.skip:
         out["accuracy"] = accuracy_leg(dataclasses.replace(w, nsteps=4000), {
             "fp32": (L.DGX_FP32, L.DGX_STEP_HALF), "blocked_fp32": (L.DGX_FP32, L.DGX_STEP_BLOCKED),
