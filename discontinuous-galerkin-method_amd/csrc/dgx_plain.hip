@@ -1339,7 +1339,8 @@ hipError_t dgx_launch_unpermute(const void *f, const void *lo, int fp32, const i
     else dgx::k_unpermute<double><<<blocks, 256, 0, st>>>((const double *)f, (const double *)lo, perm, out, (int)tri, spad, il, shot);
     return hipGetLastError();
 }
-/* Serving launch of the blocked kernel over the tiles of nb single-shot batches (ks_block). */
+/* Serving launch of the blocked kernel over the tiles of nb single-shot batches (ks_block): 512 threads per
+ * workgroup in FP32, 256 in FP64 (the tiles are built for that count). */
 hipError_t dgx_launch_serve_block(int fp32, int p2, const void *table, const int *sched, const int *step_base,
                                   int step_off, int launch, int nb, int total, hipStream_t st)
 {
@@ -1350,10 +1351,10 @@ hipError_t dgx_launch_serve_block(int fp32, int p2, const void *table, const int
         const auto *t = (const dgx::StepArgs<float> *)table;
         if (p2) dgx::ks_block<float, 512, 4, true><<<grid, 512, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
         else dgx::ks_block<float, 512, 4, false><<<grid, 512, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
-    } else {
+    } else { /* FP64: 256-thread tiles, two workgroups per CU */
         const auto *t = (const dgx::StepArgs<double> *)table;
-        if (p2) dgx::ks_block<double, 512, 1, true><<<grid, 512, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
-        else dgx::ks_block<double, 512, 1, false><<<grid, 512, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
+        if (p2) dgx::ks_block<double, 256, 2, true><<<grid, 256, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
+        else dgx::ks_block<double, 256, 2, false><<<grid, 256, 0, st>>>(t, sched, step_base, step_off, launch, nb, total, nper);
     }
     return hipGetLastError();
 }

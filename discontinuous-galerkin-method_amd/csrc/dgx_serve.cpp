@@ -50,7 +50,7 @@ namespace {
 const int MAXG = 64;      /* largest slot group (shots per batch = shot tile of the serving kernel) */
 const int MAX_BATCHES = 4096;
 const int kServeBlockSteps = 2;   /* single-shot slots: steps per launch of the blocked kernel */
-const int kServeBlockThreads = 512; /* and threads per workgroup (lanes per tile) */
+
 
 /* A material model on a resident mesh: 1/rho | lambda | mu per element in device order, and the time step
  * its requests run with.  Model 0 of every mesh is the one given at dgx_server_add_mesh and stays resident;
@@ -188,6 +188,10 @@ struct dgx_server {
 
     size_t rsize() const { return fp32 ? 4 : 8; }
 };
+
+/* Threads per workgroup (lanes per tile) of the single-shot slots' blocked kernel: 512 in FP32, 256 in FP64 (two
+ * workgroups per CU in both, as for the solver's blocked kernel; dgx_launch_serve_block launches that many). */
+static int serve_block_threads(const dgx_server *sv) { return sv->fp32 ? 512 : 256; }
 
 /* Slot groups that hold n shots. */
 static int groups_for(const dgx_server *sv, int n) { return (n + sv->group - 1) / sv->group; }
@@ -421,9 +425,10 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     std::vector<int4> bt, bl;
     std::vector<int> blv, bg;
     if (sv->group == 1 &&
-        !dgx_build_block_tiles(p->tri, mh.nbr, kServeBlockSteps, kServeBlockThreads, bt, blv, bg, bl)) {
+        !dgx_build_block_tiles(p->tri, mh.nbr, kServeBlockSteps, serve_block_threads(sv), bt, blv, bg, bl)) {
         dgx_mesh_dev_free(&mh);
-        dgx_set_error("dgx_server_add_mesh: an element and its neighbour levels exceed %d threads", kServeBlockThreads);
+        dgx_set_error("dgx_server_add_mesh: an element and its neighbour levels exceed %d threads",
+                      serve_block_threads(sv));
         return DGX_EMESH;
     }
     MeshDev m;
@@ -751,7 +756,7 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
         a.ntile = (int)m.ntile;
         a.bkmax = kServeBlockSteps;
         a.bsteps = kServeBlockSteps;
-        a.bthreads = kServeBlockThreads;
+        a.bthreads = serve_block_threads(sv);
     }
     a.nbr = m.nbr;
     a.vtx = m.vtx;

@@ -11,7 +11,8 @@
 
 The first two hold for every slot-group size (4, 8, 16, 32 and 64 shots per group, which is also the serving
 kernel's shot tile) and for single-shot slots (group 1), which the blocked kernel steps two steps per launch
-(compared with the standalone solver's blocked kernel).
+(compared with the standalone solver's blocked kernel).  The first is also checked in FP64, for single-shot
+slots (256-thread tiles) and 16-shot groups.
 """
 import numpy as np
 import pytest
@@ -64,10 +65,11 @@ def run_alone(k, group=16, precision=L.DGX_FP32):
     return out, xz, rec
 
 
-@pytest.mark.parametrize("group", [1, 4, 8, 16, 32, 64])
-def test_results_identical_alone_and_in_the_mix(group):
-    print(f"case serve mix, group {group}", flush=True)
-    sv, rms, _, _ = new_server(group=group)
+@pytest.mark.parametrize("group,precision", [(g, L.DGX_FP32) for g in (1, 4, 8, 16, 32, 64)]
+                         + [(1, L.DGX_FP64), (16, L.DGX_FP64)])
+def test_results_identical_alone_and_in_the_mix(group, precision):
+    print(f"case serve mix, group {group}, precision {precision}", flush=True)
+    sv, rms, _, _ = new_server(precision=precision, group=group)
     built = [build(sv, rms, spec, k) for k, spec in enumerate(REQS)]
     ids = [sv.submit(built[0][0]), sv.submit(built[1][0])]
     sv.advance(1)
@@ -80,7 +82,7 @@ def test_results_identical_alone_and_in_the_mix(group):
     mixed = [sv.result(rid, built[k][0]) for k, rid in enumerate(ids)]
     sv.close()
     for k in range(len(REQS)):
-        (ax, az), _, _ = run_alone(k, group)
+        (ax, az), _, _ = run_alone(k, group, precision)
         assert np.array_equal(mixed[k][0], ax) and np.array_equal(mixed[k][1], az), k
 
 
