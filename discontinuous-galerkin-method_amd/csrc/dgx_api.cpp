@@ -133,7 +133,8 @@ struct dgx_solver {
     int blocked = 0;        /* blocked kernel: bkmax steps per launch (element layout) */
     int bkmax = 0, bthreads = 0;
     long bown = 0, blanes = 0; /* own elements and lanes over all tiles of the blocked kernel */
-    int *blevels = nullptr, *bgid = nullptr;
+    int *blevels = nullptr;
+    int4 *blrec = nullptr;
     int4 *blnb = nullptr;
     int cur = 0;            /* full step: copy of the fields holding the current state */
     long vrows = 0;         /* element layout: rows per field and shot (elements, then ring rows) */
@@ -243,7 +244,8 @@ static void build_tiles(long tri, const std::vector<int4> &nbr, long vrow0, std:
 
 /*
  * Tiles of the blocked kernel: runs of consecutive elements (locality order), each grown by bisection on its
- * length to the longest run that fits `cap` threads together with every element within 2K face hops of it.
+ * length to the longest run that fits cap - 1 lanes together with every element within 2K face hops of it
+ * (`cap` is the workgroup's thread count; its last lane stays free as the kernel's zero lane).
  * Levels 0 .. 2K-1 are updated by the kernel; level 2K (ghosts) only supplies its stresses to the first
  * velocity update of level 2K-1.  tiles[t] = {first, own, first lane record, lanes};
  * levels[t*DGX_BLOCK_LEVELS + d] = end lane of level d, entries past level 2K equal to the lane count.  Lane
@@ -255,6 +257,7 @@ bool dgx_build_block_tiles(long tri, const std::vector<int4> &nbr, int K, int ca
                            std::vector<int> &levels, std::vector<int> &gid, std::vector<int4> &lnb)
 {
     const int depth = 2 * K; /* the ghost level */
+    cap -= 1;                /* lanes for elements: the last thread of the workgroup is the zero lane */
     std::vector<int> stamp(tri, 0), slot(tri, -1);
     int cur = 0;
     std::vector<int> front, next;
@@ -409,7 +412,7 @@ static void fill_args(dgx_solver *s, dgx::StepArgs<R> &a, const dgx_operators *o
     a.fstride = s->vrows;
     a.nb2 = s->nb2;
     a.blevels = s->blevels;
-    a.bgid = s->bgid;
+    a.blrec = s->blrec;
     a.blnb = s->blnb;
     a.bkmax = s->bkmax;
     a.bthreads = s->bthreads;
@@ -469,7 +472,7 @@ static void free_solver(dgx_solver *s)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {s->fields, s->fields2, s->nb2, s->tiles, s->ring, s->nbr, s->vtx, s->pt, s->mat, s->absorb,
                     s->aux, s->src, s->rec, s->wav, s->trace, s->step_base, s->d_perm, s->d_snap,
-                    s->blevels, s->bgid, s->blnb};
+                    s->blevels, s->blrec, s->blnb};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_snap) (void)hipHostFree(s->h_snap);
@@ -957,7 +960,7 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     if (s->blocked) {
         CREATE_TRY(upload(&s->tiles, btiles_host));
         CREATE_TRY(upload(&s->blevels, blevels_host));
-        CREATE_TRY(upload(&s->bgid, bgid_host));
+        CREATE_TRY(dgx_upload_lane_records(bgid_host.data(), (long)bgid_host.size(), s->vtx, &s->blrec));
         CREATE_TRY(upload(&s->blnb, blnb_host));
     }
     if (!aux_elems.empty()) /* the code words now carry their source/receiver records */

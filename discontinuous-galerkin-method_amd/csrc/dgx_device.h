@@ -74,21 +74,22 @@ __device__ __forceinline__ Geom geometry(const double2 *pt, int4 v)
     return g;
 }
 
-/* The geometric factors rounded once to the working precision R (the element kernels convert each factor
- * with the same cast where they use it), for kernels that keep them in registers over several steps. */
+/* The geometric factors rounded once to the working precision R, for kernels that keep them in registers over
+ * several steps: rx, rz, sx, sz, and each face's edge vector divided by D, (ex_j, ez_j)/D, so that the face
+ * terms carry their 1/D scale and accumulate into the volume terms directly. */
 template <typename R>
 struct GeomR {
-    R rx, rz, sx, sz, invD;
-    R ex[3], ez[3];
+    R rx, rz, sx, sz;
+    R exd[3], ezd[3];
 };
 
 template <typename R>
 __device__ __forceinline__ GeomR<R> geometry_r(const Geom &g)
 {
     GeomR<R> r;
-    r.rx = (R)g.rx; r.rz = (R)g.rz; r.sx = (R)g.sx; r.sz = (R)g.sz; r.invD = (R)g.invD;
+    r.rx = (R)g.rx; r.rz = (R)g.rz; r.sx = (R)g.sx; r.sz = (R)g.sz;
 #pragma unroll
-    for (int j = 0; j < 3; j++) { r.ex[j] = (R)g.ex[j]; r.ez[j] = (R)g.ez[j]; }
+    for (int j = 0; j < 3; j++) { r.exd[j] = (R)(g.ex[j] * g.invD); r.ezd[j] = (R)(g.ez[j] * g.invD); }
     return r;
 }
 

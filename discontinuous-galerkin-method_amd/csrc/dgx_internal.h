@@ -26,6 +26,10 @@ struct dgx_mesh_host {
 int dgx_build_mesh_host(const dgx_problem *p, int shot_inner, long tri_pad, const std::vector<char> *explosive,
                         dgx_mesh_host *out);
 
+/* The blocked kernel's lane records {element, v0, v1, v2}, built on the device from the lanes' element ids (n, host)
+ * and the device vertex records (dgx_plain.hip); *out is allocated there and owned by the caller. */
+extern "C" hipError_t dgx_upload_lane_records(const int *gid, long n, const int4 *vtx, int4 **out);
+
 /* The same mesh built on the device (dgx_meshdev.hip), bit for bit the arrays of dgx_build_mesh_host: nbr, vtx,
  * pt (npt used vertices), mat and absorb (FP64, in the layout of shot_inner) stay in device memory, owned by the
  * caller (dgx_mesh_dev_free); perm, inv and a copy of nbr come back to the host. */

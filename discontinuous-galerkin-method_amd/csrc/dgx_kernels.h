@@ -108,7 +108,7 @@ struct StepArgs {
     /* blocked kernel only (element layout, several steps per launch; tiles/ntile/tper above describe its
      * tiles {first own element, own elements, first lane record, lanes}) */
     const int *blevels;    /* per tile DGX_BLOCK_LEVELS ints: end lane of each level (hops from the own elements) */
-    const int *bgid;       /* per lane record: the element */
+    const int4 *blrec;     /* per lane record: {element, its vertices v0, v1, v2} */
     const int4 *blnb;      /* per lane record: {lane of face j's neighbour (j = 0..2), code}; -1 boundary */
     R *fl[5], *gl[5];      /* compensated FP32 (blocked kernel): the low words of f and g; NULL otherwise */
     int bkmax;             /* steps the tiles' levels allow per launch */

@@ -66,6 +66,19 @@ __device__ __forceinline__ R op_l(const StepArgs<R> &a, int k)
     else return a.op.L[k];
 }
 
+/* Whether the blocked kernel (SPARSE) leaves out the product with Dr (k < 36) or Ds (k >= 36) entry k of the
+ * compiled-in P2 table: exact zeros in both precisions (their products add nothing), and in FP32 also the
+ * entries below 1e-6 in magnitude.  Those are noise of the operator files on structural zeros (6e-10 to 2e-8);
+ * each is smaller than half an FP32 ulp of the O(1) entries beside it (6e-8), so FP32 cannot tell them from
+ * zero.  Of the 72 entries, FP64 skips 8 and FP32 28. */
+template <typename R, bool SPARSE>
+__device__ __forceinline__ constexpr bool p2_skip(int k)
+{
+    if constexpr (!SPARSE) return false;
+    const double v = dgx_p2_ops[k];
+    return v == 0.0 || (sizeof(R) == 4 && v > -1e-6 && v < 1e-6);
+}
+
 /* Workgroups are dealt to the 8 XCDs round-robin; give each XCD one contiguous slice of the
  * (Hilbert-ordered) elements so that face neighbours mostly share that XCD's L2. */
 __device__ __forceinline__ int element_of_thread(int nper)
@@ -190,7 +203,7 @@ __device__ __forceinline__ void condition_neighbour_stresses(const StepArgs<R> &
 
 /* Velocity volume term: rhx = Dr(rx Txx + rz Txz) + Ds(sx Txx + sz Txz), rhz = Dr(rx Txz + rz Tzz) +
  * Ds(sx Txz + sz Tzz). */
-template <typename R, bool P2, typename G = Geom>
+template <typename R, bool P2, typename G = Geom, bool SPARSE = false>
 __device__ __forceinline__ void velocity_volume(const StepArgs<R> &a, const G &g, const R (&txx)[6],
                                                 const R (&tzz)[6], const R (&txz)[6], R (&rhx)[6], R (&rhz)[6])
 {
@@ -203,10 +216,14 @@ __device__ __forceinline__ void velocity_volume(const StepArgs<R> &a, const G &g
         const R uz = rx * txz[n] + rz * tzz[n], wz = sx * txz[n] + sz * tzz[n];
 #pragma unroll
         for (int i = 0; i < 6; i++) { /* one accumulation per product: two fused multiply-adds, no separate add */
-            rhx[i] += op_dr<P2>(a, i * 6 + n) * ux;
-            rhx[i] += op_ds<P2>(a, i * 6 + n) * wx;
-            rhz[i] += op_dr<P2>(a, i * 6 + n) * uz;
-            rhz[i] += op_ds<P2>(a, i * 6 + n) * wz;
+            if (!p2_skip<R, SPARSE && P2>(i * 6 + n)) {
+                rhx[i] += op_dr<P2>(a, i * 6 + n) * ux;
+                rhz[i] += op_dr<P2>(a, i * 6 + n) * uz;
+            }
+            if (!p2_skip<R, SPARSE && P2>(36 + i * 6 + n)) {
+                rhx[i] += op_ds<P2>(a, i * 6 + n) * wx;
+                rhz[i] += op_ds<P2>(a, i * 6 + n) * wz;
+            }
         }
     }
 }
@@ -329,7 +346,7 @@ __device__ __forceinline__ void gather_velocities(unsigned code, const int (&nbs
 
 /* Strain-rate volume terms: exx = Dx Vx, ezz = Dz Vz, exz = Dx Vz + Dz Vx with Dx = rx Dr + sx Ds and
  * Dz = rz Dr + sz Ds (operators from the kernel argument in both variants). */
-template <typename R, typename G = Geom, bool P2 = false>
+template <typename R, typename G = Geom, bool P2 = false, bool SPARSE = false>
 __device__ __forceinline__ void strain_volume(const StepArgs<R> &a, const G &g, const R (&vx)[6],
                                               const R (&vz)[6], R (&exx)[6], R (&ezz)[6], R (&exz)[6])
 {
@@ -339,10 +356,14 @@ __device__ __forceinline__ void strain_volume(const StepArgs<R> &a, const G &g, 
         R rvx = 0, svx = 0, rvz = 0, svz = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            rvx += op_dr<P2>(a, i * 6 + n) * vx[n];
-            svx += op_ds<P2>(a, i * 6 + n) * vx[n];
-            rvz += op_dr<P2>(a, i * 6 + n) * vz[n];
-            svz += op_ds<P2>(a, i * 6 + n) * vz[n];
+            if (!p2_skip<R, SPARSE && P2>(i * 6 + n)) {
+                rvx += op_dr<P2>(a, i * 6 + n) * vx[n];
+                rvz += op_dr<P2>(a, i * 6 + n) * vz[n];
+            }
+            if (!p2_skip<R, SPARSE && P2>(36 + i * 6 + n)) {
+                svx += op_ds<P2>(a, i * 6 + n) * vx[n];
+                svz += op_ds<P2>(a, i * 6 + n) * vz[n];
+            }
         }
         exx[i] = rx * rvx + sx * svx;
         ezz[i] = rz * rvz + sz * svz;
@@ -859,6 +880,102 @@ __device__ __forceinline__ void record_block(const StepArgs<R> &a, unsigned ax, 
     }
 }
 
+/* Velocity update of one element of a blocked tile (LU_Velocity, OpenMP_Function.c:75-185): the volume term and
+ * the three face terms accumulate into one value per node (the face scalings carry 1/D), then V += c * rhs with
+ * c = dt/rho.  The neighbour across face j is lane nsl[j] of the LDS stage sts; on the outer boundary that is
+ * the workgroup's zero lane, so the jump is -U_own as in fun_DG.c:724-736. */
+template <typename R, int NT, bool P2, bool COMP>
+__device__ __forceinline__ void block_velocity(const StepArgs<R> &a, const GeomR<R> &g, unsigned code,
+                                               const int (&nsl)[3], const R *sts, R c, const R (&txx)[6],
+                                               const R (&tzz)[6], const R (&txz)[6], R (&vx)[6], R (&vz)[6],
+                                               R (&lo)[5][6])
+{
+    R rhx[6], rhz[6];
+    velocity_volume<R, P2, GeomR<R>, true>(a, g, txx, tzz, txz, rhx, rhz);
+#pragma unroll
+    for (int fc = 0; fc < 3; fc++) {
+        const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
+        const R ex = g.exd[fc], ez = g.ezd[fc];
+        const R *nb = sts + nsl[fc];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(fc, k), nn = nb_node(m, rev, k);
+            const R dxx = nb[(0 * 6 + nn) * NT] - txx[on];
+            const R dzz = nb[(1 * 6 + nn) * NT] - tzz[on];
+            const R dxz = nb[(2 * 6 + nn) * NT] - txz[on];
+            const R gx = ez * dxx - ex * dxz;
+            const R gz = ez * dxz - ex * dzz;
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                rhx[i] += op_l<P2>(a, fc * 18 + i * 3 + k) * gx;
+                rhz[i] += op_l<P2>(a, fc * 18 + i * 3 + k) * gz;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        if constexpr (COMP) {
+            comp_add(vx[i], lo[0][i], rhx[i] * c);
+            comp_add(vz[i], lo[1][i], rhz[i] * c);
+        } else {
+            vx[i] = rhx[i] * c + vx[i];
+            vz[i] = rhz[i] * c + vz[i];
+        }
+    }
+}
+
+/* Stress update of one element of a blocked tile (LU_Stress_Elastic, OpenMP_Function.c:188-306) in strain-rate
+ * form: the volume strain rates, then each face's lifted velocity jumps S_j dVx, S_j dVz scaled by its edge
+ * vector over D, then T += dt C e with the material constants already multiplied by dt (d2m = (l + 2m) dt,
+ * dl = l dt, dm = m dt).  Neighbour velocities come from the LDS stage svl at lane nsl[j] (the zero lane on the
+ * outer boundary). */
+template <typename R, int NT, bool P2, bool COMP>
+__device__ __forceinline__ void block_stress(const StepArgs<R> &a, const GeomR<R> &g, unsigned code,
+                                             const int (&nsl)[3], const R *svl, R d2m, R dl, R dm,
+                                             const R (&vx)[6], const R (&vz)[6], R (&txx)[6], R (&tzz)[6],
+                                             R (&txz)[6], R (&lo)[5][6])
+{
+    R axx[6], azz[6], axz[6];
+    strain_volume<R, GeomR<R>, P2, true>(a, g, vx, vz, axx, azz, axz);
+#pragma unroll
+    for (int fc = 0; fc < 3; fc++) {
+        const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
+        const R ex = g.exd[fc], ez = g.ezd[fc];
+        const R *nb = svl + nsl[fc];
+        R dvx[3], dvz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(fc, k), nn = nb_node(m, rev, k);
+            dvx[k] = nb[(0 * 6 + nn) * NT] - vx[on];
+            dvz[k] = nb[(1 * 6 + nn) * NT] - vz[on];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            R lx = 0, lz = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                lx += op_l<P2>(a, fc * 18 + i * 3 + k) * dvx[k];
+                lz += op_l<P2>(a, fc * 18 + i * 3 + k) * dvz[k];
+            }
+            axx[i] += ez * lx;
+            azz[i] -= ex * lz;
+            axz[i] += ez * lz - ex * lx;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        if constexpr (COMP) {
+            comp_add(txx[i], lo[2][i], d2m * axx[i] + dl * azz[i]);
+            comp_add(tzz[i], lo[3][i], dl * axx[i] + d2m * azz[i]);
+            comp_add(txz[i], lo[4][i], dm * axz[i]);
+        } else {
+            txx[i] = d2m * axx[i] + (dl * azz[i] + txx[i]);
+            tzz[i] = dl * axx[i] + (d2m * azz[i] + tzz[i]);
+            txz[i] = dm * axz[i] + txz[i];
+        }
+    }
+}
+
 /*
  * Several whole leap-frog steps of one tile (element layout), with the fields held on chip between them.
  * Tile t holds `own` consecutive elements in the locality order (level 0) and every element within 2*bkmax
@@ -886,18 +1003,29 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
     const int ghost = 2 * a.bkmax;
     const long long so = (long long)shot * a.fstride * 6;
     int lev = ghost + 1, q = 0; /* lev > ghost: no element (threads past the tile's lanes) */
-    int4 nb = make_int4(-1, -1, -1, 0);
+    int4 nb = make_int4(-1, -1, -1, 0), lr = make_int4(0, 0, 0, 0);
     if (l < td.w) {
+        /* the level of lane l: the number of level ends at or below it (the ends ascend; uniform loads) */
         const int *le = a.blevels + (long long)t * DGX_BLOCK_LEVELS;
         lev = 0;
-        while (lev < ghost && l >= le[lev]) lev++;
-        q = a.bgid[td.z + l];
+#pragma unroll
+        for (int d = 0; d < DGX_BLOCK_LEVELS; d++) lev += (d < ghost && l >= le[d]) ? 1 : 0;
+        lr = a.blrec[td.z + l];
+        q = lr.x;
         nb = a.blnb[td.z + l];
     }
     const unsigned code = (unsigned)nb.w;
     unsigned ax = 0; /* aux record + 1 of the element's sources and receivers */
     if (lev <= ghost) ax = SERVE ? ((unsigned)a.aux_of_elem[q] & DGX_DYN_AUX_MASK) : (code >> DGX_CODE_AUX_SHIFT);
-    const int nbs[3] = {nb.x, nb.y, nb.z};
+    /* Lane NT - 1 is never an element (the tile builder leaves it free): its LDS slots hold zeros for the whole
+     * launch and stand in for the neighbour across an outer-boundary face. */
+    const int nsl[3] = {nb.x >= 0 ? nb.x : NT - 1, nb.y >= 0 ? nb.y : NT - 1, nb.z >= 0 ? nb.z : NT - 1};
+    if (l == NT - 1) {
+#pragma unroll
+        for (int r = 0; r < 18; r++) sts[r * NT + l] = 0;
+#pragma unroll
+        for (int r = 0; r < 12; r++) svl[r * NT + l] = 0;
+    }
     const bool upd = lev < ghost; /* element updated by this tile (ghosts only supply stresses) */
 
     /* the stresses live in registers from their update to the next step's conditioning and in LDS (sts) in
@@ -905,7 +1033,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
     R vx[6], vz[6], txx[6], tzz[6], txz[6], ab[6];
     R lo[5][6]; /* compensated FP32: the low words of Vx, Vz, Txx, Tzz, Txz (unused otherwise) */
     GeomR<R> g;
-    R rho_inv = 0, lam = 0, mu = 0;
+    R c = 0, d2m = 0, dl = 0, dm = 0; /* dt/rho, (l + 2m) dt, l dt, m dt */
     if (lev <= ghost) {
 #pragma unroll
         for (int i = 0; i < 6; i++) {
@@ -922,10 +1050,12 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             vx[i] = f[0][so + o]; vz[i] = f[1][so + o];
             if constexpr (COMP) { lo[0][i] = a.fl[0][so + o]; lo[1][i] = a.fl[1][so + o]; }
         }
-        g = geometry_r<R>(geometry(a.pt, a.vtx[q]));
-        rho_inv = a.rho_inv[q];
-        lam = a.lam[q];
-        mu = a.mu[q];
+        g = geometry_r<R>(geometry(a.pt, make_int4(lr.y, lr.z, lr.w, 0)));
+        const R dt = (R)a.dt, lam = a.lam[q], mu = a.mu[q];
+        c = a.rho_inv[q] * dt;
+        d2m = (lam + 2 * mu) * dt;
+        dl = lam * dt;
+        dm = mu * dt;
     }
     for (int j = 0; j < nst; j++) {
         const int lstep = lstep0 + j;
@@ -940,41 +1070,8 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             }
         }
         __syncthreads();
-        if (lev <= 2 * nst - 1 - 2 * j) {
-            R rhx[6], rhz[6], flx[6], flz[6];
-            velocity_volume<R, P2>(a, g, txx, tzz, txz, rhx, rhz);
-#pragma unroll
-            for (int i = 0; i < 6; i++) { flx[i] = 0; flz[i] = 0; }
-#pragma unroll
-            for (int fc = 0; fc < 3; fc++) {
-                const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
-                const int s = nbs[fc];
-                R oxx[3], ozz[3], oxz[3], nxx[3], nzz[3], nxz[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const int on = face_node(fc, k), nn = nb_node(m, rev, k);
-                    oxx[k] = txx[on]; ozz[k] = tzz[on]; oxz[k] = txz[on];
-                    nxx[k] = 0; nzz[k] = 0; nxz[k] = 0;
-                    if (s >= 0) {
-                        nxx[k] = sts[(0 * 6 + nn) * NT + s];
-                        nzz[k] = sts[(1 * 6 + nn) * NT + s];
-                        nxz[k] = sts[(2 * 6 + nn) * NT + s];
-                    }
-                }
-                velocity_flux_face<R, P2>(a, g, fc, s >= 0, oxx, ozz, oxz, nxx, nzz, nxz, flx, flz);
-            }
-            const R c = rho_inv * (R)a.dt;
-#pragma unroll
-            for (int i = 0; i < 6; i++) {
-                if constexpr (COMP) {
-                    comp_add(vx[i], lo[0][i], (rhx[i] + flx[i] * g.invD) * c);
-                    comp_add(vz[i], lo[1][i], (rhz[i] + flz[i] * g.invD) * c);
-                } else {
-                    vx[i] = (rhx[i] + flx[i] * g.invD) * c + vx[i];
-                    vz[i] = (rhz[i] + flz[i] * g.invD) * c + vz[i];
-                }
-            }
-        }
+        if (lev <= 2 * nst - 1 - 2 * j)
+            block_velocity<R, NT, P2, COMP>(a, g, code, nsl, sts, c, txx, tzz, txz, vx, vz, lo);
         if (upd) {
 #pragma unroll
             for (int i = 0; i < 6; i++) {
@@ -984,40 +1081,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
         }
         __syncthreads();
         if (lev <= 2 * nst - 2 - 2 * j) {
-            R exx[6], ezz[6], exz[6], fx[6], fz[6], fxz[6];
-            strain_volume<R, GeomR<R>, P2>(a, g, vx, vz, exx, ezz, exz);
-#pragma unroll
-            for (int i = 0; i < 6; i++) { fx[i] = 0; fz[i] = 0; fxz[i] = 0; }
-#pragma unroll
-            for (int fc = 0; fc < 3; fc++) {
-                const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
-                const int s = nbs[fc];
-                R gvx[3], gvz[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const int nn = nb_node(m, rev, k);
-                    gvx[k] = 0; gvz[k] = 0;
-                    if (s >= 0) {
-                        gvx[k] = svl[(0 * 6 + nn) * NT + s];
-                        gvz[k] = svl[(1 * 6 + nn) * NT + s];
-                    }
-                }
-                stress_flux_face<R, P2>(a, g, fc, s >= 0, vx, vz, gvx, gvz, fx, fz, fxz);
-            }
-            const R l2m = lam + 2 * mu, dt = (R)a.dt;
-#pragma unroll
-            for (int i = 0; i < 6; i++) {
-                const R axx = exx[i] + fx[i] * g.invD, azz = ezz[i] + fz[i] * g.invD, axz = exz[i] + fxz[i] * g.invD;
-                if constexpr (COMP) {
-                    comp_add(txx[i], lo[2][i], (l2m * axx + lam * azz) * dt);
-                    comp_add(tzz[i], lo[3][i], (lam * axx + l2m * azz) * dt);
-                    comp_add(txz[i], lo[4][i], (mu * axz) * dt);
-                } else {
-                    txx[i] = (l2m * axx + lam * azz) * dt + txx[i];
-                    tzz[i] = (lam * axx + l2m * azz) * dt + tzz[i];
-                    txz[i] = (mu * axz) * dt + txz[i];
-                }
-            }
+            block_stress<R, NT, P2, COMP>(a, g, code, nsl, svl, d2m, dl, dm, vx, vz, txx, tzz, txz, lo);
             if (lev == 0) record_block<R, SERVE, COMP>(a, ax, shot, lstep, vx, vz, lo);
         }
     }
@@ -1029,7 +1093,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             gout[2][so + o] = txx[i]; gout[3][so + o] = tzz[i]; gout[4][so + o] = txz[i];
             if constexpr (COMP) {
 #pragma unroll
-                for (int c = 0; c < 5; c++) a.gl[c][so + o] = lo[c][i];
+                for (int k = 0; k < 5; k++) a.gl[k][so + o] = lo[k][i];
             }
         }
     }
@@ -1076,7 +1140,7 @@ __global__ void __launch_bounds__(NT, MINW) ks_block(const StepArgs<R> *__restri
     StepArgs<R> a = table[lo];
     a.tiles = as_global(a.tiles);
     a.blevels = as_global(a.blevels);
-    a.bgid = as_global(a.bgid);
+    a.blrec = as_global(a.blrec);
     a.blnb = as_global(a.blnb);
     a.vtx = as_global(a.vtx);
     a.pt = as_global(a.pt);
@@ -1226,6 +1290,16 @@ __global__ void __launch_bounds__(256) k_unpermute(const R *__restrict__ f, cons
     }
 }
 
+__global__ void __launch_bounds__(256) k_lane_records(const int *__restrict__ gid, const int4 *__restrict__ vtx,
+                                                     int4 *__restrict__ rec, long long n)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int q = gid[k];
+    const int4 v = vtx[q];
+    rec[k] = make_int4(q, v.x, v.y, v.z);
+}
+
 __global__ void k_advance(int *step_base, int n)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *step_base += n;
@@ -1250,15 +1324,18 @@ static void launch_step_kernels(const StepArgs<R> &a, int nshots, hipStream_t st
     case 6:
         /* one element per thread, a.bthreads threads (the tiles are built for that count): FP32 1024 (one
          * workgroup per CU) or 512 (two), FP64 256 (two) or 512 (one) */
+        {
+        const dim3 bg(DGX_NXCD * a.tper, nshots);
         if constexpr (sizeof(R) == 4) {
             /* compensated FP32: twice the registers for the low words, one 512-thread workgroup per CU */
-            if (a.fl[0]) dgx::k_block<R, 512, 2, P2, true><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
-            else if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 1024, 0, st>>>(a);
-            else dgx::k_block<R, 512, 4, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
+            if (a.fl[0]) dgx::k_block<R, 512, 2, P2, true><<<bg, 512, 0, st>>>(a);
+            else if (a.bthreads == 1024) dgx::k_block<R, 1024, 1, P2><<<bg, 1024, 0, st>>>(a);
+            else dgx::k_block<R, 512, 4, P2><<<bg, 512, 0, st>>>(a);
         } else if (a.bthreads == 256) {
-            dgx::k_block<R, 256, 2, P2><<<dim3(DGX_NXCD * a.tper, nshots), 256, 0, st>>>(a);
+            dgx::k_block<R, 256, 2, P2><<<bg, 256, 0, st>>>(a);
         } else {
-            dgx::k_block<R, 512, 1, P2><<<dim3(DGX_NXCD * a.tper, nshots), 512, 0, st>>>(a);
+            dgx::k_block<R, 512, 1, P2><<<bg, 512, 0, st>>>(a);
+        }
         }
         break;
     }
@@ -1329,6 +1406,30 @@ hipError_t dgx_noalloc_scratch(double **ptr)
     }
     *ptr = scratch_dev[dev];
     return hipSuccess;
+}
+
+/* Lane records of the blocked kernel, {element, v0, v1, v2}, from the lanes' element ids (host) and the
+ * device vertex records: the kernel then reads an element's vertices without first reading its element id.
+ * *out is allocated here (hipMalloc) and owned by the caller. */
+hipError_t dgx_upload_lane_records(const int *gid, long n, const int4 *vtx, int4 **out)
+{
+    *out = nullptr;
+    if (n <= 0) return hipSuccess;
+    int *d_gid = nullptr;
+    hipError_t e = hipMalloc((void **)&d_gid, sizeof(int) * n);
+    if (e == hipSuccess) e = hipMemcpy(d_gid, gid, sizeof(int) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void **)out, sizeof(int4) * n);
+    if (e == hipSuccess) {
+        dgx::k_lane_records<<<(unsigned)((n + 255) / 256), 256>>>(d_gid, vtx, *out, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(d_gid);
+    if (e != hipSuccess && *out) {
+        (void)hipFree(*out);
+        *out = nullptr;
+    }
+    return e;
 }
 
 hipError_t dgx_launch_unpermute(const void *f, const void *lo, int fp32, const int *perm, double *out, long tri,

@@ -71,7 +71,8 @@ struct MeshDev {
     long tri_pad = 0;      /* single-shot slots: element rows of a field (a multiple of 64) */
     /* single-shot slots: the blocked kernel's tiling of the mesh */
     int4 *btiles = nullptr, *blnb = nullptr;
-    int *blevels = nullptr, *bgid = nullptr;
+    int *blevels = nullptr;
+    int4 *blrec = nullptr;
     long ntile = 0;
     std::vector<long> inv; /* caller element -> device element */
     std::vector<int4> nbr_host; /* device-order neighbours (finding the faces next to a source) */
@@ -341,7 +342,7 @@ extern "C" int dgx_server_destroy(dgx_server *sv)
     if (sv->stream) (void)hipStreamSynchronize(sv->stream);
     if (sv->graph) (void)hipGraphExecDestroy(sv->graph);
     for (auto &m : sv->meshes) {
-        void *p[] = {m.nbr, m.vtx, m.pt, m.absorb, m.btiles, m.blnb, m.blevels, m.bgid};
+        void *p[] = {m.nbr, m.vtx, m.pt, m.absorb, m.btiles, m.blnb, m.blevels, m.blrec};
         for (void *x : p)
             if (x) (void)hipFree(x);
         for (auto &md : m.models)
@@ -446,7 +447,7 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     md.refs = 1; /* never released */
     /* a failure frees what this call allocated */
     auto fail = [&](hipError_t e) {
-        void *q[] = {m.nbr, m.vtx, m.pt, m.absorb, md.d_mat, m.btiles, m.blnb, m.blevels, m.bgid, mh.d_mat, mh.d_absorb};
+        void *q[] = {m.nbr, m.vtx, m.pt, m.absorb, md.d_mat, m.btiles, m.blnb, m.blevels, m.blrec, mh.d_mat, mh.d_absorb};
         for (void *x : q)
             if (x) (void)hipFree(x);
         dgx_set_error("dgx_server_add_mesh: %s", hipGetErrorString(e));
@@ -472,7 +473,7 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
         e = upload_vec(&m.btiles, bt);
         if (e == hipSuccess) e = upload_vec(&m.blnb, bl);
         if (e == hipSuccess) e = upload_vec(&m.blevels, blv);
-        if (e == hipSuccess) e = upload_vec(&m.bgid, bg);
+        if (e == hipSuccess) e = dgx_upload_lane_records(bg.data(), (long)bg.size(), m.vtx, &m.blrec);
     }
     if (e != hipSuccess) return fail(e);
     m.models.push_back(std::move(md));
@@ -751,7 +752,7 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
         a.fstride = 0;
         a.tiles = m.btiles;
         a.blevels = m.blevels;
-        a.bgid = m.bgid;
+        a.blrec = m.blrec;
         a.blnb = m.blnb;
         a.ntile = (int)m.ntile;
         a.bkmax = kServeBlockSteps;

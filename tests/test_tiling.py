@@ -127,14 +127,14 @@ def block_tiling(mesh, K, cap):
 
 def check_block_tiling(mesh, K, cap):
     """The blocked kernel's tiles: own runs partition the elements in order; each tile's lanes fit `cap`
-    threads; level d holds exactly the elements d face hops from the run (breadth-first distance, d <= 2K,
+    threads with the last one free (the kernel's zero lane); level d holds exactly the elements d face hops from the run (breadth-first distance, d <= 2K,
     level 2K the ghosts); every face neighbour of a lane below level 2K is a lane of the tile at level d-1, d or
     d+1, named by its lane index; the code word is the element's.  Returns lanes/own - 1 (recomputed work)."""
     tiles, levels, gid, lnb, nbr = block_tiling(mesh, K, cap)
     T = mesh.tri
     assert tiles[0, 0] == 0 and np.all(tiles[1:, 0] == tiles[:-1, 0] + tiles[:-1, 1])
     assert tiles[-1, 0] + tiles[-1, 1] == T
-    assert np.all(tiles[:, 3] <= cap) and np.all(tiles[:, 1] >= 1)
+    assert np.all(tiles[:, 3] <= cap - 1) and np.all(tiles[:, 1] >= 1)  # the last thread is the zero lane
     assert np.all(tiles[1:, 2] == tiles[:-1, 2] + tiles[:-1, 3]) and tiles[-1, 2] + tiles[-1, 3] == len(gid)
     for t in range(len(tiles)):
         first, own, base, n = tiles[t]
