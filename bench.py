@@ -6,12 +6,15 @@ surface receivers.  A step is one full leap-frog step (sponge, source, velocity 
 half-step, receiver pick) over the whole mesh; DOF-updates/s = Tri * 6 nodes * 5 fields * shots *
 steps / seconds, with all inputs resident in HBM when the timed region starts.
 
-Prints one JSON line.  `value` is the plain FP64 single-shot path; the FP32 path and its trace error
-against FP64 are reported beside it, then the batched FP32 path (64 shots on the ~1.2 M-element slope
-model, shot-innermost layout) and a sample of the open-loop serving stream on the eight resident meshes
-(p50/p99 latency and aggregate DOF-updates/s), each with its ratio to the plain FP64 rate.  `roofline` is for the dominant kernel (algorithmic field bytes
-per launch over its HIP-event-measured duration); `cpu_baseline` is the reference's own step loop
-(oracle/_ref, compiled from the reference sources) or, without it, the oracle, timed on the host.
+Prints one JSON line.  `value` is the plain FP64 single-shot path (the fused velocity and stress half-step
+kernels, whose HBM roofline `roofline` reports); beside it the FP64 default (the blocked kernel, several steps
+per launch), the FP32 paths and their trace errors against FP64, the batched FP32 path (64 shots on the
+~1.2 M-element slope model: the blocked kernel per shot, and the shot-innermost half-step kernels) and a
+sample of the open-loop serving stream on the eight resident meshes (p50/p99 latency and aggregate
+DOF-updates/s), each with its ratio to the plain FP64 rate.  `roofline` is for the dominant kernel
+(algorithmic field bytes per launch over its HIP-event-measured duration); `cpu_baseline` is the reference's
+own step loop (oracle/_ref, compiled from the reference sources) or, without it, the oracle, timed on the
+host.
 """
 from __future__ import annotations
 
@@ -186,12 +189,16 @@ def large_leg(a):
     st = dgx.build_setup(mesh, model, [dgx.SourceSpec(W / 2, H - 300.0, L.DGX_SRC_EXPLOSIVE, 0, 20.0)],
                          np.stack([np.linspace(100.0, W - 100.0, 101), np.full(101, H - 5.0)], 1), n + 200, sponge=sp)
     sec, kt, _, _, _ = run_path(st, L.DGX_FP64, 64, n, step_kernel=L.DGX_STEP_HALF)
+    secd, ktd, _, _, _ = run_path(st, L.DGX_FP64, 64, n, step_kernel=L.DGX_STEP_BLOCKED)  # the FP64 default
     T = mesh.tri
     rate = T * 30 * n / sec
     kv, ks = kt["velocity"], kt["stress"]
     return {"elements": T, "fields_MB": T * 30 * 8 / 1e6, "steps": n, "ms_per_step": sec * 1e3 / n, "value": rate,
-            "step_frac": rate * 24 / HBM_PEAK, "velocity_ms": kv, "stress_ms": ks,
-            "velocity_frac": T * 6 * 8 * 7 / (kv * 1e-3) / HBM_PEAK, "stress_frac": T * 6 * 8 * 8 / (ks * 1e-3) / HBM_PEAK}
+            "path": "half-step pair", "step_frac": rate * 24 / HBM_PEAK, "velocity_ms": kv, "stress_ms": ks,
+            "velocity_frac": T * 6 * 8 * 7 / (kv * 1e-3) / HBM_PEAK, "stress_frac": T * 6 * 8 * 8 / (ks * 1e-3) / HBM_PEAK,
+            "default": {"path": "blocked kernel (the FP64 default), 256-thread tiles, 2 steps per launch",
+                        "ms_per_step": secd * 1e3 / n, "value": T * 30 * n / secd, "launch_ms": ktd["block"],
+                        "tiling": ktd["info"], "x_vs_half_step": sec / secd}}
 
 
 def small_leg(a):
@@ -399,6 +406,8 @@ def main():
     progress(f"blocked FP32: {secb1 * 1e3 / a.steps:.4f} ms/step")
     secf, ktf, xf, zf, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps, step_kernel=L.DGX_STEP_FULL)
     progress(f"full-step FP64: {secf * 1e3 / a.steps:.4f} ms/step")
+    secd, ktd, _, _, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps, step_kernel=L.DGX_STEP_BLOCKED)  # the FP64 default
+    progress(f"default (blocked) FP64: {secd * 1e3 / a.steps:.4f} ms/step")
     rate64 = dof * a.steps / sec64
     rate32 = dof * a.steps / sec32
     rate64f = dof * a.steps / secf
@@ -454,6 +463,9 @@ def main():
                          "x_vs_plain_fp64": dof * a.steps / secb1 / rate64},
         "full_step_fp64": {"value": rate64f, "ms_per_step": secf * 1e3 / a.steps, "kernel_ms": ktf["full"],
                            "note": "one kernel per step over tiles + recomputed ring (opt-in DGX_STEP_FULL)"},
+        "default_fp64": {"value": dof * a.steps / secd, "ms_per_step": secd * 1e3 / a.steps, "launch_ms": ktd["block"],
+                         "tiling": ktd["info"], "x_vs_half_step": sec64 / secd,
+                         "path": "the FP64 default: blocked kernel, 256-thread tiles, 2 steps per launch"},
     }
     if "large" not in a.skip:
         out["plain_fp64_large"] = large_leg(a)

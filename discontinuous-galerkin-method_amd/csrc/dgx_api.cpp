@@ -706,23 +706,17 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         free_solver(s);
         return DGX_EARG;
     }
-    /* automatic choices: FP32 runs the blocked kernel in the element layout, one shot per workgroup row, at
-     * any shot count (measured fastest: 1.5x the half-step pair on one shot, 1.26x the shot-innermost kernels
-     * on 64).  FP64 runs it (256-thread tiles) while elements x shots <= 550,000, about where its two field
-     * copies (2 x 240 B per element and shot) outgrow the 256 MB Infinity Cache (measured: 2.7x the half-step
-     * pair at 20k elements, 1.05x at 500k, 0.95x at 640k), else the half-step pair, shot-innermost from 8
-     * shots.  A shot tile or workgroup shape given with the automatic layout asks for the shot-innermost
-     * kernels */
-    const bool fp32_plain = p->precision == DGX_FP32;
+    /* automatic choices: the blocked kernel in the element layout, one shot per workgroup row, at any shot count
+     * and in both precisions (measured fastest: FP32 on one shot 2.2x the half-step pair, on 64 shots 1.28x the
+     * shot-innermost kernels; FP64 on 20k elements 2.9x the pair, on 640k 1.0x, on 1.2M 1.43x, on 2M 1.38x, on
+     * 8 and 64 shots of 1.2M elements 1.35x and 1.17x the shot-innermost kernels).  A shot tile or workgroup
+     * shape given with the automatic layout asks for the shot-innermost kernels, from 8 shots */
     const bool wants_inner = p->shot_tile != 0 || p->wg_waves != 0;
     if (p->layout == DGX_LAYOUT_AUTO)
-        s->layout = (fp32_plain && !wants_inner) ? DGX_LAYOUT_ELEMENT
-                                                 : (p->nshots >= 8 ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT);
+        s->layout = (wants_inner && p->nshots >= 8) ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT;
     else
         s->layout = p->layout;
-    const bool fp64_cached = p->precision == DGX_FP64 && T * p->nshots <= 550000;
-    const bool auto_blocked =
-        p->step_kernel == DGX_STEP_AUTO && (fp32_plain || fp64_cached) && s->layout == DGX_LAYOUT_ELEMENT;
+    const bool auto_blocked = p->step_kernel == DGX_STEP_AUTO && s->layout == DGX_LAYOUT_ELEMENT;
     if (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF && p->step_kernel != DGX_STEP_FULL &&
         p->step_kernel != DGX_STEP_BLOCKED) {
         dgx_set_error("dgx_create: unknown step_kernel %d", p->step_kernel);

@@ -35,8 +35,8 @@ enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1, DGX_FP32_COMPENSATED = 2 };
 enum dgx_source_kind { DGX_SRC_FORCE_Z = 0, DGX_SRC_EXPLOSIVE = 1 };
 /* Field layout on the device.  ELEMENT: 64-element blocks, one thread per element and shot (the
  * single-shot path).  SHOT_INNER: shots innermost, lanes over shots, for many shots on one mesh.
- * AUTO: ELEMENT for DGX_FP32 (with the blocked kernel, faster at every shot count measured) unless a shot tile
- * or workgroup shape is given; otherwise SHOT_INNER from 8 shots on. */
+ * AUTO: ELEMENT (with the blocked kernel, faster at every shot count measured, in both precisions) unless a
+ * shot tile or workgroup shape is given for 8 or more shots, which selects SHOT_INNER. */
 enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_INNER = 2 };
 /* How dgx_step runs a step.  HALF: two kernels per step (velocity, then stress), each updating the
  * fields in place.  FULL (element layout only): one kernel per step that reads the fields and writes the
@@ -47,9 +47,10 @@ enum dgx_layout { DGX_LAYOUT_AUTO = 0, DGX_LAYOUT_ELEMENT = 1, DGX_LAYOUT_SHOT_I
  * per launch; each workgroup holds a tile of elements and the rings of elements within 2*block_steps - 1
  * face hops around it in registers and LDS, recomputes the rings instead of exchanging them, and reads and
  * writes each field once per launch; it keeps a second copy of the fields.  AUTO: BLOCKED in the element
- * layout for DGX_FP32 and DGX_FP32_COMPENSATED, and for DGX_FP64 while elements x shots <= 550,000 (about
- * where its two field copies outgrow the Infinity Cache); HALF otherwise, and HALF if no tile fits an element
- * and its rings (which only extreme vertex valences cause). */
+ * layout, in every precision and at any shot count (measured faster than HALF and than the shot-innermost
+ * kernels on every mesh and shot count tried); HALF in the shot-innermost layout, and HALF if no tile fits an
+ * element and its rings (which only extreme vertex valences cause).  The automatic layout is the element
+ * layout, unless a shot tile or workgroup shape is given for 8 or more shots. */
 enum dgx_step_kernel { DGX_STEP_AUTO = 0, DGX_STEP_HALF = 1, DGX_STEP_FULL = 2, DGX_STEP_BLOCKED = 3 };
 
 /* Reference-element operators as read from input_files/precomputed_matrix/ (row-major), the arrays
