@@ -905,11 +905,12 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
          * other's arithmetic.  FP32: 512 (128 VGPRs; measured 1.2-1.3x faster than one 1024-thread workgroup per
          * CU despite the larger rings).  FP64: 256 (about 235 VGPRs allow only two waves per SIMD; measured 1.07-
          * 1.16x faster than 512-thread tiles although their rings recompute 1.4-1.6 instead of 0.9 of the own
-         * work).  DGX_BLOCK_THREADS selects 1024 (FP32) or 512 (FP64) instead */
+         * work).  DGX_BLOCK_THREADS selects 1024 (FP32) or 512 (FP64) instead; compensated FP32 runs only on
+         * 512-thread tiles (its kernel is built for that count) */
         int cap = s->fp32 ? 512 : 256;
         if (const char *bt = getenv("DGX_BLOCK_THREADS")) {
             const int v = atoi(bt);
-            if (s->fp32 && v == 1024) cap = 1024;
+            if (s->fp32 && !s->comp && v == 1024) cap = 1024;
             if (!s->fp32 && v == 512) cap = 512;
         }
         s->bthreads = cap;
