@@ -244,28 +244,30 @@ static void build_tiles(long tri, const std::vector<int4> &nbr, long vrow0, std:
 
 /*
  * Tiles of the blocked kernel: runs of consecutive elements (locality order), each grown by bisection on its
- * length to the longest run that fits cap - 1 lanes together with every element within 2K face hops of it
- * (`cap` is the workgroup's thread count; its last lane stays free as the kernel's zero lane).
- * Levels 0 .. 2K-1 are updated by the kernel; level 2K (ghosts) only supplies its stresses to the first
- * velocity update of level 2K-1.  tiles[t] = {first, own, first lane record, lanes};
- * levels[t*DGX_BLOCK_LEVELS + d] = end lane of level d, entries past level 2K equal to the lane count.  Lane
- * records: the run's elements in order, then each level in increasing element id; gid = the element, lnb =
- * {lane of face j's neighbour, code}, -1 on the boundary (ghost lanes: -1 everywhere, never read).  Every
- * face neighbour of a level below 2K is in the tile.  Returns false if one element and its levels do not fit.
+ * length to the longest run whose levels fit the workgroup: levels 0 .. 2K-1 (the run and every element within
+ * 2K-1 face hops of it, the elements the kernel updates) at most `cap` lanes, one per thread (`cap` is the
+ * workgroup's thread count), and level 2K (ghosts, which only supply their stresses to the first velocity update
+ * of level 2K-1) at most cap/4 - 1 elements, held in LDS slots past the threads' (the last slot is the kernel's
+ * zero slot).  tiles[t] = {first, own, first record, records}; levels[t*DGX_BLOCK_LEVELS + d] = end record of
+ * level d, entries past level 2K equal to the record count.  Records: the run's elements in order, then each
+ * level in increasing element id; gid = the element, lnb = {LDS slot of face j's neighbour, code}: the lane for
+ * levels below 2K, cap + (record - lanes) for a ghost, -1 on the boundary (ghost records: -1 everywhere, never
+ * read).  Every face neighbour of a level below 2K is in the tile.  Returns false if one element and its levels
+ * do not fit.
  */
 bool dgx_build_block_tiles(long tri, const std::vector<int4> &nbr, int K, int cap, std::vector<int4> &tiles,
                            std::vector<int> &levels, std::vector<int> &gid, std::vector<int4> &lnb)
 {
-    const int depth = 2 * K; /* the ghost level */
-    cap -= 1;                /* lanes for elements: the last thread of the workgroup is the zero lane */
+    const int depth = 2 * K;       /* the ghost level */
+    const long gcap = cap / 4 - 1; /* ghost slots (the last of the cap/4 slots is the zero slot) */
     std::vector<int> stamp(tri, 0), slot(tri, -1);
     int cur = 0;
     std::vector<int> front, next;
     std::vector<std::vector<int>> lv(depth + 1);
-    /* lanes of the run [q0, q0 + n) and its levels, or cap + 1 once they exceed cap */
-    auto lanes_of = [&](long q0, long n, bool keep) -> long {
+    /* whether the run [q0, q0 + n) and its levels fit; *total = its records (lanes + ghosts) */
+    auto fits = [&](long q0, long n, bool keep, long *total) -> bool {
         cur++;
-        long total = n;
+        long lanes = n;
         front.clear();
         for (long q = q0; q < q0 + n; q++) {
             stamp[q] = cur;
@@ -281,31 +283,37 @@ bool dgx_build_block_tiles(long tri, const std::vector<int4> &nbr, int K, int ca
                         next.push_back(x);
                     }
             }
-            total += (long)next.size();
-            if (total > cap) return cap + 1;
+            if (d < depth) {
+                lanes += (long)next.size();
+                if (lanes > cap) return false;
+            } else if ((long)next.size() > gcap) {
+                return false;
+            }
             if (keep) {
                 lv[d] = next;
                 std::sort(lv[d].begin(), lv[d].end());
             }
             front.swap(next);
         }
-        return total;
+        if (total) *total = lanes + (long)front.size();
+        return true;
     };
     long q0 = 0, prev = 1;
     while (q0 < tri) {
         long lo = 1, hi = std::min<long>(cap, tri - q0);
-        if (lanes_of(q0, 1, false) > cap) return false;
+        if (!fits(q0, 1, false, nullptr)) return false;
         if (prev <= hi) { /* tiles along the order have similar sizes: start from the previous one */
-            if (lanes_of(q0, prev, false) <= cap) lo = prev;
+            if (fits(q0, prev, false, nullptr)) lo = prev;
             else hi = prev - 1;
         }
         while (lo < hi) {
             const long mid = (lo + hi + 1) / 2;
-            if (lanes_of(q0, mid, false) <= cap) lo = mid;
+            if (fits(q0, mid, false, nullptr)) lo = mid;
             else hi = mid - 1;
         }
         const long n = lo;
-        const long total = lanes_of(q0, n, true);
+        long total = 0;
+        fits(q0, n, true, &total);
         const int base = (int)gid.size();
         for (long q = q0; q < q0 + n; q++) gid.push_back((int)q);
         int lev[DGX_BLOCK_LEVELS];
@@ -316,11 +324,12 @@ bool dgx_build_block_tiles(long tri, const std::vector<int4> &nbr, int K, int ca
         }
         for (int d = depth + 1; d < DGX_BLOCK_LEVELS; d++) lev[d] = (int)total;
         levels.insert(levels.end(), lev, lev + DGX_BLOCK_LEVELS);
-        for (int l = 0; l < (int)total; l++) slot[gid[base + l]] = l;
+        const int lanes = lev[depth - 1];
+        for (int l = 0; l < (int)total; l++) slot[gid[base + l]] = l < lanes ? l : cap + (l - lanes);
         for (int l = 0; l < (int)total; l++) {
             const int e = gid[base + l];
             int r[3] = {-1, -1, -1};
-            if (l < lev[depth - 1]) {
+            if (l < lanes) {
                 const int nq[3] = {nbr[e].x, nbr[e].y, nbr[e].z};
                 for (int j = 0; j < 3; j++) r[j] = nq[j] < 0 ? -1 : slot[nq[j]];
             }

@@ -882,9 +882,9 @@ __device__ __forceinline__ void record_block(const StepArgs<R> &a, unsigned ax, 
 
 /* Velocity update of one element of a blocked tile (LU_Velocity, OpenMP_Function.c:75-185): the volume term and
  * the three face terms accumulate into one value per node (the face scalings carry 1/D), then V += c * rhs with
- * c = dt/rho.  The neighbour across face j is lane nsl[j] of the LDS stage sts; on the outer boundary that is
- * the workgroup's zero lane, so the jump is -U_own as in fun_DG.c:724-736. */
-template <typename R, int NT, bool P2, bool COMP>
+ * c = dt/rho.  The neighbour across face j is slot nsl[j] of the LDS stage sts (rows of NS slots); on the outer
+ * boundary that is the workgroup's zero slot, so the jump is -U_own as in fun_DG.c:724-736. */
+template <typename R, int NS, bool P2, bool COMP>
 __device__ __forceinline__ void block_velocity(const StepArgs<R> &a, const GeomR<R> &g, unsigned code,
                                                const int (&nsl)[3], const R *sts, R c, const R (&txx)[6],
                                                const R (&tzz)[6], const R (&txz)[6], R (&vx)[6], R (&vz)[6],
@@ -900,9 +900,9 @@ __device__ __forceinline__ void block_velocity(const StepArgs<R> &a, const GeomR
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int on = face_node(fc, k), nn = nb_node(m, rev, k);
-            const R dxx = nb[(0 * 6 + nn) * NT] - txx[on];
-            const R dzz = nb[(1 * 6 + nn) * NT] - tzz[on];
-            const R dxz = nb[(2 * 6 + nn) * NT] - txz[on];
+            const R dxx = nb[(0 * 6 + nn) * NS] - txx[on];
+            const R dzz = nb[(1 * 6 + nn) * NS] - tzz[on];
+            const R dxz = nb[(2 * 6 + nn) * NS] - txz[on];
             const R gx = ez * dxx - ex * dxz;
             const R gz = ez * dxz - ex * dzz;
 #pragma unroll
@@ -927,9 +927,9 @@ __device__ __forceinline__ void block_velocity(const StepArgs<R> &a, const GeomR
 /* Stress update of one element of a blocked tile (LU_Stress_Elastic, OpenMP_Function.c:188-306) in strain-rate
  * form: the volume strain rates, then each face's lifted velocity jumps S_j dVx, S_j dVz scaled by its edge
  * vector over D, then T += dt C e with the material constants already multiplied by dt (d2m = (l + 2m) dt,
- * dl = l dt, dm = m dt).  Neighbour velocities come from the LDS stage svl at lane nsl[j] (the zero lane on the
+ * dl = l dt, dm = m dt).  Neighbour velocities come from the LDS stage svl at slot nsl[j] (the zero slot on the
  * outer boundary). */
-template <typename R, int NT, bool P2, bool COMP>
+template <typename R, int NS, bool P2, bool COMP>
 __device__ __forceinline__ void block_stress(const StepArgs<R> &a, const GeomR<R> &g, unsigned code,
                                              const int (&nsl)[3], const R *svl, R d2m, R dl, R dm,
                                              const R (&vx)[6], const R (&vz)[6], R (&txx)[6], R (&tzz)[6],
@@ -946,8 +946,8 @@ __device__ __forceinline__ void block_stress(const StepArgs<R> &a, const GeomR<R
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int on = face_node(fc, k), nn = nb_node(m, rev, k);
-            dvx[k] = nb[(0 * 6 + nn) * NT] - vx[on];
-            dvz[k] = nb[(1 * 6 + nn) * NT] - vz[on];
+            dvx[k] = nb[(0 * 6 + nn) * NS] - vx[on];
+            dvz[k] = nb[(1 * 6 + nn) * NS] - vz[on];
         }
 #pragma unroll
         for (int i = 0; i < 6; i++) {
@@ -976,37 +976,50 @@ __device__ __forceinline__ void block_stress(const StepArgs<R> &a, const GeomR<R
     }
 }
 
+/* LDS slots per stage row of the blocked kernel with NT threads: one per thread, then NT/4 for ghost elements,
+ * the last of which is the zero slot (a multiple of 64, so that rows stay aligned for paired LDS stores). */
+template <int NT>
+__device__ __forceinline__ constexpr int block_slots()
+{
+    return NT + NT / 4;
+}
+
 /*
  * Several whole leap-frog steps of one tile (element layout), with the fields held on chip between them.
  * Tile t holds `own` consecutive elements in the locality order (level 0) and every element within 2*bkmax
- * face hops of them (level d = d hops), one element per thread.  Each thread reads its element's fields from f
- * once; nst steps follow (global steps step0.., local steps lstep0.. for the wavelet and traces), step j being:
+ * face hops of them (level d = d hops).  Levels 0 .. 2*bkmax - 1 take one thread each; level 2*bkmax (ghosts)
+ * only supplies stresses, conditioned at the first step, to the first velocity update of the level inside it:
+ * thread g loads ghost g besides its own element and stages it in LDS slot NT + g.  Each thread reads its
+ * element's fields from f once; nst steps follow (global steps step0.., local steps lstep0.. for the wavelet
+ * and traces), step j being:
  *   every element conditions its values (sponge, sources: Absorb_Boundary and source_kernel,
  *   OpenMP_Function.c:25-58) and stages its stresses in LDS; barrier;
  *   elements of level <= 2 nst - 1 - 2j update their velocities (LU_Velocity, :75-185), face neighbours read
  *   from LDS, and stage them; barrier;
  *   elements of level <= 2 nst - 2 - 2j update their stresses (LU_Stress_Elastic, :188-306); level 0 records
  *   its receivers (Record, :61-72);
- * then the own elements' fields go to g.  Level 2*bkmax (ghosts) holds only stresses, conditioned at the
- * first step, for the first velocity update of the level inside it.  The outer levels are recomputed from the
- * same inputs with the same code as their owning tile's, so tiles never wait on each other; f is only read
- * and every row of g is written by one workgroup.  Per launch each field is read once for every thread's
- * element and written once for every own element, against 7 reads and writes per step and element for the
- * half-step kernel pair.  Every thread of the workgroup reaches every barrier.
+ * then the own elements' fields go to g.  The outer levels are recomputed from the same inputs with the same
+ * code as their owning tile's, so tiles never wait on each other; f is only read and every row of g is written
+ * by one workgroup.  Per launch each field is read once for every element of the tile and written once for
+ * every own element, against 7 reads and writes per step and element for the half-step kernel pair.  Every
+ * thread of the workgroup reaches every barrier.
  */
 template <typename R, int NT, bool P2, bool SERVE, bool COMP = false>
 __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], R *const gout[5], int t, int shot,
                                            int nst, int lstep0, R *sts, R *svl)
 {
+    constexpr int NS = block_slots<NT>();
     const int4 td = a.tiles[t];
     const int l = threadIdx.x;
     const int ghost = 2 * a.bkmax;
     const long long so = (long long)shot * a.fstride * 6;
+    /* the tile's level ends (uniform loads): lanes = elements with a thread, the rest of its records are ghosts */
+    const int *le = a.blevels + (long long)t * DGX_BLOCK_LEVELS;
+    const int lanes = le[ghost - 1], nghost = td.w - lanes;
     int lev = ghost + 1, q = 0; /* lev > ghost: no element (threads past the tile's lanes) */
     int4 nb = make_int4(-1, -1, -1, 0), lr = make_int4(0, 0, 0, 0);
-    if (l < td.w) {
-        /* the level of lane l: the number of level ends at or below it (the ends ascend; uniform loads) */
-        const int *le = a.blevels + (long long)t * DGX_BLOCK_LEVELS;
+    if (l < lanes) {
+        /* the level of lane l: the number of level ends at or below it (the ends ascend) */
         lev = 0;
 #pragma unroll
         for (int d = 0; d < DGX_BLOCK_LEVELS; d++) lev += (d < ghost && l >= le[d]) ? 1 : 0;
@@ -1016,17 +1029,39 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
     }
     const unsigned code = (unsigned)nb.w;
     unsigned ax = 0; /* aux record + 1 of the element's sources and receivers */
-    if (lev <= ghost) ax = SERVE ? ((unsigned)a.aux_of_elem[q] & DGX_DYN_AUX_MASK) : (code >> DGX_CODE_AUX_SHIFT);
-    /* Lane NT - 1 is never an element (the tile builder leaves it free): its LDS slots hold zeros for the whole
-     * launch and stand in for the neighbour across an outer-boundary face. */
-    const int nsl[3] = {nb.x >= 0 ? nb.x : NT - 1, nb.y >= 0 ? nb.y : NT - 1, nb.z >= 0 ? nb.z : NT - 1};
+    if (lev < ghost) ax = SERVE ? ((unsigned)a.aux_of_elem[q] & DGX_DYN_AUX_MASK) : (code >> DGX_CODE_AUX_SHIFT);
+    /* Slot NS - 1 is never an element (ghosts fill at most NT/4 - 1 slots): it holds zeros for the whole launch
+     * and stands in for the neighbour across an outer-boundary face. */
+    const int nsl[3] = {nb.x >= 0 ? nb.x : NS - 1, nb.y >= 0 ? nb.y : NS - 1, nb.z >= 0 ? nb.z : NS - 1};
     if (l == NT - 1) {
 #pragma unroll
-        for (int r = 0; r < 18; r++) sts[r * NT + l] = 0;
+        for (int r = 0; r < 18; r++) sts[r * NS + NS - 1] = 0;
 #pragma unroll
-        for (int r = 0; r < 12; r++) svl[r * NT + l] = 0;
+        for (int r = 0; r < 12; r++) svl[r * NS + NS - 1] = 0;
     }
-    const bool upd = lev < ghost; /* element updated by this tile (ghosts only supply stresses) */
+    /* ghost l: its stresses, conditioned for the first step, into slot NT + l */
+    if (l < nghost) {
+        const int4 glr = a.blrec[td.z + lanes + l];
+        const unsigned gcode = (unsigned)a.blnb[td.z + lanes + l].w;
+        const int gq = glr.x;
+        const unsigned gax = SERVE ? ((unsigned)a.aux_of_elem[gq] & DGX_DYN_AUX_MASK) : (gcode >> DGX_CODE_AUX_SHIFT);
+        R gxx[6], gzz[6], gxz[6], gab[6], gv[6], glo[5][6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const int o = fidx(gq, i);
+            gxx[i] = f[2][so + o]; gzz[i] = f[3][so + o]; gxz[i] = f[4][so + o];
+            gab[i] = (gcode & DGX_CODE_SPONGE) ? a.absorb[o] : (R)0; /* used only with the sponge bit */
+            if constexpr (COMP) { glo[2][i] = a.fl[2][so + o]; glo[3][i] = a.fl[3][so + o]; glo[4][i] = a.fl[4][so + o]; }
+        }
+        condition_block<R, false, SERVE, COMP>(a, gcode, gax, shot, lstep0, gab, gv, gv, gxx, gzz, gxz, glo);
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            sts[(0 * 6 + i) * NS + NT + l] = gxx[i];
+            sts[(1 * 6 + i) * NS + NT + l] = gzz[i];
+            sts[(2 * 6 + i) * NS + NT + l] = gxz[i];
+        }
+    }
+    const bool upd = lev < ghost; /* element updated by this tile */
 
     /* the stresses live in registers from their update to the next step's conditioning and in LDS (sts) in
      * between; the velocities stay in registers */
@@ -1034,7 +1069,7 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
     R lo[5][6]; /* compensated FP32: the low words of Vx, Vz, Txx, Tzz, Txz (unused otherwise) */
     GeomR<R> g;
     R c = 0, d2m = 0, dl = 0, dm = 0; /* dt/rho, (l + 2m) dt, l dt, m dt */
-    if (lev <= ghost) {
+    if (upd) {
 #pragma unroll
         for (int i = 0; i < 6; i++) {
             const int o = fidx(q, i);
@@ -1042,8 +1077,6 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
             ab[i] = (code & DGX_CODE_SPONGE) ? a.absorb[o] : (R)0; /* used only with the sponge bit */
             if constexpr (COMP) { lo[2][i] = a.fl[2][so + o]; lo[3][i] = a.fl[3][so + o]; lo[4][i] = a.fl[4][so + o]; }
         }
-    }
-    if (upd) {
 #pragma unroll
         for (int i = 0; i < 6; i++) {
             const int o = fidx(q, i);
@@ -1059,29 +1092,28 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
     }
     for (int j = 0; j < nst; j++) {
         const int lstep = lstep0 + j;
-        if (upd || (lev == ghost && j == 0)) {
-            if (upd) condition_block<R, true, SERVE, COMP>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz, lo);
-            else condition_block<R, false, SERVE, COMP>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz, lo);
+        if (upd) {
+            condition_block<R, true, SERVE, COMP>(a, code, ax, shot, lstep, ab, vx, vz, txx, tzz, txz, lo);
 #pragma unroll
             for (int i = 0; i < 6; i++) {
-                sts[(0 * 6 + i) * NT + l] = txx[i];
-                sts[(1 * 6 + i) * NT + l] = tzz[i];
-                sts[(2 * 6 + i) * NT + l] = txz[i];
+                sts[(0 * 6 + i) * NS + l] = txx[i];
+                sts[(1 * 6 + i) * NS + l] = tzz[i];
+                sts[(2 * 6 + i) * NS + l] = txz[i];
             }
         }
         __syncthreads();
         if (lev <= 2 * nst - 1 - 2 * j)
-            block_velocity<R, NT, P2, COMP>(a, g, code, nsl, sts, c, txx, tzz, txz, vx, vz, lo);
+            block_velocity<R, NS, P2, COMP>(a, g, code, nsl, sts, c, txx, tzz, txz, vx, vz, lo);
         if (upd) {
 #pragma unroll
             for (int i = 0; i < 6; i++) {
-                svl[(0 * 6 + i) * NT + l] = vx[i];
-                svl[(1 * 6 + i) * NT + l] = vz[i];
+                svl[(0 * 6 + i) * NS + l] = vx[i];
+                svl[(1 * 6 + i) * NS + l] = vz[i];
             }
         }
         __syncthreads();
         if (lev <= 2 * nst - 2 - 2 * j) {
-            block_stress<R, NT, P2, COMP>(a, g, code, nsl, svl, d2m, dl, dm, vx, vz, txx, tzz, txz, lo);
+            block_stress<R, NS, P2, COMP>(a, g, code, nsl, svl, d2m, dl, dm, vx, vz, txx, tzz, txz, lo);
             if (lev == 0) record_block<R, SERVE, COMP>(a, ax, shot, lstep, vx, vz, lo);
         }
     }
@@ -1105,8 +1137,8 @@ __device__ __forceinline__ void block_tile(const StepArgs<R> &a, R *const f[5], 
 template <typename R, int NT, int MINW, bool P2, bool COMP = false>
 __global__ void __launch_bounds__(NT, MINW) k_block(StepArgs<R> a)
 {
-    __shared__ R sts[18 * NT]; /* conditioned stresses, (c*6 + i)*NT + lane */
-    __shared__ R svl[12 * NT]; /* new velocities */
+    __shared__ R sts[18 * block_slots<NT>()]; /* conditioned stresses, (c*6 + i)*slots + slot */
+    __shared__ R svl[12 * block_slots<NT>()]; /* new velocities */
     const int b = blockIdx.x;
     const int t = (b & (DGX_NXCD - 1)) * a.tper + (b >> 3);
     if (t >= a.ntile) return; /* uniform over the workgroup */
@@ -1126,8 +1158,8 @@ __global__ void __launch_bounds__(NT, MINW) ks_block(const StepArgs<R> *__restri
                                                     const int *__restrict__ sched, const int *step_base,
                                                     int step_off, int launch, int nb, int total, int nper)
 {
-    __shared__ R sts[18 * NT];
-    __shared__ R svl[12 * NT];
+    __shared__ R sts[18 * block_slots<NT>()];
+    __shared__ R svl[12 * block_slots<NT>()];
     const int tile = (int)(blockIdx.x & (DGX_NXCD - 1)) * nper + (int)(blockIdx.x >> 3);
     if (tile >= total) return;
     int lo = 0, hi = nb - 1;

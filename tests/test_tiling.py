@@ -126,15 +126,18 @@ def block_tiling(mesh, K, cap):
 
 
 def check_block_tiling(mesh, K, cap):
-    """The blocked kernel's tiles: own runs partition the elements in order; each tile's lanes fit `cap`
-    threads with the last one free (the kernel's zero lane); level d holds exactly the elements d face hops from the run (breadth-first distance, d <= 2K,
-    level 2K the ghosts); every face neighbour of a lane below level 2K is a lane of the tile at level d-1, d or
-    d+1, named by its lane index; the code word is the element's.  Returns lanes/own - 1 (recomputed work)."""
+    """The blocked kernel's tiles: own runs partition the elements in order; level d holds exactly the elements
+    d face hops from the run (breadth-first distance, d <= 2K, level 2K the ghosts); levels below 2K fit `cap`
+    threads, one each, and the ghosts fit the cap/4 - 1 LDS slots past them (the last of the cap/4 is the
+    kernel's zero slot); every face neighbour of an element below level 2K is an element of the tile at level
+    d-1, d or d+1, named by its LDS slot (its record index below 2K, cap + its index among the ghosts at 2K);
+    the code word is the element's.  Returns records/own - 1 (recomputed work)."""
     tiles, levels, gid, lnb, nbr = block_tiling(mesh, K, cap)
     T = mesh.tri
     assert tiles[0, 0] == 0 and np.all(tiles[1:, 0] == tiles[:-1, 0] + tiles[:-1, 1])
     assert tiles[-1, 0] + tiles[-1, 1] == T
-    assert np.all(tiles[:, 3] <= cap - 1) and np.all(tiles[:, 1] >= 1)  # the last thread is the zero lane
+    lanes, ghosts = levels[:, 2 * K - 1], tiles[:, 3] - levels[:, 2 * K - 1]
+    assert np.all(lanes <= cap) and np.all(ghosts <= cap // 4 - 1) and np.all(tiles[:, 1] >= 1)
     assert np.all(tiles[1:, 2] == tiles[:-1, 2] + tiles[:-1, 3]) and tiles[-1, 2] + tiles[-1, 3] == len(gid)
     for t in range(len(tiles)):
         first, own, base, n = tiles[t]
@@ -162,12 +165,14 @@ def check_block_tiling(mesh, K, cap):
             assert lnb[base + l, 3] == nbr[e, 3]
             if lev[l] == 2 * K:
                 continue
+            nl = le[2 * K - 1]
             for j in range(3):
                 x, s = nbr[e, j], lnb[base + l, j]
                 if x < 0:
                     assert s == -1
                 else:
-                    assert 0 <= s < n and g[s] == x and abs(lev[s] - lev[l]) <= 1
+                    r = s if s < nl else nl + (s - cap)  # the neighbour's record from its LDS slot
+                    assert (0 <= s < nl or cap <= s < cap + n - nl) and g[r] == x and abs(lev[r] - lev[l]) <= 1
     return (len(gid) - T) / T
 
 
