@@ -53,7 +53,8 @@ def parse():
                     help="CPU time of the reference step-loop sample (steps sized from a short timed run)")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batched FP32 leg (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=320)
-    ap.add_argument("--block-steps", type=int, default=2, help="steps per launch of the blocked kernel (1-3)")
+    ap.add_argument("--block-steps", type=int, default=0,
+                    help="steps per launch of the blocked kernel in the batched leg (1-3; 0 = the library's default)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the seeded serving stream to run (0 = skip; 2000 = the whole stream)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
