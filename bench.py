@@ -302,7 +302,8 @@ def serving_leg(a, rate64, group, rate64d):
     ms = dgx.configs.serving_meshes()
     arrivals, specs = dgx.configs.serving_requests(ms, n=max(a.serve_requests, 1), rate=a.serve_rate)
     arrivals, specs = arrivals[:a.serve_requests], specs[:a.serve_requests]
-    sv = SV.Server(L.DGX_FP32, int(a.serve_budget_gb * 2**30), 100, group_shots=group)
+    # blocks of 96 steps: a multiple of 6, so that single-shot slots run three steps per launch
+    sv = SV.Server(L.DGX_FP32, int(a.serve_budget_gb * 2**30), 96, group_shots=group)
     rms = [sv.add_mesh(m, md, sp) for m, md, sp in ms]
     reqs = [sv.make_request(rms[r["mesh"]], r["xz"], [r["f0"]] * len(r["xz"]), r["nsteps"], r["rec"])
             for r in specs]
@@ -311,7 +312,7 @@ def serving_leg(a, rate64, group, rate64d):
     dof = st["dof_updates"]
     return {"requests": len(reqs), "arrival_rate_per_s": a.serve_rate, "meshes": [m.tri for m, _, _ in ms],
             "group_shots": group, "hbm_budget_gb": a.serve_budget_gb or "80% of free",
-            "path": ("single-shot slots, blocked kernel (2 steps per launch)" if group == 1 else
+            "path": ("single-shot slots, blocked kernel (3 steps per launch, blocks of 96 steps)" if group == 1 else
                      f"{group}-shot slot groups, shot-innermost half-step kernels"),
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "wall_s": wall, "dof_updates": dof,

@@ -49,7 +49,6 @@ namespace {
 
 const int MAXG = 64;      /* largest slot group (shots per batch = shot tile of the serving kernel) */
 const int MAX_BATCHES = 4096;
-const int kServeBlockSteps = 2;   /* single-shot slots: steps per launch of the blocked kernel */
 
 
 /* A material model on a resident mesh: 1/rho | lambda | mu per element in device order, and the time step
@@ -161,6 +160,11 @@ struct dgx_server {
     unsigned long long *d_runmask = nullptr; /* [step of block][stepped batch]: slots running at that step */
     long long budget = 0, in_use = 0;
     int K = 100;
+    /* single-shot slots: steps per launch of the blocked kernel.  3 in FP32 when K allows it (a multiple of 6):
+     * the slots' two field copies are far beyond the Infinity Cache, where one pass over them per three steps
+     * pays for the larger rings (the solver's blocked kernel on 64 shots of 1.2M elements: 3.71 against 3.85
+     * ms per step); else 2 */
+    int bk = 2;
     dgx_operators ops;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -287,9 +291,10 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
         delete sv;
         return DGX_EARG;
     }
-    if (sv->group == 1 && sv->K % (2 * kServeBlockSteps)) {
+    sv->bk = (sv->fp32 && sv->K % 6 == 0) ? 3 : 2;
+    if (sv->group == 1 && sv->K % (2 * sv->bk)) {
         dgx_set_error("dgx_server_create: with single-shot slots steps_per_block must be a multiple of %d",
-                      2 * kServeBlockSteps);
+                      2 * sv->bk);
         delete sv;
         return DGX_EARG;
     }
@@ -426,7 +431,7 @@ extern "C" int dgx_server_add_mesh(dgx_server *sv, const dgx_problem *p, int *me
     std::vector<int4> bt, bl;
     std::vector<int> blv, bg;
     if (sv->group == 1 &&
-        !dgx_build_block_tiles(p->tri, mh.nbr, kServeBlockSteps, serve_block_threads(sv), bt, blv, bg, bl)) {
+        !dgx_build_block_tiles(p->tri, mh.nbr, sv->bk, serve_block_threads(sv), bt, blv, bg, bl)) {
         dgx_mesh_dev_free(&mh);
         dgx_set_error("dgx_server_add_mesh: an element and its neighbour levels exceed %d threads",
                       serve_block_threads(sv));
@@ -755,8 +760,8 @@ static void fill_batch_args(const dgx_server *sv, const Batch &b, const dgx::Ops
         a.blrec = m.blrec;
         a.blnb = m.blnb;
         a.ntile = (int)m.ntile;
-        a.bkmax = kServeBlockSteps;
-        a.bsteps = kServeBlockSteps;
+        a.bkmax = sv->bk;
+        a.bsteps = sv->bk;
         a.bthreads = serve_block_threads(sv);
     }
     a.nbr = m.nbr;
@@ -1123,10 +1128,10 @@ static int build_graph(dgx_server *sv)
     SV_TRY(hipStreamBeginCapture(sv->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     const void *ops = sv->fp32 ? (const void *)&sv->op32 : (const void *)&sv->op64;
-    /* single-shot slots: K / kServeBlockSteps launches of the blocked kernel (an even number: every slot's
+    /* single-shot slots: K / bk launches of the blocked kernel (an even number: every slot's
      * state is back in copy A at the end of the block) */
-    for (int k = 0; sv->group == 1 && k < sv->K / kServeBlockSteps && e == hipSuccess; k++)
-        e = dgx_launch_serve_block(sv->fp32, sv->p2, sv->d_table, sv->d_sched, sv->d_step_base, k * kServeBlockSteps, k,
+    for (int k = 0; sv->group == 1 && k < sv->K / sv->bk && e == hipSuccess; k++)
+        e = dgx_launch_serve_block(sv->fp32, sv->p2, sv->d_table, sv->d_sched, sv->d_step_base, k * sv->bk, k,
                                    sv->nb, sv->tiles, sv->stream);
     for (int k = 0; sv->group != 1 && k < sv->K && e == hipSuccess; k++) {
         e = dgx_launch_serve(sv->fp32, ops, sv->d_table, sv->d_sched, sv->d_runmask, sv->d_step_base, k, 1,

@@ -97,7 +97,7 @@ def run_slope(a):
             "fp64_reference_checks": checks}
 
 
-def serve_stream(meshes, specs, arrivals, precision=L.DGX_FP32, steps_per_block=100, group=16):
+def serve_stream(meshes, specs, arrivals, precision=L.DGX_FP32, steps_per_block=96, group=16):
     from . import serve as SV
     sv = SV.Server(precision, 0, steps_per_block, group_shots=group)
     rms = [sv.add_mesh(m, md, sp) for m, md, sp in meshes]

@@ -35,10 +35,10 @@ typedef struct {
                             * serving kernel's shot tile, one launch per half-step; each node row of a group
                             * holds 64/G elements x G shots, so every G moves whole lines; 64 is fastest per
                             * lane when groups fill, 4-16 waste fewer lanes when requests are small and meshes
-                            * many.  1: every shot is a slot of its own in the element layout, stepped two
-                            * steps per launch by the blocked kernel (tiles of each mesh with recomputed
-                            * rings, two copies of the fields per slot); steps_per_block must then be a
-                            * multiple of 4 */
+                            * many.  1: every shot is a slot of its own in the element layout, stepped by
+                            * the blocked kernel (tiles of each mesh with recomputed rings, two copies of the
+                            * fields per slot), three steps per launch in FP32 when steps_per_block is a
+                            * multiple of 6, else two; steps_per_block must then be a multiple of 4 or 6 */
 } dgx_server_config;
 
 enum dgx_request_state { DGX_REQ_QUEUED = 0, DGX_REQ_RUNNING = 1, DGX_REQ_DONE = 2 };

@@ -10,8 +10,8 @@
   group holds), and may move between them when groups are compacted, without changing its results.
 
 The first two hold for every slot-group size (4, 8, 16, 32 and 64 shots per group, which is also the serving
-kernel's shot tile) and for single-shot slots (group 1), which the blocked kernel steps two steps per launch
-(compared with the standalone solver's blocked kernel).  The first is also checked in FP64, for single-shot
+kernel's shot tile) and for single-shot slots (group 1), which the blocked kernel steps three steps per launch
+in FP32 (blocks of 48 steps), two in FP64 (compared with the standalone solver's blocked kernel).  The first is also checked in FP64, for single-shot
 slots (256-thread tiles) and 16-shot groups.
 """
 import numpy as np
@@ -48,7 +48,7 @@ def build(server, rms, spec, k):
 
 def new_server(precision=L.DGX_FP32, budget=0, K=50, group=16):
     if group == 1:
-        K = K // 4 * 4  # single-shot slots: an even number of 2-step launches per block
+        K = K // 12 * 12  # single-shot slots: an even number of 2- or 3-step launches per block
     sp, ms = meshes()
     sv = S.Server(precision, budget, K, group_shots=group)
     rms = [sv.add_mesh(m, md, sp) for m, md in ms]
@@ -176,8 +176,8 @@ def test_request_equals_standalone_solver_and_oracle(group):
     nsteps, f0 = REQS[k][2], REQS[k][3]
     srcs = [dgx.SourceSpec(float(a), float(b), L.DGX_SRC_EXPLOSIVE, s, f0) for s, (a, b) in enumerate(xz)]
     st = dgx.build_setup(mesh, model, srcs, rec, nsteps, sponge=sp)
-    if group == 1:  # the same tile code as the solver's blocked kernel (2 steps per launch, 512 threads)
-        solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_ELEMENT, step_kernel=L.DGX_STEP_BLOCKED, block_steps=2)
+    if group == 1:  # the same tile code as the solver's blocked kernel (3 steps per launch, 512 threads)
+        solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_ELEMENT, step_kernel=L.DGX_STEP_BLOCKED, block_steps=3)
     else:
         solver = dgx.Solver(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER, group)
     solver.step(nsteps)
