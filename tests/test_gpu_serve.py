@@ -86,6 +86,25 @@ def test_results_identical_alone_and_in_the_mix(group, precision):
         assert np.array_equal(mixed[k][0], ax) and np.array_equal(mixed[k][1], az), k
 
 
+def test_single_shot_slots_two_and_three_steps_per_launch_agree():
+    """FP32 single-shot slots run three steps per launch when the block of steps is a multiple of 6 (48) and two
+    otherwise (52); every element's update is the same code on the same inputs either way (the rings recompute
+    what their owning tile computes), so the traces agree bit for bit."""
+    print("case serve single-shot slots, 2 and 3 steps per launch", flush=True)
+    out = []
+    for K in (48, 52):
+        sp, ms = meshes()
+        sv = S.Server(L.DGX_FP32, 0, K, group_shots=1)
+        rms = [sv.add_mesh(m, md, sp) for m, md in ms]
+        reqs = [build(sv, rms, REQS[k], k) for k in (0, 1)]
+        ids = [sv.submit(r[0]) for r in reqs]
+        sv.drain()
+        out.append([sv.result(rid, r[0]) for rid, r in zip(ids, reqs)])
+        sv.close()
+    for (ax, az), (bx, bz) in zip(*out):
+        assert np.array_equal(ax, bx) and np.array_equal(az, bz)
+
+
 def test_requests_spread_over_groups():
     """Three requests of 10, 10 and 12 shots on one mesh with 16-shot groups fill exactly two groups (the
     second and third request take the free slots of the first group before a new one is opened), and each
