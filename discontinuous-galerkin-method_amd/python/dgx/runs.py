@@ -1,12 +1,13 @@
 """Full runs of the measured workloads, one JSON line each:
 
     python -m dgx.runs gradient  [--steps 4000]             single shot 640k elements: FP64 vs FP32
-    python -m dgx.runs slope     [--steps 5000 --shots 64]  batched FP32, FP64 reference on 4 shots
+    python -m dgx.runs slope     [--steps 5000 --shots 64]  batched FP32 (default path; --layout inner: the
+                                                            shot-innermost kernels), FP64 reference on 4 shots
     python -m dgx.runs serving   [--requests 2000 --rate 20] open-loop request stream on 8 meshes
     python -m dgx.runs carbonate [--shots 256 --steps 8000] 256 shots through the request loop, FP64 on 4
 
-The serving runs take --group (4, 8, 16, 32 or 64 shots per slot group; default 8 for the request stream of 1-16-shot
-requests over 8 meshes, 64 for carbonate's 16-shot requests on one mesh).
+The serving runs take --group (1: single-shot slots on the blocked kernel, the default; 4, 8, 16, 32 or 64 shots per
+slot group).
 
 Progress lines go to stderr while a run is going."""
 from __future__ import annotations
@@ -76,8 +77,10 @@ def run_slope(a):
     w = configs.slope_6x3km(a.steps, nshots=a.shots)
     log(f"slope mesh {w.mesh.tri} elements")
     st = build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
-    s32, sec32 = timed_run(st, L.DGX_FP32, L.DGX_LAYOUT_SHOT_INNER)
-    tinfo = s32.tile_info()
+    layout = L.DGX_LAYOUT_SHOT_INNER if a.layout == "inner" else L.DGX_LAYOUT_AUTO
+    s32, sec32 = timed_run(st, L.DGX_FP32, layout)
+    path = (f"shot-innermost half-step kernels, tile {s32.tile_info()[0]}" if a.layout == "inner" else
+            f"blocked kernel, {s32.block_info()['steps_per_launch']} steps per launch")
     picks = [0, a.shots // 3, (2 * a.shots) // 3, a.shots - 1]
     checks, sec64 = [], 0.0
     for k in picks:
@@ -92,7 +95,7 @@ def run_slope(a):
     s32.close()
     dof = w.mesh.tri * 30 * a.shots * a.steps
     return {"workload": w.name, "elements": w.mesh.tri, "shots": a.shots, "steps": a.steps, "dt": st.dt,
-            "batched_fp32_dof_updates_per_s": dof / sec32, "shot_tile": tinfo[0], "wavefronts_per_workgroup": tinfo[1],
+            "batched_fp32_dof_updates_per_s": dof / sec32, "path": path,
             "plain_fp64_dof_updates_per_s": w.mesh.tri * 30 * a.steps * len(picks) / sec64,
             "fp64_reference_checks": checks}
 
@@ -151,13 +154,15 @@ def main(argv=None):
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--group", type=int, default=0,
-                    help="shots per slot group of the server, 1 (single-shot slots on the blocked kernel), 4, 8, 16, 32 or 64 "
-                         "(default: 1 for serving, 64 for carbonate)")
+                    help="shots per slot group of the server, 1 (single-shot slots on the blocked kernel, the default), "
+                         "4, 8, 16, 32 or 64")
+    ap.add_argument("--layout", choices=["auto", "inner"], default="auto",
+                    help="slope: the default path (blocked kernel) or the shot-innermost half-step kernels")
     a = ap.parse_args(argv)
     defaults = {"gradient": (4000, 1), "slope": (5000, 64), "serving": (0, 0), "carbonate": (8000, 256)}
     a.steps = a.steps or defaults[a.which][0]
     a.shots = a.shots or defaults[a.which][1]
-    a.group = a.group or (64 if a.which == "carbonate" else 1)
+    a.group = a.group or 1
     fn = {"gradient": run_gradient, "slope": run_slope, "serving": run_serving, "carbonate": run_carbonate}[a.which]
     print(json.dumps(fn(a)), flush=True)
 
