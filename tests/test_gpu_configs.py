@@ -5,7 +5,7 @@ checks the solver on it at a reduced step count:
 
 * gradient 8 km x 4 km, 640k jittered triangles: plain FP64 against the oracle (the reference's loops,
   OpenMP_Function.c:25-306) to 1e-12 on traces and all five fields; FP32 against the oracle to 1e-4 on the
-  trace set, on every trace at or above 10 % of the peak trace norm, and on every trace against that peak;
+  trace set, on every trace at or above 0.1 % of the peak trace norm, and on every trace against that peak;
 * slope 6 km x 3 km, ~1.2M Delaunay triangles, water over three dipping layers, 64 FP32 shots in the
   shot-innermost layout and on the blocked kernel (the FP32 default): sampled shots against FP64 single-shot
   GPU runs to the per-trace bar; sources and receivers located on the GPU equal the host locator's; one shot
@@ -33,10 +33,10 @@ FP64_TOL = 1e-12
 FP32_TOL = 1e-4
 # Per-trace FP32 bar: a trace (one component of one receiver) whose FP64 norm is at least FLOOR of the set's
 # largest trace norm is within FP32_TOL of its own norm; every trace is within FP32_TOL of that largest norm.
-# FP32 rounding leaves an absolute error of about the same size on every trace (about 5e-6 of the peak on
-# these workloads), so below the floor a trace's relative error grows as its amplitude falls (about 2e-4 at
-# 1 % of the peak).
-FLOOR = 0.1
+# FP32 rounding leaves an absolute error of about the same size on every trace (well under 1e-6 of the peak on
+# these workloads), so a trace's relative error grows as its amplitude falls: at 0.1-1 % of the peak it
+# measures up to 2.5e-5, and traces still nearly silent (below 0.1 %) reach relative errors of order 1.
+FLOOR = 1e-3
 
 
 def cross(x, z, half=200.0, step=10.0):
