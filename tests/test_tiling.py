@@ -190,6 +190,29 @@ def test_block_tiling_unstructured_and_tiny():
         assert len(tiles) == 1 and tiles[0, 1] == 2 * n
 
 
+def test_block_tiling_refuses_what_does_not_fit():
+    """An element whose 2K - 1 levels need more lanes than the workgroup has threads, or whose ghost level needs
+    more than the cap/4 - 1 ghost slots, cannot be tiled: the builder reports DGX_EMESH (the solver then falls
+    back to the half-step kernels, or refuses an explicit blocked request), and never a partial tiling."""
+    mesh = dgx.mesh.structured(30, 30, 10.0, diagonal="random", seed=2)
+    lib = L.load()
+    f = lib.dgx_block_tiling
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(L.ProblemC), C.c_int, C.c_int] + [C.POINTER(C.c_long)] * 2 + [C.c_void_p] * 5
+    pt = np.ascontiguousarray(mesh.point, np.float64)
+    tr = np.ascontiguousarray(mesh.triangle, np.int64)
+    nb = np.ascontiguousarray(mesh.neighbor, np.int64)
+    ones = np.ones(mesh.tri)
+    p = L.ProblemC(mesh.tri, mesh.point_num, L.dp(pt), L.lp(tr), L.lp(nb), L.dp(ones), L.dp(ones), L.dp(ones),
+                   L.dptr(), 1.0, L.DGX_FP32, 1, 1, 0, L.dptr(), 0, None, 0, None, L.DGX_LAYOUT_ELEMENT, 0,
+                   L.DGX_STEP_BLOCKED, 0, 3)
+    for cap in (32, 64):  # 3 steps: 5 levels of lanes and a ghost level around one element exceed these
+        nt, nl = C.c_long(-7), C.c_long(-7)
+        assert f(C.byref(p), 3, cap, C.byref(nt), C.byref(nl), None, None, None, None, None) == L.DGX_EMESH
+        assert nt.value == -7 and nl.value == -7
+    check_block_tiling(mesh, 3, 128)
+
+
 def test_block_tiling_recompute_fraction_640k_order():
     """Work recomputed on the rings, for a mesh of the 640k-element kind at a quarter of its size: 2 steps per
     launch at 1024 threads recompute well under the own work."""
