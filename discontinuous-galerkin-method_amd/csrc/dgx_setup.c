@@ -730,3 +730,114 @@ int dgx_locate_receivers(const dgx_locator *L, long n, const double *xz, dgx_rec
     }
     return DGX_OK;
 }
+
+/* ---- the reference's per-element operators, for DGX_FP64_REFERENCE ----
+ * The reference forms these once per run on the host (Jacobian, Angle, Normals, Elastic_A_B, Central_flux,
+ * Dx_Dz; fun_DG.c:421-554 and 1071-1192) and its step loops read them per element.  The
+ * expressions and their order below are the reference's, so that a caller's arrays equal its arrays bit for
+ * bit on the same machine (same libm for sqrt, pow, acos and cos; no fused multiply-adds). */
+
+/* The three vertices of element e (1-based ids in triangle). */
+static void ref_vertices(const double *point, const long *triangle, long e, double P[3][2])
+{
+    for (int v = 0; v < 3; v++) {
+        const long p = triangle[e * 3 + v] - 1;
+        P[v][0] = point[p * 2 + 0];
+        P[v][1] = point[p * 2 + 1];
+    }
+}
+
+/* Edge lengths L0 = |v0 v1|, L1 = |v1 v2|, L2 = |v0 v2| (EdgeLength, fun_DG.c:122-146). */
+static void ref_lengths(const double P[3][2], double L[3])
+{
+    L[0] = sqrt(pow(P[0][0] - P[1][0], 2) + pow(P[0][1] - P[1][1], 2));
+    L[1] = sqrt(pow(P[1][0] - P[2][0], 2) + pow(P[1][1] - P[2][1], 2));
+    L[2] = sqrt(pow(P[0][0] - P[2][0], 2) + pow(P[0][1] - P[2][1], 2));
+}
+
+int dgx_reference_operators(long tri, const double *point, const long *triangle, const double *lambda,
+                            const double *miu, const double *den, const dgx_operators *ops, double *Dx, double *Dz,
+                            double *Q, double *A, double *B)
+{
+    if (tri <= 0 || !point || !triangle || !lambda || !miu || !den || !ops || !Dx || !Dz || !Q || !A || !B)
+        return DGX_EARG;
+    const double ux[2] = {1, 0}, uz[2] = {0, 1};
+    for (long e = 0; e < tri; e++) {
+        double P[3][2], len[3];
+        ref_vertices(point, triangle, e, P);
+        ref_lengths(P, len);
+        /* Jacobian and half edge lengths (Jacobian, fun_DG.c:421-455) */
+        const double xr = (P[1][0] - P[0][0]) * 0.5, yr = (P[1][1] - P[0][1]) * 0.5;
+        const double xs = (P[2][0] - P[0][0]) * 0.5, ys = (P[2][1] - P[0][1]) * 0.5;
+        const double J = xr * ys - xs * yr;
+        /* outward unit normals through the in-circle's tangent points (Angle and Normals, fun_DG.c:456-554): the in-centre Ac, the
+         * angle at each face's first vertex, the tangent point Bt on the face, and n = (Bt - Ac) / |Bt - Ac| */
+        const double L1 = len[0], L2 = len[1], L3 = len[2];
+        double ang[3];
+        ang[0] = acos((L1 * L1 + L3 * L3 - L2 * L2) / (2 * L1 * L3));
+        ang[1] = acos((L1 * L1 + L2 * L2 - L3 * L3) / (2 * L1 * L2));
+        ang[2] = acos((L2 * L2 + L3 * L3 - L1 * L1) / (2 * L2 * L3));
+        double Ac[2];
+        for (int j = 0; j < 2; j++)
+            Ac[j] = (len[1] * P[0][j] + len[2] * P[1][j] + len[0] * P[2][j]) / (len[1] + len[2] + len[0]);
+        double nrm[3][2];
+        for (int k = 0; k < 3; k++) {
+            const double *Ps = P[k], *Pe = P[(k + 1) % 3];
+            const double d = sqrt(pow(Ac[0] - Ps[0], 2) + pow(Ac[1] - Ps[1], 2)) * cos(0.5 * ang[k]);
+            double AB[2];
+            for (int j = 0; j < 2; j++) {
+                const double Bt = Ps[j] + (d / len[k]) * (Pe[j] - Ps[j]);
+                AB[j] = Bt - Ac[j];
+            }
+            const double n = sqrt(pow(AB[0], 2) + pow(AB[1], 2));
+            nrm[k][0] = (AB[0] * ux[0] + AB[1] * ux[1]) / n;
+            nrm[k][1] = (AB[0] * uz[0] + AB[1] * uz[1]) / n;
+        }
+        /* material factors (Elastic_A_B, fun_DG.c:1071-1090) */
+        double *a = &A[e * 5], *b = &B[e * 5];
+        a[0] = 1 / den[e];
+        a[1] = 1 / den[e];
+        a[2] = lambda[e] + 2 * miu[e];
+        a[3] = lambda[e];
+        a[4] = miu[e];
+        b[0] = 1 / den[e];
+        b[1] = 1 / den[e];
+        b[2] = lambda[e];
+        b[3] = lambda[e] + 2 * miu[e];
+        b[4] = miu[e];
+        /* face flux coefficients 0.5 n (A|B) (L/2) / J (Central_flux, fun_DG.c:1091-1136) */
+        for (int j = 0; j < 3; j++) {
+            const double nx = nrm[j][0], nz = nrm[j][1], je = len[j] / 2;
+            double *q = &Q[e * 30 + j * 10];
+            q[0] = 0.5 * nx * a[0] * je / J;
+            q[1] = 0.5 * nz * b[0] * je / J;
+            q[2] = 0.5 * nz * a[1] * je / J;
+            q[3] = 0.5 * nx * b[1] * je / J;
+            q[4] = 0.5 * nx * a[2] * je / J;
+            q[5] = 0.5 * nz * b[2] * je / J;
+            q[6] = 0.5 * nx * a[3] * je / J;
+            q[7] = 0.5 * nz * b[3] * je / J;
+            q[8] = 0.5 * nz * a[4] * je / J;
+            q[9] = 0.5 * nx * b[4] * je / J;
+        }
+        /* Dx = (rx Vr + sx Vs) inverV, Dz = (rz Vr + sz Vs) inverV (Dx_Dz, fun_DG.c:1137-1192) */
+        const double Rx = ys / J, Rz = -xs / J, Sx = -yr / J, Sz = xr / J;
+        double gx[6][6], gz[6][6];
+        for (int i = 0; i < 6; i++)
+            for (int n = 0; n < 6; n++) {
+                gx[i][n] = Rx * ops->Vr[i * 6 + n] + Sx * ops->Vs[i * 6 + n];
+                gz[i][n] = Rz * ops->Vr[i * 6 + n] + Sz * ops->Vs[i * 6 + n];
+            }
+        for (int p = 0; p < 6; p++)
+            for (int q = 0; q < 6; q++) {
+                double f1 = 0, f2 = 0;
+                for (int k = 0; k < 6; k++) {
+                    f1 = f1 + gx[p][k] * ops->inverV[k * 6 + q];
+                    f2 = f2 + gz[p][k] * ops->inverV[k * 6 + q];
+                }
+                Dx[e * 36 + p * 6 + q] = f1;
+                Dz[e * 36 + p * 6 + q] = f2;
+            }
+    }
+    return DGX_OK;
+}

@@ -147,6 +147,7 @@ def load() -> C.CDLL:
                                                 dptr, C.POINTER(C.c_int), C.POINTER(C.c_int), lptr,
                                                 C.POINTER(Source)]),
         "dgx_p2_basis": (None, [C.c_double, C.c_double, dptr]),
+        "dgx_reference_operators": (C.c_int, [C.c_long, dptr, lptr, dptr, dptr, dptr, C.POINTER(Operators)] + [dptr] * 5),
         "dgx_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(Operators), C.POINTER(vp)]),
         "dgx_server_destroy": (C.c_int, [vp]),
         "dgx_server_add_mesh": (C.c_int, [vp, C.POINTER(ProblemC), C.POINTER(C.c_int)]),

@@ -131,3 +131,22 @@ def test_binary_mesh_round_trips_and_matches_the_text_reader(tmp_path):
     trunc = str(tmp_path / "t.dgxm")
     open(trunc, "wb").write(open(py_path, "rb").read()[:-9])
     assert lib.dgx_read_mesh_binary(trunc.encode(), T, P, L.dp(pt2), L.lp(tr2), L.lp(nb2), L.lp(at2)) == L.DGX_EIO
+
+
+@pytest.mark.parametrize("maker", [lambda: dgx.configs.homogeneous_2km(10),
+                                   lambda: dgx.configs.gradient_8x4km(10, nx=120, nz=60),
+                                   lambda: dgx.configs.slope_6x3km(10, h=40.0, nshots=1, nrec=5)])
+def test_reference_operators_equal_oracle_bitwise(maker):
+    """dgx_reference_operators forms Dx, Dz, Q, A, B with the reference's expressions in its order: the arrays
+    equal the oracle's (itself equal to the reference build's on the fixtures) bit for bit."""
+    w = maker()
+    m, T = w.mesh, w.mesh.tri
+    ops = O.read_operators(OPDIR)
+    lay = O.setup_reference_layout(m.point, m.triangle, m.neighbor, w.model.lam, w.model.miu, w.model.den, ops, None)
+    Dx, Dz, Q, A, B = np.zeros(T * 36), np.zeros(T * 36), np.zeros(T * 30), np.zeros(T * 5), np.zeros(T * 5)
+    c = lambda a, t=np.float64: np.ascontiguousarray(a, t)
+    L.check(L.load().dgx_reference_operators(T, L.dp(c(m.point)), L.lp(c(m.triangle, np.int64)), L.dp(c(w.model.lam)),
+                                             L.dp(c(w.model.miu)), L.dp(c(w.model.den)), C.byref(L.read_operators()),
+                                             L.dp(Dx), L.dp(Dz), L.dp(Q), L.dp(A), L.dp(B)), "dgx_reference_operators")
+    for name, mine, orc in (("Dx", Dx, lay.Dx), ("Dz", Dz, lay.Dz), ("Q", Q, lay.Q), ("A", A, lay.A), ("B", B, lay.B)):
+        assert np.array_equal(mine, orc), (name, np.abs(mine - orc).max())
