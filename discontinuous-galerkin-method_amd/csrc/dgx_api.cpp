@@ -17,6 +17,7 @@
 #include "dgx.h"
 #include "dgx_kernels.h"
 #include "dgx_p2ops.h"
+#include "dgx_setup.h"
 
 extern "C" {
 hipError_t dgx_launch_f64(const void *args, int nshots, hipStream_t st, int which);
@@ -31,6 +32,8 @@ hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int varian
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
 int dgx_batched_shape_ok(int shot_tile, int wg_waves);
 hipError_t dgx_launch_batched_f32(const void *args, hipStream_t st, int which);
+hipError_t dgx_launch_reference(const void *args, const double *ops, int which, int nshots, int nsrc, int nrec, int t,
+                                double w, hipStream_t st);
 }
 
 /* ---- error reporting ---- */
@@ -112,6 +115,8 @@ inline long long fidx(long long q, int i) { return ((q >> 6) * 6 + i) * 64 + (q 
 struct dgx_solver {
     int fp32 = 0;
     int comp = 0;                    /* compensated FP32: a low word per value (blocked kernel) */
+    int exact = 0;                   /* DGX_FP64_REFERENCE: the reference's arrays and order (dgx_reference.hip) */
+    int sponge = 0;                  /* the problem has sponge factors */
     int layout = DGX_LAYOUT_ELEMENT;
     int shot_tile = 0;               /* shot-innermost: shots per lane group */
     int wg_waves = 4;                /* shot-innermost: wavefronts per workgroup */
@@ -155,6 +160,7 @@ struct dgx_solver {
     double *wav = nullptr;
     double *trace = nullptr;
     int *step_base = nullptr;
+    double *refops = nullptr; /* DGX_FP64_REFERENCE: Dx | Dz | Q | A | B per element, blocked by 64 elements */
     /* argument blocks */
     dgx::StepArgs<double> a64;
     dgx::StepArgs<float> a32;
@@ -342,9 +348,30 @@ bool dgx_build_block_tiles(long tri, const std::vector<int4> &nbr, int K, int ca
     return true;
 }
 
+/* DGX_FP64_REFERENCE: one of its kernels (dgx_launch_reference's `which`); sources read the caller-ordered list. */
+static int launch_exact(dgx_solver *s, int which, int step_off, int t = -1, double w = 0)
+{
+    dgx::StepArgs<double> a = s->a64;
+    a.step_off = step_off;
+    a.src = s->src + s->nsrc;
+    const hipError_t e = dgx_launch_reference(&a, s->refops, which, (int)s->nshots, (int)s->nsrc, (int)s->nrec, t, w,
+                                              s->stream);
+    if (e != hipSuccess) {
+        dgx_set_error("kernel launch failed: %s", hipGetErrorString(e));
+        return DGX_EHIP;
+    }
+    return DGX_OK;
+}
+
 static int launch(dgx_solver *s, int which, int step_off)
 {
     hipError_t e;
+    if (s->exact && which != K_ABSORB) {
+        if (which == K_VEL || which == K_VEL_FUSED) return launch_exact(s, 0, step_off);
+        if (which == K_STRESS || which == K_STRESS_FUSED) return launch_exact(s, 1, step_off);
+        dgx_set_error("DGX_FP64_REFERENCE runs the split step kernels only");
+        return DGX_ESTATE;
+    }
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         if (which != K_VEL_FUSED && which != K_STRESS_FUSED) {
             dgx_set_error("the split entry points and the full-step kernel need the element layout (DGX_LAYOUT_ELEMENT)");
@@ -481,7 +508,7 @@ static void free_solver(dgx_solver *s)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {s->fields, s->fields2, s->nb2, s->tiles, s->ring, s->nbr, s->vtx, s->pt, s->mat, s->absorb,
                     s->aux, s->src, s->rec, s->wav, s->trace, s->step_base, s->d_perm, s->d_snap,
-                    s->blevels, s->blrec, s->blnb};
+                    s->blevels, s->blrec, s->blnb, s->refops};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (s->h_snap) (void)hipHostFree(s->h_snap);
@@ -697,14 +724,17 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     if (T > (1L << 28) || p->point_num > (1L << 30)) { dgx_set_error("dgx_create: mesh too large"); return DGX_EARG; }
     if (p->nshots < 1 || p->nsteps < 0 || p->nrec < 0 || p->nsrc < 0 || (p->nsrc > 0 && !p->src) ||
         (p->nrec > 0 && !p->rec) || (p->nsrc > 0 && (!p->wavelet || p->nwavelets < 1)) ||
-        (p->precision != DGX_FP64 && p->precision != DGX_FP32 && p->precision != DGX_FP32_COMPENSATED) ||
+        (p->precision != DGX_FP64 && p->precision != DGX_FP32 && p->precision != DGX_FP32_COMPENSATED &&
+         p->precision != DGX_FP64_REFERENCE) ||
         !(p->dt > 0)) {
         dgx_set_error("dgx_create: invalid shots/steps/sources/receivers/precision/dt");
         return DGX_EARG;
     }
     dgx_solver *s = new dgx_solver();
-    s->fp32 = (p->precision != DGX_FP64);
+    s->fp32 = (p->precision == DGX_FP32 || p->precision == DGX_FP32_COMPENSATED);
     s->comp = (p->precision == DGX_FP32_COMPENSATED);
+    s->exact = (p->precision == DGX_FP64_REFERENCE);
+    s->sponge = p->absorb_fun != nullptr;
     s->tri = T;
     s->nshots = p->nshots;
     s->nsteps = p->nsteps;
@@ -725,7 +755,14 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         s->layout = (wants_inner && p->nshots >= 8) ? DGX_LAYOUT_SHOT_INNER : DGX_LAYOUT_ELEMENT;
     else
         s->layout = p->layout;
-    const bool auto_blocked = p->step_kernel == DGX_STEP_AUTO && s->layout == DGX_LAYOUT_ELEMENT;
+    if (s->exact && (s->layout != DGX_LAYOUT_ELEMENT || (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF) ||
+                     p->block_steps)) {
+        dgx_set_error("dgx_create: DGX_FP64_REFERENCE runs in the element layout, one kernel per step function "
+                      "(step_kernel AUTO or HALF, block_steps 0)");
+        free_solver(s);
+        return DGX_EARG;
+    }
+    const bool auto_blocked = p->step_kernel == DGX_STEP_AUTO && s->layout == DGX_LAYOUT_ELEMENT && !s->exact;
     if (p->step_kernel != DGX_STEP_AUTO && p->step_kernel != DGX_STEP_HALF && p->step_kernel != DGX_STEP_FULL &&
         p->step_kernel != DGX_STEP_BLOCKED) {
         dgx_set_error("dgx_create: unknown step_kernel %d", p->step_kernel);
@@ -858,6 +895,31 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     s->inv = std::move(md.inv);
     std::vector<int4> &nbr_host = md.nbr;
 
+    /* ---- DGX_FP64_REFERENCE: the reference's per-element arrays, formed on the host in chunks of the caller's
+     * elements and placed in device order, entry k of element q at ((q/64)*112 + k)*64 + q%64 ---- */
+    std::vector<double> refops_host;
+    if (s->exact) {
+        refops_host.assign((size_t)s->tri_pad * 112, 0.0);
+        const long chunk = 65536;
+        std::vector<double> Dx(chunk * 36), Dz(chunk * 36), Q(chunk * 30), A(chunk * 5), B(chunk * 5);
+        for (long e0 = 0; e0 < T; e0 += chunk) {
+            const long n = std::min(chunk, T - e0);
+            const int rc = dgx_reference_operators(n, p->point, p->triangle + 3 * e0, p->lambda + e0, p->miu + e0,
+                                                   p->den + e0, ops, Dx.data(), Dz.data(), Q.data(), A.data(),
+                                                   B.data());
+            if (rc) { dgx_set_error("dgx_create: reference operators failed"); free_solver(s); return rc; }
+            for (long k = 0; k < n; k++) {
+                const long q = s->inv[e0 + k];
+                double *dst = &refops_host[(size_t)(q >> 6) * 112 * 64 + (q & 63)];
+                for (int c = 0; c < 36; c++) dst[(size_t)c * 64] = Dx[k * 36 + c];
+                for (int c = 0; c < 36; c++) dst[(size_t)(36 + c) * 64] = Dz[k * 36 + c];
+                for (int c = 0; c < 30; c++) dst[(size_t)(72 + c) * 64] = Q[k * 30 + c];
+                for (int c = 0; c < 5; c++) dst[(size_t)(102 + c) * 64] = A[k * 5 + c];
+                for (int c = 0; c < 5; c++) dst[(size_t)(107 + c) * 64] = B[k * 5 + c];
+            }
+        }
+    }
+
     std::vector<dgx_aux> aux_host(aux_elems.size());
     std::vector<dgx_src_dev> src_host;
     std::vector<dgx_rec_dev> rec_host;
@@ -971,6 +1033,10 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
         CREATE_TRY(dgx_upload_lane_records(bgid_host.data(), (long)bgid_host.size(), s->vtx, &s->blrec));
         CREATE_TRY(upload(&s->blnb, blnb_host));
     }
+    if (s->exact) {
+        CREATE_TRY(upload(&s->refops, refops_host));
+        std::vector<double>().swap(refops_host);
+    }
     if (!aux_elems.empty()) /* the code words now carry their source/receiver records */
         CREATE_TRY(hipMemcpy(s->nbr, nbr_host.data(), nbr_host.size() * sizeof(int4), hipMemcpyHostToDevice));
     CREATE_TRY(upload(&s->aux, aux_host));
@@ -1061,11 +1127,26 @@ extern "C" int dgx_destroy(dgx_solver *s)
     return DGX_OK;
 }
 
+/* DGX_FP64_REFERENCE: one step as the reference composes it (DG_leap_Frog_OpenMP.c:405-425): absorb, sources,
+ * velocity, stress, record, each on the step at the device counter + off. */
+static int step_exact(dgx_solver *s, int off)
+{
+    int rc = s->sponge ? launch(s, K_ABSORB, off) : DGX_OK;
+    if (rc == DGX_OK && s->nsrc) rc = launch_exact(s, 2, off);
+    if (rc == DGX_OK) rc = launch_exact(s, 0, off);
+    if (rc == DGX_OK) rc = launch_exact(s, 1, off);
+    if (rc == DGX_OK && s->nrec) rc = launch_exact(s, 4, off);
+    return rc;
+}
+
 static int step_eager(dgx_solver *s)
 {
     int off = (int)(s->step - s->base_host);
     int rc;
-    if (s->full) {
+    if (s->exact) {
+        rc = step_exact(s, off);
+        if (rc) return rc;
+    } else if (s->full) {
         rc = launch(s, K_FULL, off);
         if (rc) return rc;
         swap_copies(s);
@@ -1109,7 +1190,9 @@ static int build_graph(dgx_solver *s)
         }
     } else {
         for (int k = 0; k < kGraphSteps && rc == DGX_OK; k++) {
-            if (s->full) {
+            if (s->exact) {
+                rc = step_exact(s, k);
+            } else if (s->full) {
                 rc = launch(s, K_FULL, k);
                 swap_copies(s);
             } else {
@@ -1213,6 +1296,7 @@ extern "C" int dgx_source_kernel(dgx_solver *s, double w)
     if (!s) return DGX_EARG;
     if (int rc = needs_element_layout(s)) return rc;
     if (s->nsrc == 0) return DGX_OK;
+    if (s->exact) return launch_exact(s, 3, 0, -1, w);
     /* the caller-ordered list follows the aux-ordered copy in the same allocation */
     dgx::StepArgs<double> a64 = s->a64;
     dgx::StepArgs<float> a32 = s->a32;
@@ -1228,6 +1312,7 @@ extern "C" int dgx_record(dgx_solver *s, long t)
 {
     if (!s || t < 0 || t >= s->nsteps) return DGX_EARG;
     if (int rc = needs_element_layout(s)) return rc;
+    if (s->exact) return launch_exact(s, 5, 0, (int)t);
     hipError_t e = dgx_launch_record(s->args(), s->fp32, (int)t, (int)s->nshots, (int)s->nrec, s->stream);
     if (e != hipSuccess) { dgx_set_error("record launch failed: %s", hipGetErrorString(e)); return DGX_EHIP; }
     return DGX_OK;

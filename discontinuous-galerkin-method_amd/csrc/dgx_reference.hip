@@ -1,0 +1,282 @@
+/*
+ * dgx_reference.hip -- the FP64 step of DGX_FP64_REFERENCE: the reference's arithmetic, operation for operation.
+ *
+ * The fast FP64 kernels regroup the reference's sums (metric terms factored out of Dx/Dz, 1/D folded into the face
+ * scalings, fused multiply-adds); their results differ from the reference's in the last bits, and over thousands
+ * of steps by about 1e-12 of the field maxima.  These kernels instead take the reference's own per-element arrays
+ * (Dx, Dz, Q, A, B as its setup forms them, computed on the host by dgx_reference_operators) and evaluate every
+ * sum in the reference's order with every product rounded on its own, so that each value equals the reference's
+ * bit for bit (up to the sign of exact zeros in the face jumps, which its additions erase):
+ *   velocity  OpenMP_Function.c:75-185    stress  OpenMP_Function.c:188-306
+ *   source    OpenMP_Function.c:49-58     record  OpenMP_Function.c:61-72 (P2 weights; a vertex receiver is the
+ *                                                  reference's nodal pick)
+ * Contraction into fused multiply-adds is switched off for this whole file.
+ *
+ * One thread per element and shot, fields in the element layout of the other kernels (64-element blocks,
+ * node-major).  The per-element arrays use the same blocking: entry k of element q lives at
+ * ((q/64)*112 + k)*64 + q%64 with k = Dx 0..35 | Dz 36..71 | Q 72..101 | A 102..106 | B 107..111, so a
+ * wavefront reading entry k of its 64 elements reads 512 contiguous bytes.
+ */
+#include <hip/hip_runtime.h>
+
+#include "dgx.h"
+#include "dgx_device.h"
+#include "dgx_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace dgx {
+namespace exact {
+
+constexpr int kWidth = 112; /* per-element reals: Dx 36 | Dz 36 | Q 30 | A 5 | B 5 */
+constexpr int kDz = 36, kQ = 72, kA = 102, kB = 107;
+
+__device__ __forceinline__ long long fidx(int q, int i) { return ((long long)(q >> 6) * 6 + i) * 64 + (q & 63); }
+__device__ __forceinline__ long long ridx(int q, int k) { return ((long long)(q >> 6) * kWidth + k) * 64 + (q & 63); }
+
+/* Elements dealt to the 8 XCDs in contiguous slices, as in the other element-layout kernels. */
+__device__ __forceinline__ int element_of_thread(int nper)
+{
+    const int b = blockIdx.x;
+    return ((b & (DGX_NXCD - 1)) * nper + (b >> 3)) * DGX_BLOCK + threadIdx.x;
+}
+
+/* Face j's neighbour (device index, -1 on the outer boundary) from the neighbour record. */
+__device__ __forceinline__ int neighbour(const int4 &nb, int j) { return j == 0 ? nb.x : j == 1 ? nb.y : nb.z; }
+
+/* LU_Velocity (OpenMP_Function.c:75-185):
+ *   Vx = (A0 (Dx Txx) + B0 (Dz Txz) + sum_j (Q_j0 S_j dTxx_j + Q_j1 S_j dTxz_j)) dt + Vx
+ *   Vz = (A1 (Dx Txz) + B1 (Dz Tzz) + sum_j (Q_j2 S_j dTzz_j + Q_j3 S_j dTxz_j)) dt + Vz
+ * with the jump dU = flag U_nb - U_own (flag 0 on the boundary). */
+__global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, const double *__restrict__ ops)
+{
+    const int q = element_of_thread(a.nper);
+    if (q >= a.tri) return;
+    const long long so = (long long)blockIdx.y * a.fstride * 6;
+    double *__restrict__ Vx = a.f[0] + so;
+    double *__restrict__ Vz = a.f[1] + so;
+    const double *__restrict__ Txx = a.f[2] + so;
+    const double *__restrict__ Tzz = a.f[3] + so;
+    const double *__restrict__ Txz = a.f[4] + so;
+    double tx[6], tz[6], tc[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = fidx(q, i);
+        tx[i] = Txx[o]; tz[i] = Tzz[o]; tc[i] = Txz[o];
+    }
+    const double A0 = ops[ridx(q, kA + 0)], A1 = ops[ridx(q, kA + 1)];
+    const double B0 = ops[ridx(q, kB + 0)], B1 = ops[ridx(q, kB + 1)];
+    double accx[6], accz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const double dx = ops[ridx(q, i * 6 + n)], dz = ops[ridx(q, kDz + i * 6 + n)];
+            s1 = s1 + dx * tx[n];
+            s2 = s2 + dx * tc[n];
+            s3 = s3 + dz * tz[n];
+            s4 = s4 + dz * tc[n];
+        }
+        accx[i] = A0 * s1 + B0 * s4;
+        accz[i] = A1 * s2 + B1 * s3;
+    }
+    const int4 nb = a.nbr[q];
+    const unsigned code = (unsigned)nb.w;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int e = neighbour(nb, j);
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+        double jxx[3], jzz[3], jxz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(j, k);
+            double nxx = 0, nzz = 0, nxz = 0;
+            if (e >= 0) {
+                const long long o = fidx(e, nb_node(m, rev, k));
+                nxx = Txx[o]; nzz = Tzz[o]; nxz = Txz[o];
+            }
+            jxx[k] = nxx - tx[on];
+            jzz[k] = nzz - tz[on];
+            jxz[k] = nxz - tc[on];
+        }
+        const double q0 = ops[ridx(q, kQ + j * 10 + 0)], q1 = ops[ridx(q, kQ + j * 10 + 1)];
+        const double q2 = ops[ridx(q, kQ + j * 10 + 2)], q3 = ops[ridx(q, kQ + j * 10 + 3)];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double sa = 0, sb = 0, sc = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double sv = a.op.L[j * 18 + i * 3 + k];
+                sa = sa + sv * jxx[k];
+                sb = sb + sv * jzz[k];
+                sc = sc + sv * jxz[k];
+            }
+            accx[i] = accx[i] + (q0 * sa + q1 * sc);
+            accz[i] = accz[i] + (q2 * sb + q3 * sc);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = fidx(q, i);
+        Vx[o] = accx[i] * a.dt + Vx[o];
+        Vz[o] = accz[i] * a.dt + Vz[o];
+    }
+}
+
+/* LU_Stress_Elastic (OpenMP_Function.c:188-306):
+ *   Txx = (A2 (Dx Vx) + B2 (Dz Vz) + sum_j (Q_j4 S_j dVx_j + Q_j5 S_j dVz_j)) dt + Txx
+ *   Tzz = (A3 (Dx Vx) + B3 (Dz Vz) + sum_j (Q_j6 S_j dVx_j + Q_j7 S_j dVz_j)) dt + Tzz
+ *   Txz = (A4 (Dx Vz) + B4 (Dz Vx) + sum_j (Q_j8 S_j dVx_j + Q_j9 S_j dVz_j)) dt + Txz */
+__global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, const double *__restrict__ ops)
+{
+    const int q = element_of_thread(a.nper);
+    if (q >= a.tri) return;
+    const long long so = (long long)blockIdx.y * a.fstride * 6;
+    const double *__restrict__ Vx = a.f[0] + so;
+    const double *__restrict__ Vz = a.f[1] + so;
+    double *__restrict__ Txx = a.f[2] + so;
+    double *__restrict__ Tzz = a.f[3] + so;
+    double *__restrict__ Txz = a.f[4] + so;
+    double vx[6], vz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = fidx(q, i);
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    double A[3], B[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) { A[c] = ops[ridx(q, kA + 2 + c)]; B[c] = ops[ridx(q, kB + 2 + c)]; }
+    double axx[6], azz[6], axz[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const double dx = ops[ridx(q, i * 6 + n)], dz = ops[ridx(q, kDz + i * 6 + n)];
+            s1 = s1 + dx * vx[n];
+            s2 = s2 + dx * vz[n];
+            s3 = s3 + dz * vx[n];
+            s4 = s4 + dz * vz[n];
+        }
+        axx[i] = A[0] * s1 + B[0] * s4;
+        azz[i] = A[1] * s1 + B[1] * s4;
+        axz[i] = A[2] * s2 + B[2] * s3;
+    }
+    const int4 nb = a.nbr[q];
+    const unsigned code = (unsigned)nb.w;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int e = neighbour(nb, j);
+        const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+        double jx[3], jz[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int on = face_node(j, k);
+            double nx = 0, nz = 0;
+            if (e >= 0) {
+                const long long o = fidx(e, nb_node(m, rev, k));
+                nx = Vx[o]; nz = Vz[o];
+            }
+            jx[k] = nx - vx[on];
+            jz[k] = nz - vz[on];
+        }
+        double qq[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) qq[c] = ops[ridx(q, kQ + j * 10 + 4 + c)];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double sa = 0, sb = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double sv = a.op.L[j * 18 + i * 3 + k];
+                sa = sa + sv * jx[k];
+                sb = sb + sv * jz[k];
+            }
+            axx[i] = axx[i] + (qq[0] * sa + qq[1] * sb);
+            azz[i] = azz[i] + (qq[2] * sa + qq[3] * sb);
+            axz[i] = axz[i] + (qq[4] * sa + qq[5] * sb);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = fidx(q, i);
+        Txx[o] = axx[i] * a.dt + Txx[o];
+        Tzz[o] = azz[i] * a.dt + Tzz[o];
+        Txz[o] = axz[i] * a.dt + Txz[o];
+    }
+}
+
+/* source_kernel (OpenMP_Function.c:49-58) for every source, in list order, in one thread: V = V + fun w, the
+ * product rounded first.  w: the sample of the source's wavelet at the device step (*step_base + step_off), or
+ * the given value when `fixed` is set. */
+__global__ void k_ref_source(StepArgs<double> a, int nsrc, int fixed, double wfix)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int step = *a.step_base + a.step_off;
+    for (int s = 0; s < nsrc; s++) {
+        const dgx_src_dev &src = a.src[s];
+        const double w = fixed ? wfix : a.wav[(long long)src.wavelet * a.nsteps + step];
+        const long long so = (long long)src.shot * a.fstride * 6;
+        for (int i = 0; i < 6; i++) {
+            const double inc = src.fun[i] * w;
+            const long long o = so + fidx(src.elem, i);
+            if (src.kind == 0) {
+                a.f[1][o] = a.f[1][o] + inc;
+            } else {
+                a.f[2][o] = a.f[2][o] + inc;
+                a.f[3][o] = a.f[3][o] + inc;
+            }
+        }
+    }
+}
+
+/* Record (OpenMP_Function.c:61-72): trace = sum_i w_i V_i in node order, into slot t, or into the device step's
+ * slot when t < 0. */
+__global__ void k_ref_record(StepArgs<double> a, int t, int nshots)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nrec * nshots) return;
+    if (t < 0) t = *a.step_base + a.step_off;
+    const int shot = k / a.nrec, r = k % a.nrec;
+    const dgx_rec_dev &rc = a.rec[r];
+    const long long so = (long long)shot * a.fstride * 6;
+    double sx = 0, sz = 0;
+    for (int i = 0; i < 6; i++) {
+        const long long o = so + fidx(rc.elem, i);
+        sx = sx + rc.w[i] * a.f[0][o];
+        sz = sz + rc.w[i] * a.f[1][o];
+    }
+    double *tr = a.trace + ((long long)shot * a.nsteps + t) * 2 * a.nrec;
+    tr[rc.index] = sx;
+    tr[a.nrec + rc.index] = sz;
+}
+
+} // namespace exact
+} // namespace dgx
+
+extern "C" {
+/* which: 0 velocity, 1 stress, 2 sources at the device step, 3 sources with the sample w, 4 record into the
+ * device step's slot, 5 record into slot t. */
+hipError_t dgx_launch_reference(const void *args, const double *ops, int which, int nshots, int nsrc, int nrec, int t,
+                                double w, hipStream_t st)
+{
+    const dgx::StepArgs<double> &a = *(const dgx::StepArgs<double> *)args;
+    const dim3 grid(DGX_NXCD * a.nper, nshots);
+    switch (which) {
+    case 0: dgx::exact::k_ref_velocity<<<grid, DGX_BLOCK, 0, st>>>(a, ops); break;
+    case 1: dgx::exact::k_ref_stress<<<grid, DGX_BLOCK, 0, st>>>(a, ops); break;
+    case 2:
+    case 3:
+        if (nsrc > 0) dgx::exact::k_ref_source<<<1, 64, 0, st>>>(a, nsrc, which == 3, w);
+        break;
+    case 4:
+    case 5:
+        if (nrec > 0) dgx::exact::k_ref_record<<<(nrec * nshots + 255) / 256, 256, 0, st>>>(a, which == 5 ? t : -1,
+                                                                                         nshots);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+}

@@ -282,6 +282,10 @@ extern "C" int dgx_server_create(const dgx_server_config *cfg, const dgx_operato
         dgx_set_error("dgx_server_create: no HIP device available");
         return DGX_EHIP;
     }
+    if (cfg && cfg->precision != DGX_FP64 && cfg->precision != DGX_FP32) {
+        dgx_set_error("dgx_server_create: precision must be DGX_FP32 or DGX_FP64");
+        return DGX_EARG;
+    }
     dgx_server *sv = new dgx_server();
     sv->fp32 = cfg ? (cfg->precision == DGX_FP64 ? 0 : 1) : 1;
     sv->K = (cfg && cfg->steps_per_block > 0) ? cfg->steps_per_block : 100;

@@ -15,7 +15,7 @@ PKG_DIR = pathlib.Path(__file__).resolve().parents[2]
 LIB_PATH = PKG_DIR / "lib" / "libdgx.so"
 OPERATOR_DIR = PKG_DIR / "input_files" / "precomputed_matrix"
 
-DGX_FP64, DGX_FP32, DGX_FP32_COMPENSATED = 0, 1, 2
+DGX_FP64, DGX_FP32, DGX_FP32_COMPENSATED, DGX_FP64_REFERENCE = 0, 1, 2, 3
 DGX_SRC_FORCE_Z, DGX_SRC_EXPLOSIVE = 0, 1
 DGX_LAYOUT_AUTO, DGX_LAYOUT_ELEMENT, DGX_LAYOUT_SHOT_INNER = 0, 1, 2
 DGX_OK, DGX_EARG, DGX_EMESH, DGX_EHIP, DGX_ENOMEM, DGX_EIO, DGX_ESTATE = 0, -1, -2, -3, -4, -5, -6

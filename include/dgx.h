@@ -31,7 +31,13 @@ extern "C" {
  * rounding error of every update (TwoSum accumulation), so that accumulated rounding does not grow with the
  * number of steps; snapshots and traces return the sum of the two words.  Runs on the blocked kernel only
  * (element layout) and moves twice the field bytes. */
-enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1, DGX_FP32_COMPENSATED = 2 };
+/* DGX_FP64_REFERENCE: FP64 fields advanced with the reference's own arithmetic: its per-element Dx, Dz, Q, A, B
+ * (formed on the host as its setup forms them, dgx_reference_operators) held on the device, every sum in its
+ * order, no fused multiply-adds, and the step split as the reference splits it (absorb, sources, velocity,
+ * stress, record), so that fields and traces equal the reference's bit for bit.  Element layout and one kernel
+ * per step function (step_kernel AUTO or HALF); reads Dx, Dz, Q, A and B (896 bytes per element) in every
+ * half-step, so it is slower than DGX_FP64. */
+enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1, DGX_FP32_COMPENSATED = 2, DGX_FP64_REFERENCE = 3 };
 enum dgx_source_kind { DGX_SRC_FORCE_Z = 0, DGX_SRC_EXPLOSIVE = 1 };
 /* Field layout on the device.  ELEMENT: 64-element blocks, one thread per element and shot (the
  * single-shot path).  SHOT_INNER: shots innermost, lanes over shots, for many shots on one mesh.

@@ -77,8 +77,8 @@ def test_create_rejects_invalid_configurations():
     an empty mesh, a negative step count, an unknown precision, layout or step kernel, the full-step
     kernel with the shot-innermost layout, a shot tile that is not a power of two from 4 to 64 or that
     is given for the element layout, a workgroup size the kernels are not built for (or given for the
-    element layout), and sources or receivers that name a missing element, shot,
-    wavelet or kind."""
+    element layout), the reference-arithmetic precision with another layout or kernel than its own, and sources
+    or receivers that name a missing element, shot, wavelet or kind."""
     lib = L.load()
     mesh = dgx.structured(4, 4, 10.0)
     ops = L.read_operators()
@@ -98,6 +98,11 @@ def test_create_rejects_invalid_configurations():
         "2 wavefronts with shot tile 8": _problem(mesh, nshots=8, layout=L.DGX_LAYOUT_SHOT_INNER, shot_tile=8,
                                                   wg_waves=2),
         "wavefronts per workgroup, element layout": _problem(mesh, layout=L.DGX_LAYOUT_ELEMENT, wg_waves=4),
+        "reference arithmetic, shot-innermost": _problem(mesh, nshots=8, precision=L.DGX_FP64_REFERENCE,
+                                                         layout=L.DGX_LAYOUT_SHOT_INNER),
+        "reference arithmetic, blocked kernel": _problem(mesh, precision=L.DGX_FP64_REFERENCE,
+                                                         step_kernel=L.DGX_STEP_BLOCKED),
+        "reference arithmetic, block steps": _problem(mesh, precision=L.DGX_FP64_REFERENCE, block_steps=2),
         "source element 0": _with_source(mesh, elem=0),
         "source element past the mesh": _with_source(mesh, elem=T + 1),
         "source shot": _with_source(mesh, shot=1),

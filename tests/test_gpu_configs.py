@@ -86,6 +86,23 @@ def test_gradient_640k_fp64_against_oracle(gradient):
 
 
 @pytest.mark.timeout(600)
+def test_gradient_640k_reference_arithmetic_bit_identical(gradient):
+    """DGX_FP64_REFERENCE on 640,000 elements: traces and both snapshots equal the oracle's bit for bit."""
+    w, oc, ox, oz, osnap = gradient
+    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
+    assert st.dt == oc.dt and np.array_equal(st.absorb_fun, oc.lay.absorb_fun)
+    for s, (e, kind, fun, shot, wi) in zip(st.sources, oc.src):
+        assert s.elem == e and np.array_equal(np.array(list(s.fun)), fun)
+    for r, (e, wgt) in zip(st.receivers, oc.rec):
+        assert r.elem == e and np.array_equal(np.array(list(r.weight)), wgt)
+    gx, gz, gsnap = product_run(st, L.DGX_FP64_REFERENCE, snap_every=600)
+    assert np.array_equal(gx, ox) and np.array_equal(gz, oz)
+    for k, f in osnap.items():
+        for c in range(5):
+            assert np.array_equal(gsnap[k][c], f[c]), (k, c)
+
+
+@pytest.mark.timeout(600)
 def test_gradient_640k_fp32_against_oracle(gradient):
     w, oc, ox, oz, osnap = gradient
     st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
