@@ -32,8 +32,8 @@ hipError_t dgx_launch_copy(const void *src, void *dst, long long n32, int varian
 hipError_t dgx_launch_batched_f64(const void *args, hipStream_t st, int which);
 int dgx_batched_shape_ok(int shot_tile, int wg_waves);
 hipError_t dgx_launch_batched_f32(const void *args, hipStream_t st, int which);
-hipError_t dgx_launch_reference(const void *args, const double *ops, int which, int nshots, int nsrc, int nrec, int t,
-                                double w, hipStream_t st);
+hipError_t dgx_launch_reference(const void *args, const double *ops, const double *mats, int which, int nshots,
+                                int nsrc, int nrec, int t, double w, hipStream_t st);
 }
 
 /* ---- error reporting ---- */
@@ -160,7 +160,8 @@ struct dgx_solver {
     double *wav = nullptr;
     double *trace = nullptr;
     int *step_base = nullptr;
-    double *refops = nullptr; /* DGX_FP64_REFERENCE: Dx | Dz | Q | A | B per element, blocked by 64 elements */
+    double *refops = nullptr; /* DGX_FP64_REFERENCE: Q | 1/rho | lambda + 2 mu | lambda | mu per element, blocked by 64 */
+    double refmats[108] = {};  /* DGX_FP64_REFERENCE: Vr | Vs | inverV */
     /* argument blocks */
     dgx::StepArgs<double> a64;
     dgx::StepArgs<float> a32;
@@ -354,8 +355,8 @@ static int launch_exact(dgx_solver *s, int which, int step_off, int t = -1, doub
     dgx::StepArgs<double> a = s->a64;
     a.step_off = step_off;
     a.src = s->src + s->nsrc;
-    const hipError_t e = dgx_launch_reference(&a, s->refops, which, (int)s->nshots, (int)s->nsrc, (int)s->nrec, t, w,
-                                              s->stream);
+    const hipError_t e = dgx_launch_reference(&a, s->refops, s->refmats, which, (int)s->nshots, (int)s->nsrc,
+                                              (int)s->nrec, t, w, s->stream);
     if (e != hipSuccess) {
         dgx_set_error("kernel launch failed: %s", hipGetErrorString(e));
         return DGX_EHIP;
@@ -895,11 +896,15 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     s->inv = std::move(md.inv);
     std::vector<int4> &nbr_host = md.nbr;
 
-    /* ---- DGX_FP64_REFERENCE: the reference's per-element arrays, formed on the host in chunks of the caller's
-     * elements and placed in device order, entry k of element q at ((q/64)*112 + k)*64 + q%64 ---- */
+    /* ---- DGX_FP64_REFERENCE: the reference's per-element Q and material factors, formed on the host in chunks of
+     * the caller's elements and placed in device order, entry k of element q at ((q/64)*34 + k)*64 + q%64 (Dx
+     * and Dz are formed by the kernels from the vertices and Vr, Vs, inverV) ---- */
     std::vector<double> refops_host;
     if (s->exact) {
-        refops_host.assign((size_t)s->tri_pad * 112, 0.0);
+        memcpy(s->refmats, ops->Vr, sizeof ops->Vr);
+        memcpy(s->refmats + 36, ops->Vs, sizeof ops->Vs);
+        memcpy(s->refmats + 72, ops->inverV, sizeof ops->inverV);
+        refops_host.assign((size_t)s->tri_pad * 34, 0.0);
         const long chunk = 65536;
         std::vector<double> Dx(chunk * 36), Dz(chunk * 36), Q(chunk * 30), A(chunk * 5), B(chunk * 5);
         for (long e0 = 0; e0 < T; e0 += chunk) {
@@ -910,12 +915,13 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
             if (rc) { dgx_set_error("dgx_create: reference operators failed"); free_solver(s); return rc; }
             for (long k = 0; k < n; k++) {
                 const long q = s->inv[e0 + k];
-                double *dst = &refops_host[(size_t)(q >> 6) * 112 * 64 + (q & 63)];
-                for (int c = 0; c < 36; c++) dst[(size_t)c * 64] = Dx[k * 36 + c];
-                for (int c = 0; c < 36; c++) dst[(size_t)(36 + c) * 64] = Dz[k * 36 + c];
-                for (int c = 0; c < 30; c++) dst[(size_t)(72 + c) * 64] = Q[k * 30 + c];
-                for (int c = 0; c < 5; c++) dst[(size_t)(102 + c) * 64] = A[k * 5 + c];
-                for (int c = 0; c < 5; c++) dst[(size_t)(107 + c) * 64] = B[k * 5 + c];
+                double *dst = &refops_host[(size_t)(q >> 6) * 34 * 64 + (q & 63)];
+                for (int c = 0; c < 30; c++) dst[(size_t)c * 64] = Q[k * 30 + c];
+                /* A = {1/rho, 1/rho, lambda + 2 mu, lambda, mu}, B = {1/rho, 1/rho, lambda, lambda + 2 mu, mu} */
+                dst[30 * 64] = A[k * 5 + 0];
+                dst[31 * 64] = A[k * 5 + 2];
+                dst[32 * 64] = A[k * 5 + 3];
+                dst[33 * 64] = A[k * 5 + 4];
             }
         }
     }

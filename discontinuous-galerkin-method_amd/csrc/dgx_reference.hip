@@ -3,19 +3,25 @@
  *
  * The fast FP64 kernels regroup the reference's sums (metric terms factored out of Dx/Dz, 1/D folded into the face
  * scalings, fused multiply-adds); their results differ from the reference's in the last bits, and over thousands
- * of steps by about 1e-12 of the field maxima.  These kernels instead take the reference's own per-element arrays
- * (Dx, Dz, Q, A, B as its setup forms them, computed on the host by dgx_reference_operators) and evaluate every
- * sum in the reference's order with every product rounded on its own, so that each value equals the reference's
- * bit for bit (up to the sign of exact zeros in the face jumps, which its additions erase):
+ * of steps by about 1e-12 of the field maxima.  These kernels instead use the reference's own per-element values
+ * (Dx, Dz, Q, A, B as its setup forms them; Q, A and B computed on the host by dgx_reference_operators) and
+ * evaluate every sum in the reference's order with every product rounded on its own, so that each value equals
+ * the reference's bit for bit (up to the sign of exact zeros in the face jumps, which its additions erase):
  *   velocity  OpenMP_Function.c:75-185    stress  OpenMP_Function.c:188-306
  *   source    OpenMP_Function.c:49-58     record  OpenMP_Function.c:61-72 (P2 weights; a vertex receiver is the
  *                                                  reference's nodal pick)
  * Contraction into fused multiply-adds is switched off for this whole file.
  *
  * One thread per element and shot, fields in the element layout of the other kernels (64-element blocks,
- * node-major).  The per-element arrays use the same blocking: entry k of element q lives at
- * ((q/64)*112 + k)*64 + q%64 with k = Dx 0..35 | Dz 36..71 | Q 72..101 | A 102..106 | B 107..111, so a
- * wavefront reading entry k of its 64 elements reads 512 contiguous bytes.
+ * node-major).  Dx and Dz (72 values per element) are not stored: each thread forms them row by row from its
+ * vertex coordinates and the reference-element matrices Vr, Vs, inverV exactly as the reference's setup does
+ * (Jacobian, fun_DG.c:421-455; Dx_Dz, fun_DG.c:1137-1192: the same products, quotients and sums in the same
+ * order, all of them correctly rounded IEEE operations, so the values are its values), which costs about 900
+ * FP64 operations per element and half-step instead of 576 bytes of memory traffic.  Q and the materials are
+ * read: entry k of element q lives at ((q/64)*34 + k)*64 + q%64, k = Q 0..29 | 1/rho, lambda + 2 mu, lambda, mu
+ * 30..33 (the values of the reference's A and B: A0 = A1 = B0 = B1 = 1/rho, A2 = B3 = lambda + 2 mu,
+ * A3 = B2 = lambda, A4 = B4 = mu; Elastic_A_B, fun_DG.c:1071-1090), so a wavefront reading entry k of its 64
+ * elements reads 512 contiguous bytes.
  */
 #include <hip/hip_runtime.h>
 
@@ -28,8 +34,14 @@
 namespace dgx {
 namespace exact {
 
-constexpr int kWidth = 112; /* per-element reals: Dx 36 | Dz 36 | Q 30 | A 5 | B 5 */
-constexpr int kDz = 36, kQ = 72, kA = 102, kB = 107;
+constexpr int kWidth = 34; /* per-element reals: Q 30 | 1/rho | lambda + 2 mu | lambda | mu */
+constexpr int kQ = 0, kRinv = 30, kL2m = 31, kLam = 32, kMu = 33;
+
+/* The reference-element matrices the reference's setup forms Dx and Dz from (row-major, as read from its
+ * operator files). */
+struct RefMats {
+    double Vr[36], Vs[36], iV[36];
+};
 
 __device__ __forceinline__ long long fidx(int q, int i) { return ((long long)(q >> 6) * 6 + i) * 64 + (q & 63); }
 __device__ __forceinline__ long long ridx(int q, int k) { return ((long long)(q >> 6) * kWidth + k) * 64 + (q & 63); }
@@ -44,12 +56,62 @@ __device__ __forceinline__ int element_of_thread(int nper)
 /* Face j's neighbour (device index, -1 on the outer boundary) from the neighbour record. */
 __device__ __forceinline__ int neighbour(const int4 &nb, int j) { return j == 0 ? nb.x : j == 1 ? nb.y : nb.z; }
 
+/* Metric terms as the reference computes them: half edge vectors, J = xr ys - xs yr (Jacobian, fun_DG.c:421-455),
+ * Rx = ys/J, Rz = -xs/J, Sx = -yr/J, Sz = xr/J (Dx_Dz, fun_DG.c:1137-1192). */
+struct Metric {
+    double Rx, Rz, Sx, Sz;
+};
+
+__device__ __forceinline__ Metric metric(const double2 *pt, int4 v)
+{
+    const double2 p0 = pt[v.x], p1 = pt[v.y], p2 = pt[v.z];
+    const double xr = (p1.x - p0.x) * 0.5, yr = (p1.y - p0.y) * 0.5;
+    const double xs = (p2.x - p0.x) * 0.5, ys = (p2.y - p0.y) * 0.5;
+    const double J = xr * ys - xs * yr;
+    return Metric{ys / J, -xs / J, -yr / J, xr / J};
+}
+
+/* Row p of Dx and Dz: g = R Vr + S Vs elementwise, then the row of g times inverV, summed from zero in k order
+ * (Dx_Dz, fun_DG.c:1137-1192). */
+__device__ __forceinline__ void dxdz_row(const double *m, const Metric &g, int p, double (&dx)[6], double (&dz)[6])
+{
+    double gx[6], gz[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        gx[k] = g.Rx * m[p * 6 + k] + g.Sx * m[36 + p * 6 + k];
+        gz[k] = g.Rz * m[p * 6 + k] + g.Sz * m[36 + p * 6 + k];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        double f1 = 0, f2 = 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            f1 = f1 + gx[k] * m[72 + k * 6 + q];
+            f2 = f2 + gz[k] * m[72 + k * 6 + q];
+        }
+        dx[q] = f1;
+        dz[q] = f2;
+    }
+}
+
+/* Vr | Vs | inverV into LDS: read there with one address per wavefront, they take no registers between uses (as
+ * arguments, the 108 values overflow the scalar registers and the spills cost vector registers). */
+__device__ __forceinline__ void stage_matrices(const RefMats &mats, double *sm)
+{
+    const double *src = &mats.Vr[0];
+    for (int k = threadIdx.x; k < 108; k += blockDim.x) sm[k] = src[k];
+    __syncthreads();
+}
+
 /* LU_Velocity (OpenMP_Function.c:75-185):
  *   Vx = (A0 (Dx Txx) + B0 (Dz Txz) + sum_j (Q_j0 S_j dTxx_j + Q_j1 S_j dTxz_j)) dt + Vx
  *   Vz = (A1 (Dx Txz) + B1 (Dz Tzz) + sum_j (Q_j2 S_j dTzz_j + Q_j3 S_j dTxz_j)) dt + Vz
  * with the jump dU = flag U_nb - U_own (flag 0 on the boundary). */
-__global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, const double *__restrict__ ops)
+__global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, const double *__restrict__ ops,
+                                                            RefMats mats)
 {
+    __shared__ double sm[108];
+    stage_matrices(mats, sm);
     const int q = element_of_thread(a.nper);
     if (q >= a.tri) return;
     const long long so = (long long)blockIdx.y * a.fstride * 6;
@@ -64,22 +126,23 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, 
         const long long o = fidx(q, i);
         tx[i] = Txx[o]; tz[i] = Tzz[o]; tc[i] = Txz[o];
     }
-    const double A0 = ops[ridx(q, kA + 0)], A1 = ops[ridx(q, kA + 1)];
-    const double B0 = ops[ridx(q, kB + 0)], B1 = ops[ridx(q, kB + 1)];
+    const Metric g = metric(a.pt, a.vtx[q]);
+    const double rinv = ops[ridx(q, kRinv)]; /* A0 = A1 = B0 = B1 */
     double accx[6], accz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
+        double dx[6], dz[6];
+        dxdz_row(sm, g, i, dx, dz);
         double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            const double dx = ops[ridx(q, i * 6 + n)], dz = ops[ridx(q, kDz + i * 6 + n)];
-            s1 = s1 + dx * tx[n];
-            s2 = s2 + dx * tc[n];
-            s3 = s3 + dz * tz[n];
-            s4 = s4 + dz * tc[n];
+            s1 = s1 + dx[n] * tx[n];
+            s2 = s2 + dx[n] * tc[n];
+            s3 = s3 + dz[n] * tz[n];
+            s4 = s4 + dz[n] * tc[n];
         }
-        accx[i] = A0 * s1 + B0 * s4;
-        accz[i] = A1 * s2 + B1 * s3;
+        accx[i] = rinv * s1 + rinv * s4;
+        accz[i] = rinv * s2 + rinv * s3;
     }
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
@@ -128,8 +191,11 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, 
  *   Txx = (A2 (Dx Vx) + B2 (Dz Vz) + sum_j (Q_j4 S_j dVx_j + Q_j5 S_j dVz_j)) dt + Txx
  *   Tzz = (A3 (Dx Vx) + B3 (Dz Vz) + sum_j (Q_j6 S_j dVx_j + Q_j7 S_j dVz_j)) dt + Tzz
  *   Txz = (A4 (Dx Vz) + B4 (Dz Vx) + sum_j (Q_j8 S_j dVx_j + Q_j9 S_j dVz_j)) dt + Txz */
-__global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, const double *__restrict__ ops)
+__global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, const double *__restrict__ ops,
+                                                          RefMats mats)
 {
+    __shared__ double sm[108];
+    stage_matrices(mats, sm);
     const int q = element_of_thread(a.nper);
     if (q >= a.tri) return;
     const long long so = (long long)blockIdx.y * a.fstride * 6;
@@ -144,24 +210,24 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, co
         const long long o = fidx(q, i);
         vx[i] = Vx[o]; vz[i] = Vz[o];
     }
-    double A[3], B[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) { A[c] = ops[ridx(q, kA + 2 + c)]; B[c] = ops[ridx(q, kB + 2 + c)]; }
+    const Metric g = metric(a.pt, a.vtx[q]);
+    const double l2m = ops[ridx(q, kL2m)], lam = ops[ridx(q, kLam)], mu = ops[ridx(q, kMu)];
     double axx[6], azz[6], axz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
+        double dx[6], dz[6];
+        dxdz_row(sm, g, i, dx, dz);
         double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            const double dx = ops[ridx(q, i * 6 + n)], dz = ops[ridx(q, kDz + i * 6 + n)];
-            s1 = s1 + dx * vx[n];
-            s2 = s2 + dx * vz[n];
-            s3 = s3 + dz * vx[n];
-            s4 = s4 + dz * vz[n];
+            s1 = s1 + dx[n] * vx[n];
+            s2 = s2 + dx[n] * vz[n];
+            s3 = s3 + dz[n] * vx[n];
+            s4 = s4 + dz[n] * vz[n];
         }
-        axx[i] = A[0] * s1 + B[0] * s4;
-        azz[i] = A[1] * s1 + B[1] * s4;
-        axz[i] = A[2] * s2 + B[2] * s3;
+        axx[i] = l2m * s1 + lam * s4; /* A2, B2 */
+        azz[i] = lam * s1 + l2m * s4; /* A3, B3 */
+        axz[i] = mu * s2 + mu * s3;   /* A4, B4 */
     }
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
@@ -257,15 +323,19 @@ __global__ void k_ref_record(StepArgs<double> a, int t, int nshots)
 
 extern "C" {
 /* which: 0 velocity, 1 stress, 2 sources at the device step, 3 sources with the sample w, 4 record into the
- * device step's slot, 5 record into slot t. */
-hipError_t dgx_launch_reference(const void *args, const double *ops, int which, int nshots, int nsrc, int nrec, int t,
-                                double w, hipStream_t st)
+ * device step's slot, 5 record into slot t.  ops: the per-element Q and materials; mats: Vr | Vs | inverV, 108
+ * doubles. */
+hipError_t dgx_launch_reference(const void *args, const double *ops, const double *mats, int which, int nshots,
+                                int nsrc, int nrec, int t, double w, hipStream_t st)
 {
     const dgx::StepArgs<double> &a = *(const dgx::StepArgs<double> *)args;
     const dim3 grid(DGX_NXCD * a.nper, nshots);
+    dgx::exact::RefMats m;
+    static_assert(sizeof m == 108 * sizeof(double), "RefMats is Vr | Vs | inverV, packed");
+    if (which <= 1) __builtin_memcpy(&m, mats, sizeof m);
     switch (which) {
-    case 0: dgx::exact::k_ref_velocity<<<grid, DGX_BLOCK, 0, st>>>(a, ops); break;
-    case 1: dgx::exact::k_ref_stress<<<grid, DGX_BLOCK, 0, st>>>(a, ops); break;
+    case 0: dgx::exact::k_ref_velocity<<<grid, DGX_BLOCK, 0, st>>>(a, ops, m); break;
+    case 1: dgx::exact::k_ref_stress<<<grid, DGX_BLOCK, 0, st>>>(a, ops, m); break;
     case 2:
     case 3:
         if (nsrc > 0) dgx::exact::k_ref_source<<<1, 64, 0, st>>>(a, nsrc, which == 3, w);
