@@ -16,8 +16,10 @@
  * node-major).  Dx and Dz (72 values per element) are not stored: each thread forms them row by row from its
  * vertex coordinates and the reference-element matrices Vr, Vs, inverV exactly as the reference's setup does
  * (Jacobian, fun_DG.c:421-455; Dx_Dz, fun_DG.c:1137-1192: the same products, quotients and sums in the same
- * order, all of them correctly rounded IEEE operations, so the values are its values), which costs about 900
- * FP64 operations per element and half-step instead of 576 bytes of memory traffic.  Q and the materials are
+ * order, all of them correctly rounded IEEE operations, so the values are its values), which costs about 1100
+ * FP64 operations per element and half-step instead of 576 bytes of memory traffic (measured on 640,000
+ * elements: 0.294 ms per step against 0.384 with Dx and Dz read from memory; with the rows in a loop and the
+ * matrices read from LDS per row, 160 registers and 3 wavefronts per SIMD instead of 256 and 1: 0.201 ms).  Q and the materials are
  * read: entry k of element q lives at ((q/64)*34 + k)*64 + q%64, k = Q 0..29 | 1/rho, lambda + 2 mu, lambda, mu
  * 30..33 (the values of the reference's A and B: A0 = A1 = B0 = B1 = 1/rho, A2 = B3 = lambda + 2 mu,
  * A3 = B2 = lambda, A4 = B4 = mu; Elastic_A_B, fun_DG.c:1071-1090), so a wavefront reading entry k of its 64
@@ -76,6 +78,10 @@ __device__ __forceinline__ Metric metric(const double2 *pt, int4 v)
 __device__ __forceinline__ void dxdz_row(const double *m, const Metric &g, int p, double (&dx)[6], double (&dz)[6])
 {
     double gx[6], gz[6];
+    int o = 0;
+    asm volatile("" : "+v"(o)); /* an offset the compiler cannot see through: the matrices are read in each row,
+                                 * not held in registers across the row loop */
+    m += o;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         gx[k] = g.Rx * m[p * 6 + k] + g.Sx * m[36 + p * 6 + k];
@@ -92,6 +98,14 @@ __device__ __forceinline__ void dxdz_row(const double *m, const Metric &g, int p
         dx[q] = f1;
         dz[q] = f2;
     }
+}
+
+/* a[0..4] = a[1..5], a[5] = v */
+__device__ __forceinline__ void shift_in(double (&a)[6], double v)
+{
+#pragma unroll
+    for (int k = 0; k < 5; k++) a[k] = a[k + 1];
+    a[5] = v;
 }
 
 /* Vr | Vs | inverV into LDS: read there with one address per wavefront, they take no registers between uses (as
@@ -128,8 +142,10 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, 
     }
     const Metric g = metric(a.pt, a.vtx[q]);
     const double rinv = ops[ridx(q, kRinv)]; /* A0 = A1 = B0 = B1 */
-    double accx[6], accz[6];
-#pragma unroll
+    /* rows one at a time (a loop, not unrolled: the unrolled rows are scheduled together and need more registers
+     * than a SIMD has), each result shifted into the last slot so that row i ends in slot i */
+    double accx[6] = {}, accz[6] = {};
+#pragma unroll 1
     for (int i = 0; i < 6; i++) {
         double dx[6], dz[6];
         dxdz_row(sm, g, i, dx, dz);
@@ -141,8 +157,8 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, 
             s3 = s3 + dz[n] * tz[n];
             s4 = s4 + dz[n] * tc[n];
         }
-        accx[i] = rinv * s1 + rinv * s4;
-        accz[i] = rinv * s2 + rinv * s3;
+        shift_in(accx, rinv * s1 + rinv * s4);
+        shift_in(accz, rinv * s2 + rinv * s3);
     }
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
@@ -212,9 +228,9 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, co
     }
     const Metric g = metric(a.pt, a.vtx[q]);
     const double l2m = ops[ridx(q, kL2m)], lam = ops[ridx(q, kLam)], mu = ops[ridx(q, kMu)];
-    double axx[6], azz[6], axz[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
+    double axx[6] = {}, azz[6] = {}, axz[6] = {};
+#pragma unroll 1
+    for (int i = 0; i < 6; i++) { /* rows one at a time, as in the velocity kernel */
         double dx[6], dz[6];
         dxdz_row(sm, g, i, dx, dz);
         double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
@@ -225,9 +241,9 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, co
             s3 = s3 + dz[n] * vx[n];
             s4 = s4 + dz[n] * vz[n];
         }
-        axx[i] = l2m * s1 + lam * s4; /* A2, B2 */
-        azz[i] = lam * s1 + l2m * s4; /* A3, B3 */
-        axz[i] = mu * s2 + mu * s3;   /* A4, B4 */
+        shift_in(axx, l2m * s1 + lam * s4); /* A2, B2 */
+        shift_in(azz, lam * s1 + l2m * s4); /* A3, B3 */
+        shift_in(axz, mu * s2 + mu * s3);   /* A4, B4 */
     }
     const int4 nb = a.nbr[q];
     const unsigned code = (unsigned)nb.w;
