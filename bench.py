@@ -412,6 +412,8 @@ def main():
     progress(f"full-step FP64: {secf * 1e3 / a.steps:.4f} ms/step")
     secd, ktd, _, _, _ = run_path(setup, L.DGX_FP64, a.warmup, a.steps, step_kernel=L.DGX_STEP_BLOCKED)  # the FP64 default
     progress(f"default (blocked) FP64: {secd * 1e3 / a.steps:.4f} ms/step")
+    secr, ktr, _, _, _ = run_path(setup, L.DGX_FP64_REFERENCE, a.warmup, a.steps, step_kernel=L.DGX_STEP_HALF)
+    progress(f"reference-arithmetic FP64: {secr * 1e3 / a.steps:.4f} ms/step")
     rate64 = dof * a.steps / sec64
     rate32 = dof * a.steps / sec32
     rate64f = dof * a.steps / secf
@@ -471,6 +473,13 @@ def main():
         "default_fp64": {"value": dof * a.steps / secd, "ms_per_step": secd * 1e3 / a.steps, "launch_ms": ktd["block"],
                          "tiling": ktd["info"], "x_vs_half_step": sec64 / secd,
                          "path": "the FP64 default: blocked kernel, 256-thread tiles, 2 steps per launch"},
+        "reference_arithmetic_fp64": {
+            "value": dof * a.steps / secr, "ms_per_step": secr * 1e3 / a.steps, "velocity_ms": ktr["velocity"],
+            "stress_ms": ktr["stress"], "time_vs_fp64_default": secr / secd,
+            "path": "DGX_FP64_REFERENCE: the reference's operations in its order, no fused multiply-adds; absorb, "
+                    "source, velocity, stress and record kernels per step",
+            "parity": "fields and traces equal the oracle's bit for bit (all 4000 steps of this workload: "
+                      "tests/test_gpu_configs.py::test_gradient_640k_all_4000_steps)"},
     }
     if "large" not in a.skip:
         out["plain_fp64_large"] = large_leg(a)

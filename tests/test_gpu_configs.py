@@ -103,6 +103,30 @@ def test_gradient_640k_reference_arithmetic_bit_identical(gradient):
 
 
 @pytest.mark.timeout(600)
+def test_gradient_640k_all_4000_steps():
+    """The 640,000-element gradient case over all 4000 steps (32 s of oracle time): the reference-arithmetic
+    precision equals the oracle bit for bit at every 1000th step and on every trace sample.  The FP64 default
+    (regrouped sums, fused multiply-adds) drifts from it by rounding that grows with the step count: measured
+    1.40e-12 / 1.28e-12 on the traces and at most 1.35e-12 on the step-4000 fields, above the 1e-12 bar that it
+    meets on the 1200-step runs; it is held here to 3e-12."""
+    w = dgx.configs.gradient_8x4km(4000)
+    print(f"case {w.name}: {w.mesh.tri} elements, 4000 steps", flush=True)
+    oc = OracleCase(w)
+    ox, oz, osnap = oc.run(snap_every=1000)
+    st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
+    gx, gz, gsnap = product_run(st, L.DGX_FP64_REFERENCE, snap_every=1000)
+    assert np.array_equal(gx, ox) and np.array_equal(gz, oz)
+    for k, f in osnap.items():
+        for c in range(5):
+            assert np.array_equal(gsnap[k][c], f[c]), (k, c)
+    gx, gz, gsnap = product_run(st, L.DGX_FP64, snap_every=1000)
+    ex, ez = rel_l2(gx, ox), rel_l2(gz, oz)
+    worst = max(rel_l2(gsnap[k][c], f[c]) for k, f in osnap.items() for c in range(5))
+    print(f"fp64 default after 4000 steps: traces {ex:.3e} {ez:.3e}, fields at most {worst:.3e}", flush=True)
+    assert max(ex, ez, worst) <= 3e-12
+
+
+@pytest.mark.timeout(600)
 def test_gradient_640k_fp32_against_oracle(gradient):
     w, oc, ox, oz, osnap = gradient
     st = dgx.build_setup(w.mesh, w.model, w.sources, w.receivers, w.nsteps, sponge=w.sponge)
