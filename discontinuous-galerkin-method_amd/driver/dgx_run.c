@@ -24,9 +24,10 @@
  *   source <x> <z> <force|explosive>  (repeatable; all in one shot)      [1200 1200 force]
  *   receiver <x> <z>         one receiver (repeatable)                   [1600 1600]
  *   receiver_line <x0> <z0> <x1> <z1> <n>   n receivers on a segment
- *   precision <fp64|fp32>                                                [fp64]
- *   step_kernel <auto|half|full|blocked>   two half-step kernels per step, one full-step kernel, or two whole
- *                                  steps per launch (auto: blocked in FP32, half in FP64)   [auto]
+ *   precision <fp64|fp32|fp32c|reference>   fp32c: compensated FP32; reference: the reference's arithmetic,
+ *                                  outputs bit-identical to its own loop (DGX_FP64_REFERENCE)   [fp64]
+ *   step_kernel <auto|half|full|blocked>   two half-step kernels per step, one full-step kernel, or whole
+ *                                  steps per launch (auto: blocked, half with precision reference)   [auto]
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -126,8 +127,13 @@ static int read_config(const char *path, config *c)
                 double t = n > 1 ? (double)k / (n - 1) : 0.0;
                 add_receiver(c, v[0] + t * (v[2] - v[0]), v[1] + t * (v[3] - v[1]));
             }
-        } else if (!strcmp(key, "precision") && sscanf(line, "%*s %31s", kind) == 1)
-            c->precision = !strcmp(kind, "fp32") ? DGX_FP32 : DGX_FP64;
+        } else if (!strcmp(key, "precision") && sscanf(line, "%*s %31s", kind) == 1 &&
+                   (!strcmp(kind, "fp64") || !strcmp(kind, "fp32") || !strcmp(kind, "fp32c") ||
+                    !strcmp(kind, "reference")))
+            c->precision = !strcmp(kind, "fp32")        ? DGX_FP32
+                           : !strcmp(kind, "fp32c")     ? DGX_FP32_COMPENSATED
+                           : !strcmp(kind, "reference") ? DGX_FP64_REFERENCE
+                                                        : DGX_FP64;
         else if (!strcmp(key, "step_kernel") && sscanf(line, "%*s %31s", kind) == 1)
             c->step_kernel = !strcmp(kind, "half")      ? DGX_STEP_HALF
                              : !strcmp(kind, "full")    ? DGX_STEP_FULL

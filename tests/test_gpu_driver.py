@@ -1,6 +1,7 @@
 """The C driver (bin/dgx_run) end to end: reference-format mesh files in, the reference's snapshot and
 seismogram files out, compared with the oracle (binary twins at 1e-12, text at its 14-decimal
-quantisation)."""
+quantisation) and, in the reference-arithmetic precision, with the reference build's own outputs (bit for
+bit)."""
 import pathlib
 import subprocess
 
@@ -81,3 +82,38 @@ def test_driver_binary_mesh_equals_text_mesh(tmp_path):
         outs.append(out)
     for f in ["Vx60.bin", "Vz120.bin", "record_x0.bin", "record_z3.bin"]:
         assert open(outs[0] / f, "rb").read() == open(outs[1] / f, "rb").read(), f
+
+
+GOLDEN = sorted((pathlib.Path(__file__).parent / "golden").glob("ref_*.npz"))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[p.stem for p in GOLDEN])
+def test_driver_reference_precision_reproduces_the_reference_build(tmp_path, path):
+    """`precision reference`: the driver's snapshot and seismogram twins equal the outputs the reference's own
+    build wrote for the same mesh (the committed fixtures), bit for bit; the mesh goes in losslessly as
+    mesh_binary, and the sponge, materials, frequency and Pi are the driver's defaults, which are the
+    reference's."""
+    assert BIN.exists(), "bin/dgx_run not built"
+    d = np.load(path, allow_pickle=False)
+    T, P = len(d["triangle"]), len(d["point"])
+    arrays = [np.ascontiguousarray(d[k].ravel(), t) for k, t in
+              (("point", np.float64), ("triangle", np.int64), ("neighbor", np.int64), ("attribute", np.int64))]
+    L.check(L.load().dgx_write_mesh_binary(str(tmp_path / "model.dgxm").encode(), T, P, L.dp(arrays[0]),
+                                           L.lp(arrays[1]), L.lp(arrays[2]), L.lp(arrays[3])))
+    n, every = int(d["nsteps"]), int(d["snap_every"])
+    out = tmp_path / "out"
+    out.mkdir()
+    (tmp_path / "run.cfg").write_text(
+        f"mesh_binary {tmp_path / 'model.dgxm'}\noperators {L.OPERATOR_DIR}\noutput {out}\nsteps {n}\n"
+        f"snapshot {every}\nsource {float(d['src'][0])!r} {float(d['src'][1])!r} force\n"
+        f"receiver {float(d['rec'][0])!r} {float(d['rec'][1])!r}\n"
+        "precision reference\n")
+    print(f"case driver reference precision {path.stem}", flush=True)
+    r = subprocess.run([str(BIN), str(tmp_path / "run.cfg")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for k in range(n // every):
+        step = (k + 1) * every
+        assert np.array_equal(np.fromfile(out / f"Vx{step}.bin"), d["snaps"][k][0]), step
+        assert np.array_equal(np.fromfile(out / f"Vz{step}.bin"), d["snaps"][k][1]), step
+    assert np.array_equal(np.fromfile(out / "record_x.bin"), d["rec_x"])
+    assert np.array_equal(np.fromfile(out / "record_z.bin"), d["rec_z"])
