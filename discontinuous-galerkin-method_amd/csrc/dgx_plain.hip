@@ -201,13 +201,78 @@ __device__ __forceinline__ void condition_neighbour_stresses(const StepArgs<R> &
     }
 }
 
+/* The P2 derivatives in factored form.  The interpolant of six nodal values u on the FeketeNew nodes
+ * (r, s) = (1,-1), (0,0), (-1,1), (0,-1), (-1,0), (-1,-1) is quadratic, so its derivatives are linear functions,
+ * du/dr = r0 + r1 r + r2 s and du/ds = s0 + s1 r + s2 s, with
+ *   r2 = s1 = u5 + u1 - u3 - u4;  X = u0 - u3 + r2, Y = u4 - u1: r0 = (X - Y)/2, r1 = X + Y;
+ *   Z = u2 - u4 + r2, W = u3 - u1: s0 = (Z - W)/2, s2 = Z + W
+ * (15 operations for both), and a linear function takes 6 operations to evaluate at the six nodes.  The
+ * operator files' Dr = Vr inverV and Ds = Vs inverV are these exact derivatives (entries 0, 1/2, 1, 3/2, 2 and
+ * their negatives) to within 4e-8, below FP32 resolution: rounded to FP32, 69 of their 72 entries equal the
+ * exact values and three are one ulp off.  The FP32 blocked kernel with the shipped operators therefore applies
+ * them in this form, 45 + 24 + 12 operations for the velocity volume term instead of 48 + 88 for the sparse
+ * matrix form (the compiled-in matrices keep FP64, which must match the files' values, and compensated FP32). */
+template <typename R>
+struct P2Lin {
+    R c0, c1, c2; /* c0 + c1 r + c2 s */
+};
+
+template <typename R>
+__device__ __forceinline__ void p2_grad(const R (&u)[6], P2Lin<R> &dr, P2Lin<R> &ds)
+{
+    const R a4 = (u[5] + u[1]) - (u[3] + u[4]);
+    const R X = (u[0] - u[3]) + a4, Y = u[4] - u[1];
+    const R Z = (u[2] - u[4]) + a4, W = u[3] - u[1];
+    dr = P2Lin<R>{(X - Y) * (R)0.5, X + Y, a4};
+    ds = P2Lin<R>{(Z - W) * (R)0.5, a4, Z + W};
+}
+
+/* p x + q y + s z + t w, coefficient by coefficient */
+template <typename R>
+__device__ __forceinline__ P2Lin<R> p2_comb(R p, const P2Lin<R> &x, R q, const P2Lin<R> &y, R s,
+                                            const P2Lin<R> &z, R t, const P2Lin<R> &w)
+{
+    return P2Lin<R>{p * x.c0 + q * y.c0 + s * z.c0 + t * w.c0, p * x.c1 + q * y.c1 + s * z.c1 + t * w.c1,
+                    p * x.c2 + q * y.c2 + s * z.c2 + t * w.c2};
+}
+
+/* p x + q y */
+template <typename R>
+__device__ __forceinline__ P2Lin<R> p2_comb(R p, const P2Lin<R> &x, R q, const P2Lin<R> &y)
+{
+    return P2Lin<R>{p * x.c0 + q * y.c0, p * x.c1 + q * y.c1, p * x.c2 + q * y.c2};
+}
+
+/* c0 + c1 r + c2 s at the six nodes */
+template <typename R>
+__device__ __forceinline__ void p2_eval(const P2Lin<R> &c, R (&out)[6])
+{
+    const R m = c.c0 - c.c1;
+    out[0] = (c.c0 + c.c1) - c.c2;
+    out[1] = c.c0;
+    out[2] = m + c.c2;
+    out[3] = c.c0 - c.c2;
+    out[4] = m;
+    out[5] = m - c.c2;
+}
+
 /* Velocity volume term: rhx = Dr(rx Txx + rz Txz) + Ds(sx Txx + sz Txz), rhz = Dr(rx Txz + rz Tzz) +
- * Ds(sx Txz + sz Tzz). */
-template <typename R, bool P2, typename G = Geom, bool SPARSE = false>
+ * Ds(sx Txz + sz Tzz).  FACTORED (FP32 with the compiled-in P2 operators): the same terms from the factored
+ * derivatives, rhx = rx dTxx/dr + sx dTxx/ds + rz dTxz/dr + sz dTxz/ds and likewise rhz. */
+template <typename R, bool P2, typename G = Geom, bool SPARSE = false, bool FACTORED = false>
 __device__ __forceinline__ void velocity_volume(const StepArgs<R> &a, const G &g, const R (&txx)[6],
                                                 const R (&tzz)[6], const R (&txz)[6], R (&rhx)[6], R (&rhz)[6])
 {
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
+    if constexpr (FACTORED && P2 && sizeof(R) == 4) {
+        P2Lin<R> rxx, sxx, rxz, sxz, rzz, szz;
+        p2_grad(txx, rxx, sxx);
+        p2_grad(txz, rxz, sxz);
+        p2_grad(tzz, rzz, szz);
+        p2_eval(p2_comb(rx, rxx, sx, sxx, rz, rxz, sz, sxz), rhx);
+        p2_eval(p2_comb(rx, rxz, sx, sxz, rz, rzz, sz, szz), rhz);
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 6; i++) { rhx[i] = 0; rhz[i] = 0; }
 #pragma unroll
@@ -346,11 +411,20 @@ __device__ __forceinline__ void gather_velocities(unsigned code, const int (&nbs
 
 /* Strain-rate volume terms: exx = Dx Vx, ezz = Dz Vz, exz = Dx Vz + Dz Vx with Dx = rx Dr + sx Ds and
  * Dz = rz Dr + sz Ds (operators from the kernel argument in both variants). */
-template <typename R, typename G = Geom, bool P2 = false, bool SPARSE = false>
+template <typename R, typename G = Geom, bool P2 = false, bool SPARSE = false, bool FACTORED = false>
 __device__ __forceinline__ void strain_volume(const StepArgs<R> &a, const G &g, const R (&vx)[6],
                                               const R (&vz)[6], R (&exx)[6], R (&ezz)[6], R (&exz)[6])
 {
     const R rx = (R)g.rx, rz = (R)g.rz, sx = (R)g.sx, sz = (R)g.sz;
+    if constexpr (FACTORED && P2 && sizeof(R) == 4) { /* the factored P2 derivatives, as in velocity_volume */
+        P2Lin<R> rvx, svx, rvz, svz;
+        p2_grad(vx, rvx, svx);
+        p2_grad(vz, rvz, svz);
+        p2_eval(p2_comb(rx, rvx, sx, svx), exx);
+        p2_eval(p2_comb(rz, rvz, sz, svz), ezz);
+        p2_eval(p2_comb(rx, rvz, sx, svz, rz, rvx, sz, svx), exz);
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         R rvx = 0, svx = 0, rvz = 0, svz = 0;
@@ -891,7 +965,7 @@ __device__ __forceinline__ void block_velocity(const StepArgs<R> &a, const GeomR
                                                R (&lo)[5][6])
 {
     R rhx[6], rhz[6];
-    velocity_volume<R, P2, GeomR<R>, true>(a, g, txx, tzz, txz, rhx, rhz);
+    velocity_volume<R, P2, GeomR<R>, true, !COMP>(a, g, txx, tzz, txz, rhx, rhz);
 #pragma unroll
     for (int fc = 0; fc < 3; fc++) {
         const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
@@ -936,7 +1010,7 @@ __device__ __forceinline__ void block_stress(const StepArgs<R> &a, const GeomR<R
                                              R (&txz)[6], R (&lo)[5][6])
 {
     R axx[6], azz[6], axz[6];
-    strain_volume<R, GeomR<R>, P2, true>(a, g, vx, vz, axx, azz, axz);
+    strain_volume<R, GeomR<R>, P2, true, !COMP>(a, g, vx, vz, axx, azz, axz);
 #pragma unroll
     for (int fc = 0; fc < 3; fc++) {
         const unsigned m = (code >> (3 * fc)) & 3u, rev = (code >> (3 * fc + 2)) & 1u;
