@@ -476,8 +476,9 @@ def main():
         "reference_arithmetic_fp64": {
             "value": dof * a.steps / secr, "ms_per_step": secr * 1e3 / a.steps, "velocity_ms": ktr["velocity"],
             "stress_ms": ktr["stress"], "time_vs_fp64_default": secr / secd,
-            "path": "DGX_FP64_REFERENCE: the reference's operations in its order, no fused multiply-adds; absorb, "
-                    "source, velocity, stress and record kernels per step",
+            "path": "DGX_FP64_REFERENCE: the reference's operations in its order, no fused multiply-adds; two "
+                    "kernels per step (sponge and sources applied as values load, receivers recorded by the stress "
+                    "kernel)",
             "parity": "fields and traces equal the oracle's bit for bit (all 4000 steps of this workload: "
                       "tests/test_gpu_configs.py::test_gradient_640k_all_4000_steps)"},
     }

@@ -116,7 +116,6 @@ struct dgx_solver {
     int fp32 = 0;
     int comp = 0;                    /* compensated FP32: a low word per value (blocked kernel) */
     int exact = 0;                   /* DGX_FP64_REFERENCE: the reference's arrays and order (dgx_reference.hip) */
-    int sponge = 0;                  /* the problem has sponge factors */
     int layout = DGX_LAYOUT_ELEMENT;
     int shot_tile = 0;               /* shot-innermost: shots per lane group */
     int wg_waves = 4;                /* shot-innermost: wavefronts per workgroup */
@@ -349,12 +348,13 @@ bool dgx_build_block_tiles(long tri, const std::vector<int4> &nbr, int K, int ca
     return true;
 }
 
-/* DGX_FP64_REFERENCE: one of its kernels (dgx_launch_reference's `which`); sources read the caller-ordered list. */
+/* DGX_FP64_REFERENCE: one of its kernels (dgx_launch_reference's `which`).  The split source kernel (2, 3) reads
+ * the caller-ordered source list, the fused step (6, 7) the aux-ordered one through the elements' aux records. */
 static int launch_exact(dgx_solver *s, int which, int step_off, int t = -1, double w = 0)
 {
     dgx::StepArgs<double> a = s->a64;
     a.step_off = step_off;
-    a.src = s->src + s->nsrc;
+    a.src = (which == 2 || which == 3) ? s->src + s->nsrc : s->src;
     const hipError_t e = dgx_launch_reference(&a, s->refops, s->refmats, which, (int)s->nshots, (int)s->nsrc,
                                               (int)s->nrec, t, w, s->stream);
     if (e != hipSuccess) {
@@ -368,8 +368,8 @@ static int launch(dgx_solver *s, int which, int step_off)
 {
     hipError_t e;
     if (s->exact && which != K_ABSORB) {
-        if (which == K_VEL || which == K_VEL_FUSED) return launch_exact(s, 0, step_off);
-        if (which == K_STRESS || which == K_STRESS_FUSED) return launch_exact(s, 1, step_off);
+        if (which == K_VEL || which == K_STRESS) return launch_exact(s, which == K_VEL ? 0 : 1, step_off);
+        if (which == K_VEL_FUSED || which == K_STRESS_FUSED) return launch_exact(s, which == K_VEL_FUSED ? 6 : 7, step_off);
         dgx_set_error("DGX_FP64_REFERENCE runs the split step kernels only");
         return DGX_ESTATE;
     }
@@ -735,7 +735,6 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     s->fp32 = (p->precision == DGX_FP32 || p->precision == DGX_FP32_COMPENSATED);
     s->comp = (p->precision == DGX_FP32_COMPENSATED);
     s->exact = (p->precision == DGX_FP64_REFERENCE);
-    s->sponge = p->absorb_fun != nullptr;
     s->tri = T;
     s->nshots = p->nshots;
     s->nsteps = p->nsteps;
@@ -1133,15 +1132,13 @@ extern "C" int dgx_destroy(dgx_solver *s)
     return DGX_OK;
 }
 
-/* DGX_FP64_REFERENCE: one step as the reference composes it (DG_leap_Frog_OpenMP.c:405-425): absorb, sources,
- * velocity, stress, record, each on the step at the device counter + off. */
+/* DGX_FP64_REFERENCE: one step as the reference composes it (DG_leap_Frog_OpenMP.c:405-425), on the step at the
+ * device counter + off, in two kernels: the velocity kernel applies the sponge and the sources to the values it
+ * loads, the stress kernel applies them again to the stresses it updates and records the receivers. */
 static int step_exact(dgx_solver *s, int off)
 {
-    int rc = s->sponge ? launch(s, K_ABSORB, off) : DGX_OK;
-    if (rc == DGX_OK && s->nsrc) rc = launch_exact(s, 2, off);
-    if (rc == DGX_OK) rc = launch_exact(s, 0, off);
-    if (rc == DGX_OK) rc = launch_exact(s, 1, off);
-    if (rc == DGX_OK && s->nrec) rc = launch_exact(s, 4, off);
+    int rc = launch_exact(s, 6, off);
+    if (rc == DGX_OK) rc = launch_exact(s, 7, off);
     return rc;
 }
 

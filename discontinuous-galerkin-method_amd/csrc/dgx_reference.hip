@@ -17,13 +17,18 @@
  * vertex coordinates and the reference-element matrices Vr, Vs, inverV exactly as the reference's setup does
  * (Jacobian, fun_DG.c:421-455; Dx_Dz, fun_DG.c:1137-1192: the same products, quotients and sums in the same
  * order, all of them correctly rounded IEEE operations, so the values are its values), which costs about 1100
- * FP64 operations per element and half-step instead of 576 bytes of memory traffic (measured on 640,000
- * elements: 0.294 ms per step against 0.384 with Dx and Dz read from memory; with the rows in a loop and the
- * matrices read from LDS per row, 160 registers and 3 wavefronts per SIMD instead of 256 and 1: 0.201 ms).  Q and the materials are
+ * FP64 operations per element and half-step instead of 576 bytes of memory traffic.  The rows run in a loop with
+ * the matrices read from LDS per row, which keeps the kernels at 2-3 wavefronts per SIMD.  Q and the materials are
  * read: entry k of element q lives at ((q/64)*34 + k)*64 + q%64, k = Q 0..29 | 1/rho, lambda + 2 mu, lambda, mu
  * 30..33 (the values of the reference's A and B: A0 = A1 = B0 = B1 = 1/rho, A2 = B3 = lambda + 2 mu,
  * A3 = B2 = lambda, A4 = B4 = mu; Elastic_A_B, fun_DG.c:1071-1090), so a wavefront reading entry k of its 64
  * elements reads 512 contiguous bytes.
+ *
+ * A step (dgx_step) is two kernels, FUSED: the velocity kernel applies the sponge and the step's sources to every
+ * value it loads, and writes the new velocities; the stress kernel applies them again to the stresses it updates
+ * (the same operations on the same stored values, so the same results) and records the receivers.  That is the
+ * reference's order, absorb, sources, velocity, stress, record, without separate passes.  The split entry points
+ * run the unfused kernels and the one-thread source and record kernels, one per reference function.
  */
 #include <hip/hip_runtime.h>
 
@@ -117,12 +122,74 @@ __device__ __forceinline__ void stage_matrices(const RefMats &mats, double *sm)
     __syncthreads();
 }
 
+/* The sponge and the sources of the step, applied to values as a kernel loads them (fused step), with the
+ * reference's operations in its order: x f (Absorb_Boundary, OpenMP_Function.c:25-46; elements outside the sponge
+ * have f = 1 and are skipped), then x + fun w for the element's sources in list order (source_kernel, :49-58).
+ * The aux record lists an element's sources in the caller's order.  Both half-step kernels condition the same
+ * stored values with the same operations, so they see the same conditioned values. */
+/* Stresses of N nodes (node[k]) of element q conditioned in place: the sponge factor of each node, then the
+ * element's explosive sources in list order. */
+template <int N>
+__device__ __forceinline__ void cond_stress(const StepArgs<double> &a, unsigned code, int q, const int (&node)[N],
+                                            int shot, int step, double (&txx)[N], double (&tzz)[N], double (&txz)[N])
+{
+    if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const double f = a.absorb[fidx(q, node[k])];
+            txx[k] = txx[k] * f; tzz[k] = tzz[k] * f; txz[k] = txz[k] * f;
+        }
+    }
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (ax) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+            const dgx_src_dev &src = a.src[s];
+            if (src.shot != shot || src.kind != DGX_SRC_EXPLOSIVE) continue;
+            const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                const double inc = src.fun[node[k]] * w;
+                txx[k] = txx[k] + inc;
+                tzz[k] = tzz[k] + inc;
+            }
+        }
+    }
+}
+
+/* Velocities of element q conditioned in place: the sponge, then the element's force sources in list order. */
+__device__ __forceinline__ void cond_velocity(const StepArgs<double> &a, unsigned code, int q, int shot, int step,
+                                              double (&vx)[6], double (&vz)[6])
+{
+    if (code & DGX_CODE_SPONGE) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            const double f = a.absorb[fidx(q, i)];
+            vx[i] = vx[i] * f; vz[i] = vz[i] * f;
+        }
+    }
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (ax) {
+        const dgx_aux x = a.aux[ax - 1];
+        for (int s = x.src_begin; s < x.src_begin + x.src_count; s++) {
+            const dgx_src_dev &src = a.src[s];
+            if (src.shot != shot || src.kind != DGX_SRC_FORCE_Z) continue;
+            const double w = a.wav[(long long)src.wavelet * a.nsteps + step];
+#pragma unroll
+            for (int i = 0; i < 6; i++) vz[i] = vz[i] + src.fun[i] * w;
+        }
+    }
+}
+
+constexpr int kAllNodes[6] = {0, 1, 2, 3, 4, 5};
+
 /* LU_Velocity (OpenMP_Function.c:75-185):
  *   Vx = (A0 (Dx Txx) + B0 (Dz Txz) + sum_j (Q_j0 S_j dTxx_j + Q_j1 S_j dTxz_j)) dt + Vx
  *   Vz = (A1 (Dx Txz) + B1 (Dz Tzz) + sum_j (Q_j2 S_j dTzz_j + Q_j3 S_j dTxz_j)) dt + Vz
  * with the jump dU = flag U_nb - U_own (flag 0 on the boundary). */
-__global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, const double *__restrict__ ops,
-                                                            RefMats mats)
+template <bool FUSED>
+__global__ void __launch_bounds__(DGX_BLOCK, FUSED ? 2 : 3)
+    k_ref_velocity(StepArgs<double> a, const double *__restrict__ ops, RefMats mats)
 {
     __shared__ double sm[108];
     stage_matrices(mats, sm);
@@ -134,12 +201,16 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, 
     const double *__restrict__ Txx = a.f[2] + so;
     const double *__restrict__ Tzz = a.f[3] + so;
     const double *__restrict__ Txz = a.f[4] + so;
+    const int shot = blockIdx.y, step = FUSED ? *a.step_base + a.step_off : 0;
+    const int4 nb = a.nbr[q];
+    const unsigned code = (unsigned)nb.w;
     double tx[6], tz[6], tc[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const long long o = fidx(q, i);
         tx[i] = Txx[o]; tz[i] = Tzz[o]; tc[i] = Txz[o];
     }
+    if (FUSED) cond_stress(a, code, q, kAllNodes, shot, step, tx, tz, tc);
     const Metric g = metric(a.pt, a.vtx[q]);
     const double rinv = ops[ridx(q, kRinv)]; /* A0 = A1 = B0 = B1 */
     /* rows one at a time (a loop, not unrolled: the unrolled rows are scheduled together and need more registers
@@ -160,24 +231,29 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, 
         shift_in(accx, rinv * s1 + rinv * s4);
         shift_in(accz, rinv * s2 + rinv * s3);
     }
-    const int4 nb = a.nbr[q];
-    const unsigned code = (unsigned)nb.w;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const int e = neighbour(nb, j);
         const unsigned m = (code >> (3 * j)) & 3u, rev = (code >> (3 * j + 2)) & 1u;
+        double nxx[3] = {}, nzz[3] = {}, nxz[3] = {};
+        int nn[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            nn[k] = nb_node(m, rev, k);
+            if (e >= 0) {
+                const long long o = fidx(e, nn[k]);
+                nxx[k] = Txx[o]; nzz[k] = Tzz[o]; nxz[k] = Txz[o];
+            }
+        }
+        if (FUSED && e >= 0 && (code & DGX_CODE_NBSPECIAL(j)))
+            cond_stress(a, (unsigned)a.nbr[e].w, e, nn, shot, step, nxx, nzz, nxz);
         double jxx[3], jzz[3], jxz[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int on = face_node(j, k);
-            double nxx = 0, nzz = 0, nxz = 0;
-            if (e >= 0) {
-                const long long o = fidx(e, nb_node(m, rev, k));
-                nxx = Txx[o]; nzz = Tzz[o]; nxz = Txz[o];
-            }
-            jxx[k] = nxx - tx[on];
-            jzz[k] = nzz - tz[on];
-            jxz[k] = nxz - tc[on];
+            jxx[k] = nxx[k] - tx[on];
+            jzz[k] = nzz[k] - tz[on];
+            jxz[k] = nxz[k] - tc[on];
         }
         const double q0 = ops[ridx(q, kQ + j * 10 + 0)], q1 = ops[ridx(q, kQ + j * 10 + 1)];
         const double q2 = ops[ridx(q, kQ + j * 10 + 2)], q3 = ops[ridx(q, kQ + j * 10 + 3)];
@@ -195,11 +271,18 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, 
             accz[i] = accz[i] + (q2 * sb + q3 * sc);
         }
     }
+    double vx[6], vz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const long long o = fidx(q, i);
-        Vx[o] = accx[i] * a.dt + Vx[o];
-        Vz[o] = accz[i] * a.dt + Vz[o];
+        vx[i] = Vx[o]; vz[i] = Vz[o];
+    }
+    if (FUSED) cond_velocity(a, code, q, shot, step, vx, vz);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = fidx(q, i);
+        Vx[o] = accx[i] * a.dt + vx[i];
+        Vz[o] = accz[i] * a.dt + vz[i];
     }
 }
 
@@ -207,8 +290,9 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_velocity(StepArgs<double> a, 
  *   Txx = (A2 (Dx Vx) + B2 (Dz Vz) + sum_j (Q_j4 S_j dVx_j + Q_j5 S_j dVz_j)) dt + Txx
  *   Tzz = (A3 (Dx Vx) + B3 (Dz Vz) + sum_j (Q_j6 S_j dVx_j + Q_j7 S_j dVz_j)) dt + Tzz
  *   Txz = (A4 (Dx Vz) + B4 (Dz Vx) + sum_j (Q_j8 S_j dVx_j + Q_j9 S_j dVz_j)) dt + Txz */
-__global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, const double *__restrict__ ops,
-                                                          RefMats mats)
+template <bool FUSED>
+__global__ void __launch_bounds__(DGX_BLOCK, FUSED ? 2 : 3)
+    k_ref_stress(StepArgs<double> a, const double *__restrict__ ops, RefMats mats)
 {
     __shared__ double sm[108];
     stage_matrices(mats, sm);
@@ -228,6 +312,7 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, co
     }
     const Metric g = metric(a.pt, a.vtx[q]);
     const double l2m = ops[ridx(q, kL2m)], lam = ops[ridx(q, kLam)], mu = ops[ridx(q, kMu)];
+    const int shot = blockIdx.y, step = FUSED ? *a.step_base + a.step_off : 0;
     double axx[6] = {}, azz[6] = {}, axz[6] = {};
 #pragma unroll 1
     for (int i = 0; i < 6; i++) { /* rows one at a time, as in the velocity kernel */
@@ -280,12 +365,35 @@ __global__ void __launch_bounds__(DGX_BLOCK) k_ref_stress(StepArgs<double> a, co
             axz[i] = axz[i] + (qq[4] * sa + qq[5] * sb);
         }
     }
+    double txx[6], tzz[6], txz[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const long long o = fidx(q, i);
-        Txx[o] = axx[i] * a.dt + Txx[o];
-        Tzz[o] = azz[i] * a.dt + Tzz[o];
-        Txz[o] = axz[i] * a.dt + Txz[o];
+        txx[i] = Txx[o]; tzz[i] = Tzz[o]; txz[i] = Txz[o];
+    }
+    if (FUSED) cond_stress(a, code, q, kAllNodes, shot, step, txx, tzz, txz);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const long long o = fidx(q, i);
+        Txx[o] = axx[i] * a.dt + txx[i];
+        Tzz[o] = azz[i] * a.dt + tzz[i];
+        Txz[o] = axz[i] * a.dt + txz[i];
+    }
+    /* Record (OpenMP_Function.c:61-72) of this element's receivers: the velocities are final for the step */
+    const unsigned ax = code >> DGX_CODE_AUX_SHIFT;
+    if (FUSED && ax) {
+        const dgx_aux x = a.aux[ax - 1];
+        double *tr = a.trace + ((long long)shot * a.nsteps + step) * 2 * a.nrec;
+        for (int r = x.rec_begin; r < x.rec_begin + x.rec_count; r++) {
+            const dgx_rec_dev &rc = a.rec[r];
+            double sx = 0, sz = 0;
+            for (int i = 0; i < 6; i++) {
+                sx = sx + rc.w[i] * vx[i];
+                sz = sz + rc.w[i] * vz[i];
+            }
+            tr[rc.index] = sx;
+            tr[a.nrec + rc.index] = sz;
+        }
     }
 }
 
@@ -339,7 +447,9 @@ __global__ void k_ref_record(StepArgs<double> a, int t, int nshots)
 
 extern "C" {
 /* which: 0 velocity, 1 stress, 2 sources at the device step, 3 sources with the sample w, 4 record into the
- * device step's slot, 5 record into slot t.  ops: the per-element Q and materials; mats: Vr | Vs | inverV, 108
+ * device step's slot, 5 record into slot t; 6 and 7 the velocity and stress kernels of the fused step (sponge and
+ * sources applied as values are loaded, receivers recorded by the stress kernel), sources read from the aux-ordered
+ * list.  ops: the per-element Q and materials; mats: Vr | Vs | inverV, 108
  * doubles. */
 hipError_t dgx_launch_reference(const void *args, const double *ops, const double *mats, int which, int nshots,
                                 int nsrc, int nrec, int t, double w, hipStream_t st)
@@ -348,10 +458,12 @@ hipError_t dgx_launch_reference(const void *args, const double *ops, const doubl
     const dim3 grid(DGX_NXCD * a.nper, nshots);
     dgx::exact::RefMats m;
     static_assert(sizeof m == 108 * sizeof(double), "RefMats is Vr | Vs | inverV, packed");
-    if (which <= 1) __builtin_memcpy(&m, mats, sizeof m);
+    if (which <= 1 || which >= 6) __builtin_memcpy(&m, mats, sizeof m);
     switch (which) {
-    case 0: dgx::exact::k_ref_velocity<<<grid, DGX_BLOCK, 0, st>>>(a, ops, m); break;
-    case 1: dgx::exact::k_ref_stress<<<grid, DGX_BLOCK, 0, st>>>(a, ops, m); break;
+    case 0: dgx::exact::k_ref_velocity<false><<<grid, DGX_BLOCK, 0, st>>>(a, ops, m); break;
+    case 1: dgx::exact::k_ref_stress<false><<<grid, DGX_BLOCK, 0, st>>>(a, ops, m); break;
+    case 6: dgx::exact::k_ref_velocity<true><<<grid, DGX_BLOCK, 0, st>>>(a, ops, m); break;
+    case 7: dgx::exact::k_ref_stress<true><<<grid, DGX_BLOCK, 0, st>>>(a, ops, m); break;
     case 2:
     case 3:
         if (nsrc > 0) dgx::exact::k_ref_source<<<1, 64, 0, st>>>(a, nsrc, which == 3, w);

@@ -32,12 +32,11 @@ extern "C" {
  * number of steps; snapshots and traces return the sum of the two words.  Runs on the blocked kernel only
  * (element layout) and moves twice the field bytes. */
 /* DGX_FP64_REFERENCE: FP64 fields advanced with the reference's own arithmetic: its per-element Dx, Dz, Q, A, B
- * (formed on the host as its setup forms them, dgx_reference_operators) held on the device, every sum in its
- * order, no fused multiply-adds, and the step split as the reference splits it (absorb, sources, velocity,
- * stress, record), so that fields and traces equal the reference's bit for bit.  Element layout and one kernel
- * per step function (step_kernel AUTO or HALF).  It forms Dx and Dz in every half-step (about 1100 FP64 operations
- * per element, without fused multiply-adds) and reads Q and the materials (272 bytes per element): 2.8x the time
- * per step of DGX_FP64 (640,000 elements 0.201 against 0.072 ms, 2M 0.589 against 0.208 ms). */
+ * (Q, A, B formed on the host as its setup forms them, dgx_reference_operators; Dx, Dz in the kernels with its
+ * operations), every sum in its order, no fused multiply-adds, sponge and sources applied before the velocity
+ * update and receivers recorded after the stress update, so that fields and traces equal the reference's bit for
+ * bit.  Element layout, two kernels per step (step_kernel AUTO or HALF): 2.4x the time per step of DGX_FP64
+ * (640,000 elements 0.171 against 0.072 ms, 2M 0.532 against 0.208 ms). */
 enum dgx_precision { DGX_FP64 = 0, DGX_FP32 = 1, DGX_FP32_COMPENSATED = 2, DGX_FP64_REFERENCE = 3 };
 enum dgx_source_kind { DGX_SRC_FORCE_Z = 0, DGX_SRC_EXPLOSIVE = 1 };
 /* Field layout on the device.  ELEMENT: 64-element blocks, one thread per element and shot (the
