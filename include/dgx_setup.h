@@ -90,7 +90,8 @@ int dgx_locate_sources_device(long tri, long point_num, const double *point, con
  * fun_DG.c:1137-1192), the face flux coefficients Q [tri*30] (Central_flux, fun_DG.c:1091-1136, from the
  * normals and edge Jacobians of fun_DG.c:421-554) and the material factors A, B [tri*5] (Elastic_A_B,
  * fun_DG.c:1071-1090), with the reference's expressions in its order, so that they equal its arrays bit for
- * bit.  DGX_FP64_REFERENCE steps with these; the caller needs them only to inspect them. */
+ * bit.  DGX_FP64_REFERENCE steps with Q, A and B from here and forms Dx and Dz in its kernels with the same
+ * operations; the caller needs these arrays only to inspect them. */
 int dgx_reference_operators(long tri, const double *point, const long *triangle, const double *lambda,
                             const double *miu, const double *den, const dgx_operators *ops, double *Dx, double *Dz,
                             double *Q, double *A, double *B);
