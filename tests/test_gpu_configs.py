@@ -254,6 +254,14 @@ def test_carbonate_2400k():
     assert ex <= FP64_TOL and ez <= FP64_TOL
     for c in range(5):
         assert rel_l2(gsnap[100][c], osnap[100][c]) <= FP64_TOL, c
+    # the reference-arithmetic precision on the same inputs (source vector, receiver weights and sponge equal the
+    # oracle's bit for bit): the oracle's traces and fields exactly
+    assert np.array_equal(np.array(list(st1.sources[0].fun)), oc.src[0][2])
+    assert all(np.array_equal(np.array(list(r.weight)), wgt) for r, (e, wgt) in zip(st1.receivers, oc.rec))
+    gx, gz, gsnap = product_run(st1, L.DGX_FP64_REFERENCE, snap_every=100)
+    assert np.array_equal(gx, ox) and np.array_equal(gz, oz)
+    for c in range(5):
+        assert np.array_equal(gsnap[100][c], osnap[100][c]), c
     # four shots served in FP32 (one request, 1000 steps) against FP64 single-shot runs
     picks = [0, 85, 170, 255]
     sv = S.Server(L.DGX_FP32, 0, 100, group_shots=4)
