@@ -789,15 +789,19 @@ extern "C" int dgx_create(const dgx_problem *p, const dgx_operators *ops, dgx_so
     }
     s->full = p->step_kernel == DGX_STEP_FULL;
     s->blocked = p->step_kernel == DGX_STEP_BLOCKED || s->comp || auto_blocked;
-    /* steps per launch by default: FP32 3 up to 50,000 elements, where the launch itself is a large part of a
-     * step, and 3 where the two field copies exceed the 256 MB Infinity Cache, so that fewer passes over them
-     * pay for the larger rings (ms per step, K = 2 / 3: 1.2M elements 0.0670 / 0.0627, 64 shots of them
-     * 3.845 / 3.706, 2M 0.1056 / 0.0971, 640k x 2 shots 0.0658 / 0.0613); 2 in between, where the rings'
-     * recomputation dominates (120k 0.0104 / 0.0110, 640k 0.0282 / 0.0297); FP64 2 (its 256-thread tiles:
-     * 20k 0.0063 / 0.0073, 640k 0.0714 / 0.0817, 2M 0.2059 / 0.2500) */
+    /* steps per launch by default: FP32 3 up to 300,000 elements (compensated FP32: 50,000), where the launch and
+     * the tile's loads are a large part of a step, and 3 where the two field copies exceed the 256 MB Infinity
+     * Cache, so that fewer passes over them pay for the larger rings; 2 in between, where the rings'
+     * recomputation dominates.  FP32 with the factored P2 derivatives, ms per step, K = 2 / 3: 120k 0.0097 /
+     * 0.0079, 250k 0.0154 / 0.0142, 400k 0.0195 / 0.0196, 500k 0.0243 / 0.0226, 560k 0.0256 / 0.0254, 640k
+     * 0.0284 / 0.0293, 640k x 2 shots 0.0656 / 0.0602, 2M 0.1059 / 0.0969; 64 shots of 1.2M elements 3.845 /
+     * 3.706 (before the factored derivatives).  FP64 2 (its 256-thread tiles: 20k 0.0063 / 0.0073, 640k 0.0714 /
+     * 0.0817, 2M 0.2059 / 0.2500) */
     const double copies_bytes = 2.0 * 30 * (s->fp32 ? 4 : 8) * (s->comp ? 2 : 1) * (double)T * (double)p->nshots;
     const bool beyond_cache = copies_bytes > 256.0 * 1024 * 1024;
-    s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : (s->fp32 && (T <= 50000 || beyond_cache) ? 3 : 2)) : 0;
+    const long k3_upto = s->comp ? 50000 : 300000;
+    s->bkmax = s->blocked ? (p->block_steps ? p->block_steps : (s->fp32 && (T <= k3_upto || beyond_cache) ? 3 : 2))
+                          : 0;
     if (s->layout == DGX_LAYOUT_SHOT_INNER) {
         const int t = p->shot_tile;
         if (t != 0 && t != 4 && t != 8 && t != 16 && t != 32 && t != 64) {

@@ -113,8 +113,8 @@ typedef struct {
                             * tiles 16-64, 4 with tiles 4 and 8.  0: autotuned with the shot tile when
                             * shot_tile is 0, else 4.  Given with shot_tile 0, only the tile is tuned. */
     int block_steps;       /* steps per launch of the blocked kernel (DGX_STEP_BLOCKED, or AUTO when it picks the
-                            * blocked kernel): 1, 2 or 3; 0 = 2, and 3 in FP32 up to 50,000 elements or where
-                            * the two FP32 field copies of all shots exceed 256 MB */
+                            * blocked kernel): 1, 2 or 3; 0 = 2, and 3 in FP32 up to 300,000 elements (compensated
+                            * FP32: 50,000) or where the two FP32 field copies of all shots exceed 256 MB */
 } dgx_problem;
 
 typedef struct dgx_solver dgx_solver;
