@@ -460,3 +460,41 @@ def test_point_source_symmetry_and_isotropy(kind):
         bx[t], bz[t] = run.record(f)
     bad_ratio, bad_tang = pattern(bx[:, :k], bz[:, :k])
     assert bad_ratio < 0.75 and bad_tang > 0.1, (bad_ratio, bad_tang)
+
+
+def test_operator_files_hold_the_exact_p2_derivatives():
+    """Dr = Vr inverV and Ds = Vs inverV of the operator files are the derivatives of the P2 interpolant at the
+    FeketeNew nodes to within 4e-8; rounded to FP32, 69 of their 72 entries equal the exact values and three are
+    one ulp off.  The factored form the FP32 blocked kernel uses (15 adds and a halving for both derivatives, then
+    an evaluation of a linear function at the six nodes) applies exactly these derivatives."""
+    ops = O.read_operators(OPDIR)
+    Dr, Ds = ops["Vr"] @ ops["inverV"], ops["Vs"] @ ops["inverV"]
+    r, s = ops["FeketeNew"][:, 0], ops["FeketeNew"][:, 1]
+    V = np.stack([np.ones(6), r, s, r * r, r * s, s * s], 1)
+    Vr = np.stack([np.zeros(6), np.ones(6), np.zeros(6), 2 * r, s, np.zeros(6)], 1)
+    Vs = np.stack([np.zeros(6), np.zeros(6), np.ones(6), np.zeros(6), r, 2 * s], 1)
+    Dre, Dse = Vr @ np.linalg.inv(V), Vs @ np.linalg.inv(V)
+    Dre, Dse = np.round(Dre * 2) / 2, np.round(Dse * 2) / 2  # entries are multiples of 1/2
+    assert np.abs(Dr - Dre).max() < 4e-8 and np.abs(Ds - Dse).max() < 4e-8
+    f32 = lambda m: np.where(np.abs(m) < 1e-6, 0.0, m).astype(np.float32)  # sub-ulp file noise on zeros
+    off = np.concatenate([(f32(Dr) != f32(Dre)).ravel(), (f32(Ds) != f32(Dse)).ravel()])
+    assert off.sum() == 3
+    ulp = np.spacing(np.float32(0.5))
+    assert np.all(np.abs(f32(Dr) - f32(Dre)) <= ulp) and np.all(np.abs(f32(Ds) - f32(Dse)) <= ulp)
+
+    def grad(u):  # the kernel's p2_grad
+        a4 = (u[5] + u[1]) - (u[3] + u[4])
+        X, Y = (u[0] - u[3]) + a4, u[4] - u[1]
+        Z, W = (u[2] - u[4]) + a4, u[3] - u[1]
+        return ((X - Y) * 0.5, X + Y, a4), ((Z - W) * 0.5, a4, Z + W)
+
+    def evaluate(c):  # the kernel's p2_eval
+        m = c[0] - c[1]
+        return np.array([(c[0] + c[1]) - c[2], c[0], m + c[2], c[0] - c[2], m, m - c[2]])
+
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        u = rng.standard_normal(6)
+        gr, gs = grad(u)
+        assert np.allclose(evaluate(gr), Dre @ u, rtol=0, atol=1e-14)
+        assert np.allclose(evaluate(gs), Dse @ u, rtol=0, atol=1e-14)
