@@ -577,7 +577,7 @@ __device__ __forceinline__ void velocity_update(const StepArgs<R> &a, int q, int
         }
     }
     R rhx[6], rhz[6];
-    velocity_volume<R, P2>(a, g, txx, tzz, txz, rhx, rhz);
+    velocity_volume<R, P2, Geom, false, true>(a, g, txx, tzz, txz, rhx, rhz);
 
     const int nbs[3] = {nb.x, nb.y, nb.z};
     bool global[3];
@@ -647,7 +647,7 @@ __device__ __forceinline__ void velocity_update_staged(const StepArgs<R> &a, int
         wsh[(2 * 6 + i) * DGX_BLOCK + threadIdx.x] = txz[i];
     }
     R rhx[6], rhz[6];
-    velocity_volume<R, P2>(a, g, txx, tzz, txz, rhx, rhz);
+    velocity_volume<R, P2, Geom, false, true>(a, g, txx, tzz, txz, rhx, rhz);
 
     __syncthreads();
 #pragma unroll
@@ -753,7 +753,7 @@ __device__ __forceinline__ void stress_update_staged(const StepArgs<R> &a, int q
         wsh[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = vz[i];
     }
     R exx[6], ezz[6], exz[6];
-    strain_volume(a, g, vx, vz, exx, ezz, exz);
+    strain_volume<R, Geom, P2, false, true>(a, g, vx, vz, exx, ezz, exz);
 
     __syncthreads();
 #pragma unroll
