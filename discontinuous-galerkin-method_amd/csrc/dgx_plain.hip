@@ -753,7 +753,9 @@ __device__ __forceinline__ void stress_update_staged(const StepArgs<R> &a, int q
         wsh[(1 * 6 + i) * DGX_BLOCK + threadIdx.x] = vz[i];
     }
     R exx[6], ezz[6], exz[6];
-    strain_volume<R, Geom, P2, false, true>(a, g, vx, vz, exx, ezz, exz);
+    /* FP64 reads Dr/Ds from the argument here (compiled in, they cost the kernel its occupancy); FP32 with the
+     * shipped operators uses the factored derivatives */
+    strain_volume<R, Geom, P2 && sizeof(R) == 4, false, true>(a, g, vx, vz, exx, ezz, exz);
 
     __syncthreads();
 #pragma unroll
